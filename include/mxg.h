@@ -1,0 +1,199 @@
+/*
+ * mxg.h — C ABI of the MI355X engine for mdcore's force-and-advance step.
+ *
+ * Plain C types only (pointers, sizes, scalars).  Every function returns 0
+ * or a negative error code that mirrors mdcore's engine_err_* table
+ * (src/mdcore/include/engine.h:33-63); the engine keeps the first error
+ * sticky, as mdcore's engine_err does (MxUniverse.cpp:551-559), and refuses
+ * further steps until it is destroyed.  Each declaration names the reference
+ * interface it replaces.  Paths are relative to the reference repository.
+ */
+#ifndef MXG_H_
+#define MXG_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes (engine.h:33-63). */
+#define MXG_ERR_OK 0
+#define MXG_ERR_NULL (-1)
+#define MXG_ERR_MALLOC (-2)
+#define MXG_ERR_SPACE (-3)
+#define MXG_ERR_RANGE (-6)
+#define MXG_ERR_BOND (-11)
+#define MXG_ERR_POTENTIAL (-17)
+#define MXG_ERR_DEVICE (-21)
+#define MXG_ERR_MAXPARTS (-24)
+#define MXG_ERR_CUTOFF (-27)
+#define MXG_ERR_TOOFAST (-29)
+
+/* Particle dynamics (MxParticle.h PARTICLE_NEWTONIAN / PARTICLE_OVERDAMPED). */
+#define MXG_NEWTONIAN 0
+#define MXG_OVERDAMPED 1
+
+/* Particle flags honoured by the advance (MxParticle.h PARTICLE_FROZEN_*). */
+#define MXG_FROZEN_X (1 << 2)
+#define MXG_FROZEN_Y (1 << 3)
+#define MXG_FROZEN_Z (1 << 4)
+
+/* Potential kinds and flags (MxPotential.h:55-110). */
+#define MXG_POTENTIAL_KIND_TABLE 0
+#define MXG_POTENTIAL_KIND_DPD 1
+#define MXG_POTENTIAL_SCALED (1u << 11)
+#define MXG_POTENTIAL_SHIFTED (1u << 12)
+
+typedef struct mxg_potential mxg_potential;
+typedef struct mxg_engine mxg_engine;
+
+typedef struct mxg_potential_info {
+  uint32_t kind;
+  uint32_t flags;
+  int32_t n;           /* number of intervals (table potentials) */
+  float alpha[4];      /* interval map coefficients */
+  float a, b;          /* domain */
+  float r0_plusone;    /* shift of SHIFTED potentials */
+  float dpd_alpha, dpd_gamma, dpd_sigma;
+} mxg_potential_info;
+
+/* ---- Potentials (host-side table construction) ------------------------ */
+
+/* potential_create_LJ126 (src/mdcore/src/MxPotential.cpp:976-991), Python
+ * m.Potential.lennard_jones_12_6 (:2459-2473). */
+mxg_potential *mxg_potential_create_lj126(double min, double max, double A, double B, double tol);
+
+/* potential_create_morse (MxPotential.cpp:3395-3424), m.Potential.morse
+ * (:2652-2667). */
+mxg_potential *mxg_potential_create_morse(double d, double a, double r0, double min, double max,
+                                          double tol);
+
+/* potential_create_harmonic (MxPotential.cpp:348-366), m.Potential.harmonic
+ * (:2545-2559). */
+mxg_potential *mxg_potential_create_harmonic(double min, double max, double K, double r0, double tol);
+
+/* DPDPotential_New (DissapativeParticleDynamics.cpp:134-148). */
+mxg_potential *mxg_potential_create_dpd(float alpha, float gamma, float sigma, float cutoff, int shifted);
+
+/* Py_DECREF of an MxPotential. */
+void mxg_potential_free(mxg_potential *p);
+
+/* Last error of the potential constructors in this thread (potential_err,
+ * MxPotential.h:31-36). */
+int mxg_potential_error(void);
+
+/* Fields of struct MxPotential (MxPotential.h:121-155). */
+int mxg_potential_get_info(const mxg_potential *p, mxg_potential_info *out);
+
+/* The (n+1)*8 table coefficients (MxPotential.h:131); returns the count or a
+ * negative code if capacity is too small. */
+int mxg_potential_coefficients(const mxg_potential *p, float *c, int capacity);
+
+/* potential_eval (potential_eval.hpp:67-104). */
+int mxg_potential_eval(const mxg_potential *p, float r2, float *e, float *f);
+
+/* potential_eval_ex (potential_eval.hpp:106-161); returns 1 in range, 0 not. */
+int mxg_potential_eval_ex(const mxg_potential *p, float ri, float rj, float r2, float *e, float *f);
+
+/* ---- Engine ------------------------------------------------------------ */
+
+/* engine_init (engine.h:600-601, engine.cpp:1502-1679) with the periodic
+ * boundary default and universe_init's dt (MxSimulator.cpp:349-422). */
+int mxg_engine_init(mxg_engine **e, const double origin[3], const double dim[3], const int cells[3],
+                    double cutoff, double dt);
+
+/* A batch of independent periodic universes stepped together (no reference
+ * counterpart; each replica behaves as its own engine_init). origins, dims
+ * and cells hold 3 values per replica. */
+int mxg_ensemble_init(mxg_engine **e, int nrep, const double *origins, const double *dims, const int *cells,
+                      double cutoff, double dt);
+
+/* engine_finalize. */
+void mxg_engine_free(mxg_engine *e);
+
+/* engine_addtype (engine.h:573-574); returns the type id. */
+int mxg_engine_addtype(mxg_engine *e, double mass, float radius, int dynamics);
+
+/* engine_addpot (engine.h:440, engine.cpp:960-981). */
+int mxg_engine_addpot(mxg_engine *e, const mxg_potential *p, int i, int j);
+
+/* engine_add_singlebody_force with random_create (MxForce.cpp:370-407):
+ * a persistent random force of magnitude N(mean, std) in a uniform
+ * direction, refreshed every ceil(duration/dt) steps. */
+int mxg_engine_add_random_force(mxg_engine *e, int type, float mean, float std, float duration);
+
+/* Seed of the counter-based generator (no reference counterpart: the
+ * reference uses a global Mersenne twister). */
+int mxg_engine_set_seed(mxg_engine *e, uint64_t seed);
+
+/* 0: DPD random term as the reference computes it (sigma*omega/sqrt(dt),
+ * dpd_eval.hpp:57); 1: a Gaussian per pair and step (counter-based). */
+int mxg_engine_set_dpd_noise(mxg_engine *e, int mode);
+
+/* engine::particle_max_dist_fraction (engine.h:383). */
+int mxg_engine_set_max_dist_fraction(mxg_engine *e, float fraction);
+
+/* Accumulate the virial tensor sum r_ij (x) f_ij (metrics.cpp:41-167 has no
+ * working counterpart). */
+int mxg_engine_set_virial(mxg_engine *e, int on);
+
+/* engine_load (engine.h:612-613): x, v hold 3N doubles; particle ids are
+ * 0..N-1 in argument order; replica and flags may be NULL. */
+int mxg_engine_load(mxg_engine *e, int N, const double *x, const double *v, const int *type,
+                    const int *replica, const uint16_t *flags);
+
+/* engine_bond_alloc + _bond_init (engine_bonded.cpp:1317-1360,
+ * bond.cpp:497-529): M harmonic-table bonds between ids i[b], j[b]. */
+int mxg_engine_set_bonds(mxg_engine *e, int M, const int32_t *i, const int32_t *j,
+                         const mxg_potential *const *pots, const double *dissociation_energy);
+
+/* engine_step (engine.h:628, engine.cpp:1198-1228), nsteps times. */
+int mxg_engine_step(mxg_engine *e, int nsteps);
+
+/* Waits for the engine's stream. */
+int mxg_engine_sync(mxg_engine *e);
+
+/* engine_unload (engine.h:634-635): global positions and velocities by id. */
+int mxg_engine_unload(mxg_engine *e, double *x, double *v);
+
+/* Per-particle state by id: cell-local position (MxParticle::x), cell index,
+ * velocity, last forces, persistent random force.  NULL skips an array. */
+int mxg_engine_state(mxg_engine *e, float *xlocal, int *cell, float *v, float *f, float *pf);
+
+/* Number of in-cutoff neighbours of each particle in the current
+ * configuration (pairs runner_dopair/runner_doself evaluate). */
+int mxg_engine_neighbour_counts(mxg_engine *e, int *counts);
+
+/* Observables of the last step, replica r or the sum over replicas (r < 0):
+ * out[0] non-bonded energy, out[1] bond energy, out[2] kinetic energy at
+ * the start of the step (engine_kinetic_energy, engine.cpp:1698-1719),
+ * out[3..11] virial tensor (row-major), all accumulated in FP64. */
+int mxg_engine_observables(mxg_engine *e, int replica, double out[12]);
+
+/* BOND_ACTIVE of each bond (bond.h:40-43). */
+int mxg_engine_bond_active(mxg_engine *e, uint8_t *active);
+
+/* Cell grid of a replica (space.h cdim/h). */
+int mxg_engine_geometry(mxg_engine *e, int replica, int cdim[3], double h[3]);
+
+/* engine::time (engine.h). */
+int mxg_engine_time(mxg_engine *e, uint64_t *time);
+
+/* Sticky error code and message of the engine (engine_err / errs_getstring). */
+int mxg_engine_error(const mxg_engine *e);
+const char *mxg_engine_error_message(const mxg_engine *e);
+
+/* HIP-event timing of the force kernel and of the whole step over the last
+ * mxg_engine_step call (engine timers, engine.h:111-135). */
+int mxg_engine_set_timing(mxg_engine *e, int on);
+int mxg_engine_timing(mxg_engine *e, double *force_ms, double *step_ms);
+
+/* Number of visible HIP devices (0 when none). */
+int mxg_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MXG_H_ */

@@ -1,0 +1,318 @@
+"""MI355X engine for Mechanica's mdcore force-and-advance step.
+
+The Python surface mirrors the parts of Mechanica's API that set up and run
+the step (src/__init__.py, src/MxPotential.cpp:2459-2751,
+src/MxUniverse.cpp:121-127):
+
+    import mechanica_amd as m
+    u = m.Universe(dim=[10, 10, 10], cells=[10, 10, 10], cutoff=1.0, dt=0.01)
+    Argon = u.add_type(mass=39.4, radius=0.1)
+    pot = m.Potential.lennard_jones_12_6(0.275, 1.0, 9.5075e-06, 6.1545e-03, 1e-3)
+    u.bind(pot, Argon, Argon)
+    u.load(positions, velocities, types)
+    u.step(100)
+
+Everything below calls the C ABI of libmxg.so (include/mxg.h); the step
+runs only in the HIP kernels.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+
+NEWTONIAN = 0
+OVERDAMPED = 1
+FROZEN_X = 1 << 2
+FROZEN_Y = 1 << 3
+FROZEN_Z = 1 << 4
+
+ERR_TOOFAST = -29
+
+
+class EngineError(RuntimeError):
+    """Raised with the engine's error code and message (mdcore engine_err)."""
+
+    def __init__(self, code, message):
+        super().__init__("engine error %d: %s" % (code, message))
+        self.code = code
+
+
+def _dp(a):
+    return a.ctypes.data_as(_capi.c_double_p)
+
+
+def _fp(a):
+    return a.ctypes.data_as(_capi.c_float_p)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_capi.c_int_p)
+
+
+class Potential:
+    """A tabulated pair potential or a DPD potential (MxPotential)."""
+
+    def __init__(self, handle, name):
+        if not handle:
+            raise ValueError("potential construction failed (potential_err %d)" % _capi.lib().mxg_potential_error())
+        self._h = C.c_void_p(handle)
+        self.name = name
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            try:
+                _capi.lib().mxg_potential_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # Factories with Mechanica's defaults (MxPotential.cpp:2459-2751).
+    @staticmethod
+    def lennard_jones_12_6(min, max, A, B, tol=None):
+        if tol is None:
+            tol = 0.001 * (max - min)
+        return Potential(_capi.lib().mxg_potential_create_lj126(min, max, A, B, tol), "Lennard-Jones")
+
+    @staticmethod
+    def morse(d=1.0, a=6.0, r0=0.0, min=0.0001, max=3.0, tol=0.001):
+        return Potential(_capi.lib().mxg_potential_create_morse(d, a, r0, min, max, tol), "Morse")
+
+    @staticmethod
+    def harmonic(k, r0, min=None, max=None, tol=None):
+        if min is None:
+            min = r0 - r0
+        if max is None:
+            max = r0 + r0
+        if tol is None:
+            tol = 0.01 * (max - min)
+        return Potential(_capi.lib().mxg_potential_create_harmonic(min, max, k, r0, tol), "Harmonic")
+
+    @staticmethod
+    def dpd(alpha=1.0, gamma=1.0, sigma=1.0, cutoff=1.0, shifted=False):
+        return Potential(_capi.lib().mxg_potential_create_dpd(alpha, gamma, sigma, cutoff, int(bool(shifted))),
+                         "Dissapative Particle Dynamics")
+
+    @property
+    def info(self):
+        inf = _capi.PotentialInfo()
+        rc = _capi.lib().mxg_potential_get_info(self._h, C.byref(inf))
+        if rc:
+            raise ValueError("bad potential")
+        return inf
+
+    @property
+    def kind(self):
+        return self.info.kind
+
+    @property
+    def flags(self):
+        return self.info.flags
+
+    @property
+    def n(self):
+        return self.info.n
+
+    @property
+    def domain(self):
+        inf = self.info
+        return inf.a, inf.b
+
+    @property
+    def alpha(self):
+        return np.array(self.info.alpha[:], dtype=np.float32)
+
+    @property
+    def r0_plusone(self):
+        return self.info.r0_plusone
+
+    @property
+    def coefficients(self):
+        L = _capi.lib()
+        n = L.mxg_potential_coefficients(self._h, None, 0)
+        out = np.zeros(max(n, 0), dtype=np.float32)
+        if n > 0:
+            L.mxg_potential_coefficients(self._h, _fp(out), n)
+        return out
+
+    def eval(self, r2):
+        """potential_eval at r^2: (energy, force / r)."""
+        e, f = C.c_float(), C.c_float()
+        rc = _capi.lib().mxg_potential_eval(self._h, float(r2), C.byref(e), C.byref(f))
+        if rc:
+            raise ValueError("not a table potential")
+        return e.value, f.value
+
+    def eval_ex(self, r2, ri=1.0, rj=1.0):
+        """potential_eval_ex at r^2: (in_range, energy, force / r)."""
+        e, f = C.c_float(), C.c_float()
+        ok = _capi.lib().mxg_potential_eval_ex(self._h, float(ri), float(rj), float(r2), C.byref(e), C.byref(f))
+        if ok < 0:
+            raise ValueError("not a table potential")
+        return bool(ok), e.value, f.value
+
+
+class Universe:
+    """One engine: a periodic universe (or a batch of them) on the GPU."""
+
+    def __init__(self, dim=(10.0, 10.0, 10.0), cells=(4, 4, 4), cutoff=1.0, dt=0.01, origin=(0.0, 0.0, 0.0),
+                 replicas=None):
+        L = _capi.lib()
+        h = C.c_void_p()
+        if replicas is None:
+            o = np.ascontiguousarray(origin, dtype=np.float64)
+            d = np.ascontiguousarray(dim, dtype=np.float64)
+            c = np.ascontiguousarray(cells, dtype=np.int32)
+            rc = L.mxg_engine_init(C.byref(h), _dp(o), _dp(d), _ip(c), float(cutoff), float(dt))
+            self.nreplicas = 1
+        else:
+            dims = np.ascontiguousarray([r[0] for r in replicas], dtype=np.float64).reshape(-1, 3)
+            cl = np.ascontiguousarray([r[1] for r in replicas], dtype=np.int32).reshape(-1, 3)
+            org = np.zeros_like(dims)
+            rc = L.mxg_ensemble_init(C.byref(h), len(replicas), _dp(org), _dp(dims), _ip(cl), float(cutoff),
+                                     float(dt))
+            self.nreplicas = len(replicas)
+        self._h = h
+        self.cutoff = cutoff
+        self.dt = dt
+        self._pots = []
+        self.N = 0
+        self.M = 0
+        self._check(rc)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _capi.lib().mxg_engine_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _check(self, rc):
+        if rc is not None and rc < 0:
+            msg = _capi.lib().mxg_engine_error_message(self._h)
+            raise EngineError(rc, msg.decode() if msg else "")
+        return rc
+
+    # set-up -------------------------------------------------------------
+    def add_type(self, mass=1.0, radius=1.0, dynamics=NEWTONIAN):
+        return self._check(_capi.lib().mxg_engine_addtype(self._h, float(mass), float(radius), int(dynamics)))
+
+    def bind(self, pot, a, b):
+        """m.bind(pot, A, B) -> engine_addpot (MxBind.cpp:18-60)."""
+        self._pots.append(pot)
+        self._check(_capi.lib().mxg_engine_addpot(self._h, pot._h, int(a), int(b)))
+
+    def add_random_force(self, type_id, mean, std, duration):
+        self._check(_capi.lib().mxg_engine_add_random_force(self._h, int(type_id), float(mean), float(std),
+                                                            float(duration)))
+
+    def set_seed(self, seed):
+        self._check(_capi.lib().mxg_engine_set_seed(self._h, int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def set_dpd_noise(self, mode):
+        self._check(_capi.lib().mxg_engine_set_dpd_noise(self._h, int(mode)))
+
+    def set_max_dist_fraction(self, fraction):
+        self._check(_capi.lib().mxg_engine_set_max_dist_fraction(self._h, float(fraction)))
+
+    def set_virial(self, on=True):
+        self._check(_capi.lib().mxg_engine_set_virial(self._h, int(bool(on))))
+
+    def load(self, x, v=None, types=None, replica=None, flags=None):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        N = x.shape[0]
+        v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+        t = np.zeros(N, np.int32) if types is None else np.ascontiguousarray(types, dtype=np.int32)
+        r = None if replica is None else np.ascontiguousarray(replica, dtype=np.int32)
+        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint16)
+        self._check(_capi.lib().mxg_engine_load(
+            self._h, N, _dp(x), _dp(v), _ip(t), _ip(r) if r is not None else None,
+            fl.ctypes.data_as(_capi.c_u16_p) if fl is not None else None))
+        self.N = N
+
+    def set_bonds(self, i, j, pots, dissociation=None):
+        i = np.ascontiguousarray(i, dtype=np.int32)
+        j = np.ascontiguousarray(j, dtype=np.int32)
+        M = len(i)
+        if isinstance(pots, Potential):
+            pots = [pots] * M
+        self._pots.extend(set(pots))
+        arr = (C.c_void_p * max(M, 1))(*[p._h.value for p in pots])
+        d = None
+        if dissociation is not None:
+            d = np.ascontiguousarray(np.broadcast_to(np.asarray(dissociation, np.float64), (M,)))
+        self._check(_capi.lib().mxg_engine_set_bonds(self._h, M, _ip(i), _ip(j), arr,
+                                                     _dp(d) if d is not None else None))
+        self.M = M
+
+    # stepping -----------------------------------------------------------
+    def step(self, nsteps=1):
+        """engine_step, nsteps times (MxUniverse_Step)."""
+        return self._check(_capi.lib().mxg_engine_step(self._h, int(nsteps)))
+
+    def sync(self):
+        self._check(_capi.lib().mxg_engine_sync(self._h))
+
+    # read-back ----------------------------------------------------------
+    def state(self):
+        """Per-particle arrays by id: local x, cell, v, f, persistent force."""
+        N = self.N
+        xl = np.zeros((N, 3), np.float32)
+        cell = np.zeros(N, np.int32)
+        v = np.zeros((N, 3), np.float32)
+        f = np.zeros((N, 3), np.float32)
+        pf = np.zeros((N, 3), np.float32)
+        self._check(_capi.lib().mxg_engine_state(self._h, _fp(xl), _ip(cell), _fp(v), _fp(f), _fp(pf)))
+        return {"x": xl, "cell": cell, "v": v, "f": f, "pf": pf}
+
+    def unload(self):
+        x = np.zeros((self.N, 3), np.float64)
+        v = np.zeros((self.N, 3), np.float64)
+        self._check(_capi.lib().mxg_engine_unload(self._h, _dp(x), _dp(v)))
+        return x, v
+
+    def neighbour_counts(self):
+        c = np.zeros(self.N, np.int32)
+        self._check(_capi.lib().mxg_engine_neighbour_counts(self._h, _ip(c)))
+        return c
+
+    def observables(self, replica=-1):
+        out = np.zeros(12, np.float64)
+        self._check(_capi.lib().mxg_engine_observables(self._h, int(replica), _dp(out)))
+        return {"epot_nonbond": out[0], "epot_bond": out[1], "ekin": out[2], "virial": out[3:].reshape(3, 3)}
+
+    def bond_active(self):
+        a = np.zeros(max(self.M, 1), np.uint8)
+        self._check(_capi.lib().mxg_engine_bond_active(self._h, a.ctypes.data_as(_capi.c_u8_p)))
+        return a[: self.M].astype(bool)
+
+    def geometry(self, replica=0):
+        cd = np.zeros(3, np.int32)
+        h = np.zeros(3, np.float64)
+        self._check(_capi.lib().mxg_engine_geometry(self._h, int(replica), _ip(cd), _dp(h)))
+        return cd, h
+
+    @property
+    def time(self):
+        t = C.c_uint64()
+        _capi.lib().mxg_engine_time(self._h, C.byref(t))
+        return t.value
+
+    @property
+    def error(self):
+        return _capi.lib().mxg_engine_error(self._h)
+
+    def set_timing(self, on=True):
+        _capi.lib().mxg_engine_set_timing(self._h, int(bool(on)))
+
+    def timing(self):
+        a, b = C.c_double(), C.c_double()
+        _capi.lib().mxg_engine_timing(self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+
+def device_count():
+    return _capi.lib().mxg_device_count()

@@ -1,0 +1,255 @@
+// C ABI over the engine and the potential builder (declarations in
+// include/mxg.h).
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/mxg.h"
+#include "mx_potential.h"
+#include "mxg_engine.h"
+
+struct mxg_potential {
+  mxg::Potential p;
+};
+
+struct mxg_engine {
+  mxg::Engine e;
+};
+
+namespace {
+thread_local int g_pot_err = 0;
+
+mxg_potential *finish(int rc, mxg_potential *h) {
+  g_pot_err = rc;
+  if (rc < 0) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+}  // namespace
+
+extern "C" {
+
+mxg_potential *mxg_potential_create_lj126(double min, double max, double A, double B, double tol) {
+  auto *h = new (std::nothrow) mxg_potential();
+  if (!h) return finish(mxg::kPotErrMalloc, nullptr);
+  return finish(mxg::make_lj126(h->p, min, max, A, B, tol), h);
+}
+
+mxg_potential *mxg_potential_create_morse(double d, double a, double r0, double min, double max, double tol) {
+  auto *h = new (std::nothrow) mxg_potential();
+  if (!h) return finish(mxg::kPotErrMalloc, nullptr);
+  return finish(mxg::make_morse(h->p, d, a, r0, min, max, tol), h);
+}
+
+mxg_potential *mxg_potential_create_harmonic(double min, double max, double K, double r0, double tol) {
+  auto *h = new (std::nothrow) mxg_potential();
+  if (!h) return finish(mxg::kPotErrMalloc, nullptr);
+  return finish(mxg::make_harmonic(h->p, min, max, K, r0, tol), h);
+}
+
+mxg_potential *mxg_potential_create_dpd(float alpha, float gamma, float sigma, float cutoff, int shifted) {
+  auto *h = new (std::nothrow) mxg_potential();
+  if (!h) return finish(mxg::kPotErrMalloc, nullptr);
+  return finish(mxg::make_dpd(h->p, alpha, gamma, sigma, cutoff, shifted != 0), h);
+}
+
+void mxg_potential_free(mxg_potential *p) { delete p; }
+
+int mxg_potential_error(void) { return g_pot_err; }
+
+int mxg_potential_get_info(const mxg_potential *h, mxg_potential_info *out) {
+  if (!h || !out) return MXG_ERR_NULL;
+  const mxg::Potential &p = h->p;
+  out->kind = p.kind;
+  out->flags = p.flags;
+  out->n = p.n;
+  for (int k = 0; k < 4; ++k) out->alpha[k] = p.alpha[k];
+  out->a = p.a;
+  out->b = p.b;
+  out->r0_plusone = p.r0_plusone;
+  out->dpd_alpha = p.dpd_alpha;
+  out->dpd_gamma = p.dpd_gamma;
+  out->dpd_sigma = p.dpd_sigma;
+  return MXG_ERR_OK;
+}
+
+int mxg_potential_coefficients(const mxg_potential *h, float *c, int capacity) {
+  if (!h) return MXG_ERR_NULL;
+  int n = (int)h->p.c.size();
+  if (!c) return n;
+  if (capacity < n) return MXG_ERR_RANGE;
+  std::memcpy(c, h->p.c.data(), n * sizeof(float));
+  return n;
+}
+
+int mxg_potential_eval(const mxg_potential *h, float r2, float *e, float *f) {
+  if (!h || !e || !f) return MXG_ERR_NULL;
+  if (h->p.kind != mxg::kPotKindTable || h->p.c.empty()) return MXG_ERR_POTENTIAL;
+  mxg::eval_plain(h->p, r2, e, f);
+  return MXG_ERR_OK;
+}
+
+int mxg_potential_eval_ex(const mxg_potential *h, float ri, float rj, float r2, float *e, float *f) {
+  if (!h || !e || !f) return MXG_ERR_NULL;
+  if (h->p.kind != mxg::kPotKindTable || h->p.c.empty()) return MXG_ERR_POTENTIAL;
+  return mxg::eval_ex(h->p, ri, rj, r2, e, f) ? 1 : 0;
+}
+
+int mxg_ensemble_init(mxg_engine **out, int nrep, const double *origins, const double *dims, const int *cells,
+                      double cutoff, double dt) {
+  if (!out || !origins || !dims || !cells || nrep < 1) return MXG_ERR_NULL;
+  *out = nullptr;
+  auto *h = new (std::nothrow) mxg_engine();
+  if (!h) return MXG_ERR_MALLOC;
+  std::vector<mxg::ReplicaSpec> specs(nrep);
+  for (int r = 0; r < nrep; ++r)
+    for (int k = 0; k < 3; ++k) {
+      specs[r].origin[k] = origins[3 * r + k];
+      specs[r].dim[k] = dims[3 * r + k];
+      specs[r].cells[k] = cells[3 * r + k];
+    }
+  int rc = h->e.init(specs.data(), nrep, cutoff, dt);
+  // The handle is returned even on failure so the caller can read the
+  // message; it refuses every further call.
+  *out = h;
+  return rc;
+}
+
+int mxg_engine_init(mxg_engine **out, const double origin[3], const double dim[3], const int cells[3], double cutoff,
+                    double dt) {
+  return mxg_ensemble_init(out, 1, origin, dim, cells, cutoff, dt);
+}
+
+void mxg_engine_free(mxg_engine *e) { delete e; }
+
+#define MXG_CHECK(e)                            \
+  do {                                          \
+    if (!(e)) return MXG_ERR_NULL;              \
+    if ((e)->e.error()) return (e)->e.error();  \
+  } while (0)
+
+int mxg_engine_addtype(mxg_engine *e, double mass, float radius, int dynamics) {
+  MXG_CHECK(e);
+  return e->e.add_type(mass, radius, dynamics);
+}
+
+int mxg_engine_addpot(mxg_engine *e, const mxg_potential *p, int i, int j) {
+  MXG_CHECK(e);
+  if (!p) return MXG_ERR_NULL;
+  return e->e.add_pot(p->p, i, j);
+}
+
+int mxg_engine_add_random_force(mxg_engine *e, int type, float mean, float std, float duration) {
+  MXG_CHECK(e);
+  return e->e.set_random_force(type, mean, std, duration);
+}
+
+int mxg_engine_set_seed(mxg_engine *e, uint64_t seed) {
+  MXG_CHECK(e);
+  return e->e.set_seed(seed);
+}
+
+int mxg_engine_set_dpd_noise(mxg_engine *e, int mode) {
+  MXG_CHECK(e);
+  return e->e.set_dpd_counter_noise(mode);
+}
+
+int mxg_engine_set_max_dist_fraction(mxg_engine *e, float fraction) {
+  MXG_CHECK(e);
+  return e->e.set_max_dist_fraction(fraction);
+}
+
+int mxg_engine_set_virial(mxg_engine *e, int on) {
+  MXG_CHECK(e);
+  return e->e.set_virial(on);
+}
+
+int mxg_engine_load(mxg_engine *e, int N, const double *x, const double *v, const int *type, const int *replica,
+                    const uint16_t *flags) {
+  MXG_CHECK(e);
+  return e->e.load(N, x, v, type, replica, flags);
+}
+
+int mxg_engine_set_bonds(mxg_engine *e, int M, const int32_t *i, const int32_t *j, const mxg_potential *const *pots,
+                         const double *dissociation_energy) {
+  MXG_CHECK(e);
+  if (M > 0 && (!i || !j || !pots)) return MXG_ERR_NULL;
+  std::vector<const mxg::Potential *> pp(M);
+  for (int b = 0; b < M; ++b) pp[b] = pots[b] ? &pots[b]->p : nullptr;
+  return e->e.set_bonds(M, i, j, pp.data(), dissociation_energy);
+}
+
+int mxg_engine_step(mxg_engine *e, int nsteps) {
+  MXG_CHECK(e);
+  return e->e.step(nsteps);
+}
+
+int mxg_engine_sync(mxg_engine *e) {
+  if (!e) return MXG_ERR_NULL;
+  return e->e.sync();
+}
+
+int mxg_engine_unload(mxg_engine *e, double *x, double *v) {
+  if (!e) return MXG_ERR_NULL;
+  return e->e.get_global(x, v);
+}
+
+int mxg_engine_state(mxg_engine *e, float *xlocal, int *cell, float *v, float *f, float *pf) {
+  if (!e) return MXG_ERR_NULL;
+  return e->e.get_state(xlocal, cell, v, f, pf);
+}
+
+int mxg_engine_neighbour_counts(mxg_engine *e, int *counts) {
+  MXG_CHECK(e);
+  if (!counts) return MXG_ERR_NULL;
+  return e->e.neighbour_counts(counts);
+}
+
+int mxg_engine_observables(mxg_engine *e, int replica, double out[12]) {
+  if (!e || !out) return MXG_ERR_NULL;
+  return e->e.observables(replica, out);
+}
+
+int mxg_engine_bond_active(mxg_engine *e, uint8_t *active) {
+  if (!e || !active) return MXG_ERR_NULL;
+  return e->e.bond_active(active);
+}
+
+int mxg_engine_geometry(mxg_engine *e, int replica, int cdim[3], double h[3]) {
+  if (!e || !cdim || !h) return MXG_ERR_NULL;
+  return e->e.geometry(replica, cdim, h);
+}
+
+int mxg_engine_time(mxg_engine *e, uint64_t *time) {
+  if (!e || !time) return MXG_ERR_NULL;
+  *time = e->e.time();
+  return MXG_ERR_OK;
+}
+
+int mxg_engine_error(const mxg_engine *e) { return e ? e->e.error() : MXG_ERR_NULL; }
+
+const char *mxg_engine_error_message(const mxg_engine *e) { return e ? e->e.message().c_str() : "null engine"; }
+
+int mxg_engine_set_timing(mxg_engine *e, int on) {
+  if (!e) return MXG_ERR_NULL;
+  return e->e.set_timing(on);
+}
+
+int mxg_engine_timing(mxg_engine *e, double *force_ms, double *step_ms) {
+  if (!e) return MXG_ERR_NULL;
+  if (force_ms) *force_ms = e->e.force_kernel_ms();
+  if (step_ms) *step_ms = e->e.step_ms();
+  return MXG_ERR_OK;
+}
+
+int mxg_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// Host side of the MI355X engine: geometry, device buffers and the step
+// driver.  One Engine holds one or more independent replica universes that
+// share particle types and potentials; a single universe is the case R = 1.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mx_potential.h"
+
+namespace mxg {
+
+// Error codes mirror mdcore's engine_err_* (src/mdcore/include/engine.h:33-63).
+enum : int {
+  kErrOk = 0,
+  kErrNull = -1,
+  kErrMalloc = -2,
+  kErrSpace = -3,
+  kErrRange = -6,
+  kErrCell = -7,
+  kErrBond = -11,
+  kErrPotential = -17,
+  kErrDevice = -21,  // engine_err_cuda: a device call failed
+  kErrMaxParts = -24,
+  kErrCutoff = -27,
+  kErrToofast = -29,
+};
+
+enum : int { kDynNewtonian = 0, kDynOverdamped = 1 };
+
+// Particle flags understood by the advance (MxParticle.h PARTICLE_FROZEN_*).
+enum : uint16_t { kFrozenX = 1 << 2, kFrozenY = 1 << 3, kFrozenZ = 1 << 4 };
+
+struct ReplicaSpec {
+  double origin[3];
+  double dim[3];
+  int cells[3];
+};
+
+struct HostReplica {
+  double origin[3], dim[3], h[3], ih[3];
+  int cdim[3], span[3];
+  int cell_base = 0, ncells = 0;
+  int slot_base = 0, nparts = 0;
+  int tile_begin = 0, tile_end = 0;
+  std::vector<double> corg[3];  // cell origins per axis
+};
+
+struct DeviceState;  // device buffers (defined in mxg_engine.hip)
+
+class Engine {
+ public:
+  Engine();
+  ~Engine();
+  Engine(const Engine &) = delete;
+  Engine &operator=(const Engine &) = delete;
+
+  int init(const ReplicaSpec *reps, int nrep, double cutoff, double dt);
+  int add_type(double mass, float radius, int dynamics);
+  int add_pot(const Potential &p, int i, int j);
+  int set_random_force(int type, float mean, float std, float duration);
+  int set_seed(uint64_t seed);
+  int set_dpd_counter_noise(int on);
+  int set_max_dist_fraction(float f);
+  int load(int N, const double *x, const double *v, const int *type, const int *replica,
+           const uint16_t *flags);
+  int set_bonds(int M, const int *bi, const int *bj, const Potential *const *pots,
+                const double *dissociation);
+  int step(int nsteps);
+  int sync();
+
+  // Per particle (by id) snapshots; any pointer may be null.
+  int get_state(float *xlocal, int *cell, float *v, float *f, float *pf);
+  int get_global(double *x, double *v);
+  int neighbour_counts(int *counts);
+  int observables(int replica, double *out12);
+  int bond_active(uint8_t *active);
+  int geometry(int replica, int cdim[3], double h[3]);
+
+  int nparts() const { return N_; }
+  int nreplicas() const { return (int)reps_.size(); }
+  uint64_t time() const { return time_; }
+  int error() const { return err_; }
+  const std::string &message() const { return msg_; }
+
+  int fail(int code, const std::string &msg);
+
+  // Adds the virial tensor to the per-step reductions.
+  int set_virial(int on);
+
+  // Timing of the force kernel over the last step() call (HIP events on the
+  // engine's stream); -1 when not recorded.
+  int set_timing(int on);
+  double force_kernel_ms() const { return force_ms_; }
+  double step_ms() const { return step_ms_; }
+
+  // Autotuned launch shape of the force kernel.
+  int block_size() const { return block_; }
+  int set_block_size(int b);
+
+ private:
+  int build_geometry();
+  int upload_potentials();
+  int launch_step();
+  int check_device(int hip_status, const char *what);
+
+  std::vector<HostReplica> reps_;
+  double cutoff_ = 1.0, dt_ = 0.01;
+  float max_frac_ = 0.05f;
+  uint64_t seed_ = 0x5eed5eedull;
+  int dpd_noise_ = 0;
+
+  struct TypeInfo {
+    double mass, imass;
+    float radius;
+    int dynamics;
+    bool random = false;
+    float rnd_mean = 0, rnd_std = 0;
+    unsigned rnd_steps = 1;
+  };
+  std::vector<TypeInfo> types_;
+  std::vector<Potential> pots_;
+  std::vector<int> potmat_;  // ntypes * ntypes
+  std::vector<Potential> bpots_;
+
+  int N_ = 0, M_ = 0, ncells_ = 0, ntiles_ = 0;
+  int block_ = 256;
+  bool any_dpd_ = false, any_table_ = false, any_random_ = false;
+  bool potentials_dirty_ = true;
+  uint64_t time_ = 0;
+  int err_ = 0;
+  std::string msg_;
+  bool timing_ = false;
+  double force_ms_ = -1, step_ms_ = -1;
+  bool virial_ = false;
+  bool pid_prev_valid_ = false;  // pid[1] holds the order forces were computed in
+  int bact_cur_ = 0;             // current bond-activity buffer
+  DeviceState *d_ = nullptr;
+};
+
+}  // namespace mxg

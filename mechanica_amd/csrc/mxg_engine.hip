@@ -1,0 +1,1622 @@
+// MI355X (gfx950) engine for mdcore's force-and-advance step.
+//
+// One step is five launches on the engine's stream:
+//   k_step      per particle slot: non-bonded gather over the neighbour
+//               cells, bonds, single-body force, then the Euler/overdamped
+//               advance and the cell-migration rule; per-tile FP64 partial
+//               sums of energy, kinetic energy and virial
+//   k_count     per cell: how many particles land in it (pull over the 27
+//               cells they can come from)
+//   k_scan_*    exclusive scan of the counts into the new cell offsets
+//   k_scatter   per cell: gather its particles in a fixed source order
+//               (stable, no atomics) into the slot arrays of the next step
+//   k_finalize  per replica: sum the tile partials in tile order
+//
+// Every pair is evaluated from both sides (no Newton-3 write-back), so each
+// particle's force is written once by the thread that owns it and the result
+// is independent of scheduling.  Each side computes the pair in the
+// reference's orientation (space_getsid, src/mdcore/src/space.cpp:95-130:
+// the cell whose shift to the other points "forward" is i, and
+// dx = (x_i - shift) - x_j), so r^2, the cutoff test and the pair force are
+// bit-identical to the reference's runner_dopair/runner_doself
+// (src/mdcore/src/runner_dopair.cpp:91-340, 446-682).  Compile with
+// -ffp-contract=off: the reference never fuses multiply-adds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "mxg_engine.h"
+#include "mxg_philox.h"
+
+namespace mxg {
+
+namespace {
+
+constexpr int kNRed = 12;  // epot_nonbond, epot_bond, ekin, virial[9]
+constexpr int kTile = 256;
+
+struct DevReplica {
+  int cdim[3];
+  int cell_base;
+  int off_base, n_off;
+  int shift_base[3];
+  int span[3];
+  float h[3], h2[3];
+  float maxv[3], maxv2[3], maxx[3], maxx2[3];
+  double hd[3];
+  float bias, dscale;
+  int dmaxdist;
+  int pad;
+};
+
+// A neighbour-cell offset.  side: 0 same cell, +1 this cell is the
+// reference's i cell, -1 the neighbour is.  n: unit direction of the
+// forward shift (the sort axis runner_dosort uses for this pair).
+struct DevOffset {
+  int ox, oy, oz, side;
+  float nx, ny, nz;
+  int pad;
+};
+
+struct DevPot {
+  uint32_t kind, flags;
+  float alpha0, alpha1, alpha2;
+  float a, b, r0_plusone;
+  int n, coff;
+  float dpd_alpha, dpd_gamma, dpd_sigma;
+  float a2;
+  int pad[2];
+};
+
+struct DevType {
+  double mass, imass;
+  float radius;
+  int dynamics;
+  int random;
+  float rnd_mean, rnd_std;
+  unsigned rnd_steps;
+};
+
+}  // namespace
+
+struct DeviceState {
+  hipStream_t stream = nullptr;
+  // particle state, ping-pong
+  float4 *x[2] = {nullptr, nullptr};
+  float4 *v[2] = {nullptr, nullptr};
+  float4 *pf[2] = {nullptr, nullptr};
+  int *pid[2] = {nullptr, nullptr};
+  int *cell[2] = {nullptr, nullptr};
+  int *cstart[2] = {nullptr, nullptr};
+  float4 *f = nullptr;
+  int *slot_of = nullptr;
+  int *ncount = nullptr;
+  int *cnt = nullptr;
+  int *bsum = nullptr;
+  int *cell_rep = nullptr;
+  DevReplica *rep = nullptr;
+  DevOffset *offs = nullptr;
+  float *shifts = nullptr;
+  int4 *tiles = nullptr;
+  int2 *rep_tiles = nullptr;
+  DevPot *pots = nullptr;
+  DevPot *bpots = nullptr;
+  float *coef = nullptr;
+  int *potmat = nullptr;
+  DevType *types = nullptr;
+  int *boff = nullptr, *bent = nullptr, *bi = nullptr, *bj = nullptr, *bpot = nullptr;
+  double *bdis = nullptr;
+  unsigned char *bact[2] = {nullptr, nullptr};
+  double *partials = nullptr;
+  double *obs = nullptr;
+  unsigned *time = nullptr;
+  int *err = nullptr;  // [0] error code, [1] pid, [2] error state at step start
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<void *> owned;
+};
+
+namespace {
+
+struct StepArgs {
+  const float4 *__restrict__ x;
+  const float4 *__restrict__ v;
+  const float4 *__restrict__ pf;
+  const int *__restrict__ pid;
+  const int *__restrict__ cell;
+  const int *__restrict__ cstart;
+  float4 *__restrict__ x_out;
+  float4 *__restrict__ v_out;
+  float4 *__restrict__ pf_out;
+  float4 *__restrict__ f_out;
+  int *__restrict__ cell_out;
+  const DevReplica *__restrict__ rep;
+  const DevOffset *__restrict__ offs;
+  const float *__restrict__ shifts;
+  const int4 *__restrict__ tiles;
+  int ntiles;
+  const DevPot *__restrict__ pots;
+  const DevPot *__restrict__ bpots;
+  const float *__restrict__ coef;
+  const int *__restrict__ potmat;
+  int ntypes;
+  const DevType *__restrict__ types;
+  const int *__restrict__ boff;
+  const int *__restrict__ bent;
+  const int *__restrict__ bi;
+  const int *__restrict__ bj;
+  const int *__restrict__ bpot;
+  const double *__restrict__ bdis;
+  const unsigned char *__restrict__ bact_in;
+  unsigned char *__restrict__ bact_out;
+  const int *__restrict__ slot_of;
+  float dt;
+  float cut2_pair, cut2_self;
+  float dpd_delta;
+  uint32_t seed_lo, seed_hi;
+  int dpd_noise;
+  const unsigned *time;
+  int *err;
+  double *partials;
+  int *ncount;  // neighbour-count mode only
+  int random;   // some type carries a persistent random force
+};
+
+enum : int {
+  kModeTable = 1,
+  kModeDPD = 2,
+  kModeVirial = 4,
+  kModeBonds = 8,
+};
+
+__device__ __forceinline__ float gauss_from(uint32_t a, uint32_t b) {
+  float u1 = u01_open_closed(a);
+  float u2 = u01_closed_open(b);
+  return sqrtf(-2.0f * logf(u1)) * cosf(6.2831853f * u2);
+}
+
+// Random direction of length len for a pair closer than the potential's
+// lower bound (the reference draws it from a per-cell Mersenne twister,
+// dpd_eval.hpp:122-131; here it is keyed by the pair and the time).
+__device__ __forceinline__ float3 overlap_dir(uint32_t pi, uint32_t pj, uint32_t t, uint32_t klo,
+                                              uint32_t khi, float len) {
+  u32x4 ctr = {{pi, pj, t, kStreamOverlap}};
+  u32x4 r = philox4x32_10(ctr, klo, khi);
+  u32x4 r2 = philox4x32_10(r, klo ^ 0x9E3779B9u, khi);
+  float g0 = gauss_from(r.v[0], r.v[1]);
+  float g1 = gauss_from(r.v[2], r.v[3]);
+  float g2 = gauss_from(r2.v[0], r2.v[1]);
+  float l = sqrtf(g0 * g0 + g1 * g1 + g2 * g2);
+  return make_float3(g0 * len / l, g1 * len / l, g2 * len / l);
+}
+
+// potential_eval_ex (potential_eval.hpp:106-161).
+__device__ __forceinline__ bool table_eval_ex(const DevPot &P, const float *__restrict__ coef, float ri,
+                                              float rj, float r2, float &e, float &f) {
+  const float eps = 1.1920928955078125e-07f;
+  float r = sqrtf(r2);
+  float ro = r < eps ? eps : r;
+  r = r < P.a ? P.a : r;
+  if (P.flags & kPotFlagScaled)
+    r = r / (ri + rj);
+  else if (P.flags & kPotFlagShifted)
+    r = r - (ri + rj) + P.r0_plusone;
+  float indf = fmaxf(0.0f, P.alpha0 + r * (P.alpha1 + r * P.alpha2));
+  unsigned ind = (unsigned)indf;
+  if (r > P.b || ind > (unsigned)P.n) return false;
+  const float4 *c4 = reinterpret_cast<const float4 *>(coef + P.coff + (size_t)ind * kPotChunk);
+  float4 c0 = c4[0], c1 = c4[1];
+  float x = (r - c0.x) * c0.y;
+  float ee = c0.z * x + c0.w;
+  float eff = c0.z;
+  eff = eff * x + ee;
+  ee = ee * x + c1.x;
+  eff = eff * x + ee;
+  ee = ee * x + c1.y;
+  eff = eff * x + ee;
+  ee = ee * x + c1.z;
+  eff = eff * x + ee;
+  ee = ee * x + c1.w;
+  e = ee;
+  f = eff * c0.y / ro;
+  return true;
+}
+
+// potential_eval (potential_eval.hpp:67-104), used for bonds.
+__device__ __forceinline__ void table_eval(const DevPot &P, const float *__restrict__ coef, float r2,
+                                           float &e, float &f) {
+  float r = sqrtf(r2);
+  int ind = (int)fmaxf(0.0f, P.alpha0 + r * (P.alpha1 + r * P.alpha2));
+  // r^2 is clamped to [a^2, b^2] by the caller, so ind <= n; the clamp only
+  // keeps non-finite input inside the table.
+  ind = min(max(ind, 0), P.n);
+  const float4 *c4 = reinterpret_cast<const float4 *>(coef + P.coff + (size_t)ind * kPotChunk);
+  float4 c0 = c4[0], c1 = c4[1];
+  float x = (r - c0.x) * c0.y;
+  float ee = c0.z * x + c0.w;
+  float eff = c0.z;
+  eff = eff * x + ee;
+  ee = ee * x + c1.x;
+  eff = eff * x + ee;
+  ee = ee * x + c1.y;
+  eff = eff * x + ee;
+  ee = ee * x + c1.z;
+  eff = eff * x + ee;
+  ee = ee * x + c1.w;
+  e = ee;
+  f = eff * c0.y / r;
+}
+
+__device__ __forceinline__ unsigned sort_key(float bias, float dscale, float x, float y, float z, float nx,
+                                             float ny, float nz) {
+  return ((unsigned)(dscale * (bias + x * nx + y * ny + z * nz))) & 0xffffu;
+}
+
+__device__ __forceinline__ void cell_coords(int lc, const DevReplica &R, int &lx, int &ly, int &lz) {
+  lz = lc % R.cdim[2];
+  int t = lc / R.cdim[2];
+  ly = t % R.cdim[1];
+  lx = t / R.cdim[1];
+}
+
+__device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Accumulates one pair into the owner's force.  dx is in the reference
+// orientation (x_i - x_j); p_is_i says which end the owner is.
+template <int MODE>
+__device__ __forceinline__ void pair_force(const StepArgs &A, int ti, int tj, bool p_is_i, int mypid, int q,
+                                           bool self_pair, float dxx, float dxy, float dxz, float r2,
+                                           const float4 &vs, float &fx, float &fy, float &fz, double *acc,
+                                           unsigned t1) {
+  int pidx = A.potmat[ti * A.ntypes + tj];
+  if (pidx < 0) return;
+  const DevPot P = A.pots[pidx];
+  if (r2 < P.a2) {
+    int qpid = A.pid[q];
+    uint32_t ki, kj;
+    float sgn = 1.0f;
+    if (self_pair) {
+      ki = (uint32_t)min(mypid, qpid);
+      kj = (uint32_t)max(mypid, qpid);
+      sgn = (mypid == (int)ki) ? 1.0f : -1.0f;
+    } else {
+      ki = (uint32_t)(p_is_i ? mypid : qpid);
+      kj = (uint32_t)(p_is_i ? qpid : mypid);
+    }
+    float3 d = overlap_dir(ki, kj, t1, A.seed_lo, A.seed_hi, P.a);
+    dxx = sgn * d.x;
+    dxy = sgn * d.y;
+    dxz = sgn * d.z;
+    r2 = P.a * P.a;
+  }
+  const float ri = A.types[ti].radius, rj = A.types[tj].radius;
+  if ((MODE & kModeDPD) && P.kind == kPotKindDPD) {
+    bool shifted = P.flags & kPotFlagShifted;
+    float cutoff = shifted ? (P.b + ri + rj) : P.b;
+    if (r2 > cutoff * cutoff) return;
+    float r = sqrtf(r2);
+    float ex = dxx / r, ey = dxy / r, ez = dxz / r;
+    float4 vq = A.v[q];
+    float vx, vy, vz;
+    if (p_is_i) {
+      vx = vs.x - vq.x;
+      vy = vs.y - vq.y;
+      vz = vs.z - vq.z;
+    } else {
+      vx = vq.x - vs.x;
+      vy = vq.y - vs.y;
+      vz = vq.z - vs.z;
+    }
+    float shifted_r = shifted ? r - ri - rj : r;
+    float omega_c = shifted_r < 0.f ? 1.f : (1 - shifted_r / cutoff);
+    float fc = P.dpd_alpha * omega_c;
+    float omega_d = omega_c * omega_c;
+    float ev = ex * vx + ey * vy + ez * vz;
+    float fd = -P.dpd_gamma * omega_d * ev;
+    float fr;
+    if (A.dpd_noise) {
+      int qpid = A.pid[q];
+      u32x4 ctr = {{(uint32_t)min(mypid, qpid), (uint32_t)max(mypid, qpid), t1, kStreamDPD}};
+      u32x4 rr = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
+      fr = P.dpd_sigma * omega_c * gauss_from(rr.v[0], rr.v[1]) * A.dpd_delta;
+    } else {
+      fr = P.dpd_sigma * omega_c * A.dpd_delta;
+    }
+    float f = fc + fd + fr;
+    float wx = f * ex, wy = f * ey, wz = f * ez;  // force on i
+    if (p_is_i) {
+      fx += wx;
+      fy += wy;
+      fz += wz;
+    } else {
+      fx -= wx;
+      fy -= wy;
+      fz -= wz;
+    }
+    if (MODE & kModeVirial) {
+      acc[3] += 0.5 * dxx * wx;
+      acc[4] += 0.5 * dxx * wy;
+      acc[5] += 0.5 * dxx * wz;
+      acc[6] += 0.5 * dxy * wx;
+      acc[7] += 0.5 * dxy * wy;
+      acc[8] += 0.5 * dxy * wz;
+      acc[9] += 0.5 * dxz * wx;
+      acc[10] += 0.5 * dxz * wy;
+      acc[11] += 0.5 * dxz * wz;
+    }
+    return;
+  }
+  if (MODE & kModeTable) {
+    float e, f;
+    if (!table_eval_ex(P, A.coef, ri, rj, r2, e, f)) return;
+    float wx = f * dxx, wy = f * dxy, wz = f * dxz;
+    if (p_is_i) {
+      fx -= wx;
+      fy -= wy;
+      fz -= wz;
+    } else {
+      fx += wx;
+      fy += wy;
+      fz += wz;
+    }
+    acc[0] += 0.5 * (double)e;
+    if (MODE & kModeVirial) {
+      acc[3] -= 0.5 * dxx * wx;
+      acc[4] -= 0.5 * dxx * wy;
+      acc[5] -= 0.5 * dxx * wz;
+      acc[6] -= 0.5 * dxy * wx;
+      acc[7] -= 0.5 * dxy * wy;
+      acc[8] -= 0.5 * dxy * wz;
+      acc[9] -= 0.5 * dxz * wx;
+      acc[10] -= 0.5 * dxz * wy;
+      acc[11] -= 0.5 * dxz * wz;
+    }
+  }
+}
+
+// Visits every in-cutoff neighbour of slot s in the reference orientation,
+// applying the sorted-axis window of runner_dopair (keys of runner_dosort,
+// runner_dosort.cpp:126-137; window test runner_dopair.cpp:170,182) so the
+// pair set is exactly the reference's.  fn(q, p_is_i, self, dx, r2).
+template <typename Fn>
+__device__ __forceinline__ void for_each_neighbour(const StepArgs &A, const DevReplica &R, int s, int lx,
+                                                   int ly, int lz, float4 xs, Fn &&fn) {
+  const int cd0 = R.cdim[0], cd1 = R.cdim[1], cd2 = R.cdim[2];
+  const int w0 = 2 * R.span[0] + 1, w1 = 2 * R.span[1] + 1, w2 = 2 * R.span[2] + 1;
+  for (int o = 0; o < R.n_off; ++o) {
+    const DevOffset off = A.offs[R.off_base + o];
+    int nx = wrap(lx + off.ox, cd0), ny = wrap(ly + off.oy, cd1), nz = wrap(lz + off.oz, cd2);
+    int c2 = R.cell_base + (nx * cd1 + ny) * cd2 + nz;
+    int qb = A.cstart[c2], qe = A.cstart[c2 + 1];
+    if (off.side == 0) {
+      for (int q = qb; q < qe; ++q) {
+        if (q == s) continue;
+        float4 xq = A.x[q];
+        float dx = xs.x - xq.x, dy = xs.y - xq.y, dz = xs.z - xq.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 > A.cut2_self) continue;
+        fn(q, true, true, dx, dy, dz, r2, xq);
+      }
+      continue;
+    }
+    // Shift from the reference i cell to the j cell.
+    float sx, sy, sz;
+    if (off.side > 0) {
+      sx = A.shifts[R.shift_base[0] + lx * w0 + (off.ox + R.span[0])];
+      sy = A.shifts[R.shift_base[1] + ly * w1 + (off.oy + R.span[1])];
+      sz = A.shifts[R.shift_base[2] + lz * w2 + (off.oz + R.span[2])];
+    } else {
+      sx = A.shifts[R.shift_base[0] + nx * w0 + (-off.ox + R.span[0])];
+      sy = A.shifts[R.shift_base[1] + ny * w1 + (-off.oy + R.span[1])];
+      sz = A.shifts[R.shift_base[2] + nz * w2 + (-off.oz + R.span[2])];
+    }
+    float nshift = sqrtf(sx * sx + sy * sy + sz * sz);
+    unsigned dnshift = (unsigned)(int)(R.dscale * nshift);
+    unsigned dmax = (unsigned)R.dmaxdist;
+    unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, off.nx, off.ny, off.nz);
+    if (off.side > 0) {
+      float pix = xs.x - sx, piy = xs.y - sy, piz = xs.z - sz;
+      for (int q = qb; q < qe; ++q) {
+        float4 xq = A.x[q];
+        float dx = pix - xq.x, dy = piy - xq.y, dz = piz - xq.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 > A.cut2_pair) continue;
+        unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, off.nx, off.ny, off.nz);
+        if (!(kq + dnshift - kp < dmax)) continue;
+        fn(q, true, false, dx, dy, dz, r2, xq);
+      }
+    } else {
+      for (int q = qb; q < qe; ++q) {
+        float4 xq = A.x[q];
+        float qix = xq.x - sx, qiy = xq.y - sy, qiz = xq.z - sz;
+        float dx = qix - xs.x, dy = qiy - xs.y, dz = qiz - xs.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 > A.cut2_pair) continue;
+        unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, off.nx, off.ny, off.nz);
+        if (!(kp + dnshift - kq < dmax)) continue;
+        fn(q, false, false, dx, dy, dz, r2, xq);
+      }
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kTile) void k_step(StepArgs A) {
+  __shared__ double red[kNRed][kTile / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int passthrough = A.err[2];
+  const unsigned t1 = *A.time + 1u;
+
+  for (int t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
+    const int4 tile = A.tiles[t];
+    const int s = tile.x + threadIdx.x;
+    double acc[kNRed];
+#pragma unroll
+    for (int k = 0; k < kNRed; ++k) acc[k] = 0.0;
+
+    if (s < tile.y) {
+      const DevReplica &R = A.rep[tile.z];
+      const float4 xs = A.x[s];
+      const float4 vs = A.v[s];
+      const int tw = __float_as_int(xs.w);
+      const int ti = tw & 0xffff;
+      const int pflags = (unsigned)tw >> 16;
+      const int c = A.cell[s];
+      const int mypid = A.pid[s];
+      int lx, ly, lz;
+      cell_coords(c - R.cell_base, R, lx, ly, lz);
+      const DevType T = A.types[ti];
+
+      // engine_kinetic_energy (engine.cpp:1698-1719): FP32 term, FP64 sum.
+      {
+        float e = 0.5 * T.mass * (vs.x * vs.x + vs.y * vs.y + vs.z * vs.z);
+        acc[2] = (double)e;
+      }
+
+      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+      float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+      if (!passthrough) {
+        // Single-body persistent random force (gaussian_force,
+        // MxForce.cpp:370-385), refreshed when (time + id) % steps == 0.
+        if (A.random) {
+          pfs = A.pf[s];
+          if (T.random) {
+            if (((uint64_t)t1 + (uint64_t)mypid) % T.rnd_steps == 0) {
+              u32x4 ctr = {{(uint32_t)mypid, t1, kStreamSingle, 0u}};
+              u32x4 r = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
+              float g = gauss_from(r.v[0], r.v[1]);
+              float mag = T.rnd_mean + T.rnd_std * g;
+              float theta = 6.2831853f * u01_closed_open(r.v[2]);
+              float phi = acosf(1.0f - 2.0f * u01_closed_open(r.v[3]));
+              pfs.x = mag * sinf(phi) * cosf(theta);
+              pfs.y = mag * sinf(phi) * sinf(theta);
+              pfs.z = mag * cosf(phi);
+            }
+            fx += pfs.x;
+            fy += pfs.y;
+            fz += pfs.z;
+          }
+        }
+
+        for_each_neighbour(A, R, s, lx, ly, lz, xs,
+                           [&](int q, bool p_is_i, bool self, float dx, float dy, float dz, float r2,
+                               const float4 &xq) {
+                             const int tj = __float_as_int(xq.w) & 0xffff;
+                             pair_force<MODE>(A, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, vs, fx,
+                                              fy, fz, acc, t1);
+                           });
+
+        // Bonds (bond_eval, bond.cpp:99-295), gathered per particle.  Both
+        // ends evaluate the bond in its (i, j) orientation; the i end keeps
+        // the energy and writes the dissociation flag.
+        if (MODE & kModeBonds) {
+          const int b0 = A.boff[mypid], b1 = A.boff[mypid + 1];
+          for (int k = b0; k < b1; ++k) {
+            const int ent = A.bent[k];
+            const int bid = ent >> 1;
+            const bool p_is_i = (ent & 1) == 0;
+            if (!A.bact_in[bid]) {
+              if (p_is_i) A.bact_out[bid] = 0;
+              continue;
+            }
+            const int si = p_is_i ? s : A.slot_of[A.bi[bid]];
+            const int sj = p_is_i ? A.slot_of[A.bj[bid]] : s;
+            const float4 xi = A.x[si], xj = A.x[sj];
+            int lix, liy, liz, ljx, ljy, ljz;
+            cell_coords(A.cell[si] - R.cell_base, R, lix, liy, liz);
+            cell_coords(A.cell[sj] - R.cell_base, R, ljx, ljy, ljz);
+            int sh[3] = {lix - ljx, liy - ljy, liz - ljz};
+            for (int d = 0; d < 3; ++d) sh[d] = sh[d] > 1 ? -1 : (sh[d] < -1 ? 1 : sh[d]);
+            const float pix = (float)((double)xi.x + R.hd[0] * sh[0]);
+            const float piy = (float)((double)xi.y + R.hd[1] * sh[1]);
+            const float piz = (float)((double)xi.z + R.hd[2] * sh[2]);
+            const float dx = pix - xj.x, dy = piy - xj.y, dz = piz - xj.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            const DevPot P = A.bpots[A.bpot[bid]];
+            const float lo = P.a * P.a, hi = P.b * P.b;
+            if (r2 < lo || r2 > hi) r2 = fmaxf(lo, fminf(hi, r2));
+            float ee, eff;
+            table_eval(P, A.coef, r2, ee, eff);
+            if ((double)eff >= A.bdis[bid]) {
+              if (p_is_i) A.bact_out[bid] = 0;
+              continue;
+            }
+            if (p_is_i) A.bact_out[bid] = 1;
+            const float wx = eff * dx, wy = eff * dy, wz = eff * dz;
+            if (p_is_i) {
+              fx -= wx;
+              fy -= wy;
+              fz -= wz;
+              acc[1] += (double)ee;
+              if (MODE & kModeVirial) {
+                acc[3] -= dx * wx;
+                acc[4] -= dx * wy;
+                acc[5] -= dx * wz;
+                acc[6] -= dy * wx;
+                acc[7] -= dy * wy;
+                acc[8] -= dy * wz;
+                acc[9] -= dz * wx;
+                acc[10] -= dz * wy;
+                acc[11] -= dz * wz;
+              }
+            } else {
+              fx += wx;
+              fy += wy;
+              fz += wz;
+            }
+          }
+        }
+      }
+
+      // Advance (cell_advance_forward_euler, engine_advance.cpp:126-239).
+      float X[3] = {xs.x, xs.y, xs.z};
+      float V[3] = {vs.x, vs.y, vs.z};
+      const float F[3] = {fx, fy, fz};
+      int newc = c;
+      const bool all_frozen = (pflags & kFrozenX) && (pflags & kFrozenY) && (pflags & kFrozenZ);
+      if (!passthrough && !all_frozen) {
+        const float mask[3] = {(pflags & kFrozenX) ? 0.0f : 1.0f, (pflags & kFrozenY) ? 0.0f : 1.0f,
+                               (pflags & kFrozenZ) ? 0.0f : 1.0f};
+        int delta[3];
+        bool toofast = false;
+        if (T.dynamics == kDynNewtonian) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            float vn = (float)((double)mask[k] * ((double)V[k] + (double)(A.dt * F[k]) * T.imass));
+            if (vn * vn <= R.maxv2[k]) {
+              V[k] = vn;
+            } else {
+              float neg = vn / fabsf(vn);
+              V[k] = neg * R.maxv[k];
+            }
+            X[k] += A.dt * V[k];
+            delta[k] = (X[k] >= R.h[k] ? 1 : 0) - (X[k] < 0.0f ? 1 : 0);
+            toofast = toofast || (X[k] >= R.h2[k] || X[k] <= -R.h[k]);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            float d = (float)((double)mask[k] * ((double)(A.dt * F[k]) * T.imass));
+            float step;
+            if (d * d <= R.maxx2[k]) {
+              step = d;
+            } else {
+              float neg = d / fabsf(d);
+              step = neg * R.maxx[k];
+            }
+            X[k] += step;
+            delta[k] = (X[k] >= R.h[k] ? 1 : 0) - (X[k] < 0.0f ? 1 : 0);
+            toofast = toofast || (X[k] >= R.h2[k] || X[k] <= -R.h[k]);
+          }
+        }
+        if (toofast) {
+          if (atomicCAS(&A.err[0], 0, (int)kErrToofast) == 0) A.err[1] = mypid;
+        }
+        if (delta[0] | delta[1] | delta[2]) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) X[k] -= delta[k] * R.h[k];
+          const int nx = (lx + delta[0] + R.cdim[0]) % R.cdim[0];
+          const int ny = (ly + delta[1] + R.cdim[1]) % R.cdim[1];
+          const int nz = (lz + delta[2] + R.cdim[2]) % R.cdim[2];
+          newc = R.cell_base + (nx * R.cdim[1] + ny) * R.cdim[2] + nz;
+        }
+      }
+      A.x_out[s] = make_float4(X[0], X[1], X[2], xs.w);
+      A.v_out[s] = make_float4(V[0], V[1], V[2], 0.0f);
+      A.cell_out[s] = newc;
+      A.f_out[s] = make_float4(fx, fy, fz, 0.0f);
+      if (A.random) A.pf_out[s] = pfs;
+    }
+
+    // Deterministic block reduction of the FP64 partials for this tile.
+#pragma unroll
+    for (int k = 0; k < kNRed; ++k) {
+      if (!(MODE & kModeVirial) && k >= 3) break;
+      double w = wave_sum(acc[k]);
+      if (lane == 0) red[k][wave] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNRed) {
+      double sum = 0.0;
+      for (int w = 0; w < kTile / 64; ++w) sum += ((MODE & kModeVirial) || threadIdx.x < 3) ? red[threadIdx.x][w] : 0.0;
+      A.partials[(size_t)t * kNRed + threadIdx.x] = sum;
+    }
+    __syncthreads();
+  }
+}
+
+// Per-particle count of in-cutoff neighbours (pairs the reference would
+// evaluate), for bit-exact comparison with the CPU loops.
+__global__ __launch_bounds__(kTile) void k_ncount(StepArgs A) {
+  for (int t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
+    const int4 tile = A.tiles[t];
+    const int s = tile.x + threadIdx.x;
+    if (s >= tile.y) continue;
+    const DevReplica &R = A.rep[tile.z];
+    const float4 xs = A.x[s];
+    int lx, ly, lz;
+    cell_coords(A.cell[s] - R.cell_base, R, lx, ly, lz);
+    int n = 0;
+    for_each_neighbour(A, R, s, lx, ly, lz, xs,
+                       [&](int, bool, bool, float, float, float, float, const float4 &) { ++n; });
+    A.ncount[A.pid[s]] = n;
+  }
+}
+
+struct RebuildArgs {
+  const int *__restrict__ cstart;   // old offsets
+  int *__restrict__ cstart_new;     // new offsets
+  const int *__restrict__ cell_out; // new cell of each old slot
+  const int *__restrict__ cell_rep;
+  const DevReplica *__restrict__ rep;
+  int *__restrict__ cnt;
+  int ncells;
+  const float4 *__restrict__ x_in;
+  const float4 *__restrict__ v_in;
+  const float4 *__restrict__ pf_in;
+  const int *__restrict__ pid_in;
+  float4 *__restrict__ x;
+  float4 *__restrict__ v;
+  float4 *__restrict__ pf;
+  int *__restrict__ pid;
+  int *__restrict__ cell;
+  int *__restrict__ slot_of;
+  int with_pf;
+};
+
+// Source cells of destination c in the fixed order: itself, then the 26
+// neighbours in lexicographic offset order.
+__device__ __forceinline__ int source_cell(const DevReplica &R, int lx, int ly, int lz, int k) {
+  int o = (k == 0) ? 13 : (k <= 13 ? k - 1 : k);
+  int dx = o / 9 - 1, dy = (o / 3) % 3 - 1, dz = o % 3 - 1;
+  int sx = wrap(lx - dx, R.cdim[0]), sy = wrap(ly - dy, R.cdim[1]), sz = wrap(lz - dz, R.cdim[2]);
+  return R.cell_base + (sx * R.cdim[1] + sy) * R.cdim[2] + sz;
+}
+
+__global__ __launch_bounds__(256) void k_count(RebuildArgs A) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= A.ncells) return;
+  const DevReplica &R = A.rep[A.cell_rep[c]];
+  int lx, ly, lz;
+  cell_coords(c - R.cell_base, R, lx, ly, lz);
+  int n = 0;
+  for (int k = 0; k < 27; ++k) {
+    const int src = source_cell(R, lx, ly, lz, k);
+    const int qe = A.cstart[src + 1];
+    for (int q = A.cstart[src]; q < qe; ++q) n += (A.cell_out[q] == c);
+  }
+  A.cnt[c] = n;
+}
+
+constexpr int kScanBlock = 256, kScanItems = 4, kScanChunk = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total) {
+  // Inclusive scan over the block with wave shuffles, then exclusive.
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[wave] = x;
+  __syncthreads();
+  int wsum = 0, tot = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    if (w < wave) wsum += lds[w];
+    tot += lds[w];
+  }
+  __syncthreads();
+  total = tot;
+  return wsum + x - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_block(const int *__restrict__ in, int *__restrict__ out,
+                                                          int *__restrict__ bsum, int n) {
+  __shared__ int lds[kScanBlock / 64];
+  const int base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+  int vals[kScanItems];
+  int local = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    vals[i] = (base + i < n) ? in[base + i] : 0;
+    local += vals[i];
+  }
+  int total;
+  int off = block_exclusive_scan(local, lds, total);
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) out[base + i] = off;
+    off += vals[i];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_top(int *__restrict__ bsum, int nb, int *__restrict__ out,
+                                                        int n) {
+  __shared__ int lds[kScanBlock / 64];
+  int carry = 0;
+  for (int base = 0; base < nb; base += kScanBlock) {
+    const int i = base + threadIdx.x;
+    int v = i < nb ? bsum[i] : 0;
+    int total;
+    int ex = block_exclusive_scan(v, lds, total);
+    if (i < nb) bsum[i] = ex + carry;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int *__restrict__ out, const int *__restrict__ bsum, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += bsum[i / kScanChunk];
+}
+
+__global__ __launch_bounds__(256) void k_scatter(RebuildArgs A) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= A.ncells) return;
+  const DevReplica &R = A.rep[A.cell_rep[c]];
+  int lx, ly, lz;
+  cell_coords(c - R.cell_base, R, lx, ly, lz);
+  int k = A.cstart_new[c];
+  for (int j = 0; j < 27; ++j) {
+    const int src = source_cell(R, lx, ly, lz, j);
+    const int qe = A.cstart[src + 1];
+    for (int q = A.cstart[src]; q < qe; ++q) {
+      if (A.cell_out[q] != c) continue;
+      A.x[k] = A.x_in[q];
+      A.v[k] = A.v_in[q];
+      if (A.with_pf) A.pf[k] = A.pf_in[q];
+      const int p = A.pid_in[q];
+      A.pid[k] = p;
+      A.cell[k] = c;
+      A.slot_of[p] = k;
+      ++k;
+    }
+  }
+}
+
+// Sums the tile partials of each replica in tile order (one block, so the
+// order is fixed); advances the time unless the step was skipped because of
+// an earlier error, and records the error state for the next step.
+__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ partials,
+                                                  const int2 *__restrict__ rep_tiles, int nrep,
+                                                  double *__restrict__ obs, unsigned *time, int *err) {
+  const int skipped = err[2];
+  if (!skipped) {
+    for (int item = threadIdx.x; item < nrep * kNRed; item += blockDim.x) {
+      const int r = item / kNRed, k = item % kNRed;
+      const int2 tr = rep_tiles[r];
+      double s = 0.0;
+      for (int t = tr.x; t < tr.y; ++t) s += partials[(size_t)t * kNRed + k];
+      obs[item] = s;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (!skipped) *time += 1u;
+    err[2] = err[0];
+  }
+}
+
+template <int MODE>
+void launch_step_mode(const StepArgs &A, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_step<MODE>, dim3(grid), dim3(kTile), 0, st, A);
+}
+
+using StepLauncher = void (*)(const StepArgs &, int, hipStream_t);
+
+// One instantiation per combination of table / DPD / virial / bonds.
+const StepLauncher kStepLaunchers[16] = {
+    launch_step_mode<0>,  launch_step_mode<1>,  launch_step_mode<2>,  launch_step_mode<3>,
+    launch_step_mode<4>,  launch_step_mode<5>,  launch_step_mode<6>,  launch_step_mode<7>,
+    launch_step_mode<8>,  launch_step_mode<9>,  launch_step_mode<10>, launch_step_mode<11>,
+    launch_step_mode<12>, launch_step_mode<13>, launch_step_mode<14>, launch_step_mode<15>,
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+Engine::Engine() {}
+
+Engine::~Engine() {
+  if (d_) {
+    if (d_->stream) hipStreamSynchronize(d_->stream);
+    for (void *p : d_->owned) hipFree(p);
+    for (auto &e : d_->ev)
+      if (e) hipEventDestroy(e);
+    if (d_->stream) hipStreamDestroy(d_->stream);
+    delete d_;
+  }
+}
+
+int Engine::fail(int code, const std::string &msg) {
+  err_ = code;
+  msg_ = msg;
+  return code;
+}
+
+int Engine::check_device(int st, const char *what) {
+  if (st != hipSuccess) return fail(kErrDevice, std::string(what) + ": " + hipGetErrorString((hipError_t)st));
+  return kErrOk;
+}
+
+// Allocates n elements into *p, releasing what *p held before.
+template <typename T>
+static int dalloc(DeviceState *d, T **p, size_t n) {
+  if (*p) {
+    auto it = std::find(d->owned.begin(), d->owned.end(), (void *)*p);
+    if (it != d->owned.end()) {
+      d->owned.erase(it);
+      hipFree(*p);
+    }
+    *p = nullptr;
+  }
+  void *q = nullptr;
+  if (n == 0) n = 1;
+  hipError_t st = hipMalloc(&q, n * sizeof(T));
+  if (st != hipSuccess) return (int)st;
+  d->owned.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+
+int Engine::init(const ReplicaSpec *specs, int nrep, double cutoff, double dt) {
+  if (!specs || nrep < 1) return fail(kErrNull, "no replica given");
+  if (!(cutoff > 0)) return fail(kErrCutoff, "cutoff must be positive");
+  cutoff_ = cutoff;
+  dt_ = dt;
+  reps_.assign(nrep, HostReplica());
+  int base = 0;
+  for (int r = 0; r < nrep; ++r) {
+    HostReplica &R = reps_[r];
+    int cells[3] = {specs[r].cells[0], specs[r].cells[1], specs[r].cells[2]};
+    // Periodic boundaries need at least three cells per axis
+    // (MxBoundaryConditions.cpp:669-686).
+    for (int k = 0; k < 3; ++k)
+      if (cells[k] < 3) cells[k] = 3;
+    double L[3];
+    for (int k = 0; k < 3; ++k) L[k] = (specs[r].dim[k] - specs[r].origin[k]) / cells[k];
+    // space_init (space.cpp:645-682): cdim = floor(dim / L), h = dim / cdim.
+    for (int k = 0; k < 3; ++k) {
+      R.origin[k] = specs[r].origin[k];
+      R.dim[k] = specs[r].dim[k];
+      R.cdim[k] = (int)std::floor(specs[r].dim[k] / L[k]);
+      if (R.cdim[k] < 3) return fail(kErrSpace, "periodic space needs at least 3 cells per axis");
+      R.h[k] = R.dim[k] / R.cdim[k];
+      R.ih[k] = 1.0 / R.h[k];
+      R.span[k] = (int)std::ceil(cutoff * R.ih[k]);
+      if (R.cdim[k] < 2 * R.span[k] + 1)
+        return fail(kErrSpace, "cell grid too coarse for the cutoff (needs cdim >= 2*span+1)");
+      R.corg[k].resize(R.cdim[k]);
+      for (int l = 0; l < R.cdim[k]; ++l) R.corg[k][l] = R.origin[k] + l * R.h[k];
+    }
+    R.cell_base = base;
+    R.ncells = R.cdim[0] * R.cdim[1] * R.cdim[2];
+    base += R.ncells;
+  }
+  ncells_ = base;
+  d_ = new DeviceState();
+  if (int rc = check_device(hipStreamCreateWithFlags(&d_->stream, hipStreamNonBlocking), "hipStreamCreate"))
+    return rc;
+  for (auto &e : d_->ev)
+    if (int rc = check_device(hipEventCreate(&e), "hipEventCreate")) return rc;
+  return build_geometry();
+}
+
+// Offsets, shift tables and per-replica constants.
+int Engine::build_geometry() {
+  std::vector<DevReplica> drep(reps_.size());
+  std::vector<DevOffset> offs;
+  std::vector<float> shifts;
+  std::vector<int> cell_rep(ncells_);
+  const float cutoff_f = (float)cutoff_;
+  for (size_t r = 0; r < reps_.size(); ++r) {
+    HostReplica &R = reps_[r];
+    DevReplica &D = drep[r];
+    std::memset(&D, 0, sizeof(D));
+    for (int k = 0; k < 3; ++k) {
+      D.cdim[k] = R.cdim[k];
+      D.span[k] = R.span[k];
+    }
+    D.cell_base = R.cell_base;
+    for (int c = 0; c < R.ncells; ++c) cell_rep[R.cell_base + c] = (int)r;
+    // engine_advance_forward_euler constants (engine_advance.cpp:262-281).
+    const float dt = (float)dt_;
+    for (int k = 0; k < 3; ++k) {
+      D.h[k] = (float)R.h[k];
+      D.h2[k] = (float)(2. * R.h[k]);
+      D.maxv[k] = (D.h[k] * max_frac_) / dt;
+      D.maxv2[k] = D.maxv[k] * D.maxv[k];
+      D.maxx[k] = D.h[k] * max_frac_;
+      D.maxx2[k] = D.maxx[k] * D.maxx[k];
+      D.hd[k] = R.h[k];
+    }
+    // Sort-key scale of runner_dosort/runner_dopair.
+    D.bias = (float)std::sqrt(R.h[0] * R.h[0] + R.h[1] * R.h[1] + R.h[2] * R.h[2]);
+    D.dscale = (float)SHRT_MAX / (2 * D.bias);
+    D.dmaxdist = (int)(2 + D.dscale * (cutoff_f + 2 * 0.0f));
+    // Shift tables: shift[k][l][d] = min-image float(origin[l+d] - origin[l]).
+    for (int k = 0; k < 3; ++k) {
+      D.shift_base[k] = (int)shifts.size();
+      const int w = 2 * R.span[k] + 1;
+      for (int l = 0; l < R.cdim[k]; ++l)
+        for (int d = -R.span[k]; d <= R.span[k]; ++d) {
+          int l2 = ((l + d) % R.cdim[k] + R.cdim[k]) % R.cdim[k];
+          float s = R.corg[k][l2] - R.corg[k][l];
+          if (s * 2 > R.dim[k])
+            s -= R.dim[k];
+          else if (s * 2 < -R.dim[k])
+            s += R.dim[k];
+          shifts.push_back(s);
+        }
+      (void)w;
+    }
+    // Neighbour offsets within the span whose cells can hold a pair within
+    // the cutoff (space.cpp:837-843), self first.
+    D.off_base = (int)offs.size();
+    DevOffset self = {0, 0, 0, 0, 0.f, 0.f, 0.f, 0};
+    offs.push_back(self);
+    for (int a = -R.span[0]; a <= R.span[0]; ++a)
+      for (int b = -R.span[1]; b <= R.span[1]; ++b)
+        for (int c = -R.span[2]; c <= R.span[2]; ++c) {
+          if (a == 0 && b == 0 && c == 0) continue;
+          double lh[3] = {R.h[0] * std::fmax(std::abs(a) - 1, 0), R.h[1] * std::fmax(std::abs(b) - 1, 0),
+                          R.h[2] * std::fmax(std::abs(c) - 1, 0)};
+          if (lh[0] * lh[0] + lh[1] * lh[1] + lh[2] * lh[2] > cutoff_ * cutoff_) continue;
+          int sg[3] = {(a > 0) - (a < 0), (b > 0) - (b < 0), (c > 0) - (c < 0)};
+          // forward = first non-zero sign component positive
+          bool fwd = sg[0] > 0 || (sg[0] == 0 && (sg[1] > 0 || (sg[1] == 0 && sg[2] > 0)));
+          int f[3] = {fwd ? sg[0] : -sg[0], fwd ? sg[1] : -sg[1], fwd ? sg[2] : -sg[2]};
+          double len = std::sqrt((double)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]));
+          DevOffset o;
+          o.ox = a;
+          o.oy = b;
+          o.oz = c;
+          o.side = fwd ? 1 : -1;
+          o.nx = (float)(f[0] / len);
+          o.ny = (float)(f[1] / len);
+          o.nz = (float)(f[2] / len);
+          o.pad = 0;
+          offs.push_back(o);
+        }
+    D.n_off = (int)offs.size() - D.off_base;
+  }
+  DeviceState *d = d_;
+  if (int st = dalloc(d, &d->rep, drep.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->offs, offs.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->shifts, shifts.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->cell_rep, cell_rep.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->cnt, (size_t)ncells_)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->bsum, (size_t)ncells_ / kScanChunk + 2)) return check_device(st, "hipMalloc");
+  for (int b = 0; b < 2; ++b)
+    if (int st = dalloc(d, &d->cstart[b], (size_t)ncells_ + 1)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->time, 1)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->err, 4)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->obs, reps_.size() * kNRed)) return check_device(st, "hipMalloc");
+  hipStream_t s = d->stream;
+  int rc = 0;
+  rc |= hipMemcpyAsync(d->rep, drep.data(), drep.size() * sizeof(DevReplica), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->offs, offs.data(), offs.size() * sizeof(DevOffset), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->shifts, shifts.data(), shifts.size() * sizeof(float), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->cell_rep, cell_rep.data(), cell_rep.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  rc |= hipMemsetAsync(d->time, 0, sizeof(unsigned), s);
+  rc |= hipMemsetAsync(d->err, 0, 4 * sizeof(int), s);
+  rc |= hipMemsetAsync(d->obs, 0, reps_.size() * kNRed * sizeof(double), s);
+  rc |= hipStreamSynchronize(s);
+  return check_device(rc, "geometry upload");
+}
+
+int Engine::add_type(double mass, float radius, int dynamics) {
+  if (!(mass > 0)) return fail(kErrRange, "mass must be positive");
+  if (dynamics != kDynNewtonian && dynamics != kDynOverdamped) return fail(kErrRange, "unknown dynamics");
+  TypeInfo t;
+  t.mass = mass;
+  t.imass = 1.0 / mass;
+  t.radius = radius;
+  t.dynamics = dynamics;
+  types_.push_back(t);
+  int nt = (int)types_.size();
+  std::vector<int> m((size_t)nt * nt, -1);
+  for (int i = 0; i < nt - 1; ++i)
+    for (int j = 0; j < nt - 1; ++j) m[(size_t)i * nt + j] = potmat_[(size_t)i * (nt - 1) + j];
+  potmat_.swap(m);
+  potentials_dirty_ = true;
+  return nt - 1;
+}
+
+// engine_addpot (engine.cpp:960-981): the same potential for (i, j), (j, i).
+int Engine::add_pot(const Potential &p, int i, int j) {
+  int nt = (int)types_.size();
+  if (i < 0 || i >= nt || j < 0 || j >= nt) return fail(kErrRange, "type index out of range");
+  if (p.flags & kPotFlagBound) return fail(kErrPotential, "cluster-bound potentials are not supported");
+  if (p.kind == kPotKindTable && (p.n < 1 || p.c.size() < (size_t)(p.n + 1) * kPotChunk))
+    return fail(kErrPotential, "potential has no table");
+  pots_.push_back(p);
+  int idx = (int)pots_.size() - 1;
+  potmat_[(size_t)i * nt + j] = idx;
+  potmat_[(size_t)j * nt + i] = idx;
+  potentials_dirty_ = true;
+  return kErrOk;
+}
+
+int Engine::set_random_force(int type, float mean, float std, float duration) {
+  if (type < 0 || type >= (int)types_.size()) return fail(kErrRange, "type index out of range");
+  TypeInfo &t = types_[type];
+  t.random = true;
+  t.rnd_mean = mean;
+  t.rnd_std = std;
+  // durration_steps = ceil(durration / dt) (MxForce.cpp:397-407).
+  double steps = std::ceil(duration / dt_);
+  t.rnd_steps = steps < 1 ? 1u : (unsigned)steps;
+  any_random_ = true;
+  potentials_dirty_ = true;
+  return kErrOk;
+}
+
+int Engine::set_seed(uint64_t seed) {
+  seed_ = seed;
+  return kErrOk;
+}
+
+int Engine::set_dpd_counter_noise(int on) {
+  dpd_noise_ = on ? 1 : 0;
+  return kErrOk;
+}
+
+int Engine::set_max_dist_fraction(float f) {
+  if (!(f > 0)) return fail(kErrRange, "max distance fraction must be positive");
+  max_frac_ = f;
+  if (d_) return build_geometry();
+  return kErrOk;
+}
+
+int Engine::set_block_size(int b) {
+  if (b != kTile) return fail(kErrRange, "the force kernel is compiled for 256 threads per block");
+  block_ = b;
+  return kErrOk;
+}
+
+int Engine::upload_potentials() {
+  DeviceState *d = d_;
+  std::vector<float> coef;
+  auto pack = [&](const Potential &p) {
+    DevPot D;
+    std::memset(&D, 0, sizeof(D));
+    D.kind = p.kind;
+    D.flags = p.flags;
+    D.alpha0 = p.alpha[0];
+    D.alpha1 = p.alpha[1];
+    D.alpha2 = p.alpha[2];
+    D.a = p.a;
+    D.b = p.b;
+    D.r0_plusone = p.r0_plusone;
+    D.n = p.n;
+    D.dpd_alpha = p.dpd_alpha;
+    D.dpd_gamma = p.dpd_gamma;
+    D.dpd_sigma = p.dpd_sigma;
+    D.a2 = p.a * p.a;
+    D.coff = (int)coef.size();
+    if (p.kind == kPotKindTable) coef.insert(coef.end(), p.c.begin(), p.c.end());
+    return D;
+  };
+  std::vector<DevPot> dp, dbp;
+  any_dpd_ = any_table_ = false;
+  for (auto &p : pots_) {
+    dp.push_back(pack(p));
+    if (p.kind == kPotKindDPD)
+      any_dpd_ = true;
+    else
+      any_table_ = true;
+  }
+  for (auto &p : bpots_) dbp.push_back(pack(p));
+  std::vector<DevType> dt(types_.size());
+  for (size_t i = 0; i < types_.size(); ++i) {
+    dt[i].mass = types_[i].mass;
+    dt[i].imass = types_[i].imass;
+    dt[i].radius = types_[i].radius;
+    dt[i].dynamics = types_[i].dynamics;
+    dt[i].random = types_[i].random ? 1 : 0;
+    dt[i].rnd_mean = types_[i].rnd_mean;
+    dt[i].rnd_std = types_[i].rnd_std;
+    dt[i].rnd_steps = types_[i].rnd_steps;
+  }
+  // Potentials change at set-up only, so reallocating here is fine.
+  auto upload = [&](auto **dst, const auto &vec) -> int {
+    using T = typename std::remove_reference<decltype(vec)>::type::value_type;
+    if (int st = dalloc(d, dst, vec.size())) return st;
+    if (!vec.empty())
+      return (int)hipMemcpyAsync(*dst, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice, d->stream);
+    return 0;
+  };
+  int rc = 0;
+  rc |= upload(&d->pots, dp);
+  rc |= upload(&d->bpots, dbp);
+  rc |= upload(&d->coef, coef);
+  rc |= upload(&d->potmat, potmat_);
+  rc |= upload(&d->types, dt);
+  rc |= hipStreamSynchronize(d->stream);
+  if (rc) return check_device(rc, "potential upload");
+  potentials_dirty_ = false;
+  return kErrOk;
+}
+
+// engine_load / space_addpart (space.cpp:397-477): cell index
+// (x - origin) * ih in double, local coordinate x - cell origin, particles
+// kept in insertion order inside each cell.
+int Engine::load(int N, const double *x, const double *v, const int *type, const int *replica,
+                 const uint16_t *flags) {
+  if (!d_) return fail(kErrNull, "engine not initialised");
+  if (N <= 0 || !x || !type) return fail(kErrNull, "empty particle set");
+  const int nt = (int)types_.size();
+  std::vector<int> cellof(N);
+  std::vector<float4> xl(N);
+  for (int i = 0; i < N; ++i) {
+    const int r = replica ? replica[i] : 0;
+    if (r < 0 || r >= (int)reps_.size()) return fail(kErrRange, "replica index out of range");
+    const int t = type[i];
+    if (t < 0 || t >= nt) return fail(kErrRange, "type index out of range");
+    if (types_[t].radius > cutoff_)
+      return fail(kErrRange, "particles with radius larger than the cutoff (large particles) are not supported");
+    const HostReplica &R = reps_[r];
+    int ind[3];
+    for (int k = 0; k < 3; ++k) ind[k] = (x[3 * (size_t)i + k] - R.origin[k]) * R.ih[k];
+    for (int k = 0; k < 3; ++k)
+      if (ind[k] < 0 || ind[k] >= R.cdim[k])
+        return fail(kErrSpace, "particle " + std::to_string(i) + " lies outside the space");
+    float loc[3];
+    for (int k = 0; k < 3; ++k) loc[k] = x[3 * (size_t)i + k] - R.corg[k][ind[k]];
+    cellof[i] = R.cell_base + (ind[0] * R.cdim[1] + ind[1]) * R.cdim[2] + ind[2];
+    const uint16_t fl = flags ? flags[i] : 0;
+    const unsigned bits = (unsigned)t | ((unsigned)fl << 16);
+    float wbits;
+    std::memcpy(&wbits, &bits, sizeof(wbits));
+    xl[i] = make_float4(loc[0], loc[1], loc[2], wbits);
+  }
+  // Stable counting sort by cell.
+  std::vector<int> cstart(ncells_ + 1, 0);
+  for (int i = 0; i < N; ++i) cstart[cellof[i] + 1]++;
+  for (int c = 0; c < ncells_; ++c) {
+    if (cstart[c + 1] > 65535) return fail(kErrMaxParts, "more than 65535 particles in one cell");
+  }
+  for (int c = 0; c < ncells_; ++c) cstart[c + 1] += cstart[c];
+  std::vector<int> fill(cstart.begin(), cstart.end() - 1);
+  std::vector<float4> xs(N), vs(N);
+  std::vector<int> pid(N), cell(N), slot_of(N);
+  for (int i = 0; i < N; ++i) {
+    const int k = fill[cellof[i]]++;
+    xs[k] = xl[i];
+    vs[k] = v ? make_float4((float)v[3 * (size_t)i], (float)v[3 * (size_t)i + 1], (float)v[3 * (size_t)i + 2], 0.f)
+              : make_float4(0.f, 0.f, 0.f, 0.f);
+    pid[k] = i;
+    cell[k] = cellof[i];
+    slot_of[i] = k;
+  }
+  // Tiles: per replica, slot ranges of kTile.
+  std::vector<int4> tiles;
+  std::vector<int2> rep_tiles(reps_.size());
+  for (size_t r = 0; r < reps_.size(); ++r) {
+    HostReplica &R = reps_[r];
+    R.slot_base = cstart[R.cell_base];
+    R.nparts = cstart[R.cell_base + R.ncells] - R.slot_base;
+    R.tile_begin = (int)tiles.size();
+    for (int s = 0; s < R.nparts; s += kTile)
+      tiles.push_back(make_int4(R.slot_base + s, R.slot_base + std::min(s + kTile, R.nparts), (int)r, 0));
+    R.tile_end = (int)tiles.size();
+    rep_tiles[r] = make_int2(R.tile_begin, R.tile_end);
+  }
+  N_ = N;
+  ntiles_ = (int)tiles.size();
+
+  DeviceState *d = d_;
+  // (Re)allocate particle buffers.
+  for (int b = 0; b < 2; ++b) {
+    if (int st = dalloc(d, &d->x[b], N)) return check_device(st, "hipMalloc");
+    if (int st = dalloc(d, &d->v[b], N)) return check_device(st, "hipMalloc");
+    if (int st = dalloc(d, &d->pf[b], N)) return check_device(st, "hipMalloc");
+    if (int st = dalloc(d, &d->pid[b], N)) return check_device(st, "hipMalloc");
+    if (int st = dalloc(d, &d->cell[b], N)) return check_device(st, "hipMalloc");
+  }
+  if (int st = dalloc(d, &d->f, N)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->slot_of, N)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->ncount, N)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->tiles, tiles.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->partials, tiles.size() * kNRed)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->rep_tiles, rep_tiles.size())) return check_device(st, "hipMalloc");
+  hipStream_t s = d->stream;
+  int rc = 0;
+  rc |= hipMemcpyAsync(d->x[0], xs.data(), N * sizeof(float4), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->v[0], vs.data(), N * sizeof(float4), hipMemcpyHostToDevice, s);
+  rc |= hipMemsetAsync(d->pf[0], 0, N * sizeof(float4), s);
+  rc |= hipMemsetAsync(d->pf[1], 0, N * sizeof(float4), s);
+  rc |= hipMemsetAsync(d->f, 0, N * sizeof(float4), s);
+  rc |= hipMemcpyAsync(d->pid[0], pid.data(), N * sizeof(int), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->pid[1], pid.data(), N * sizeof(int), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->cell[0], cell.data(), N * sizeof(int), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->cstart[0], cstart.data(), (ncells_ + 1) * sizeof(int), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->slot_of, slot_of.data(), N * sizeof(int), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->rep_tiles, rep_tiles.data(), rep_tiles.size() * sizeof(int2), hipMemcpyHostToDevice, s);
+  rc |= hipMemsetAsync(d->partials, 0, tiles.size() * kNRed * sizeof(double), s);
+  rc |= hipStreamSynchronize(s);
+  if (rc) return check_device(rc, "particle upload");
+  pid_prev_valid_ = false;
+  return kErrOk;
+}
+
+// engine_bond_alloc + MxBond fields (bond.h:47-70): bonds are kept in a CSR
+// by particle id so each particle gathers its own bond forces.
+int Engine::set_bonds(int M, const int *bi, const int *bj, const Potential *const *pots, const double *dissociation) {
+  if (!d_ || N_ == 0) return fail(kErrNull, "load particles before bonds");
+  if (M < 0) return fail(kErrRange, "negative bond count");
+  std::vector<int> vi(M), vj(M), vp(M);
+  std::vector<double> vd(M);
+  bpots_.clear();
+  std::vector<const Potential *> seen;
+  for (int b = 0; b < M; ++b) {
+    if (bi[b] < 0 || bi[b] >= N_ || bj[b] < 0 || bj[b] >= N_ || bi[b] == bj[b])
+      return fail(kErrBond, "bond " + std::to_string(b) + " has an invalid particle id");
+    const Potential *p = pots[b];
+    if (!p || p->kind != kPotKindTable || p->n < 1) return fail(kErrBond, "bond potential must be a table potential");
+    auto it = std::find(seen.begin(), seen.end(), p);
+    int idx;
+    if (it == seen.end()) {
+      seen.push_back(p);
+      bpots_.push_back(*p);
+      idx = (int)bpots_.size() - 1;
+    } else {
+      idx = (int)(it - seen.begin());
+    }
+    vi[b] = bi[b];
+    vj[b] = bj[b];
+    vp[b] = idx;
+    vd[b] = dissociation ? dissociation[b] : std::numeric_limits<double>::infinity();
+  }
+  // Bond replicas must match (a bond never crosses universes).
+  std::vector<int> off(N_ + 1, 0);
+  for (int b = 0; b < M; ++b) {
+    off[vi[b] + 1]++;
+    off[vj[b] + 1]++;
+  }
+  for (int i = 0; i < N_; ++i) off[i + 1] += off[i];
+  std::vector<int> ent(2 * (size_t)M), fill(off.begin(), off.end() - 1);
+  for (int b = 0; b < M; ++b) {
+    ent[fill[vi[b]]++] = 2 * b;
+    ent[fill[vj[b]]++] = 2 * b + 1;
+  }
+  DeviceState *d = d_;
+  if (int st = dalloc(d, &d->boff, N_ + 1)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->bent, ent.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->bi, M)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->bj, M)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->bpot, M)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->bdis, M)) return check_device(st, "hipMalloc");
+  for (int b = 0; b < 2; ++b)
+    if (int st = dalloc(d, &d->bact[b], M)) return check_device(st, "hipMalloc");
+  hipStream_t s = d->stream;
+  int rc = 0;
+  rc |= hipMemcpyAsync(d->boff, off.data(), (N_ + 1) * sizeof(int), hipMemcpyHostToDevice, s);
+  if (M > 0) {
+    rc |= hipMemcpyAsync(d->bent, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice, s);
+    rc |= hipMemcpyAsync(d->bi, vi.data(), M * sizeof(int), hipMemcpyHostToDevice, s);
+    rc |= hipMemcpyAsync(d->bj, vj.data(), M * sizeof(int), hipMemcpyHostToDevice, s);
+    rc |= hipMemcpyAsync(d->bpot, vp.data(), M * sizeof(int), hipMemcpyHostToDevice, s);
+    rc |= hipMemcpyAsync(d->bdis, vd.data(), M * sizeof(double), hipMemcpyHostToDevice, s);
+    rc |= hipMemsetAsync(d->bact[0], 1, M, s);
+    rc |= hipMemsetAsync(d->bact[1], 1, M, s);
+  }
+  rc |= hipStreamSynchronize(s);
+  if (rc) return check_device(rc, "bond upload");
+  M_ = M;
+  bact_cur_ = 0;
+  potentials_dirty_ = true;
+  return kErrOk;
+}
+
+int Engine::launch_step() {
+  DeviceState *d = d_;
+  const int a = 0, b = 1;  // buffer 0 is the step's input, 1 its output
+  StepArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.x = d->x[a];
+  A.v = d->v[a];
+  A.pf = d->pf[a];
+  A.pid = d->pid[a];
+  A.cell = d->cell[a];
+  A.cstart = d->cstart[a];
+  A.x_out = d->x[b];
+  A.v_out = d->v[b];
+  A.pf_out = d->pf[b];
+  A.f_out = d->f;
+  A.cell_out = d->cell[b];
+  A.rep = d->rep;
+  A.offs = d->offs;
+  A.shifts = d->shifts;
+  A.tiles = d->tiles;
+  A.ntiles = ntiles_;
+  A.pots = d->pots;
+  A.bpots = d->bpots;
+  A.coef = d->coef;
+  A.potmat = d->potmat;
+  A.ntypes = (int)types_.size();
+  A.types = d->types;
+  A.boff = d->boff;
+  A.bent = d->bent;
+  A.bi = d->bi;
+  A.bj = d->bj;
+  A.bpot = d->bpot;
+  A.bdis = d->bdis;
+  A.bact_in = M_ > 0 ? d->bact[bact_cur_] : nullptr;
+  A.bact_out = M_ > 0 ? d->bact[bact_cur_ ^ 1] : nullptr;
+  A.slot_of = d->slot_of;
+  A.dt = (float)dt_;
+  const float cf = (float)cutoff_;
+  A.cut2_pair = cf * cf;
+  A.cut2_self = (float)(cutoff_ * cutoff_);
+  // dpd_eval's delta = 1/sqrt(dt) (dpd_eval.hpp:23).
+  A.dpd_delta = 1.f / std::sqrt(dt_);
+  A.seed_lo = (uint32_t)seed_;
+  A.seed_hi = (uint32_t)(seed_ >> 32);
+  A.dpd_noise = dpd_noise_;
+  A.time = d->time;
+  A.err = d->err;
+  A.partials = d->partials;
+
+  int mode = 0;
+  if (any_table_) mode |= kModeTable;
+  if (any_dpd_) mode |= kModeDPD;
+  if (virial_) mode |= kModeVirial;
+  if (M_ > 0) mode |= kModeBonds;
+  A.random = any_random_ ? 1 : 0;
+  hipStream_t s = d->stream;
+  if (timing_) hipEventRecord(d->ev[0], s);
+  kStepLaunchers[mode](A, ntiles_, s);
+  if (timing_) hipEventRecord(d->ev[1], s);
+
+  RebuildArgs B;
+  B.cstart = d->cstart[a];
+  B.cstart_new = d->cstart[b];
+  B.cell_out = d->cell[b];
+  B.cell_rep = d->cell_rep;
+  B.rep = d->rep;
+  B.cnt = d->cnt;
+  B.ncells = ncells_;
+  B.x_in = d->x[b];
+  B.v_in = d->v[b];
+  B.pf_in = d->pf[b];
+  B.pid_in = d->pid[a];
+  B.x = d->x[a];
+  B.v = d->v[a];
+  B.pf = d->pf[a];
+  B.pid = d->pid[b];
+  B.cell = d->cell[a];
+  B.slot_of = d->slot_of;
+  B.with_pf = any_random_ ? 1 : 0;
+  const int cg = (ncells_ + 255) / 256;
+  hipLaunchKernelGGL(k_count, dim3(cg), dim3(256), 0, s, B);
+  const int nb = (ncells_ + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, d->cnt, d->cstart[b], d->bsum, ncells_);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, d->bsum, nb, d->cstart[b], ncells_);
+  hipLaunchKernelGGL(k_scan_add, dim3(cg), dim3(256), 0, s, d->cstart[b], d->bsum, ncells_);
+  hipLaunchKernelGGL(k_scatter, dim3(cg), dim3(256), 0, s, B);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d->partials, d->rep_tiles,
+                     (int)reps_.size(), d->obs, d->time, d->err);
+  if (timing_) hipEventRecord(d->ev[2], s);
+  // After the scatter the particle arrays of buffer a hold the new order,
+  // except pid/cstart which were written into buffer b.
+  std::swap(d->pid[0], d->pid[1]);
+  std::swap(d->cstart[0], d->cstart[1]);
+  // cell[a] holds the new cells; x/v/pf[a] hold the new state.
+  if (M_ > 0) bact_cur_ ^= 1;
+  pid_prev_valid_ = true;
+  return (int)hipGetLastError();
+}
+
+int Engine::step(int nsteps) {
+  if (!d_ || N_ == 0) return fail(kErrNull, "no particles loaded");
+  if (err_ != 0) return err_;
+  if (potentials_dirty_)
+    if (int rc = upload_potentials()) return rc;
+  for (int k = 0; k < nsteps; ++k) {
+    if (int st = launch_step()) return check_device(st, "kernel launch");
+  }
+  int herr[4];
+  unsigned htime = 0;
+  int rc = hipMemcpyAsync(herr, d_->err, sizeof(herr), hipMemcpyDeviceToHost, d_->stream);
+  rc |= hipMemcpyAsync(&htime, d_->time, sizeof(htime), hipMemcpyDeviceToHost, d_->stream);
+  rc |= hipStreamSynchronize(d_->stream);
+  if (rc) return check_device(rc, "step");
+  time_ = htime;
+  if (timing_) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, d_->ev[0], d_->ev[1]);
+    force_ms_ = ms;
+    hipEventElapsedTime(&ms, d_->ev[0], d_->ev[2]);
+    step_ms_ = ms;
+  }
+  if (herr[0] != 0) {
+    return fail(herr[0], "particle " + std::to_string(herr[1]) + " moved too fast (more than one cell in a step)");
+  }
+  return kErrOk;
+}
+
+int Engine::sync() {
+  if (!d_) return kErrOk;
+  return check_device(hipStreamSynchronize(d_->stream), "sync");
+}
+
+int Engine::set_timing(int on) {
+  timing_ = on != 0;
+  return kErrOk;
+}
+
+int Engine::set_virial(int on) {
+  virial_ = on != 0;
+  return kErrOk;
+}
+
+int Engine::get_state(float *xlocal, int *cell, float *v, float *f, float *pf) {
+  if (!d_ || N_ == 0) return fail(kErrNull, "no particles loaded");
+  DeviceState *d = d_;
+  std::vector<float4> hx(N_), hv(N_), hf(N_), hpf(N_);
+  std::vector<int> hpid(N_), hcell(N_), hpidf(N_);
+  hipStream_t s = d->stream;
+  int rc = 0;
+  rc |= hipMemcpyAsync(hx.data(), d->x[0], N_ * sizeof(float4), hipMemcpyDeviceToHost, s);
+  rc |= hipMemcpyAsync(hv.data(), d->v[0], N_ * sizeof(float4), hipMemcpyDeviceToHost, s);
+  rc |= hipMemcpyAsync(hf.data(), d->f, N_ * sizeof(float4), hipMemcpyDeviceToHost, s);
+  rc |= hipMemcpyAsync(hpf.data(), d->pf[0], N_ * sizeof(float4), hipMemcpyDeviceToHost, s);
+  rc |= hipMemcpyAsync(hpid.data(), d->pid[0], N_ * sizeof(int), hipMemcpyDeviceToHost, s);
+  rc |= hipMemcpyAsync(hpidf.data(), d->pid[1], N_ * sizeof(int), hipMemcpyDeviceToHost, s);
+  rc |= hipMemcpyAsync(hcell.data(), d->cell[0], N_ * sizeof(int), hipMemcpyDeviceToHost, s);
+  rc |= hipStreamSynchronize(s);
+  if (rc) return check_device(rc, "state download");
+  for (int k = 0; k < N_; ++k) {
+    const size_t i = hpid[k];
+    if (xlocal) {
+      xlocal[3 * i] = hx[k].x;
+      xlocal[3 * i + 1] = hx[k].y;
+      xlocal[3 * i + 2] = hx[k].z;
+    }
+    if (v) {
+      v[3 * i] = hv[k].x;
+      v[3 * i + 1] = hv[k].y;
+      v[3 * i + 2] = hv[k].z;
+    }
+    if (pf) {
+      pf[3 * i] = hpf[k].x;
+      pf[3 * i + 1] = hpf[k].y;
+      pf[3 * i + 2] = hpf[k].z;
+    }
+    if (cell) cell[i] = hcell[k];
+  }
+  if (f) {
+    // Forces are stored in the slot order of the step that computed them.
+    const std::vector<int> &order = pid_prev_valid_ ? hpidf : hpid;
+    for (int k = 0; k < N_; ++k) {
+      const size_t i = order[k];
+      f[3 * i] = hf[k].x;
+      f[3 * i + 1] = hf[k].y;
+      f[3 * i + 2] = hf[k].z;
+    }
+  }
+  return kErrOk;
+}
+
+int Engine::get_global(double *x, double *v) {
+  std::vector<float> xl(3 * (size_t)N_), vv(3 * (size_t)N_);
+  std::vector<int> cell(N_);
+  if (int rc = get_state(xl.data(), cell.data(), vv.data(), nullptr, nullptr)) return rc;
+  for (int i = 0; i < N_; ++i) {
+    const int c = cell[i];
+    const HostReplica *R = nullptr;
+    for (const auto &r : reps_)
+      if (c >= r.cell_base && c < r.cell_base + r.ncells) R = &r;
+    const int lc = c - R->cell_base;
+    const int l[3] = {lc / (R->cdim[1] * R->cdim[2]), (lc / R->cdim[2]) % R->cdim[1], lc % R->cdim[2]};
+    for (int k = 0; k < 3; ++k) {
+      if (x) x[3 * (size_t)i + k] = R->corg[k][l[k]] + xl[3 * (size_t)i + k];
+      if (v) v[3 * (size_t)i + k] = vv[3 * (size_t)i + k];
+    }
+  }
+  return kErrOk;
+}
+
+int Engine::neighbour_counts(int *counts) {
+  if (!d_ || N_ == 0) return fail(kErrNull, "no particles loaded");
+  if (potentials_dirty_)
+    if (int rc = upload_potentials()) return rc;
+  DeviceState *d = d_;
+  StepArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.x = d->x[0];
+  A.pid = d->pid[0];
+  A.cell = d->cell[0];
+  A.cstart = d->cstart[0];
+  A.rep = d->rep;
+  A.offs = d->offs;
+  A.shifts = d->shifts;
+  A.tiles = d->tiles;
+  A.ntiles = ntiles_;
+  const float cf = (float)cutoff_;
+  A.cut2_pair = cf * cf;
+  A.cut2_self = (float)(cutoff_ * cutoff_);
+  A.ncount = d->ncount;
+  hipLaunchKernelGGL(k_ncount, dim3(ntiles_), dim3(kTile), 0, d->stream, A);
+  int rc = hipMemcpyAsync(counts, d->ncount, N_ * sizeof(int), hipMemcpyDeviceToHost, d->stream);
+  rc |= hipStreamSynchronize(d->stream);
+  return check_device(rc, "neighbour counts");
+}
+
+int Engine::observables(int replica, double *out) {
+  if (!d_) return fail(kErrNull, "engine not initialised");
+  const int R = (int)reps_.size();
+  std::vector<double> h((size_t)R * kNRed);
+  int rc = hipMemcpyAsync(h.data(), d_->obs, h.size() * sizeof(double), hipMemcpyDeviceToHost, d_->stream);
+  rc |= hipStreamSynchronize(d_->stream);
+  if (rc) return check_device(rc, "observables");
+  if (replica >= 0) {
+    if (replica >= R) return fail(kErrRange, "replica index out of range");
+    for (int k = 0; k < kNRed; ++k) out[k] = h[(size_t)replica * kNRed + k];
+  } else {
+    for (int k = 0; k < kNRed; ++k) {
+      double s = 0;
+      for (int r = 0; r < R; ++r) s += h[(size_t)r * kNRed + k];
+      out[k] = s;
+    }
+  }
+  return kErrOk;
+}
+
+int Engine::bond_active(uint8_t *active) {
+  if (M_ == 0) return kErrOk;
+  int rc = hipMemcpyAsync(active, d_->bact[bact_cur_], M_, hipMemcpyDeviceToHost, d_->stream);
+  rc |= hipStreamSynchronize(d_->stream);
+  return check_device(rc, "bond flags");
+}
+
+int Engine::geometry(int replica, int cdim[3], double h[3]) {
+  if (replica < 0 || replica >= (int)reps_.size()) return fail(kErrRange, "replica index out of range");
+  for (int k = 0; k < 3; ++k) {
+    cdim[k] = reps_[replica].cdim[k];
+    h[k] = reps_[replica].h[k];
+  }
+  return kErrOk;
+}
+
+}  // namespace mxg

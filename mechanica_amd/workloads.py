@@ -1,0 +1,210 @@
+"""Seeded synthetic inputs for the benchmark workloads.
+
+Each builder returns a dict with the universe geometry, particle arrays,
+types, potentials (as constructor arguments, so the same tables can be built
+for the GPU engine and handed to the CPU checker) and bonds.  Sizes are
+parameters so tests can build the same workload at small N.
+
+  argon_lj   N LJ 12-6 argon particles (mass 39.4, radius 0.1) uniform in a
+             periodic box at density 10, Gaussian velocities at kT = 1,
+             LJ(0.275, 1.0, A=9.5075e-6, B=6.1545e-3, tol=1e-3), cutoff 1,
+             Euler dt = 0.01 (examples/argon.py).
+  dpd_fluid  N DPD particles at number density 3 (box side (N/3)^(1/3)),
+             alpha = 25, gamma = 4.5, sigma = 0 or 3, cutoff 1, dt = 0.01,
+             cells of side ~1 (examples/dpd1.py, dpd2.py at scale).
+  lattice    n^3 simple-cubic lattice (spacing 1, jitter U(+-0.05)) with
+             harmonic bonds to +x/+y/+z neighbours inside the box (the open
+             lattice of src/lattice.py:449-522; 3 n^2 (n-1) bonds), K = 100,
+             r0 = 1, dissociation energy 5, Morse(d=1, a=6, r0=1, max=2)
+             between all pairs, cutoff 2, overdamped, dt = 0.005.
+  cell_sort  two-type Morse cell-sorting universe (examples/cell_sorting.py)
+             at density 0.8: AA and BB d = 3, AB d = 1, a = 6, r0 = 1,
+             cutoff 2.5, overdamped with a persistent random force whose
+             per-component variance is 2 kT / (dt / m) at kT = 0.1.
+"""
+import math
+
+import numpy as np
+
+LJ_A = 9.5075e-06
+LJ_B = 6.1545e-03
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def _min_dist_fill(rng, n, L, dmin, max_tries=200):
+    """Uniform positions in [0, L)^3 with all periodic pair distances >= dmin
+    (rejection on a cell grid)."""
+    pts = np.zeros((n, 3))
+    nc = max(1, int(L / dmin))
+    h = L / nc
+    grid = {}
+    k = 0
+    tries = 0
+    while k < n:
+        p = rng.random(3) * L
+        ci = tuple((p / h).astype(int) % nc)
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    cc = ((ci[0] + dx) % nc, (ci[1] + dy) % nc, (ci[2] + dz) % nc)
+                    for q in grid.get(cc, ()):
+                        d = pts[q] - p
+                        d -= L * np.round(d / L)
+                        if d @ d < dmin * dmin:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if ok:
+            pts[k] = p
+            grid.setdefault(ci, []).append(k)
+            k += 1
+            tries = 0
+        else:
+            tries += 1
+            if tries > max_tries * n:
+                raise RuntimeError("could not place particles")
+    return pts
+
+
+def argon_lj(N=10000, density=10.0, seed=1, min_dist=None):
+    rng = _rng(seed)
+    L = (N / density) ** (1.0 / 3.0)
+    cells = max(3, int(math.floor(L / 1.0)))
+    mass = 39.4
+    if min_dist:
+        x = _min_dist_fill(rng, N, L, min_dist)
+    else:
+        x = rng.random((N, 3)) * L
+    v = rng.normal(0.0, math.sqrt(1.0 / mass), size=(N, 3))
+    return {
+        "name": "argon_lj",
+        "dim": [L, L, L], "cells": [cells] * 3, "cutoff": 1.0, "dt": 0.01,
+        "types": [{"mass": mass, "radius": 0.1, "dynamics": 0}],
+        "x": x, "v": v, "type": np.zeros(N, np.int32),
+        "pots": [("lj126", (0.275, 1.0, LJ_A, LJ_B, 1e-3), 0, 0)],
+        "bonds": None,
+    }
+
+
+def dpd_fluid(N=1048576, density=3.0, sigma=0.0, seed=2):
+    rng = _rng(seed)
+    L = (N / density) ** (1.0 / 3.0)
+    cells = max(3, int(math.floor(L / 1.0)))
+    x = rng.random((N, 3)) * L
+    v = rng.normal(0.0, 1.0, size=(N, 3))
+    return {
+        "name": "dpd_fluid",
+        "dim": [L, L, L], "cells": [cells] * 3, "cutoff": 1.0, "dt": 0.01,
+        "types": [{"mass": 1.0, "radius": 1.0, "dynamics": 0}],
+        "x": x, "v": v, "type": np.zeros(N, np.int32),
+        "pots": [("dpd", (25.0, 4.5, sigma, 1.0, 0), 0, 0)],
+        "bonds": None,
+    }
+
+
+def lattice(n=64, seed=3, jitter=0.05, dissociation=5.0):
+    rng = _rng(seed)
+    idx = np.arange(n)
+    g = np.stack(np.meshgrid(idx, idx, idx, indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    x = g + 0.5 + rng.uniform(-jitter, jitter, size=g.shape)
+    N = n ** 3
+    lin = np.arange(N).reshape(n, n, n)
+    bi, bj = [], []
+    for ax in range(3):
+        sl_a = [slice(None)] * 3
+        sl_b = [slice(None)] * 3
+        sl_a[ax] = slice(0, n - 1)
+        sl_b[ax] = slice(1, n)
+        bi.append(lin[tuple(sl_a)].ravel())
+        bj.append(lin[tuple(sl_b)].ravel())
+    bi = np.concatenate(bi).astype(np.int32)
+    bj = np.concatenate(bj).astype(np.int32)
+    cells = max(3, n // 2)
+    return {
+        "name": "lattice",
+        "dim": [float(n)] * 3, "cells": [cells] * 3, "cutoff": 2.0, "dt": 0.005,
+        "types": [{"mass": 1.0, "radius": 1.0, "dynamics": 1}],
+        "x": x, "v": np.zeros_like(x), "type": np.zeros(N, np.int32),
+        "pots": [("morse", (1.0, 6.0, 1.0, 1e-4, 2.0, 1e-3), 0, 0)],
+        "bonds": {"i": bi, "j": bj, "pot": ("harmonic", (0.0, 2.0, 100.0, 1.0, 0.02)),
+                  "dissociation": dissociation},
+    }
+
+
+def cell_sort_replica(N, seed, kT=0.1, dt=0.01, density=0.8):
+    rng = _rng(seed)
+    L = (N / density) ** (1.0 / 3.0)
+    cells = max(3, int(math.floor(L / 2.5)))
+    x = rng.random((N, 3)) * L
+    t = np.zeros(N, np.int32)
+    t[N // 2:] = 1
+    rng.shuffle(t)
+    return {"dim": [L, L, L], "cells": [cells] * 3, "x": x, "type": t}
+
+
+def cell_sort_params(kT=0.1, dt=0.01, mass=1.0):
+    # Per-component variance of the random force 2 kT m / dt; the magnitude
+    # of MxRandomVector is N(0, std) over a uniform direction, so each
+    # component has variance std^2 / 3.
+    std = math.sqrt(3.0 * 2.0 * kT * mass / dt)
+    return {
+        "cutoff": 2.5, "dt": dt,
+        "types": [{"mass": mass, "radius": 1.0, "dynamics": 1}, {"mass": mass, "radius": 1.0, "dynamics": 1}],
+        "pots": [("morse", (3.0, 6.0, 1.0, 1e-4, 2.5, 1e-3), 0, 0),
+                 ("morse", (3.0, 6.0, 1.0, 1e-4, 2.5, 1e-3), 1, 1),
+                 ("morse", (1.0, 6.0, 1.0, 1e-4, 2.5, 1e-3), 0, 1)],
+        "random": {"mean": 0.0, "std": std, "duration": dt},
+    }
+
+
+def ensemble_sizes(R=256, lo=500, hi=20000, seed=4):
+    rng = _rng(seed)
+    return np.exp(rng.uniform(math.log(lo), math.log(hi), size=R)).astype(np.int64)
+
+
+def make_potential(m, spec):
+    """Builds a Potential from a (kind, args) spec with the given module."""
+    kind, args = spec[0], spec[1]
+    if kind == "lj126":
+        return m.Potential.lennard_jones_12_6(*args)
+    if kind == "morse":
+        d, a, r0, lo, hi, tol = args
+        return m.Potential.morse(d=d, a=a, r0=r0, min=lo, max=hi, tol=tol)
+    if kind == "harmonic":
+        lo, hi, k, r0, tol = args
+        return m.Potential.harmonic(k, r0, min=lo, max=hi, tol=tol)
+    if kind == "dpd":
+        al, g, s, c, sh = args
+        return m.Potential.dpd(alpha=al, gamma=g, sigma=s, cutoff=c, shifted=bool(sh))
+    raise ValueError(kind)
+
+
+def build_universe(m, w, seed=None):
+    """Creates and loads a Universe for workload dict w."""
+    u = m.Universe(dim=w["dim"], cells=w["cells"], cutoff=w["cutoff"], dt=w["dt"])
+    for t in w["types"]:
+        u.add_type(t["mass"], t["radius"], t["dynamics"])
+    pots = []
+    for spec in w["pots"]:
+        p = make_potential(m, spec)
+        u.bind(p, spec[2], spec[3])
+        pots.append(p)
+    if seed is not None:
+        u.set_seed(seed)
+    u.load(w["x"], w["v"], w["type"])
+    if w.get("bonds"):
+        b = w["bonds"]
+        bp = make_potential(m, b["pot"])
+        u.set_bonds(b["i"], b["j"], bp, b["dissociation"])
+        pots.append(bp)
+    u._keep = pots
+    return u

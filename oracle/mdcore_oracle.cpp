@@ -1,0 +1,1299 @@
+// ORACLE — test infrastructure only.
+//
+// A CPU restatement of mdcore's force-and-advance step (engine_step) used as
+// the checker for the HIP engine and as the CPU baseline.  Nothing in the
+// product links or calls this file; only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg load the library built from it.
+//
+// What it restates, with the reference lines it follows (paths relative to
+// src/mdcore/src of the reference):
+//   space geometry, cell origins, task graph ........ space.cpp:626-978
+//   space_getsid (min-image shift, flip) ............ space.cpp:95-130
+//   sort-direction tables ........................... space_cell.cpp:54-99
+//   space_addpart (cell index, local coordinate) .... space.cpp:397-477
+//   engine_kinetic_energy ........................... engine.cpp:1698-1719
+//   space_prepare (force reset) ..................... space.cpp:202-242
+//   space_shuffle_local / welcome ................... space.cpp:333-393,
+//                                                     space_cell.cpp:224-392
+//   single-runner task order ........................ queue.cpp:84-246,
+//                                                     runner.cpp:244-418
+//   runner_dosort + runner_sort_descending .......... runner_dosort.cpp:84-146,
+//                                                     runner.cpp:102-165
+//   runner_dopair (sorted, pruned cell pair) ........ runner_dopair.cpp:91-340
+//   runner_doself (in-cell pairs, single body) ...... runner_dopair.cpp:446-682
+//   potential_eval / potential_eval_ex .............. potential_eval.hpp:67-161
+//   potential_eval_super_ex / dpd_eval .............. dpd_eval.hpp:21-169
+//   get_potential ................................... potential_eval.hpp:1543
+//   bond_eval (periodic shift, clamp, dissociation).. bond.cpp:99-295
+//   cell_advance_forward_euler ...................... engine_advance.cpp:126-239
+//   engine_advance_forward_euler .................... engine_advance.cpp:248-422
+//   gaussian_force (persistent random force) ........ MxForce.cpp:370-385
+//
+// Deviations, each stated where it happens:
+//   * the number density f[3] (smoothing_kernel.hpp, x86 rsqrt) is not
+//     computed: it never feeds a force;
+//   * gaussian_force draws from Philox4x32-10 instead of carbon's CRandom
+//     (absent from the reference tree), keyed by (particle id, time);
+//   * the optional counter-based DPD noise mode is this project's own; the
+//     default DPD mode is the reference's deterministic sigma term.
+//
+// Two precisions: the faithful FP32 path (every float/double step as the
+// reference), and an FP64 evaluation of the same pair set and tables
+// (forces64) that states the FP32 path's rounding error.
+//
+// Build with -O3 -mavx -msse4.2 -ffp-contract=off (no FMA), as the reference
+// is built (CMakeLists.txt:79-104).
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <random>
+#include <set>
+#include <thread>
+#include <vector>
+
+#include "../mechanica_amd/csrc/mxg_philox.h"
+
+namespace {
+
+constexpr int kChunk = 8;
+
+enum : uint32_t { kFlagScaled = 1u << 11, kFlagShifted = 1u << 12 };
+enum : uint32_t { kKindTable = 0, kKindDPD = 1 };
+enum : uint16_t { kFrozenX = 1 << 2, kFrozenY = 1 << 3, kFrozenZ = 1 << 4 };
+enum { kNewtonian = 0, kOverdamped = 1 };
+enum { kTaskSort = 0, kTaskSelf = 1, kTaskPair = 2 };
+constexpr int kErrToofast = -29;
+constexpr int kErrRange = -6;
+
+// Sort-direction tables (space_cell.cpp:54-99): the ID of the sorted list
+// for each of the 27 shift sign triples, the unit direction of each list, and
+// whether the pair has to be flipped so the shift points "forward".
+const char kSortlistID[27] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 0,
+                              12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0};
+const char kFlip[27] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0,
+                        0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+struct SortDirs {
+  float n[13 * 3];
+  SortDirs() {
+    // The 13 forward directions in list order: sign triples 26, 25, ..., 14.
+    for (int id = 0; id < 13; ++id) {
+      int s27 = 26 - id;
+      int t[3] = {s27 / 9 - 1, (s27 / 3) % 3 - 1, s27 % 3 - 1};
+      double len = std::sqrt((double)(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+      for (int k = 0; k < 3; ++k) n[3 * id + k] = (float)(t[k] / len);
+    }
+  }
+};
+const SortDirs &sortdirs() {
+  static const SortDirs d;
+  return d;
+}
+
+struct Pot {
+  uint32_t kind = kKindTable, flags = 0;
+  float alpha[4] = {0, 0, 0, 0};
+  std::vector<float> c;
+  float r0_plusone = 0, a = 0, b = 0;
+  int n = 0;
+  float dpd_alpha = 0, dpd_gamma = 0, dpd_sigma = 0;
+};
+
+struct Type {
+  double mass = 1, imass = 1;
+  float radius = 1;
+  int dynamics = kNewtonian;
+  // single-body persistent random force (gaussian_force)
+  bool has_random = false;
+  float rnd_mean = 0, rnd_std = 0;
+  unsigned rnd_steps = 1;
+};
+
+struct Part {
+  float f[4];
+  float v[4];
+  float x[4];
+  float pf[4];  // persistent random force
+  double imass;
+  float radius;
+  double mass;
+  int id;
+  int16_t type;
+  uint16_t flags;
+  int ncount;  // in-cutoff pairs seen during the last force evaluation
+};
+
+struct Cell {
+  int loc[3];
+  double origin[3];
+  std::vector<Part> parts;
+  std::vector<Part> incoming;
+  std::vector<uint32_t> sortlist;
+  double epot = 0;
+  int sort_flags = 0;
+  std::mt19937 gen;
+  std::normal_distribution<float> gauss{0.f, 1.f};
+  std::atomic<char> taboo{0};
+  std::mutex mtx;
+};
+
+struct Task {
+  int type, i, j, sid;
+  std::vector<int> unlock;
+  int wait = 0;
+};
+
+struct Bond {
+  int i, j, pot;
+  double dissociation;
+  bool active;
+};
+
+struct Oracle {
+  // geometry
+  double origin[3], dim[3], h[3], ih[3];
+  int cdim[3], span[3], ncells;
+  double cutoff, cutoff2;
+  float dt;
+  double dt_d;
+  float max_frac = 0.05f;
+  int nthreads = 1;
+  uint64_t time = 0;
+  uint32_t seed_lo = 0, seed_hi = 0;
+  int dpd_counter_noise = 0;
+  int err = 0;
+  int err_pid = -1;
+
+  std::vector<Cell> cells;
+  std::vector<Task> tasks;
+  std::vector<int> order;  // single-runner execution order
+  std::vector<Type> types;
+  std::vector<Pot> pots;       // pair potentials
+  std::vector<int> potmat;     // ntypes x ntypes -> index into pots, -1 none
+  std::vector<Pot> bpots;      // bond potentials
+  std::vector<Bond> bonds;
+  int nparts = 0;
+
+  double ekin = 0, epot_nb = 0, epot_bond = 0;
+  float dpd_delta = 0;
+  bool dpd_delta_set = false;
+
+  int cellid(int i, int j, int k) const { return (i * cdim[1] + j) * cdim[2] + k; }
+};
+
+// ---------------------------------------------------------------------------
+// Geometry and task graph (space.cpp:626-978, engine.cpp:1502-1560).
+
+// space_getsid: min-image shift from ci to cj (float), flipping so it points
+// forward.  Returns the sort-list ID.
+int getsid(const Oracle &o, int &ci, int &cj, float shift[3]) {
+  for (int k = 0; k < 3; ++k) {
+    shift[k] = o.cells[cj].origin[k] - o.cells[ci].origin[k];
+    if (shift[k] * 2 > o.dim[k])
+      shift[k] -= o.dim[k];
+    else if (shift[k] * 2 < -o.dim[k])
+      shift[k] += o.dim[k];
+  }
+  int sid = 0;
+  for (int k = 0; k < 3; ++k) sid = 3 * sid + ((shift[k] < 0) ? 0 : ((shift[k] > 0) ? 2 : 1));
+  if (kFlip[sid]) {
+    std::swap(ci, cj);
+    shift[0] = -shift[0];
+    shift[1] = -shift[1];
+    shift[2] = -shift[2];
+  }
+  return kSortlistID[sid];
+}
+
+int init_space(Oracle &o, const double origin[3], const double dim[3], const int cells_in[3],
+               double cutoff) {
+  int cells[3] = {cells_in[0], cells_in[1], cells_in[2]};
+  for (int k = 0; k < 3; ++k)
+    if (cells[k] < 3) cells[k] = 3;  // periodic needs >= 3 cells
+  double L[3];
+  for (int k = 0; k < 3; ++k) L[k] = (dim[k] - origin[k]) / cells[k];
+  for (int k = 0; k < 3; ++k) {
+    o.origin[k] = origin[k];
+    o.dim[k] = dim[k];
+    o.cdim[k] = (int)std::floor(dim[k] / L[k]);
+  }
+  o.cutoff = cutoff;
+  o.cutoff2 = cutoff * cutoff;
+  o.ncells = o.cdim[0] * o.cdim[1] * o.cdim[2];
+  for (int k = 0; k < 3; ++k) {
+    o.h[k] = o.dim[k] / o.cdim[k];
+    o.ih[k] = 1.0 / o.h[k];
+  }
+  o.cells = std::vector<Cell>(o.ncells);
+  for (int i = 0; i < o.ncells; ++i) o.cells[i].gen = std::mt19937(i);
+  int l[3];
+  double org[3];
+  for (l[0] = 0; l[0] < o.cdim[0]; l[0]++) {
+    org[0] = origin[0] + l[0] * o.h[0];
+    for (l[1] = 0; l[1] < o.cdim[1]; l[1]++) {
+      org[1] = origin[1] + l[1] * o.h[1];
+      for (l[2] = 0; l[2] < o.cdim[2]; l[2]++) {
+        org[2] = origin[2] + l[2] * o.h[2];
+        Cell &c = o.cells[o.cellid(l[0], l[1], l[2])];
+        for (int k = 0; k < 3; ++k) {
+          c.loc[k] = l[k];
+          c.origin[k] = org[k];
+        }
+      }
+    }
+  }
+  for (int k = 0; k < 3; ++k) o.span[k] = (int)std::ceil(cutoff * o.ih[k]);
+  for (int k = 0; k < 3; ++k)
+    if (o.cdim[k] < 2 * o.span[k] + 1) return kErrRange;
+
+  // Cell pairs and self tasks, in the reference's loop order.
+  o.tasks.clear();
+  for (int i = 0; i < o.cdim[0]; i++)
+    for (int j = 0; j < o.cdim[1]; j++)
+      for (int k = 0; k < o.cdim[2]; k++) {
+        int id1 = o.cellid(i, j, k);
+        for (l[0] = -o.span[0]; l[0] <= o.span[0]; l[0]++) {
+          int ii = (i + l[0] + o.cdim[0]) % o.cdim[0];
+          for (l[1] = -o.span[1]; l[1] <= o.span[1]; l[1]++) {
+            int jj = (j + l[1] + o.cdim[1]) % o.cdim[1];
+            for (l[2] = -o.span[2]; l[2] <= o.span[2]; l[2]++) {
+              double lh[3];
+              for (int q = 0; q < 3; ++q) lh[q] = o.h[q] * std::fmax(std::abs(l[q]) - 1, 0);
+              if (lh[0] * lh[0] + lh[1] * lh[1] + lh[2] * lh[2] > o.cutoff2) continue;
+              int kk = (k + l[2] + o.cdim[2]) % o.cdim[2];
+              int id2 = o.cellid(ii, jj, kk);
+              int ci = id1, cj = id2;
+              float sh[3];
+              int sid = getsid(o, ci, cj, sh);
+              if (id1 < id2 || (id1 == id2 && l[0] == 0 && l[1] == 0 && l[2] == 0)) {
+                Task t;
+                t.type = (id1 == id2) ? kTaskSelf : kTaskPair;
+                t.i = ci;
+                t.j = cj;
+                t.sid = sid;
+                o.tasks.push_back(t);
+              }
+            }
+          }
+        }
+      }
+  // Sort tasks, and pair -> sort dependencies.
+  int first_sort = (int)o.tasks.size();
+  for (int c = 0; c < o.ncells; ++c) {
+    Task t;
+    t.type = kTaskSort;
+    t.i = c;
+    t.j = -1;
+    t.sid = 0;
+    o.tasks.push_back(t);
+  }
+  for (int t = 0; t < first_sort; ++t)
+    if (o.tasks[t].type == kTaskPair) {
+      o.tasks[first_sort + o.tasks[t].i].unlock.push_back(t);
+      o.tasks[first_sort + o.tasks[t].j].unlock.push_back(t);
+      o.cells[o.tasks[t].i].sort_flags |= 1 << o.tasks[t].sid;
+      o.cells[o.tasks[t].j].sort_flags |= 1 << o.tasks[t].sid;
+    }
+
+  // One runner, one queue: the first ready task in queue order runs next
+  // (queue_get with a single runner always takes the first task whose wait
+  // count is zero).  The order is fixed from the first step on.
+  std::vector<int> wait(o.tasks.size(), 0);
+  for (auto &t : o.tasks)
+    for (int u : t.unlock) wait[u]++;
+  std::set<int> ready;
+  for (int t = 0; t < (int)o.tasks.size(); ++t)
+    if (wait[t] == 0) ready.insert(t);
+  o.order.clear();
+  while (!ready.empty()) {
+    int t = *ready.begin();
+    ready.erase(ready.begin());
+    o.order.push_back(t);
+    for (int u : o.tasks[t].unlock)
+      if (--wait[u] == 0) ready.insert(u);
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Potential evaluation (potential_eval.hpp:67-161, dpd_eval.hpp:21-169).
+
+inline void eval_plain(const Pot &p, float r2, float *e, float *f) {
+  float r = std::sqrt(r2);
+  int ind = (int)std::fmax(0.0f, p.alpha[0] + r * (p.alpha[1] + r * p.alpha[2]));
+  const float *c = &p.c[(size_t)ind * kChunk];
+  float x = (r - c[0]) * c[1];
+  float ee = c[2] * x + c[3];
+  float eff = c[2];
+  for (int k = 4; k < kChunk; ++k) {
+    eff = eff * x + ee;
+    ee = ee * x + c[k];
+  }
+  *e = ee;
+  *f = eff * c[1] / r;
+}
+
+inline bool eval_ex(const Pot &p, float ri, float rj, float r2, float *e, float *f) {
+  const float eps = std::numeric_limits<float>::epsilon();
+  float r = std::sqrt(r2);
+  float ro = r < eps ? eps : r;
+  r = r < p.a ? p.a : r;
+  if (p.flags & kFlagScaled)
+    r = r / (ri + rj);
+  else if (p.flags & kFlagShifted)
+    r = r - (ri + rj) + p.r0_plusone;
+  unsigned ind = std::max(0.0f, p.alpha[0] + r * (p.alpha[1] + r * p.alpha[2]));
+  if (r > p.b || ind > (unsigned)p.n) return false;
+  const float *c = &p.c[(size_t)ind * kChunk];
+  float x = (r - c[0]) * c[1];
+  float ee = c[2] * x + c[3];
+  float eff = c[2];
+  for (int k = 4; k < kChunk; ++k) {
+    eff = eff * x + ee;
+    ee = ee * x + c[k];
+  }
+  *e = ee;
+  *f = eff * c[1] / ro;
+  return true;
+}
+
+float gaussian_pair(uint32_t a, uint32_t b, uint64_t time, uint32_t klo, uint32_t khi) {
+  mxg::u32x4 ctr = {{a, b, (uint32_t)time, mxg::kStreamDPD}};
+  mxg::u32x4 r = mxg::philox4x32_10(ctr, klo, khi);
+  float u1 = mxg::u01_open_closed(r.v[0]);
+  float u2 = mxg::u01_closed_open(r.v[1]);
+  return std::sqrt(-2.0f * std::log(u1)) * std::cos(6.2831853f * u2);
+}
+
+// dpd_eval: forces on the pair along e = dx / r.  Returns false out of range.
+inline bool dpd_eval(Oracle &o, const Pot &p, Part *pi, Part *pj, const float dx[3], float r2) {
+  float ri = pi->radius, rj = pj->radius;
+  bool shifted = p.flags & kFlagShifted;
+  float cutoff = shifted ? (p.b + ri + rj) : p.b;
+  if (r2 > cutoff * cutoff) return false;
+  float r = std::sqrt(r2);
+  float e[3] = {dx[0] / r, dx[1] / r, dx[2] / r};
+  float v[3] = {pi->v[0] - pj->v[0], pi->v[1] - pj->v[1], pi->v[2] - pj->v[2]};
+  float shifted_r = shifted ? r - ri - rj : r;
+  float omega_c = shifted_r < 0.f ? 1.f : (1 - shifted_r / cutoff);
+  float fc = p.dpd_alpha * omega_c;
+  float omega_d = omega_c * omega_c;
+  float ev = e[0] * v[0] + e[1] * v[1] + e[2] * v[2];
+  float fd = -p.dpd_gamma * omega_d * ev;
+  float fr;
+  if (o.dpd_counter_noise) {
+    uint32_t a = (uint32_t)std::min(pi->id, pj->id), b = (uint32_t)std::max(pi->id, pj->id);
+    fr = p.dpd_sigma * omega_c * gaussian_pair(a, b, o.time, o.seed_lo, o.seed_hi) * o.dpd_delta;
+  } else {
+    fr = p.dpd_sigma * omega_c * o.dpd_delta;
+  }
+  float f = fc + fd + fr;
+  for (int k = 0; k < 3; ++k) {
+    pj->f[k] = pj->f[k] - f * e[k];
+    pi->f[k] = pi->f[k] + f * e[k];
+  }
+  return true;
+}
+
+// potential_eval_super_ex: overlap randomisation, then DPD or table.
+inline void eval_pair(Oracle &o, Cell &cell, int pot, Part *pi, Part *pj, float dx[3], float r2,
+                      float *epot) {
+  const Pot &p = o.pots[pot];
+  if (r2 < p.a * p.a) {
+    dx[0] = cell.gauss(cell.gen);
+    dx[1] = cell.gauss(cell.gen);
+    dx[2] = cell.gauss(cell.gen);
+    float len = std::sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+    dx[0] = dx[0] * p.a / len;
+    dx[1] = dx[1] * p.a / len;
+    dx[2] = dx[2] * p.a / len;
+    r2 = p.a * p.a;
+  }
+  if (p.kind == kKindDPD) {
+    // the reference draws (and discards) one more gaussian here
+    (void)cell.gauss(cell.gen);
+    if (dpd_eval(o, p, pi, pj, dx, r2)) *epot += 0.0f;
+  } else {
+    float e, f;
+    if (eval_ex(p, pi->radius, pj->radius, r2, &e, &f)) {
+      for (int k = 0; k < 3; ++k) {
+        float w = f * dx[k];
+        pi->f[k] -= w;
+        pj->f[k] += w;
+      }
+      *epot += e;
+    }
+  }
+}
+
+inline float fr2(const float *x1, const float *x2, float *dx) {
+  dx[0] = x1[0] - x2[0];
+  dx[1] = x1[1] - x2[1];
+  dx[2] = x1[2] - x2[2];
+  return dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+}
+
+// ---------------------------------------------------------------------------
+// Sorting (runner_dosort.cpp:84-146, runner.cpp:102-165).
+
+void sort_descending(uint32_t *parts, int N) {
+  struct {
+    short lo, hi;
+  } qstack[10];
+  int qpos, i, j, lo, hi, imax;
+  unsigned pivot;
+  uint32_t temp;
+  qstack[0].lo = 0;
+  qstack[0].hi = N - 1;
+  qpos = 0;
+  while (qpos >= 0) {
+    lo = qstack[qpos].lo;
+    hi = qstack[qpos].hi;
+    qpos -= 1;
+    if (hi - lo < 15) {
+      for (i = lo; i < hi; i++) {
+        imax = i;
+        for (j = i + 1; j <= hi; j++)
+          if ((parts[j] & 0xffff) > (parts[imax] & 0xffff)) imax = j;
+        if (imax != i) {
+          temp = parts[imax];
+          parts[imax] = parts[i];
+          parts[i] = temp;
+        }
+      }
+    } else {
+      pivot = parts[(lo + hi) / 2] & 0xffff;
+      i = lo;
+      j = hi;
+      while (i <= j) {
+        while ((parts[i] & 0xffff) > pivot) i++;
+        while ((parts[j] & 0xffff) < pivot) j--;
+        if (i <= j) {
+          if (i < j) {
+            temp = parts[i];
+            parts[i] = parts[j];
+            parts[j] = temp;
+          }
+          i += 1;
+          j -= 1;
+        }
+      }
+      if (j > (lo + hi) / 2) {
+        if (lo < j) {
+          qpos += 1;
+          qstack[qpos].lo = lo;
+          qstack[qpos].hi = j;
+        }
+        if (i < hi) {
+          qpos += 1;
+          qstack[qpos].lo = i;
+          qstack[qpos].hi = hi;
+        }
+      } else {
+        if (i < hi) {
+          qpos += 1;
+          qstack[qpos].lo = i;
+          qstack[qpos].hi = hi;
+        }
+        if (lo < j) {
+          qpos += 1;
+          qstack[qpos].lo = lo;
+          qstack[qpos].hi = j;
+        }
+      }
+    }
+  }
+}
+
+struct SortScale {
+  float bias, dscale;
+};
+SortScale sort_scale(const Oracle &o) {
+  SortScale s;
+  s.bias = std::sqrt(o.h[0] * o.h[0] + o.h[1] * o.h[1] + o.h[2] * o.h[2]);
+  s.dscale = (float)SHRT_MAX / (2 * s.bias);
+  return s;
+}
+
+void do_sort(Oracle &o, Cell &c) {
+  int count = (int)c.parts.size();
+  c.sortlist.resize((size_t)13 * std::max(count, 1));
+  if (count == 0) return;
+  SortScale s = sort_scale(o);
+  const SortDirs &D = sortdirs();
+  for (int sid = 0; sid < 13; ++sid) {
+    if (!(c.sort_flags & (1 << sid))) continue;
+    const float *n = &D.n[3 * sid];
+    uint32_t *ip = &c.sortlist[(size_t)count * sid];
+    for (int i = 0; i < count; ++i) {
+      const Part &p = c.parts[i];
+      ip[i] = ((uint32_t)i << 16) |
+              (unsigned int)(s.dscale * (s.bias + p.x[0] * n[0] + p.x[1] * n[1] + p.x[2] * n[2]));
+    }
+    sort_descending(ip, count);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pair and self loops (runner_dopair.cpp:91-340, 446-682).
+
+void do_pair(Oracle &o, int ci_, int cj_) {
+  int ci = ci_, cj = cj_;
+  Cell *cell_i = &o.cells[ci], *cell_j = &o.cells[cj];
+  if (cell_i->parts.empty() || cell_j->parts.empty()) return;
+  float cutoff = o.cutoff;
+  float cutoff2 = cutoff * cutoff;
+  SortScale s = sort_scale(o);
+  float maxdx = 0.0f;
+  int dmaxdist = 2 + s.dscale * (cutoff + 2 * maxdx);
+  float shift[3];
+  int sid = getsid(o, ci, cj, shift);
+  cell_i = &o.cells[ci];
+  cell_j = &o.cells[cj];
+  int count_i = (int)cell_i->parts.size(), count_j = (int)cell_j->parts.size();
+  float nshift = std::sqrt(shift[0] * shift[0] + shift[1] * shift[1] + shift[2] * shift[2]);
+  int dnshift = s.dscale * nshift;
+  const uint32_t *iparts = &cell_i->sortlist[(size_t)count_i * sid];
+  const uint32_t *jparts = &cell_j->sortlist[(size_t)count_j * sid];
+  float epot = 0.0f;
+  float pix[3], dx[3];
+  for (int i = 0; i < count_i; i++) {
+    if ((jparts[count_j - 1] & 0xffff) + dnshift - (iparts[i] & 0xffff) > (unsigned)dmaxdist) break;
+    Part *part_i = &cell_i->parts[iparts[i] >> 16];
+    pix[0] = part_i->x[0] - shift[0];
+    pix[1] = part_i->x[1] - shift[1];
+    pix[2] = part_i->x[2] - shift[2];
+    for (int j = count_j - 1;
+         j >= 0 && (jparts[j] & 0xffff) + dnshift - (iparts[i] & 0xffff) < (unsigned)dmaxdist; j--) {
+      Part *part_j = &cell_j->parts[jparts[j] >> 16];
+      float r2 = fr2(pix, part_j->x, dx);
+      if (r2 > cutoff2) continue;
+      part_i->ncount++;
+      part_j->ncount++;
+      int pot = o.potmat[part_i->type * o.types.size() + part_j->type];
+      if (pot < 0) continue;
+      eval_pair(o, *cell_i, pot, part_i, part_j, dx, r2, &epot);
+    }
+  }
+  cell_i->epot += epot;
+}
+
+void single_body(Oracle &o, Part *p) {
+  const Type &t = o.types[p->type];
+  if (!t.has_random) return;
+  if ((o.time + (uint64_t)p->id) % t.rnd_steps == 0) {
+    // MxRandomVector(mean, std) with Philox draws.
+    mxg::u32x4 ctr = {{(uint32_t)p->id, (uint32_t)o.time, mxg::kStreamSingle, 0u}};
+    mxg::u32x4 r = mxg::philox4x32_10(ctr, o.seed_lo, o.seed_hi);
+    float u1 = mxg::u01_open_closed(r.v[0]);
+    float u2 = mxg::u01_closed_open(r.v[1]);
+    float ut = mxg::u01_closed_open(r.v[2]);
+    float up = mxg::u01_closed_open(r.v[3]);
+    float g = std::sqrt(-2.0f * std::log(u1)) * std::cos(6.2831853f * u2);
+    float mag = t.rnd_mean + t.rnd_std * g;
+    float theta = 6.2831853f * ut;
+    float phi = std::acos(1.0f - 2.0f * up);
+    p->pf[0] = mag * std::sin(phi) * std::cos(theta);
+    p->pf[1] = mag * std::sin(phi) * std::sin(theta);
+    p->pf[2] = mag * std::cos(phi);
+  }
+  p->f[0] += p->pf[0];
+  p->f[1] += p->pf[1];
+  p->f[2] += p->pf[2];
+}
+
+void do_self(Oracle &o, int cid) {
+  Cell &c = o.cells[cid];
+  int count = (int)c.parts.size();
+  if (count == 0) return;
+  float cutoff2 = o.cutoff2;
+  float epot = 0.0f;
+  float pix[3], dx[3];
+  for (int i = 0; i < count; i++) {
+    Part *part_i = &c.parts[i];
+    pix[0] = part_i->x[0];
+    pix[1] = part_i->x[1];
+    pix[2] = part_i->x[2];
+    single_body(o, part_i);
+    for (int j = i + 1; j < count; j++) {
+      Part *part_j = &c.parts[j];
+      float r2 = fr2(pix, part_j->x, dx);
+      if (r2 > cutoff2) continue;
+      part_i->ncount++;
+      part_j->ncount++;
+      int pot = o.potmat[part_i->type * o.types.size() + part_j->type];
+      if (pot < 0) continue;
+      eval_pair(o, c, pot, part_i, part_j, dx, r2, &epot);
+    }
+  }
+  c.epot += epot;
+}
+
+void run_task(Oracle &o, const Task &t) {
+  if (t.type == kTaskSort)
+    do_sort(o, o.cells[t.i]);
+  else if (t.type == kTaskSelf)
+    do_self(o, t.i);
+  else
+    do_pair(o, t.i, t.j);
+}
+
+// Threaded runners: sorts in parallel, then self/pair tasks with CAS cell
+// locks (the cells_taboo scheme of queue.cpp:128-182).
+void run_tasks_threaded(Oracle &o) {
+  int T = o.nthreads;
+  std::vector<int> sorts, work;
+  for (int t = 0; t < (int)o.tasks.size(); ++t)
+    (o.tasks[t].type == kTaskSort ? sorts : work).push_back(t);
+  {
+    std::atomic<int> next{0};
+    auto fn = [&]() {
+      for (int k; (k = next.fetch_add(1)) < (int)sorts.size();) run_task(o, o.tasks[sorts[k]]);
+    };
+    std::vector<std::thread> th;
+    for (int i = 0; i < T; ++i) th.emplace_back(fn);
+    for (auto &x : th) x.join();
+  }
+  std::vector<std::atomic<char>> done(work.size());
+  for (auto &d : done) d = 0;
+  std::atomic<int> remaining{(int)work.size()};
+  auto fn = [&](int rid) {
+    size_t start = work.size() * rid / T;
+    while (remaining.load() > 0) {
+      bool any = false;
+      for (size_t q = 0; q < work.size(); ++q) {
+        size_t k = (start + q) % work.size();
+        if (done[k].load(std::memory_order_relaxed)) continue;
+        const Task &t = o.tasks[work[k]];
+        char z = 0;
+        if (!o.cells[t.i].taboo.compare_exchange_strong(z, 1)) continue;
+        if (t.type == kTaskPair) {
+          z = 0;
+          if (!o.cells[t.j].taboo.compare_exchange_strong(z, 1)) {
+            o.cells[t.i].taboo = 0;
+            continue;
+          }
+        }
+        char d0 = 0;
+        if (done[k].compare_exchange_strong(d0, 1)) {
+          run_task(o, t);
+          remaining.fetch_sub(1);
+          any = true;
+        }
+        o.cells[t.i].taboo = 0;
+        if (t.type == kTaskPair) o.cells[t.j].taboo = 0;
+      }
+      if (!any) std::this_thread::yield();
+    }
+  };
+  std::vector<std::thread> th;
+  for (int i = 0; i < T; ++i) th.emplace_back(fn, i);
+  for (auto &x : th) x.join();
+}
+
+// ---------------------------------------------------------------------------
+// Bonds (bond.cpp:99-295).
+
+struct Loc {
+  int cell, idx;
+};
+
+void bond_eval(Oracle &o, const std::vector<Loc> &where) {
+  double epot = 0.0;
+  float pix[3], dx[3];
+  for (auto &b : o.bonds) {
+    if (!b.active) continue;
+    const Loc &li = where[b.i], &lj = where[b.j];
+    if (li.cell < 0 || lj.cell < 0) continue;
+    Part *pi = &o.cells[li.cell].parts[li.idx];
+    Part *pj = &o.cells[lj.cell].parts[lj.idx];
+    const Pot &pot = o.bpots[b.pot];
+    const int *loci = o.cells[li.cell].loc, *locj = o.cells[lj.cell].loc;
+    for (int k = 0; k < 3; ++k) {
+      int shift = loci[k] - locj[k];
+      if (shift > 1)
+        shift = -1;
+      else if (shift < -1)
+        shift = 1;
+      pix[k] = pi->x[k] + o.h[k] * shift;
+    }
+    float r2 = fr2(pix, pj->x, dx);
+    if (r2 < pot.a * pot.a || r2 > pot.b * pot.b)
+      r2 = std::fmax(pot.a * pot.a, std::fmin(pot.b * pot.b, r2));
+    float ee, eff;
+    eval_plain(pot, r2, &ee, &eff);
+    if (eff >= b.dissociation) {
+      b.active = false;
+    } else {
+      for (int k = 0; k < 3; ++k) {
+        float w = eff * dx[k];
+        pi->f[k] -= w;
+        pj->f[k] += w;
+      }
+      epot += ee;
+    }
+  }
+  o.epot_bond = epot;
+}
+
+// ---------------------------------------------------------------------------
+// Advance (engine_advance.cpp:126-239, 248-422).
+
+struct AdvanceConst {
+  float dt, h[3], h2[3], maxv[3], maxv2[3], maxx[3], maxx2[3];
+};
+
+void advance_cell(Oracle &o, const AdvanceConst &K, int cid) {
+  Cell &c = o.cells[cid];
+  size_t pid = 0;
+  while (pid < c.parts.size()) {
+    Part *p = &c.parts[pid];
+    bool toofast = false;
+    int delta[3];
+    if ((p->flags & kFrozenX) && (p->flags & kFrozenY) && (p->flags & kFrozenZ)) {
+      pid++;
+      continue;
+    }
+    float mask[3] = {(p->flags & kFrozenX) ? 0.0f : 1.0f, (p->flags & kFrozenY) ? 0.0f : 1.0f,
+                     (p->flags & kFrozenZ) ? 0.0f : 1.0f};
+    if (o.types[p->type].dynamics == kNewtonian) {
+      for (int k = 0; k < 3; k++) {
+        float v = mask[k] * (p->v[k] + K.dt * p->f[k] * p->imass);
+        float neg = v / std::abs(v);
+        p->v[k] = v * v <= K.maxv2[k] ? v : neg * K.maxv[k];
+        p->x[k] += K.dt * p->v[k];
+        delta[k] = std::isgreaterequal(p->x[k], K.h[k]) - std::isless(p->x[k], 0.0);
+        toofast = toofast || (p->x[k] >= K.h2[k] || p->x[k] <= -K.h[k]);
+      }
+    } else {
+      for (int k = 0; k < 3; k++) {
+        float dx = mask[k] * (K.dt * p->f[k] * p->imass);
+        float neg = dx / std::abs(dx);
+        p->x[k] += dx * dx <= K.maxx2[k] ? dx : neg * K.maxx[k];
+        delta[k] = std::isgreaterequal(p->x[k], K.h[k]) - std::isless(p->x[k], 0.0);
+        toofast = toofast || (p->x[k] >= K.h2[k] || p->x[k] <= -K.h[k]);
+      }
+    }
+    if (toofast && o.err == 0) {
+      o.err = kErrToofast;
+      o.err_pid = p->id;
+    }
+    if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
+      for (int k = 0; k < 3; k++) p->x[k] -= delta[k] * K.h[k];
+      int dest = o.cellid((c.loc[0] + delta[0] + o.cdim[0]) % o.cdim[0],
+                          (c.loc[1] + delta[1] + o.cdim[1]) % o.cdim[1],
+                          (c.loc[2] + delta[2] + o.cdim[2]) % o.cdim[2]);
+      {
+        std::lock_guard<std::mutex> g(o.cells[dest].mtx);
+        o.cells[dest].incoming.push_back(*p);
+      }
+      size_t last = c.parts.size() - 1;
+      if (pid < last) c.parts[pid] = c.parts[last];
+      c.parts.pop_back();
+    } else {
+      pid += 1;
+    }
+  }
+}
+
+void welcome_all(Oracle &o) {
+  for (auto &c : o.cells) {
+    for (auto &p : c.incoming) c.parts.push_back(p);
+    c.incoming.clear();
+  }
+}
+
+std::vector<Loc> locate(const Oracle &o) {
+  std::vector<Loc> where(o.nparts, Loc{-1, -1});
+  for (int c = 0; c < o.ncells; ++c)
+    for (int k = 0; k < (int)o.cells[c].parts.size(); ++k) where[o.cells[c].parts[k].id] = Loc{c, k};
+  return where;
+}
+
+// engine_force without the advance: kinetic energy, prepare, shuffle,
+// non-bonded tasks, bonds.
+void force(Oracle &o) {
+  if (!o.dpd_delta_set) {
+    o.dpd_delta = 1.f / std::sqrt(o.dt_d);
+    o.dpd_delta_set = true;
+  }
+  double total = 0;
+  for (auto &c : o.cells)
+    for (auto &p : c.parts) {
+      float e = 0.5 * o.types[p.type].mass * (p.v[0] * p.v[0] + p.v[1] * p.v[1] + p.v[2] * p.v[2]);
+      total += e;
+    }
+  o.ekin = total;
+  for (auto &c : o.cells) {
+    c.epot = 0.0;
+    for (auto &p : c.parts) {
+      p.f[0] = p.f[1] = p.f[2] = 0.0f;
+      p.f[3] = 0.0f;
+      p.ncount = 0;
+    }
+  }
+  // space_shuffle_local: particles outside [0, h) (only after a load).
+  float h[3] = {(float)o.h[0], (float)o.h[1], (float)o.h[2]};
+  for (int cid = 0; cid < o.ncells; ++cid) {
+    Cell &c = o.cells[cid];
+    size_t pid = 0;
+    while (pid < c.parts.size()) {
+      Part &p = c.parts[pid];
+      int delta[3];
+      for (int k = 0; k < 3; k++)
+        delta[k] = std::isgreaterequal(p.x[k], h[k]) - std::isless(p.x[k], 0.0);
+      if (delta[0] || delta[1] || delta[2]) {
+        for (int k = 0; k < 3; k++) p.x[k] -= delta[k] * h[k];
+        int dest = o.cellid((c.loc[0] + delta[0] + o.cdim[0]) % o.cdim[0],
+                            (c.loc[1] + delta[1] + o.cdim[1]) % o.cdim[1],
+                            (c.loc[2] + delta[2] + o.cdim[2]) % o.cdim[2]);
+        o.cells[dest].incoming.push_back(p);
+        size_t last = c.parts.size() - 1;
+        if (pid < last) c.parts[pid] = c.parts[last];
+        c.parts.pop_back();
+      } else {
+        pid++;
+      }
+    }
+  }
+  welcome_all(o);
+
+  if (o.nthreads <= 1) {
+    for (int t : o.order) run_task(o, o.tasks[t]);
+  } else {
+    run_tasks_threaded(o);
+  }
+  o.epot_bond = 0;
+  if (!o.bonds.empty()) bond_eval(o, locate(o));
+}
+
+AdvanceConst advance_const(const Oracle &o) {
+  AdvanceConst K;
+  K.dt = o.dt;
+  for (int k = 0; k < 3; ++k) {
+    K.h[k] = o.h[k];
+    K.h2[k] = 2. * o.h[k];
+    K.maxv[k] = (K.h[k] * o.max_frac) / K.dt;
+    K.maxv2[k] = K.maxv[k] * K.maxv[k];
+    K.maxx[k] = K.h[k] * o.max_frac;
+    K.maxx2[k] = K.maxx[k] * K.maxx[k];
+  }
+  return K;
+}
+
+void advance(Oracle &o);
+
+int step(Oracle &o) {
+  if (o.err != 0) return o.err;
+  o.time += 1;
+  force(o);
+  double epot = 0;
+  for (auto &c : o.cells) epot += c.epot;
+  o.epot_nb = epot;
+  advance(o);
+  return o.err;
+}
+
+void advance(Oracle &o) {
+  AdvanceConst K = advance_const(o);
+  if (o.nthreads <= 1) {
+    for (int cid = 0; cid < o.ncells; ++cid) advance_cell(o, K, cid);
+  } else {
+    std::atomic<int> next{0};
+    auto fn = [&]() {
+      for (int c; (c = next.fetch_add(1)) < o.ncells;) advance_cell(o, K, c);
+    };
+    std::vector<std::thread> th;
+    for (int i = 0; i < o.nthreads; ++i) th.emplace_back(fn);
+    for (auto &x : th) x.join();
+  }
+  welcome_all(o);
+}
+
+// FP64 evaluation of the forces on the current configuration: the same pair
+// set as the FP32 loops (in-cutoff in FP32, reference orientation) and the
+// same tables, but every subsequent operation in double.
+void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_out) {
+  std::fill(F, F + 3 * (size_t)o.nparts, 0.0);
+  double epot = 0;
+  auto table64 = [](const Pot &p, double ri, double rj, double r2, double *e, double *f) {
+    double r = std::sqrt(r2);
+    double ro = r < FLT_EPSILON ? FLT_EPSILON : r;
+    r = r < p.a ? p.a : r;
+    if (p.flags & kFlagScaled)
+      r = r / (ri + rj);
+    else if (p.flags & kFlagShifted)
+      r = r - (ri + rj) + p.r0_plusone;
+    double indf = std::max(0.0, p.alpha[0] + r * (p.alpha[1] + r * (double)p.alpha[2]));
+    unsigned ind = (unsigned)indf;
+    if (r > p.b || ind > (unsigned)p.n) return false;
+    const float *c = &p.c[(size_t)ind * kChunk];
+    double x = (r - c[0]) * (double)c[1];
+    double ee = c[2] * x + c[3], eff = c[2];
+    for (int k = 4; k < kChunk; ++k) {
+      eff = eff * x + ee;
+      ee = ee * x + c[k];
+    }
+    *e = ee;
+    *f = eff * c[1] / ro;
+    return true;
+  };
+  auto pair64 = [&](const Part &pi, const Part &pj, const double dx[3], double r2) {
+    int pot = o.potmat[pi.type * o.types.size() + pj.type];
+    if (pot < 0) return;
+    const Pot &p = o.pots[pot];
+    if (r2 < (double)p.a * p.a) return;  // overlap path is random; skipped
+    if (p.kind == kKindDPD) {
+      double ri = pi.radius, rj = pj.radius;
+      bool shifted = p.flags & kFlagShifted;
+      double cut = shifted ? (p.b + ri + rj) : p.b;
+      if (r2 > cut * cut) return;
+      double r = std::sqrt(r2);
+      double e[3] = {dx[0] / r, dx[1] / r, dx[2] / r};
+      double v[3] = {(double)pi.v[0] - pj.v[0], (double)pi.v[1] - pj.v[1], (double)pi.v[2] - pj.v[2]};
+      double sr = shifted ? r - ri - rj : r;
+      double wc = sr < 0 ? 1.0 : 1 - sr / cut;
+      double f = p.dpd_alpha * wc - p.dpd_gamma * wc * wc * (e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
+      if (!o.dpd_counter_noise) f += p.dpd_sigma * wc * (double)o.dpd_delta;
+      for (int k = 0; k < 3; ++k) {
+        F[3 * (size_t)pi.id + k] += f * e[k];
+        F[3 * (size_t)pj.id + k] -= f * e[k];
+      }
+    } else {
+      double e, f;
+      if (!table64(p, pi.radius, pj.radius, r2, &e, &f)) return;
+      for (int k = 0; k < 3; ++k) {
+        F[3 * (size_t)pi.id + k] -= f * dx[k];
+        F[3 * (size_t)pj.id + k] += f * dx[k];
+      }
+      epot += e;
+    }
+  };
+  for (const Task &t : o.tasks) {
+    if (t.type == kTaskSort) continue;
+    if (t.type == kTaskSelf) {
+      const Cell &c = o.cells[t.i];
+      for (size_t i = 0; i < c.parts.size(); ++i)
+        for (size_t j = i + 1; j < c.parts.size(); ++j) {
+          float dxf[3];
+          float r2f = fr2(c.parts[i].x, c.parts[j].x, dxf);
+          if (r2f > (float)o.cutoff2) continue;
+          double dx[3];
+          for (int k = 0; k < 3; ++k) dx[k] = (double)c.parts[i].x[k] - c.parts[j].x[k];
+          pair64(c.parts[i], c.parts[j], dx, dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+        }
+    } else {
+      int ci = t.i, cj = t.j;
+      float shift[3];
+      getsid(o, ci, cj, shift);
+      double shift64[3];
+      for (int k = 0; k < 3; ++k) {
+        shift64[k] = o.cells[cj].origin[k] - o.cells[ci].origin[k];
+        if (shift64[k] * 2 > o.dim[k])
+          shift64[k] -= o.dim[k];
+        else if (shift64[k] * 2 < -o.dim[k])
+          shift64[k] += o.dim[k];
+      }
+      float cutoff = o.cutoff;
+      float cutoff2 = cutoff * cutoff;
+      const Cell &A = o.cells[ci], &B = o.cells[cj];
+      for (const Part &pi : A.parts) {
+        float pix[3] = {pi.x[0] - shift[0], pi.x[1] - shift[1], pi.x[2] - shift[2]};
+        for (const Part &pj : B.parts) {
+          float dxf[3];
+          float r2f = fr2(pix, pj.x, dxf);
+          if (r2f > cutoff2) continue;
+          double dx[3];
+          for (int k = 0; k < 3; ++k) dx[k] = ((double)pi.x[k] - shift64[k]) - pj.x[k];
+          pair64(pi, pj, dx, dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+        }
+      }
+    }
+  }
+  // single-body persistent forces as stored
+  for (const Cell &c : o.cells)
+    for (const Part &p : c.parts)
+      if (o.types[p.type].has_random)
+        for (int k = 0; k < 3; ++k) F[3 * (size_t)p.id + k] += p.pf[k];
+  // bonds
+  std::vector<Loc> where = locate(o);
+  for (const Bond &b : o.bonds) {
+    if (!b.active) continue;
+    const Part &pi = o.cells[where[b.i].cell].parts[where[b.i].idx];
+    const Part &pj = o.cells[where[b.j].cell].parts[where[b.j].idx];
+    const int *loci = o.cells[where[b.i].cell].loc, *locj = o.cells[where[b.j].cell].loc;
+    double dx[3];
+    for (int k = 0; k < 3; ++k) {
+      int s = loci[k] - locj[k];
+      if (s > 1)
+        s = -1;
+      else if (s < -1)
+        s = 1;
+      dx[k] = ((double)pi.x[k] + o.h[k] * s) - pj.x[k];
+    }
+    const Pot &pot = o.bpots[b.pot];
+    double r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+    double lo = (double)pot.a * pot.a, hi = (double)pot.b * pot.b;
+    r2 = std::fmax(lo, std::fmin(hi, r2));
+    double r = std::sqrt(r2);
+    int ind = (int)std::fmax(0.0, pot.alpha[0] + r * (pot.alpha[1] + r * (double)pot.alpha[2]));
+    const float *c = &pot.c[(size_t)ind * kChunk];
+    double x = (r - c[0]) * (double)c[1];
+    double ee = c[2] * x + c[3], eff = c[2];
+    for (int k = 4; k < kChunk; ++k) {
+      eff = eff * x + ee;
+      ee = ee * x + c[k];
+    }
+    eff = eff * c[1] / r;
+    for (int k = 0; k < 3; ++k) {
+      F[3 * (size_t)pi.id + k] -= eff * dx[k];
+      F[3 * (size_t)pj.id + k] += eff * dx[k];
+    }
+    epot += ee;
+  }
+  if (epot_out) *epot_out = epot;
+}
+
+Pot make_pot(uint32_t kind, uint32_t flags, const float alpha[4], float a, float b, float r0_plusone,
+             int n, const float *c, float dpd_alpha, float dpd_gamma, float dpd_sigma) {
+  Pot p;
+  p.kind = kind;
+  p.flags = flags;
+  for (int k = 0; k < 4; ++k) p.alpha[k] = alpha ? alpha[k] : 0.0f;
+  p.a = a;
+  p.b = b;
+  p.r0_plusone = r0_plusone;
+  p.n = n;
+  if (kind == kKindTable && c) p.c.assign(c, c + (size_t)(n + 1) * kChunk);
+  p.dpd_alpha = dpd_alpha;
+  p.dpd_gamma = dpd_gamma;
+  p.dpd_sigma = dpd_sigma;
+  return p;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C interface for the tests and the CPU-baseline leg of bench.py.
+
+extern "C" {
+
+void *oracle_create(const double origin[3], const double dim[3], const int cells[3], double cutoff,
+                    double dt, int nthreads) {
+  Oracle *o = new Oracle();
+  if (init_space(*o, origin, dim, cells, cutoff) != 0) {
+    delete o;
+    return nullptr;
+  }
+  o->dt = (float)dt;
+  o->dt_d = dt;
+  o->nthreads = nthreads < 1 ? 1 : nthreads;
+  return o;
+}
+
+void oracle_free(void *h) { delete (Oracle *)h; }
+
+int oracle_geometry(void *h, int cdim[3], double hh[3], int *ntasks) {
+  Oracle *o = (Oracle *)h;
+  for (int k = 0; k < 3; ++k) {
+    cdim[k] = o->cdim[k];
+    hh[k] = o->h[k];
+  }
+  *ntasks = (int)o->tasks.size();
+  return 0;
+}
+
+void oracle_set_threads(void *h, int n) { ((Oracle *)h)->nthreads = n < 1 ? 1 : n; }
+void oracle_set_seed(void *h, uint64_t seed) {
+  ((Oracle *)h)->seed_lo = (uint32_t)seed;
+  ((Oracle *)h)->seed_hi = (uint32_t)(seed >> 32);
+}
+void oracle_set_dpd_counter_noise(void *h, int on) { ((Oracle *)h)->dpd_counter_noise = on; }
+void oracle_set_time(void *h, uint64_t t) { ((Oracle *)h)->time = t; }
+
+int oracle_add_type(void *h, double mass, float radius, int dynamics) {
+  Oracle *o = (Oracle *)h;
+  Type t;
+  t.mass = mass;
+  t.imass = 1.0 / mass;
+  t.radius = radius;
+  t.dynamics = dynamics;
+  o->types.push_back(t);
+  int nt = (int)o->types.size();
+  std::vector<int> m(nt * nt, -1);
+  for (int i = 0; i < nt - 1; ++i)
+    for (int j = 0; j < nt - 1; ++j) m[i * nt + j] = o->potmat[i * (nt - 1) + j];
+  o->potmat.swap(m);
+  return nt - 1;
+}
+
+int oracle_set_random_force(void *h, int type, float mean, float std, float duration) {
+  Oracle *o = (Oracle *)h;
+  Type &t = o->types[type];
+  t.has_random = true;
+  t.rnd_mean = mean;
+  t.rnd_std = std;
+  t.rnd_steps = (unsigned)std::ceil(duration / o->dt_d);
+  if (t.rnd_steps == 0) t.rnd_steps = 1;
+  return 0;
+}
+
+int oracle_add_pot(void *h, int ti, int tj, uint32_t kind, uint32_t flags, const float alpha[4], float a,
+                   float b, float r0_plusone, int n, const float *c, float dpd_alpha, float dpd_gamma,
+                   float dpd_sigma) {
+  Oracle *o = (Oracle *)h;
+  o->pots.push_back(make_pot(kind, flags, alpha, a, b, r0_plusone, n, c, dpd_alpha, dpd_gamma, dpd_sigma));
+  int nt = (int)o->types.size();
+  o->potmat[ti * nt + tj] = (int)o->pots.size() - 1;
+  o->potmat[tj * nt + ti] = (int)o->pots.size() - 1;
+  return 0;
+}
+
+int oracle_add_bond_pot(void *h, const float alpha[4], float a, float b, int n, const float *c) {
+  Oracle *o = (Oracle *)h;
+  o->bpots.push_back(make_pot(kKindTable, 0, alpha, a, b, 0.0f, n, c, 0, 0, 0));
+  return (int)o->bpots.size() - 1;
+}
+
+// Adds particles as space_addpart does: cell index (x - origin) * ih in
+// double, local coordinate x - cell origin.  Particle ids are 0..N-1 in
+// argument order.
+int oracle_load(void *h, int N, const double *x, const double *v, const int *type, const uint16_t *flags) {
+  Oracle *o = (Oracle *)h;
+  for (auto &c : o->cells) c.parts.clear();
+  o->nparts = N;
+  for (int i = 0; i < N; ++i) {
+    Part p;
+    std::memset(&p, 0, sizeof(p));
+    const Type &t = o->types[type[i]];
+    p.radius = t.radius;
+    p.mass = t.mass;
+    p.imass = t.imass;
+    p.id = i;
+    p.type = (int16_t)type[i];
+    p.flags = flags ? flags[i] : 0;
+    for (int k = 0; k < 3; ++k) p.v[k] = (float)v[3 * (size_t)i + k];
+    int ind[3];
+    for (int k = 0; k < 3; ++k) ind[k] = (x[3 * (size_t)i + k] - o->origin[k]) * o->ih[k];
+    for (int k = 0; k < 3; ++k)
+      if (ind[k] < 0 || ind[k] >= o->cdim[k]) return kErrRange;
+    Cell &c = o->cells[o->cellid(ind[0], ind[1], ind[2])];
+    for (int k = 0; k < 3; ++k) p.x[k] = x[3 * (size_t)i + k] - c.origin[k];
+    c.parts.push_back(p);
+  }
+  return 0;
+}
+
+int oracle_set_bonds(void *h, int M, const int *bi, const int *bj, const int *pot, const double *dissoc) {
+  Oracle *o = (Oracle *)h;
+  o->bonds.clear();
+  for (int b = 0; b < M; ++b) o->bonds.push_back(Bond{bi[b], bj[b], pot[b], dissoc[b], true});
+  return 0;
+}
+
+int oracle_step(void *h, int nsteps) {
+  Oracle *o = (Oracle *)h;
+  int rc = 0;
+  for (int s = 0; s < nsteps; ++s) {
+    rc = step(*o);
+    if (rc != 0) break;
+  }
+  return rc;
+}
+
+// Force evaluation only (engine_force without the advance), so forces can be
+// compared on a fixed configuration.
+int oracle_force(void *h) {
+  Oracle *o = (Oracle *)h;
+  o->time += 1;
+  force(*o);
+  double epot = 0;
+  for (auto &c : o->cells) epot += c.epot;
+  o->epot_nb = epot;
+  return 0;
+}
+
+// One step whose forces are replaced by the given ones (3 floats per id)
+// after the force evaluation: checks the advance and the migration rule on
+// identical inputs.
+int oracle_step_with_forces(void *h, const float *f) {
+  Oracle *o = (Oracle *)h;
+  if (o->err != 0) return o->err;
+  o->time += 1;
+  force(*o);
+  double epot = 0;
+  for (auto &c : o->cells) epot += c.epot;
+  o->epot_nb = epot;
+  for (auto &c : o->cells)
+    for (auto &p : c.parts)
+      for (int k = 0; k < 3; ++k) p.f[k] = f[3 * (size_t)p.id + k];
+  advance(*o);
+  return o->err;
+}
+
+// Per-particle state by id: local position, cell id, velocity, force,
+// in-cutoff pair count of the last force evaluation, persistent force.
+int oracle_get_state(void *h, float *xl, int *cell, float *v, float *f, int *ncount, float *pf) {
+  Oracle *o = (Oracle *)h;
+  for (int c = 0; c < o->ncells; ++c)
+    for (const Part &p : o->cells[c].parts) {
+      size_t i = p.id;
+      for (int k = 0; k < 3; ++k) {
+        if (xl) xl[3 * i + k] = p.x[k];
+        if (v) v[3 * i + k] = p.v[k];
+        if (f) f[3 * i + k] = p.f[k];
+        if (pf) pf[3 * i + k] = p.pf[k];
+      }
+      if (cell) cell[i] = c;
+      if (ncount) ncount[i] = p.ncount;
+    }
+  return 0;
+}
+
+// Per-cell particle ids in storage order (count per cell, then ids).
+int oracle_cell_counts(void *h, int *counts) {
+  Oracle *o = (Oracle *)h;
+  for (int c = 0; c < o->ncells; ++c) counts[c] = (int)o->cells[c].parts.size();
+  return 0;
+}
+
+int oracle_global_positions(void *h, double *x) {
+  Oracle *o = (Oracle *)h;
+  for (const Cell &c : o->cells)
+    for (const Part &p : c.parts)
+      for (int k = 0; k < 3; ++k) x[3 * (size_t)p.id + k] = c.origin[k] + p.x[k];
+  return 0;
+}
+
+int oracle_observables(void *h, double *epot_nb, double *epot_bond, double *ekin) {
+  Oracle *o = (Oracle *)h;
+  if (epot_nb) *epot_nb = o->epot_nb;
+  if (epot_bond) *epot_bond = o->epot_bond;
+  if (ekin) *ekin = o->ekin;
+  return 0;
+}
+
+int oracle_bond_active(void *h, uint8_t *active) {
+  Oracle *o = (Oracle *)h;
+  for (size_t b = 0; b < o->bonds.size(); ++b) active[b] = o->bonds[b].active ? 1 : 0;
+  return 0;
+}
+
+int oracle_forces64(void *h, double *F, double *epot) {
+  forces64(*(Oracle *)h, F, epot);
+  return 0;
+}
+
+int oracle_error(void *h, int *pid) {
+  Oracle *o = (Oracle *)h;
+  if (pid) *pid = o->err_pid;
+  return o->err;
+}
+
+}  // extern "C"

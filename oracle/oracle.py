@@ -1,0 +1,221 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper of liboracle.so, the CPU restatement of mdcore's step
+(mdcore_oracle.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it, as the checker or as the timed CPU baseline.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+_fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+_ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+
+_SIG = {
+    "oracle_create": (C.c_void_p, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_double,
+                                   C.c_double, C.c_int]),
+    "oracle_free": (None, [C.c_void_p]),
+    "oracle_geometry": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "oracle_set_threads": (None, [C.c_void_p, C.c_int]),
+    "oracle_set_seed": (None, [C.c_void_p, C.c_uint64]),
+    "oracle_set_dpd_counter_noise": (None, [C.c_void_p, C.c_int]),
+    "oracle_set_time": (None, [C.c_void_p, C.c_uint64]),
+    "oracle_add_type": (C.c_int, [C.c_void_p, C.c_double, C.c_float, C.c_int]),
+    "oracle_set_random_force": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "oracle_add_pot": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_float),
+                                 C.c_float, C.c_float, C.c_float, C.c_int, C.POINTER(C.c_float), C.c_float,
+                                 C.c_float, C.c_float]),
+    "oracle_add_bond_pot": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int,
+                                      C.POINTER(C.c_float)]),
+    "oracle_load": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                              C.POINTER(C.c_int), C.POINTER(C.c_uint16)]),
+    "oracle_set_bonds": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "oracle_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "oracle_force": (C.c_int, [C.c_void_p]),
+    "oracle_step_with_forces": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "oracle_get_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    "oracle_cell_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "oracle_global_positions": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "oracle_observables": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
+    "oracle_bond_active": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8)]),
+    "oracle_forces64": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "oracle_error": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+}
+
+_lib = None
+
+
+def build():
+    """Compiles liboracle.so with the recipe in oracle/Makefile."""
+    r = subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        for k, (res, args) in _SIG.items():
+            f = getattr(L, k)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _pot_tables(pot):
+    inf = pot.info
+    alpha = np.array(inf.alpha[:], np.float32)
+    c = pot.coefficients
+    return inf, alpha, c
+
+
+class Oracle:
+    def __init__(self, dim, cells, cutoff, dt, threads=1, origin=(0.0, 0.0, 0.0)):
+        L = lib()
+        o = np.ascontiguousarray(origin, np.float64)
+        d = np.ascontiguousarray(dim, np.float64)
+        c = np.ascontiguousarray(cells, np.int32)
+        self._h = L.oracle_create(_dp(o), _dp(d), _ip(c), float(cutoff), float(dt), int(threads))
+        if not self._h:
+            raise ValueError("oracle geometry rejected")
+        self.N = 0
+        self.M = 0
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_free(self._h)
+            self._h = None
+
+    def geometry(self):
+        cd = np.zeros(3, np.int32)
+        h = np.zeros(3, np.float64)
+        nt = C.c_int()
+        lib().oracle_geometry(self._h, _ip(cd), _dp(h), C.byref(nt))
+        return cd, h, nt.value
+
+    def set_threads(self, n):
+        lib().oracle_set_threads(self._h, int(n))
+
+    def set_seed(self, s):
+        lib().oracle_set_seed(self._h, int(s) & 0xFFFFFFFFFFFFFFFF)
+
+    def set_dpd_noise(self, on):
+        lib().oracle_set_dpd_counter_noise(self._h, int(on))
+
+    def add_type(self, mass, radius, dynamics):
+        return lib().oracle_add_type(self._h, float(mass), float(radius), int(dynamics))
+
+    def add_random_force(self, t, mean, std, duration):
+        lib().oracle_set_random_force(self._h, int(t), float(mean), float(std), float(duration))
+
+    def bind(self, pot, ti, tj):
+        inf, alpha, c = _pot_tables(pot)
+        cp = _fp(c) if c.size else None
+        lib().oracle_add_pot(self._h, int(ti), int(tj), inf.kind, inf.flags, _fp(alpha), inf.a, inf.b,
+                             inf.r0_plusone, inf.n, cp, inf.dpd_alpha, inf.dpd_gamma, inf.dpd_sigma)
+
+    def load(self, x, v, types, flags=None):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(types, np.int32)
+        fl = None if flags is None else np.ascontiguousarray(flags, np.uint16)
+        rc = lib().oracle_load(self._h, x.shape[0], _dp(x), _dp(v), _ip(t),
+                               fl.ctypes.data_as(C.POINTER(C.c_uint16)) if fl is not None else None)
+        if rc:
+            raise ValueError("oracle load failed %d" % rc)
+        self.N = x.shape[0]
+
+    def set_bonds(self, i, j, pot, dissociation):
+        inf, alpha, c = _pot_tables(pot)
+        pidx = lib().oracle_add_bond_pot(self._h, _fp(alpha), inf.a, inf.b, inf.n, _fp(c))
+        i = np.ascontiguousarray(i, np.int32)
+        j = np.ascontiguousarray(j, np.int32)
+        M = len(i)
+        p = np.full(M, pidx, np.int32)
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dissociation, np.float64), (M,)))
+        lib().oracle_set_bonds(self._h, M, _ip(i), _ip(j), _ip(p), _dp(d))
+        self.M = M
+
+    def step(self, n=1):
+        return lib().oracle_step(self._h, int(n))
+
+    def force(self):
+        return lib().oracle_force(self._h)
+
+    def step_with_forces(self, f):
+        f = np.ascontiguousarray(f, np.float32).reshape(-1)
+        return lib().oracle_step_with_forces(self._h, _fp(f))
+
+    def state(self):
+        N = self.N
+        xl = np.zeros((N, 3), np.float32)
+        cell = np.zeros(N, np.int32)
+        v = np.zeros((N, 3), np.float32)
+        f = np.zeros((N, 3), np.float32)
+        nc = np.zeros(N, np.int32)
+        pf = np.zeros((N, 3), np.float32)
+        lib().oracle_get_state(self._h, _fp(xl), _ip(cell), _fp(v), _fp(f), _ip(nc), _fp(pf))
+        return {"x": xl, "cell": cell, "v": v, "f": f, "ncount": nc, "pf": pf}
+
+    def global_positions(self):
+        x = np.zeros((self.N, 3), np.float64)
+        lib().oracle_global_positions(self._h, _dp(x))
+        return x
+
+    def observables(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        lib().oracle_observables(self._h, C.byref(a), C.byref(b), C.byref(c))
+        return {"epot_nonbond": a.value, "epot_bond": b.value, "ekin": c.value}
+
+    def bond_active(self):
+        a = np.zeros(max(self.M, 1), np.uint8)
+        lib().oracle_bond_active(self._h, a.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return a[: self.M].astype(bool)
+
+    def forces64(self):
+        F = np.zeros((self.N, 3), np.float64)
+        e = C.c_double()
+        lib().oracle_forces64(self._h, _dp(F), C.byref(e))
+        return F, e.value
+
+    def error(self):
+        pid = C.c_int()
+        rc = lib().oracle_error(self._h, C.byref(pid))
+        return rc, pid.value
+
+
+def from_workload(w, m, threads=1, seed=None):
+    """Builds an Oracle for workload dict w, with tables from module m."""
+    from mechanica_amd import workloads as W
+    o = Oracle(w["dim"], w["cells"], w["cutoff"], w["dt"], threads=threads)
+    for t in w["types"]:
+        o.add_type(t["mass"], t["radius"], t["dynamics"])
+    keep = []
+    for spec in w["pots"]:
+        p = W.make_potential(m, spec)
+        o.bind(p, spec[2], spec[3])
+        keep.append(p)
+    if seed is not None:
+        o.set_seed(seed)
+    o.load(w["x"], w["v"], w["type"])
+    if w.get("bonds"):
+        b = w["bonds"]
+        bp = W.make_potential(m, b["pot"])
+        o.set_bonds(b["i"], b["j"], bp, b["dissociation"])
+        keep.append(bp)
+    o._keep = keep
+    return o
