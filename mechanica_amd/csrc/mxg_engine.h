@@ -89,8 +89,8 @@ class Engine {
   // Adds the virial tensor to the per-step reductions.
   int set_virial(int on);
 
-  // Timing of the force kernel over the last step() call (HIP events on the
-  // engine's stream); -1 when not recorded.
+  // Average duration of the force kernel and of a whole step over the last
+  // step() call (HIP events on the engine's stream); -1 when not recorded.
   int set_timing(int on);
   double force_kernel_ms() const { return force_ms_; }
   double step_ms() const { return step_ms_; }
@@ -132,6 +132,7 @@ class Engine {
   int err_ = 0;
   std::string msg_;
   bool timing_ = false;
+  int timed_steps_ = 0;
   double force_ms_ = -1, step_ms_ = -1;
   bool virial_ = false;
   bool pid_prev_valid_ = false;  // pid[1] holds the order forces were computed in

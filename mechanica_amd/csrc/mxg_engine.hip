@@ -120,6 +120,8 @@ struct DeviceState {
   unsigned *time = nullptr;
   int *err = nullptr;  // [0] error code, [1] pid, [2] error state at step start
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // Per-step events of the force kernel (start, end) while timing is on.
+  std::vector<hipEvent_t> kev;
   std::vector<void *> owned;
 };
 
@@ -860,6 +862,8 @@ Engine::~Engine() {
     for (void *p : d_->owned) hipFree(p);
     for (auto &e : d_->ev)
       if (e) hipEventDestroy(e);
+    for (auto &e : d_->kev)
+      if (e) hipEventDestroy(e);
     if (d_->stream) hipStreamDestroy(d_->stream);
     delete d_;
   }
@@ -1404,9 +1408,11 @@ int Engine::launch_step() {
   if (M_ > 0) mode |= kModeBonds;
   A.random = any_random_ ? 1 : 0;
   hipStream_t s = d->stream;
-  if (timing_) hipEventRecord(d->ev[0], s);
+  const bool rec = timing_ && timed_steps_ < (int)d->kev.size() / 2;
+  if (rec) hipEventRecord(d->kev[2 * timed_steps_], s);
   kStepLaunchers[mode](A, ntiles_, s);
-  if (timing_) hipEventRecord(d->ev[1], s);
+  if (rec) hipEventRecord(d->kev[2 * timed_steps_ + 1], s);
+  if (rec) ++timed_steps_;
 
   RebuildArgs B;
   B.cstart = d->cstart[a];
@@ -1436,7 +1442,6 @@ int Engine::launch_step() {
   hipLaunchKernelGGL(k_scatter, dim3(cg), dim3(256), 0, s, B);
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d->partials, d->rep_tiles,
                      (int)reps_.size(), d->obs, d->time, d->err);
-  if (timing_) hipEventRecord(d->ev[2], s);
   // After the scatter the particle arrays of buffer a hold the new order,
   // except pid/cstart which were written into buffer b.
   std::swap(d->pid[0], d->pid[1]);
@@ -1452,9 +1457,21 @@ int Engine::step(int nsteps) {
   if (err_ != 0) return err_;
   if (potentials_dirty_)
     if (int rc = upload_potentials()) return rc;
+  if (timing_) {
+    // Two events per step, for up to 4096 steps per call.
+    const size_t want = 2 * (size_t)std::min(nsteps, 4096);
+    while (d_->kev.size() < want) {
+      hipEvent_t e;
+      if (int rc = check_device(hipEventCreate(&e), "hipEventCreate")) return rc;
+      d_->kev.push_back(e);
+    }
+    hipEventRecord(d_->ev[0], d_->stream);
+  }
+  timed_steps_ = 0;
   for (int k = 0; k < nsteps; ++k) {
     if (int st = launch_step()) return check_device(st, "kernel launch");
   }
+  if (timing_) hipEventRecord(d_->ev[2], d_->stream);
   int herr[4];
   unsigned htime = 0;
   int rc = hipMemcpyAsync(herr, d_->err, sizeof(herr), hipMemcpyDeviceToHost, d_->stream);
@@ -1462,12 +1479,17 @@ int Engine::step(int nsteps) {
   rc |= hipStreamSynchronize(d_->stream);
   if (rc) return check_device(rc, "step");
   time_ = htime;
-  if (timing_) {
+  if (timing_ && nsteps > 0) {
+    double sum = 0;
+    for (int k = 0; k < timed_steps_; ++k) {
+      float ms = 0;
+      hipEventElapsedTime(&ms, d_->kev[2 * k], d_->kev[2 * k + 1]);
+      sum += ms;
+    }
+    force_ms_ = timed_steps_ ? sum / timed_steps_ : -1;
     float ms = 0;
-    hipEventElapsedTime(&ms, d_->ev[0], d_->ev[1]);
-    force_ms_ = ms;
     hipEventElapsedTime(&ms, d_->ev[0], d_->ev[2]);
-    step_ms_ = ms;
+    step_ms_ = ms / nsteps;
   }
   if (herr[0] != 0) {
     return fail(herr[0], "particle " + std::to_string(herr[1]) + " moved too fast (more than one cell in a step)");

@@ -660,36 +660,35 @@ void run_tasks_threaded(Oracle &o) {
     for (int i = 0; i < T; ++i) th.emplace_back(fn);
     for (auto &x : th) x.join();
   }
-  std::vector<std::atomic<char>> done(work.size());
-  for (auto &d : done) d = 0;
-  std::atomic<int> remaining{(int)work.size()};
-  auto fn = [&](int rid) {
-    size_t start = work.size() * rid / T;
-    while (remaining.load() > 0) {
-      bool any = false;
-      for (size_t q = 0; q < work.size(); ++q) {
-        size_t k = (start + q) % work.size();
-        if (done[k].load(std::memory_order_relaxed)) continue;
-        const Task &t = o.tasks[work[k]];
-        char z = 0;
-        if (!o.cells[t.i].taboo.compare_exchange_strong(z, 1)) continue;
-        if (t.type == kTaskPair) {
-          z = 0;
-          if (!o.cells[t.j].taboo.compare_exchange_strong(z, 1)) {
-            o.cells[t.i].taboo = 0;
-            continue;
-          }
-        }
-        char d0 = 0;
-        if (done[k].compare_exchange_strong(d0, 1)) {
-          run_task(o, t);
-          remaining.fetch_sub(1);
-          any = true;
-        }
+  // Each runner owns a contiguous block of the task list (tasks are ordered
+  // by cell, so blocks are spatially apart and rarely contend); a task whose
+  // cells are held by another runner is retried after the block.
+  auto try_run = [&](const Task &t) {
+    char z = 0;
+    if (!o.cells[t.i].taboo.compare_exchange_strong(z, 1)) return false;
+    if (t.type == kTaskPair) {
+      z = 0;
+      if (!o.cells[t.j].taboo.compare_exchange_strong(z, 1)) {
         o.cells[t.i].taboo = 0;
-        if (t.type == kTaskPair) o.cells[t.j].taboo = 0;
+        return false;
       }
-      if (!any) std::this_thread::yield();
+    }
+    run_task(o, t);
+    o.cells[t.i].taboo = 0;
+    if (t.type == kTaskPair) o.cells[t.j].taboo = 0;
+    return true;
+  };
+  auto fn = [&](int rid) {
+    size_t b = work.size() * rid / T, e = work.size() * (rid + 1) / T;
+    std::vector<int> pending;
+    for (size_t k = b; k < e; ++k)
+      if (!try_run(o.tasks[work[k]])) pending.push_back(work[k]);
+    while (!pending.empty()) {
+      std::vector<int> again;
+      for (int t : pending)
+        if (!try_run(o.tasks[t])) again.push_back(t);
+      if (again.size() == pending.size()) std::this_thread::yield();
+      pending.swap(again);
     }
   };
   std::vector<std::thread> th;
@@ -1110,6 +1109,14 @@ int oracle_geometry(void *h, int cdim[3], double hh[3], int *ntasks) {
 }
 
 void oracle_set_threads(void *h, int n) { ((Oracle *)h)->nthreads = n < 1 ? 1 : n; }
+void oracle_set_max_dist_fraction(void *h, float f) { ((Oracle *)h)->max_frac = f; }
+
+// Philox4x32-10 of the shared header, for known-answer checks.
+void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  mxg::u32x4 c = {{ctr[0], ctr[1], ctr[2], ctr[3]}};
+  mxg::u32x4 r = mxg::philox4x32_10(c, key[0], key[1]);
+  for (int k = 0; k < 4; ++k) out[k] = r.v[k];
+}
 void oracle_set_seed(void *h, uint64_t seed) {
   ((Oracle *)h)->seed_lo = (uint32_t)seed;
   ((Oracle *)h)->seed_hi = (uint32_t)(seed >> 32);

@@ -24,6 +24,8 @@ _SIG = {
     "oracle_geometry": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "oracle_set_threads": (None, [C.c_void_p, C.c_int]),
     "oracle_set_seed": (None, [C.c_void_p, C.c_uint64]),
+    "oracle_set_max_dist_fraction": (None, [C.c_void_p, C.c_float]),
+    "oracle_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "oracle_set_dpd_counter_noise": (None, [C.c_void_p, C.c_int]),
     "oracle_set_time": (None, [C.c_void_p, C.c_uint64]),
     "oracle_add_type": (C.c_int, [C.c_void_p, C.c_double, C.c_float, C.c_int]),
@@ -113,6 +115,9 @@ class Oracle:
     def set_seed(self, s):
         lib().oracle_set_seed(self._h, int(s) & 0xFFFFFFFFFFFFFFFF)
 
+    def set_max_dist_fraction(self, f):
+        lib().oracle_set_max_dist_fraction(self._h, float(f))
+
     def set_dpd_noise(self, on):
         lib().oracle_set_dpd_counter_noise(self._h, int(on))
 
@@ -196,6 +201,14 @@ class Oracle:
         pid = C.c_int()
         rc = lib().oracle_error(self._h, C.byref(pid))
         return rc, pid.value
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    out = (C.c_uint32 * 4)()
+    lib().oracle_philox(c, k, out)
+    return list(out)
 
 
 def from_workload(w, m, threads=1, seed=None):

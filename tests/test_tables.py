@@ -1,0 +1,150 @@
+"""Potential tables (host builder of libmxg.so), pinned by closed forms.
+
+The builder restates potential_init (src/mdcore/src/MxPotential.cpp:
+1690-2340).  No reference binary can be built here, so the tables are pinned
+by what the mathematics fixes:
+  * the interpolant matches the closed-form potential to the requested
+    tolerance, measured like the builder measures it (relative to
+    max(max |f| on the interval, 1));
+  * the force it returns is the derivative of its own energy;
+  * the interval map is monotone and sends a to interval 1, b to <= n;
+  * harmonic (a quadratic) is reproduced exactly by one interval;
+  * the continuation below a matches the first interval at a;
+  * an unreachable tolerance fails with potential_err_ivalsmax (-5).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mechanica_amd import workloads as W
+
+LJ = lambda r: W.LJ_A / r ** 12 - W.LJ_B / r ** 6
+
+
+def morse(d, a):
+    return lambda r: d * (np.exp(-2 * a * (r - 1)) - 2 * np.exp(-a * (r - 1)))
+
+
+def table_e(p, r):
+    return np.array([p.eval(np.float32(x) ** 2)[0] for x in r])
+
+
+def table_f(p, r):
+    return np.array([p.eval(np.float32(x) ** 2)[1] for x in r])
+
+
+def interval_of(p, r):
+    al = p.alpha.astype(np.float32)
+    r = np.float32(r)
+    return int(max(np.float32(0), al[0] + r * (al[1] + r * al[2])))
+
+
+def check_tolerance(p, f, tol, samples=4000):
+    a, b = p.domain
+    r = np.linspace(a, b, samples, dtype=np.float64)[1:-1]
+    e = table_e(p, r)
+    ex = f(r)
+    ind = np.array([interval_of(p, x) for x in r])
+    worst = 0.0
+    for i in np.unique(ind):
+        sel = ind == i
+        scale = max(np.abs(ex[sel]).max(), 1.0)
+        worst = max(worst, np.abs(e[sel] - ex[sel]).max() / scale)
+    return worst
+
+
+def test_lj_table_within_tolerance(m):
+    p = m.Potential.lennard_jones_12_6(0.275, 1.0, W.LJ_A, W.LJ_B, 1e-3)
+    assert 1 < p.n < 3048
+    assert check_tolerance(p, LJ, 1e-3) <= 1.5e-3
+
+
+def test_morse_table_within_tolerance(m):
+    p = m.Potential.morse(d=1.0, a=6.0, r0=1.0, min=1e-4, max=2.0, tol=1e-3)
+    assert p.flags & (1 << 12)  # SHIFTED
+    assert p.r0_plusone == pytest.approx(2.0)
+    assert check_tolerance(p, morse(1.0, 6.0), 1e-3) <= 1.5e-3
+
+
+def test_tighter_tolerance_needs_more_intervals(m):
+    n = [m.Potential.lennard_jones_12_6(0.275, 1.0, W.LJ_A, W.LJ_B, t).n for t in (1e-2, 1e-3, 1e-4)]
+    assert n[0] <= n[1] <= n[2] and n[0] < n[2]
+
+
+def test_force_is_derivative_of_energy(m):
+    """f * r equals dE/dr of the interpolant (central differences in r)."""
+    for p in (m.Potential.lennard_jones_12_6(0.275, 1.0, W.LJ_A, W.LJ_B, 1e-3),
+              m.Potential.morse(d=1.0, a=6.0, r0=1.0, min=1e-4, max=2.0, tol=1e-3)):
+        a, b = p.domain
+        r = np.linspace(a + 0.02, b - 0.02, 60)
+        h = 1e-3
+        de = (table_e(p, r + h) - table_e(p, r - h)) / (2 * h)
+        fr = table_f(p, r) * r
+        scale = np.maximum(np.abs(de), 1.0)
+        assert (np.abs(fr - de) / scale).max() < 2e-2
+
+
+def test_force_matches_closed_form_derivative(m):
+    """The table bounds the energy error (tol relative to the interval's
+    energy scale); its derivative is looser: within 10% of max(|E'|, 1)."""
+    p = m.Potential.lennard_jones_12_6(0.275, 1.0, W.LJ_A, W.LJ_B, 1e-3)
+    r = np.linspace(0.3, 0.99, 50)
+    dlj = -12 * W.LJ_A / r ** 13 + 6 * W.LJ_B / r ** 7
+    fr = table_f(p, r) * r
+    assert (np.abs(fr - dlj) / np.maximum(np.abs(dlj), 1.0)).max() < 0.1
+
+
+def test_interval_map_monotone(m):
+    p = m.Potential.lennard_jones_12_6(0.275, 1.0, W.LJ_A, W.LJ_B, 1e-3)
+    a, b = p.domain
+    r = np.linspace(a, b, 5000)
+    ind = np.array([interval_of(p, x) for x in r])
+    assert ind[0] == 1
+    assert np.all(np.diff(ind) >= 0)
+    assert ind[-1] <= p.n
+
+
+def test_harmonic_is_exact(m):
+    p = m.Potential.harmonic(100.0, 1.0)  # min 0, max 2, tol 0.02
+    assert p.n == 1
+    assert p.flags == 0  # POTENTIAL_HARMONIC & POTENTIAL_R2
+    r = np.linspace(0.05, 1.95, 200)
+    e = table_e(p, r)
+    ex = 100.0 * (r - 1.0) ** 2
+    assert np.abs(e - ex).max() <= 1e-4 * 100.0
+    fr = table_f(p, r) * r
+    assert np.abs(fr - 200.0 * (r - 1.0)).max() <= 1e-3 * 200.0
+
+
+def test_continuation_below_lower_bound(m):
+    p = m.Potential.lennard_jones_12_6(0.275, 1.0, W.LJ_A, W.LJ_B, 1e-3)
+    a, _ = p.domain
+    # LJ changes by ~12 dr / r = 0.24% over this step; the quadratic
+    # continuation must not jump beyond that.
+    e_a = p.eval(np.float32(a * 1.0001) ** 2)[0]
+    e_below = p.eval(np.float32(a * 0.9999) ** 2)[0]
+    assert 0 < e_below - e_a <= 5e-3 * abs(e_a)
+
+
+def test_eval_ex_range_and_shift(m):
+    p = m.Potential.morse(d=1.0, a=6.0, r0=1.0, min=1e-4, max=2.0, tol=1e-3)
+    ok, e, f = p.eval_ex(1.0, 1.0, 1.0)  # radius 1 + 1 with r0 = 1: minimum at r = 1
+    assert ok and e == pytest.approx(-1.0, abs=2e-3) and abs(f) < 5e-2
+    ok, _, _ = p.eval_ex(2.1 ** 2, 1.0, 1.0)
+    assert not ok  # shifted r beyond b
+
+
+def test_unreachable_tolerance_fails(m):
+    with pytest.raises(ValueError):
+        m.Potential.lennard_jones_12_6(0.275, 1.0, W.LJ_A, W.LJ_B, 1e-14)
+    from mechanica_amd import _capi
+    assert _capi.lib().mxg_potential_error() == -5
+
+
+def test_dpd_potential_fields(m):
+    p = m.Potential.dpd(alpha=25, gamma=4.5, sigma=3, cutoff=1.0)
+    inf = p.info
+    assert inf.kind == 1 and inf.dpd_alpha == 25 and inf.dpd_gamma == pytest.approx(4.5)
+    assert inf.a == pytest.approx(math.sqrt(np.finfo(np.float32).eps))
+    assert inf.b == 1.0
