@@ -100,6 +100,8 @@ struct DeviceState {
   int *slot_of = nullptr;
   int *ncount = nullptr;
   int *cnt = nullptr;
+  int *inc = nullptr;
+  int *cursor = nullptr;
   int *bsum = nullptr;
   int *cell_rep = nullptr;
   DevReplica *rep = nullptr;
@@ -169,6 +171,8 @@ struct StepArgs {
   double *partials;
   int *ncount;  // neighbour-count mode only
   int random;   // some type carries a persistent random force
+  int *cnt;     // particles per cell of the next step
+  int *inc;     // particles entering each cell
 };
 
 enum : int {
@@ -177,6 +181,11 @@ enum : int {
   kModeVirial = 4,
   kModeBonds = 8,
 };
+
+// Uniform noise of zero mean and unit variance on [-sqrt(3), sqrt(3)).
+__device__ __forceinline__ float unit_uniform(uint32_t a) {
+  return 1.7320508f * (2.0f * u01_closed_open(a) - 1.0f);
+}
 
 __device__ __forceinline__ float gauss_from(uint32_t a, uint32_t b) {
   float u1 = u01_open_closed(a);
@@ -227,7 +236,9 @@ __device__ __forceinline__ bool table_eval_ex(const DevPot &P, const float *__re
   eff = eff * x + ee;
   ee = ee * x + c1.w;
   e = ee;
-  f = eff * c0.y / ro;
+  // One-ulp division: the force only has to agree to 1e-5; r^2, the cutoff
+  // test and the advance stay exact.
+  f = __fdividef(eff * c0.y, ro);
   return true;
 }
 
@@ -309,7 +320,8 @@ __device__ __forceinline__ void pair_force(const StepArgs &A, int ti, int tj, bo
     float cutoff = shifted ? (P.b + ri + rj) : P.b;
     if (r2 > cutoff * cutoff) return;
     float r = sqrtf(r2);
-    float ex = dxx / r, ey = dxy / r, ez = dxz / r;
+    const float ir = __fdividef(1.0f, r);
+    float ex = dxx * ir, ey = dxy * ir, ez = dxz * ir;
     float4 vq = A.v[q];
     float vx, vy, vz;
     if (p_is_i) {
@@ -322,7 +334,7 @@ __device__ __forceinline__ void pair_force(const StepArgs &A, int ti, int tj, bo
       vz = vq.z - vs.z;
     }
     float shifted_r = shifted ? r - ri - rj : r;
-    float omega_c = shifted_r < 0.f ? 1.f : (1 - shifted_r / cutoff);
+    float omega_c = shifted_r < 0.f ? 1.f : (1 - __fdividef(shifted_r, cutoff));
     float fc = P.dpd_alpha * omega_c;
     float omega_d = omega_c * omega_c;
     float ev = ex * vx + ey * vy + ez * vz;
@@ -332,7 +344,7 @@ __device__ __forceinline__ void pair_force(const StepArgs &A, int ti, int tj, bo
       int qpid = A.pid[q];
       u32x4 ctr = {{(uint32_t)min(mypid, qpid), (uint32_t)max(mypid, qpid), t1, kStreamDPD}};
       u32x4 rr = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
-      fr = P.dpd_sigma * omega_c * gauss_from(rr.v[0], rr.v[1]) * A.dpd_delta;
+      fr = P.dpd_sigma * omega_c * unit_uniform(rr.v[0]) * A.dpd_delta;
     } else {
       fr = P.dpd_sigma * omega_c * A.dpd_delta;
     }
@@ -638,7 +650,9 @@ __global__ __launch_bounds__(kTile) void k_step(StepArgs A) {
       }
       A.x_out[s] = make_float4(X[0], X[1], X[2], xs.w);
       A.v_out[s] = make_float4(V[0], V[1], V[2], 0.0f);
-      A.cell_out[s] = newc;
+      A.cell_out[s] = newc != c ? (int)((unsigned)newc | 0x80000000u) : newc;
+      atomicAdd(&A.cnt[newc], 1);
+      if (newc != c) atomicAdd(&A.inc[newc], 1);
       A.f_out[s] = make_float4(fx, fy, fz, 0.0f);
       if (A.random) A.pf_out[s] = pfs;
     }
@@ -679,12 +693,12 @@ __global__ __launch_bounds__(kTile) void k_ncount(StepArgs A) {
 }
 
 struct RebuildArgs {
-  const int *__restrict__ cstart;   // old offsets
-  int *__restrict__ cstart_new;     // new offsets
-  const int *__restrict__ cell_out; // new cell of each old slot
-  const int *__restrict__ cell_rep;
-  const DevReplica *__restrict__ rep;
-  int *__restrict__ cnt;
+  const int *__restrict__ cstart;      // old cell offsets
+  const int *__restrict__ cstart_new;  // new cell offsets
+  const int *__restrict__ cell_out;    // new cell of each old slot, bit 31 set if it moved
+  int *__restrict__ inc;               // particles entering each cell
+  int *__restrict__ cursor;            // per-cell cursor for entering particles
+  int nslots;
   int ncells;
   const float4 *__restrict__ x_in;
   const float4 *__restrict__ v_in;
@@ -698,30 +712,6 @@ struct RebuildArgs {
   int *__restrict__ slot_of;
   int with_pf;
 };
-
-// Source cells of destination c in the fixed order: itself, then the 26
-// neighbours in lexicographic offset order.
-__device__ __forceinline__ int source_cell(const DevReplica &R, int lx, int ly, int lz, int k) {
-  int o = (k == 0) ? 13 : (k <= 13 ? k - 1 : k);
-  int dx = o / 9 - 1, dy = (o / 3) % 3 - 1, dz = o % 3 - 1;
-  int sx = wrap(lx - dx, R.cdim[0]), sy = wrap(ly - dy, R.cdim[1]), sz = wrap(lz - dz, R.cdim[2]);
-  return R.cell_base + (sx * R.cdim[1] + sy) * R.cdim[2] + sz;
-}
-
-__global__ __launch_bounds__(256) void k_count(RebuildArgs A) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= A.ncells) return;
-  const DevReplica &R = A.rep[A.cell_rep[c]];
-  int lx, ly, lz;
-  cell_coords(c - R.cell_base, R, lx, ly, lz);
-  int n = 0;
-  for (int k = 0; k < 27; ++k) {
-    const int src = source_cell(R, lx, ly, lz, k);
-    const int qe = A.cstart[src + 1];
-    for (int q = A.cstart[src]; q < qe; ++q) n += (A.cell_out[q] == c);
-  }
-  A.cnt[c] = n;
-}
 
 constexpr int kScanBlock = 256, kScanItems = 4, kScanChunk = kScanBlock * kScanItems;
 
@@ -745,7 +735,9 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
   return wsum + x - v;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_block(const int *__restrict__ in, int *__restrict__ out,
+// Exclusive scan of the per-cell counts (first pass); the counts are reset
+// to zero for the next step once read.
+__global__ __launch_bounds__(kScanBlock) void k_scan_block(int *__restrict__ cnt, int *__restrict__ out,
                                                           int *__restrict__ bsum, int n) {
   __shared__ int lds[kScanBlock / 64];
   const int base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
@@ -753,9 +745,12 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_block(const int *__restrict
   int local = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    vals[i] = (base + i < n) ? in[base + i] : 0;
+    vals[i] = (base + i < n) ? cnt[base + i] : 0;
     local += vals[i];
   }
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) cnt[base + i] = 0;
   int total;
   int off = block_exclusive_scan(local, lds, total);
 #pragma unroll
@@ -766,8 +761,11 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_block(const int *__restrict
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
+// Scan of the block totals (one block).  Also advances the time unless the
+// step was skipped because of an earlier error, and records the error state:
+// err[3] = this step was skipped, err[2] = error state for the next step.
 __global__ __launch_bounds__(kScanBlock) void k_scan_top(int *__restrict__ bsum, int nb, int *__restrict__ out,
-                                                        int n) {
+                                                        int n, unsigned *time, int *err) {
   __shared__ int lds[kScanBlock / 64];
   int carry = 0;
   for (int base = 0; base < nb; base += kScanBlock) {
@@ -779,7 +777,13 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_top(int *__restrict__ bsum,
     carry += total;
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[n] = carry;
+  if (threadIdx.x == 0) {
+    out[n] = carry;
+    const int skipped = err[2];
+    err[3] = skipped;
+    if (!skipped) *time += 1u;
+    err[2] = err[0];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_scan_add(int *__restrict__ out, const int *__restrict__ bsum, int n) {
@@ -787,50 +791,92 @@ __global__ __launch_bounds__(256) void k_scan_add(int *__restrict__ out, const i
   if (i < n) out[i] += bsum[i / kScanChunk];
 }
 
-__global__ __launch_bounds__(256) void k_scatter(RebuildArgs A) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= A.ncells) return;
-  const DevReplica &R = A.rep[A.cell_rep[c]];
-  int lx, ly, lz;
-  cell_coords(c - R.cell_base, R, lx, ly, lz);
-  int k = A.cstart_new[c];
-  for (int j = 0; j < 27; ++j) {
-    const int src = source_cell(R, lx, ly, lz, j);
-    const int qe = A.cstart[src + 1];
-    for (int q = A.cstart[src]; q < qe; ++q) {
-      if (A.cell_out[q] != c) continue;
-      A.x[k] = A.x_in[q];
-      A.v[k] = A.v_in[q];
-      if (A.with_pf) A.pf[k] = A.pf_in[q];
-      const int p = A.pid_in[q];
-      A.pid[k] = p;
-      A.cell[k] = c;
-      A.slot_of[p] = k;
-      ++k;
-    }
+// Moves each particle to its slot of the next step.  A particle that stays in
+// its cell keeps its rank among the cell's stayers; particles entering a cell
+// follow its stayers (their order is made deterministic by k_order_incoming).
+__global__ __launch_bounds__(256) void k_place(RebuildArgs A) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= A.nslots) return;
+  const int raw = A.cell_out[q];
+  const int c1 = raw & 0x7fffffff;
+  int dst;
+  if (raw >= 0) {
+    // Stayer: rank among the stayers of its cell (old slot order).
+    int rank = 0;
+    for (int q2 = A.cstart[c1]; q2 < q; ++q2) rank += (A.cell_out[q2] == c1);
+    dst = A.cstart_new[c1] + rank;
+  } else {
+    const int stay = (A.cstart_new[c1 + 1] - A.cstart_new[c1]) - A.inc[c1];
+    dst = A.cstart_new[c1] + stay + atomicAdd(&A.cursor[c1], 1);
   }
+  A.x[dst] = A.x_in[q];
+  A.v[dst] = A.v_in[q];
+  if (A.with_pf) A.pf[dst] = A.pf_in[q];
+  const int p = A.pid_in[q];
+  A.pid[dst] = p;
+  A.cell[dst] = c1;
+  A.slot_of[p] = dst;
 }
 
-// Sums the tile partials of each replica in tile order (one block, so the
-// order is fixed); advances the time unless the step was skipped because of
-// an earlier error, and records the error state for the next step.
+// Sorts the particles that entered each cell by id (their arrival order in
+// k_place depends on scheduling) and resets the per-cell counters.
+__global__ __launch_bounds__(256) void k_order_incoming(RebuildArgs A) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= A.ncells) return;
+  const int n_in = A.inc[c];
+  if (n_in >= 2) {
+    const int base = A.cstart_new[c + 1] - n_in;
+    for (int i = 1; i < n_in; ++i) {
+      for (int j = i; j > 0 && A.pid[base + j - 1] > A.pid[base + j]; --j) {
+        const int a = base + j - 1, b = base + j;
+        const int tp = A.pid[a];
+        A.pid[a] = A.pid[b];
+        A.pid[b] = tp;
+        const float4 tx = A.x[a];
+        A.x[a] = A.x[b];
+        A.x[b] = tx;
+        const float4 tv = A.v[a];
+        A.v[a] = A.v[b];
+        A.v[b] = tv;
+        if (A.with_pf) {
+          const float4 tf = A.pf[a];
+          A.pf[a] = A.pf[b];
+          A.pf[b] = tf;
+        }
+      }
+    }
+    for (int k = base; k < base + n_in; ++k) A.slot_of[A.pid[k]] = k;
+  }
+  if (n_in) A.inc[c] = 0;
+  A.cursor[c] = 0;
+}
+
+// Sums the tile partials of each replica (one block per replica, each
+// thread a fixed strided subset, then a fixed-order block reduction).
 __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ partials,
                                                   const int2 *__restrict__ rep_tiles, int nrep,
-                                                  double *__restrict__ obs, unsigned *time, int *err) {
-  const int skipped = err[2];
-  if (!skipped) {
-    for (int item = threadIdx.x; item < nrep * kNRed; item += blockDim.x) {
-      const int r = item / kNRed, k = item % kNRed;
-      const int2 tr = rep_tiles[r];
-      double s = 0.0;
-      for (int t = tr.x; t < tr.y; ++t) s += partials[(size_t)t * kNRed + k];
-      obs[item] = s;
-    }
+                                                  double *__restrict__ obs, const int *err) {
+  __shared__ double red[kNRed][4];
+  const int r = blockIdx.x;
+  if (r >= nrep || err[3]) return;
+  const int2 tr = rep_tiles[r];
+  double acc[kNRed];
+#pragma unroll
+  for (int k = 0; k < kNRed; ++k) acc[k] = 0.0;
+  for (int t = tr.x + (int)threadIdx.x; t < tr.y; t += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < kNRed; ++k) acc[k] += partials[(size_t)t * kNRed + k];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kNRed; ++k) {
+    double w = wave_sum(acc[k]);
+    if (lane == 0) red[k][wave] = w;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (!skipped) *time += 1u;
-    err[2] = err[0];
+  if (threadIdx.x < kNRed) {
+    double s = 0.0;
+    for (int w = 0; w < 4; ++w) s += red[threadIdx.x][w];
+    obs[(size_t)r * kNRed + threadIdx.x] = s;
   }
 }
 
@@ -1027,6 +1073,8 @@ int Engine::build_geometry() {
   if (int st = dalloc(d, &d->shifts, shifts.size())) return check_device(st, "hipMalloc");
   if (int st = dalloc(d, &d->cell_rep, cell_rep.size())) return check_device(st, "hipMalloc");
   if (int st = dalloc(d, &d->cnt, (size_t)ncells_)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->inc, (size_t)ncells_)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->cursor, (size_t)ncells_)) return check_device(st, "hipMalloc");
   if (int st = dalloc(d, &d->bsum, (size_t)ncells_ / kScanChunk + 2)) return check_device(st, "hipMalloc");
   for (int b = 0; b < 2; ++b)
     if (int st = dalloc(d, &d->cstart[b], (size_t)ncells_ + 1)) return check_device(st, "hipMalloc");
@@ -1039,6 +1087,9 @@ int Engine::build_geometry() {
   rc |= hipMemcpyAsync(d->offs, offs.data(), offs.size() * sizeof(DevOffset), hipMemcpyHostToDevice, s);
   rc |= hipMemcpyAsync(d->shifts, shifts.data(), shifts.size() * sizeof(float), hipMemcpyHostToDevice, s);
   rc |= hipMemcpyAsync(d->cell_rep, cell_rep.data(), cell_rep.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  rc |= hipMemsetAsync(d->cnt, 0, (size_t)ncells_ * sizeof(int), s);
+  rc |= hipMemsetAsync(d->inc, 0, (size_t)ncells_ * sizeof(int), s);
+  rc |= hipMemsetAsync(d->cursor, 0, (size_t)ncells_ * sizeof(int), s);
   rc |= hipMemsetAsync(d->time, 0, sizeof(unsigned), s);
   rc |= hipMemsetAsync(d->err, 0, 4 * sizeof(int), s);
   rc |= hipMemsetAsync(d->obs, 0, reps_.size() * kNRed * sizeof(double), s);
@@ -1400,6 +1451,8 @@ int Engine::launch_step() {
   A.time = d->time;
   A.err = d->err;
   A.partials = d->partials;
+  A.cnt = d->cnt;
+  A.inc = d->inc;
 
   int mode = 0;
   if (any_table_) mode |= kModeTable;
@@ -1418,9 +1471,9 @@ int Engine::launch_step() {
   B.cstart = d->cstart[a];
   B.cstart_new = d->cstart[b];
   B.cell_out = d->cell[b];
-  B.cell_rep = d->cell_rep;
-  B.rep = d->rep;
-  B.cnt = d->cnt;
+  B.inc = d->inc;
+  B.cursor = d->cursor;
+  B.nslots = N_;
   B.ncells = ncells_;
   B.x_in = d->x[b];
   B.v_in = d->v[b];
@@ -1434,16 +1487,18 @@ int Engine::launch_step() {
   B.slot_of = d->slot_of;
   B.with_pf = any_random_ ? 1 : 0;
   const int cg = (ncells_ + 255) / 256;
-  hipLaunchKernelGGL(k_count, dim3(cg), dim3(256), 0, s, B);
+  const int pg = (N_ + 255) / 256;
   const int nb = (ncells_ + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, d->cnt, d->cstart[b], d->bsum, ncells_);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, d->bsum, nb, d->cstart[b], ncells_);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, d->bsum, nb, d->cstart[b], ncells_, d->time,
+                     d->err);
   hipLaunchKernelGGL(k_scan_add, dim3(cg), dim3(256), 0, s, d->cstart[b], d->bsum, ncells_);
-  hipLaunchKernelGGL(k_scatter, dim3(cg), dim3(256), 0, s, B);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d->partials, d->rep_tiles,
-                     (int)reps_.size(), d->obs, d->time, d->err);
-  // After the scatter the particle arrays of buffer a hold the new order,
-  // except pid/cstart which were written into buffer b.
+  hipLaunchKernelGGL(k_place, dim3(pg), dim3(256), 0, s, B);
+  hipLaunchKernelGGL(k_order_incoming, dim3(cg), dim3(256), 0, s, B);
+  hipLaunchKernelGGL(k_finalize, dim3(reps_.size()), dim3(256), 0, s, d->partials, d->rep_tiles,
+                     (int)reps_.size(), d->obs, d->err);
+  // The placement wrote the new order into buffer a, except pid and cstart
+  // which went to buffer b.
   std::swap(d->pid[0], d->pid[1]);
   std::swap(d->cstart[0], d->cstart[1]);
   // cell[a] holds the new cells; x/v/pf[a] hold the new state.

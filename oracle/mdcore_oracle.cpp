@@ -363,12 +363,12 @@ inline bool eval_ex(const Pot &p, float ri, float rj, float r2, float *e, float 
   return true;
 }
 
-float gaussian_pair(uint32_t a, uint32_t b, uint64_t time, uint32_t klo, uint32_t khi) {
+// Pair noise of the counter-based DPD mode: uniform on [-sqrt(3), sqrt(3))
+// (zero mean, unit variance), keyed by the pair and the time.
+float pair_noise(uint32_t a, uint32_t b, uint64_t time, uint32_t klo, uint32_t khi) {
   mxg::u32x4 ctr = {{a, b, (uint32_t)time, mxg::kStreamDPD}};
   mxg::u32x4 r = mxg::philox4x32_10(ctr, klo, khi);
-  float u1 = mxg::u01_open_closed(r.v[0]);
-  float u2 = mxg::u01_closed_open(r.v[1]);
-  return std::sqrt(-2.0f * std::log(u1)) * std::cos(6.2831853f * u2);
+  return 1.7320508f * (2.0f * mxg::u01_closed_open(r.v[0]) - 1.0f);
 }
 
 // dpd_eval: forces on the pair along e = dx / r.  Returns false out of range.
@@ -389,7 +389,7 @@ inline bool dpd_eval(Oracle &o, const Pot &p, Part *pi, Part *pj, const float dx
   float fr;
   if (o.dpd_counter_noise) {
     uint32_t a = (uint32_t)std::min(pi->id, pj->id), b = (uint32_t)std::max(pi->id, pj->id);
-    fr = p.dpd_sigma * omega_c * gaussian_pair(a, b, o.time, o.seed_lo, o.seed_hi) * o.dpd_delta;
+    fr = p.dpd_sigma * omega_c * pair_noise(a, b, o.time, o.seed_lo, o.seed_hi) * o.dpd_delta;
   } else {
     fr = p.dpd_sigma * omega_c * o.dpd_delta;
   }
