@@ -128,7 +128,8 @@ int mxg_engine_add_random_force(mxg_engine *e, int type, float mean, float std, 
 int mxg_engine_set_seed(mxg_engine *e, uint64_t seed);
 
 /* 0: DPD random term as the reference computes it (sigma*omega/sqrt(dt),
- * dpd_eval.hpp:57); 1: a Gaussian per pair and step (counter-based). */
+ * dpd_eval.hpp:57); 1: a zero-mean, unit-variance uniform number per pair
+ * and step from a counter-based generator. */
 int mxg_engine_set_dpd_noise(mxg_engine *e, int mode);
 
 /* engine::particle_max_dist_fraction (engine.h:383). */
@@ -160,6 +161,11 @@ int mxg_engine_unload(mxg_engine *e, double *x, double *v);
 /* Per-particle state by id: cell-local position (MxParticle::x), cell index,
  * velocity, last forces, persistent random force.  NULL skips an array. */
 int mxg_engine_state(mxg_engine *e, float *xlocal, int *cell, float *v, float *f, float *pf);
+
+/* Number density of each particle from the last step (MxParticle::f[3]:
+ * space_prepare's self term plus W(r^2, cutoff) of every in-cutoff pair,
+ * space.cpp:202-242, runner_dopair.cpp:194-203), by id. */
+int mxg_engine_number_density(mxg_engine *e, float *nd);
 
 /* Number of in-cutoff neighbours of each particle in the current
  * configuration (pairs runner_dopair/runner_doself evaluate). */

@@ -274,6 +274,11 @@ class Universe:
         self._check(_capi.lib().mxg_engine_unload(self._h, _dp(x), _dp(v)))
         return x, v
 
+    def number_density(self):
+        nd = np.zeros(self.N, np.float32)
+        self._check(_capi.lib().mxg_engine_number_density(self._h, _fp(nd)))
+        return nd
+
     def neighbour_counts(self):
         c = np.zeros(self.N, np.int32)
         self._check(_capi.lib().mxg_engine_neighbour_counts(self._h, _ip(c)))

@@ -63,6 +63,7 @@ SIGNATURES = {
     "mxg_engine_unload": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     "mxg_engine_state": (C.c_int, [C.c_void_p, c_float_p, c_int_p, c_float_p, c_float_p, c_float_p]),
     "mxg_engine_neighbour_counts": (C.c_int, [C.c_void_p, c_int_p]),
+    "mxg_engine_number_density": (C.c_int, [C.c_void_p, c_float_p]),
     "mxg_engine_observables": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "mxg_engine_bond_active": (C.c_int, [C.c_void_p, c_u8_p]),
     "mxg_engine_geometry": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_double_p]),

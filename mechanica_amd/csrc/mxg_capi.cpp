@@ -203,6 +203,11 @@ int mxg_engine_state(mxg_engine *e, float *xlocal, int *cell, float *v, float *f
   return e->e.get_state(xlocal, cell, v, f, pf);
 }
 
+int mxg_engine_number_density(mxg_engine *e, float *nd) {
+  if (!e || !nd) return MXG_ERR_NULL;
+  return e->e.get_state(nullptr, nullptr, nullptr, nullptr, nullptr, nd);
+}
+
 int mxg_engine_neighbour_counts(mxg_engine *e, int *counts) {
   MXG_CHECK(e);
   if (!counts) return MXG_ERR_NULL;

@@ -71,7 +71,7 @@ class Engine {
   int sync();
 
   // Per particle (by id) snapshots; any pointer may be null.
-  int get_state(float *xlocal, int *cell, float *v, float *f, float *pf);
+  int get_state(float *xlocal, int *cell, float *v, float *f, float *pf, float *nd = nullptr);
   int get_global(double *x, double *v);
   int neighbour_counts(int *counts);
   int observables(int replica, double *out12);
@@ -95,7 +95,15 @@ class Engine {
   double force_kernel_ms() const { return force_ms_; }
   double step_ms() const { return step_ms_; }
 
-  // Autotuned launch shape of the force kernel.
+  // Shape of the force kernel's work unit: a brick of bx*by*bz cells staged
+  // with its halo in LDS, and the per-thread neighbour-list length.
+  // set_brick(0, 0, 0, 0) picks the largest brick whose halo fits.
+  int set_brick(int bx, int by, int bz, int cap);
+  void brick(int dims[3], int *cap, int *pmax) const {
+    for (int k = 0; k < 3; ++k) dims[k] = bdim_[k];
+    *cap = cap_;
+    *pmax = pmax_;
+  }
   int block_size() const { return block_; }
   int set_block_size(int b);
 
@@ -103,6 +111,8 @@ class Engine {
   int build_geometry();
   int upload_potentials();
   int launch_step();
+  void fill_step_args(void *out, int parity) const;
+  int refresh_args();
   int check_device(int hip_status, const char *what);
 
   std::vector<HostReplica> reps_;
@@ -125,6 +135,10 @@ class Engine {
   std::vector<Potential> bpots_;
 
   int N_ = 0, M_ = 0, ncells_ = 0, ntiles_ = 0;
+  int bdim_[3] = {4, 4, 4};
+  int cap_ = 16, pmax_ = 0, hmax_ = 0, nbricks_ = 0;
+  bool brick_dpd_ = false;
+  std::vector<double> mean_cell_count_;
   int block_ = 256;
   bool any_dpd_ = false, any_table_ = false, any_random_ = false;
   bool potentials_dirty_ = true;
@@ -136,7 +150,8 @@ class Engine {
   double force_ms_ = -1, step_ms_ = -1;
   bool virial_ = false;
   bool pid_prev_valid_ = false;  // pid[1] holds the order forces were computed in
-  int bact_cur_ = 0;             // current bond-activity buffer
+  int parity_ = 0;               // which ping-pong ids/offsets/bond flags are current
+  bool args_dirty_ = true;        // device copy of the kernel arguments is stale
   DeviceState *d_ = nullptr;
 };
 

@@ -109,6 +109,11 @@ struct DeviceState {
   float *shifts = nullptr;
   int4 *tiles = nullptr;
   int2 *rep_tiles = nullptr;
+  int4 *bricks = nullptr;
+  int2 *rep_bricks = nullptr;
+  int *ovf_list = nullptr;
+  int *ovf_count = nullptr;
+  unsigned char *dargs = nullptr;  // two StepArgs, one per step parity
   DevPot *pots = nullptr;
   DevPot *bpots = nullptr;
   float *coef = nullptr;
@@ -171,6 +176,8 @@ struct StepArgs {
   double *partials;
   int *ncount;  // neighbour-count mode only
   int random;   // some type carries a persistent random force
+  float cutoff;       // number-density kernel width
+  float nd_norm;      // 1 / (pi cutoff^3)
   int *cnt;     // particles per cell of the next step
   int *inc;     // particles entering each cell
 };
@@ -272,6 +279,23 @@ __device__ __forceinline__ unsigned sort_key(float bias, float dscale, float x, 
   return ((unsigned)(dscale * (bias + x * nx + y * ny + z * nz))) & 0xffffu;
 }
 
+// Cubic-spline number-density kernel W(r^2, h) (smoothing_kernel.hpp:28-50)
+// with the exact square root (the reference uses an approximate x86 rsqrt).
+__device__ __forceinline__ float w_cubic(float r2, float h, float inv_norm) {
+  const float x = __fdividef(sqrtf(r2), h);
+  float y;
+  if (x < 1.f) {
+    const float x2 = x * x;
+    y = 1.f - 1.5f * x2 + 0.75f * x2 * x;
+  } else if (x < 2.f) {
+    const float a = 2.f - x;
+    y = 0.25f * a * a * a;
+  } else {
+    y = 0.f;
+  }
+  return y * inv_norm;
+}
+
 __device__ __forceinline__ void cell_coords(int lc, const DevReplica &R, int &lx, int &ly, int &lz) {
   lz = lc % R.cdim[2];
   int t = lc / R.cdim[2];
@@ -286,18 +310,37 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Neighbour data read from global memory (slot q) or from LDS (index q).
+struct GlobalNbr {
+  const float4 *__restrict__ vv;
+  const int *__restrict__ pp;
+  __device__ __forceinline__ float4 v(int q) const { return vv[q]; }
+  __device__ __forceinline__ int pid(int q) const { return pp[q]; }
+};
+
+struct LdsNbr {
+  const float4 *vv;
+  const int *pp;
+  __device__ __forceinline__ float4 v(int q) const { return vv[q]; }
+  __device__ __forceinline__ int pid(int q) const { return pp[q]; }
+};
+
 // Accumulates one pair into the owner's force.  dx is in the reference
 // orientation (x_i - x_j); p_is_i says which end the owner is.
-template <int MODE>
-__device__ __forceinline__ void pair_force(const StepArgs &A, int ti, int tj, bool p_is_i, int mypid, int q,
+// Returns whether the pair adds to the number density (no potential, or the
+// potential was evaluated in range: potential_eval_super_ex adds it only on
+// success, dpd_eval.hpp:135-160).
+template <int MODE, typename NB>
+__device__ __forceinline__ bool pair_force(const StepArgs &A, const NB &nb, int ti, int tj, bool p_is_i,
+                                           int mypid, int q,
                                            bool self_pair, float dxx, float dxy, float dxz, float r2,
                                            const float4 &vs, float &fx, float &fy, float &fz, double *acc,
                                            unsigned t1) {
   int pidx = A.potmat[ti * A.ntypes + tj];
-  if (pidx < 0) return;
+  if (pidx < 0) return true;
   const DevPot P = A.pots[pidx];
   if (r2 < P.a2) {
-    int qpid = A.pid[q];
+    int qpid = nb.pid(q);
     uint32_t ki, kj;
     float sgn = 1.0f;
     if (self_pair) {
@@ -318,11 +361,11 @@ __device__ __forceinline__ void pair_force(const StepArgs &A, int ti, int tj, bo
   if ((MODE & kModeDPD) && P.kind == kPotKindDPD) {
     bool shifted = P.flags & kPotFlagShifted;
     float cutoff = shifted ? (P.b + ri + rj) : P.b;
-    if (r2 > cutoff * cutoff) return;
+    if (r2 > cutoff * cutoff) return false;
     float r = sqrtf(r2);
     const float ir = __fdividef(1.0f, r);
     float ex = dxx * ir, ey = dxy * ir, ez = dxz * ir;
-    float4 vq = A.v[q];
+    float4 vq = nb.v(q);
     float vx, vy, vz;
     if (p_is_i) {
       vx = vs.x - vq.x;
@@ -341,7 +384,7 @@ __device__ __forceinline__ void pair_force(const StepArgs &A, int ti, int tj, bo
     float fd = -P.dpd_gamma * omega_d * ev;
     float fr;
     if (A.dpd_noise) {
-      int qpid = A.pid[q];
+      int qpid = nb.pid(q);
       u32x4 ctr = {{(uint32_t)min(mypid, qpid), (uint32_t)max(mypid, qpid), t1, kStreamDPD}};
       u32x4 rr = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
       fr = P.dpd_sigma * omega_c * unit_uniform(rr.v[0]) * A.dpd_delta;
@@ -370,11 +413,11 @@ __device__ __forceinline__ void pair_force(const StepArgs &A, int ti, int tj, bo
       acc[10] += 0.5 * dxz * wy;
       acc[11] += 0.5 * dxz * wz;
     }
-    return;
+    return true;
   }
   if (MODE & kModeTable) {
     float e, f;
-    if (!table_eval_ex(P, A.coef, ri, rj, r2, e, f)) return;
+    if (!table_eval_ex(P, A.coef, ri, rj, r2, e, f)) return false;
     float wx = f * dxx, wy = f * dxy, wz = f * dxz;
     if (p_is_i) {
       fx -= wx;
@@ -398,6 +441,7 @@ __device__ __forceinline__ void pair_force(const StepArgs &A, int ti, int tj, bo
       acc[11] -= 0.5 * dxz * wz;
     }
   }
+  return true;
 }
 
 // Visits every in-cutoff neighbour of slot s in the reference orientation,
@@ -466,198 +510,505 @@ __device__ __forceinline__ void for_each_neighbour(const StepArgs &A, const DevR
   }
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kTile) void k_step(StepArgs A) {
-  __shared__ double red[kNRed][kTile / 64];
+constexpr int kScanBlock = 256, kScanItems = 4, kScanChunk = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total) {
+  // Inclusive scan over the block with wave shuffles, then exclusive.
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[wave] = x;
+  __syncthreads();
+  int wsum = 0, tot = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    if (w < wave) wsum += lds[w];
+    tot += lds[w];
+  }
+  __syncthreads();
+  total = tot;
+  return wsum + x - v;
+}
+
+// Single-body persistent random force (gaussian_force, MxForce.cpp:370-385),
+// refreshed when (time + id) % steps == 0.
+__device__ __forceinline__ void single_body(const StepArgs &A, int s, int mypid, const DevType &T, unsigned t1,
+                                            float &fx, float &fy, float &fz, float4 &pfs) {
+  pfs = A.pf[s];
+  if (!T.random) return;
+  if (((uint64_t)t1 + (uint64_t)mypid) % T.rnd_steps == 0) {
+    u32x4 ctr = {{(uint32_t)mypid, t1, kStreamSingle, 0u}};
+    u32x4 r = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
+    float g = gauss_from(r.v[0], r.v[1]);
+    float mag = T.rnd_mean + T.rnd_std * g;
+    float theta = 6.2831853f * u01_closed_open(r.v[2]);
+    float phi = acosf(1.0f - 2.0f * u01_closed_open(r.v[3]));
+    pfs.x = mag * sinf(phi) * cosf(theta);
+    pfs.y = mag * sinf(phi) * sinf(theta);
+    pfs.z = mag * cosf(phi);
+  }
+  fx += pfs.x;
+  fy += pfs.y;
+  fz += pfs.z;
+}
+
+// Bonds of one particle (bond_eval, bond.cpp:99-295), gathered per particle.
+// Both ends evaluate the bond in its (i, j) orientation; the i end keeps the
+// energy and writes the dissociation flag.
+template <int MODE>
+__device__ __forceinline__ void bond_forces(const StepArgs &A, const DevReplica &R, int s, int mypid, float &fx,
+                                            float &fy, float &fz, double *acc) {
+  const int b0 = A.boff[mypid], b1 = A.boff[mypid + 1];
+  for (int k = b0; k < b1; ++k) {
+    const int ent = A.bent[k];
+    const int bid = ent >> 1;
+    const bool p_is_i = (ent & 1) == 0;
+    if (!A.bact_in[bid]) {
+      if (p_is_i) A.bact_out[bid] = 0;
+      continue;
+    }
+    const int si = p_is_i ? s : A.slot_of[A.bi[bid]];
+    const int sj = p_is_i ? A.slot_of[A.bj[bid]] : s;
+    const float4 xi = A.x[si], xj = A.x[sj];
+    int lix, liy, liz, ljx, ljy, ljz;
+    cell_coords(A.cell[si] - R.cell_base, R, lix, liy, liz);
+    cell_coords(A.cell[sj] - R.cell_base, R, ljx, ljy, ljz);
+    int sh[3] = {lix - ljx, liy - ljy, liz - ljz};
+    for (int d = 0; d < 3; ++d) sh[d] = sh[d] > 1 ? -1 : (sh[d] < -1 ? 1 : sh[d]);
+    const float pix = (float)((double)xi.x + R.hd[0] * sh[0]);
+    const float piy = (float)((double)xi.y + R.hd[1] * sh[1]);
+    const float piz = (float)((double)xi.z + R.hd[2] * sh[2]);
+    const float dx = pix - xj.x, dy = piy - xj.y, dz = piz - xj.z;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    const DevPot P = A.bpots[A.bpot[bid]];
+    const float lo = P.a * P.a, hi = P.b * P.b;
+    if (r2 < lo || r2 > hi) r2 = fmaxf(lo, fminf(hi, r2));
+    float ee, eff;
+    table_eval(P, A.coef, r2, ee, eff);
+    if ((double)eff >= A.bdis[bid]) {
+      if (p_is_i) A.bact_out[bid] = 0;
+      continue;
+    }
+    if (p_is_i) A.bact_out[bid] = 1;
+    const float wx = eff * dx, wy = eff * dy, wz = eff * dz;
+    if (p_is_i) {
+      fx -= wx;
+      fy -= wy;
+      fz -= wz;
+      acc[1] += (double)ee;
+      if (MODE & kModeVirial) {
+        acc[3] -= dx * wx;
+        acc[4] -= dx * wy;
+        acc[5] -= dx * wz;
+        acc[6] -= dy * wx;
+        acc[7] -= dy * wy;
+        acc[8] -= dy * wz;
+        acc[9] -= dz * wx;
+        acc[10] -= dz * wy;
+        acc[11] -= dz * wz;
+      }
+    } else {
+      fx += wx;
+      fy += wy;
+      fz += wz;
+    }
+  }
+}
+
+// Advance of one particle (cell_advance_forward_euler,
+// engine_advance.cpp:126-239) and the store of its next-step state, the
+// migration flag and the per-cell counts.
+__device__ __forceinline__ void advance_store(const StepArgs &A, const DevReplica &R, int s, int c, int lx, int ly,
+                                              int lz, const float4 &xs, const float4 &vs, const DevType &T,
+                                              int passthrough, float fx, float fy, float fz, const float4 &pfs,
+                                              int mypid, float nd) {
+  const int pflags = (unsigned)__float_as_int(xs.w) >> 16;
+  float X[3] = {xs.x, xs.y, xs.z};
+  float V[3] = {vs.x, vs.y, vs.z};
+  const float F[3] = {fx, fy, fz};
+  int newc = c;
+  const bool all_frozen = (pflags & kFrozenX) && (pflags & kFrozenY) && (pflags & kFrozenZ);
+  if (!passthrough && !all_frozen) {
+    const float mask[3] = {(pflags & kFrozenX) ? 0.0f : 1.0f, (pflags & kFrozenY) ? 0.0f : 1.0f,
+                           (pflags & kFrozenZ) ? 0.0f : 1.0f};
+    int delta[3];
+    bool toofast = false;
+    if (T.dynamics == kDynNewtonian) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float vn = (float)((double)mask[k] * ((double)V[k] + (double)(A.dt * F[k]) * T.imass));
+        if (vn * vn <= R.maxv2[k]) {
+          V[k] = vn;
+        } else {
+          float neg = vn / fabsf(vn);
+          V[k] = neg * R.maxv[k];
+        }
+        X[k] += A.dt * V[k];
+        delta[k] = (X[k] >= R.h[k] ? 1 : 0) - (X[k] < 0.0f ? 1 : 0);
+        toofast = toofast || (X[k] >= R.h2[k] || X[k] <= -R.h[k]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float d = (float)((double)mask[k] * ((double)(A.dt * F[k]) * T.imass));
+        float st;
+        if (d * d <= R.maxx2[k]) {
+          st = d;
+        } else {
+          float neg = d / fabsf(d);
+          st = neg * R.maxx[k];
+        }
+        X[k] += st;
+        delta[k] = (X[k] >= R.h[k] ? 1 : 0) - (X[k] < 0.0f ? 1 : 0);
+        toofast = toofast || (X[k] >= R.h2[k] || X[k] <= -R.h[k]);
+      }
+    }
+    if (toofast) {
+      if (atomicCAS(&A.err[0], 0, (int)kErrToofast) == 0) A.err[1] = mypid;
+    }
+    if (delta[0] | delta[1] | delta[2]) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) X[k] -= delta[k] * R.h[k];
+      const int nx = (lx + delta[0] + R.cdim[0]) % R.cdim[0];
+      const int ny = (ly + delta[1] + R.cdim[1]) % R.cdim[1];
+      const int nz = (lz + delta[2] + R.cdim[2]) % R.cdim[2];
+      newc = R.cell_base + (nx * R.cdim[1] + ny) * R.cdim[2] + nz;
+    }
+  }
+  A.x_out[s] = make_float4(X[0], X[1], X[2], xs.w);
+  A.v_out[s] = make_float4(V[0], V[1], V[2], 0.0f);
+  A.cell_out[s] = newc != c ? (int)((unsigned)newc | 0x80000000u) : newc;
+  atomicAdd(&A.cnt[newc], 1);
+  if (newc != c) atomicAdd(&A.inc[newc], 1);
+  A.f_out[s] = make_float4(fx, fy, fz, nd);
+  if (A.random) A.pf_out[s] = pfs;
+}
+
+// Bricks of cells, the work unit of the force kernel: a block stages the
+// brick and a halo of span cells around it (positions, velocities, ids) in
+// LDS with coalesced row loads, then each thread owns one particle of the
+// brick.  A brick whose halo holds more than pmax particles is computed from
+// global memory instead.
+struct BrickArgs {
+  const int4 *__restrict__ bricks;  // (replica, x0, y0, z0)
+  int nbricks;
+  int bdim[3];  // brick size in cells
+  int pmax;     // LDS particle capacity
+  int cap;      // neighbour-list entries per thread
+  int hmax;     // halo-cell capacity
+  int *ovf_list;   // bricks whose halo overflowed the LDS capacity
+  int *ovf_count;
+};
+
+constexpr int kBT = 256;        // threads per brick block
+constexpr int kHaloAxis = 64;   // max halo cells per axis
+constexpr int kSpanMax = 2;     // max cell span
+constexpr int kOffMax = 125;    // (2 * kSpanMax + 1)^3
+
+__host__ __device__ inline size_t brick_lds_bytes(int pmax, int hmax, int cap, bool dpd) {
+  size_t b = 0;
+  b += (size_t)pmax * 16;              // positions
+  if (dpd) b += (size_t)pmax * 16;     // velocities
+  b += (size_t)pmax * 4;               // ids
+  b += (size_t)(hmax + 1) * 4 * 3;     // halo starts, global starts, home starts
+  b += 3 * kHaloAxis * (2 * kSpanMax + 1) * 4;  // shifts
+  b += (size_t)kOffMax * sizeof(DevOffset);
+  b += (size_t)cap * kBT * 4;          // neighbour lists
+  return b;
+}
+
+__device__ __forceinline__ int upper_index(const int *a, int n, int f) {
+  // largest i in [0, n) with a[i] <= f (a ascending, a[0] = 0)
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= f)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, BrickArgs B) {
+  const StepArgs &A = *Ap;
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr bool kDPD = (MODE & kModeDPD) != 0;
+  float4 *hx = reinterpret_cast<float4 *>(smem);
+  float4 *hv = hx + B.pmax;
+  int *hpid = reinterpret_cast<int *>(hv + (kDPD ? B.pmax : 0));
+  int *hstart = hpid + B.pmax;
+  int *hgb = hstart + (B.hmax + 1);
+  int *homestart = hgb + (B.hmax + 1);
+  float *hs = reinterpret_cast<float *>(homestart + (B.hmax + 1));
+  DevOffset *hoff = reinterpret_cast<DevOffset *>(hs + 3 * kHaloAxis * (2 * kSpanMax + 1));
+  unsigned *nlist = reinterpret_cast<unsigned *>(hoff + kOffMax);
+  __shared__ int scan_lds[kBT / 64];
+  __shared__ double red[kNRed][kBT / 64];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int passthrough = A.err[2];
   const unsigned t1 = *A.time + 1u;
 
-  for (int t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
-    const int4 tile = A.tiles[t];
-    const int s = tile.x + threadIdx.x;
+  for (int bi = blockIdx.x; bi < B.nbricks; bi += gridDim.x) {
+    const int4 br = B.bricks[bi];
+    const DevReplica &R = A.rep[br.x];
+    const int sp0 = R.span[0], sp1 = R.span[1], sp2 = R.span[2];
+    const int nb0 = min(B.bdim[0], R.cdim[0] - br.y);
+    const int nb1 = min(B.bdim[1], R.cdim[1] - br.z);
+    const int nb2 = min(B.bdim[2], R.cdim[2] - br.w);
+    const int hd0 = nb0 + 2 * sp0, hd1 = nb1 + 2 * sp1, hd2 = nb2 + 2 * sp2;
+    const int nhalo = hd0 * hd1 * hd2;
+    const int nhome = nb0 * nb1 * nb2;
+
+    // 1. Halo cell ranges and their exclusive scan (kScanItems per thread).
+    {
+      int cnt[kScanItems];
+      int local = 0;
+#pragma unroll
+      for (int i = 0; i < kScanItems; ++i) {
+        const int h = tid * kScanItems + i;
+        cnt[i] = 0;
+        if (h < nhalo) {
+          const int hi = h / (hd1 * hd2), hj = (h / hd2) % hd1, hk = h % hd2;
+          const int gx = wrap(br.y - sp0 + hi, R.cdim[0]);
+          const int gy = wrap(br.z - sp1 + hj, R.cdim[1]);
+          const int gz = wrap(br.w - sp2 + hk, R.cdim[2]);
+          const int c = R.cell_base + (gx * R.cdim[1] + gy) * R.cdim[2] + gz;
+          const int gb = A.cstart[c];
+          cnt[i] = A.cstart[c + 1] - gb;
+          hgb[h] = gb;
+        }
+        local += cnt[i];
+      }
+      int total;
+      int off = block_exclusive_scan(local, scan_lds, total);
+#pragma unroll
+      for (int i = 0; i < kScanItems; ++i) {
+        const int h = tid * kScanItems + i;
+        if (h < nhalo) hstart[h] = off;
+        off += cnt[i];
+      }
+      if (tid == 0) hstart[nhalo] = total;
+    }
+    // Shift table of the brick and the offset list.
+    for (int i = tid; i < hd0 * (2 * sp0 + 1); i += kBT) {
+      const int hl = i / (2 * sp0 + 1), d = i % (2 * sp0 + 1);
+      hs[(0 * kHaloAxis + hl) * (2 * kSpanMax + 1) + d] =
+          A.shifts[R.shift_base[0] + wrap(br.y - sp0 + hl, R.cdim[0]) * (2 * sp0 + 1) + d];
+    }
+    for (int i = tid; i < hd1 * (2 * sp1 + 1); i += kBT) {
+      const int hl = i / (2 * sp1 + 1), d = i % (2 * sp1 + 1);
+      hs[(1 * kHaloAxis + hl) * (2 * kSpanMax + 1) + d] =
+          A.shifts[R.shift_base[1] + wrap(br.z - sp1 + hl, R.cdim[1]) * (2 * sp1 + 1) + d];
+    }
+    for (int i = tid; i < hd2 * (2 * sp2 + 1); i += kBT) {
+      const int hl = i / (2 * sp2 + 1), d = i % (2 * sp2 + 1);
+      hs[(2 * kHaloAxis + hl) * (2 * kSpanMax + 1) + d] =
+          A.shifts[R.shift_base[2] + wrap(br.w - sp2 + hl, R.cdim[2]) * (2 * sp2 + 1) + d];
+    }
+    for (int i = tid; i < R.n_off; i += kBT) hoff[i] = A.offs[R.off_base + i];
+    __syncthreads();
+
+    // 2. Home cells: exclusive scan of their particle counts.
+    {
+      int cnt[kScanItems];
+      int local = 0;
+#pragma unroll
+      for (int i = 0; i < kScanItems; ++i) {
+        const int hc = tid * kScanItems + i;
+        cnt[i] = 0;
+        if (hc < nhome) {
+          const int a = hc / (nb1 * nb2), b = (hc / nb2) % nb1, cc = hc % nb2;
+          const int h = ((a + sp0) * hd1 + (b + sp1)) * hd2 + (cc + sp2);
+          cnt[i] = hstart[h + 1] - hstart[h];
+        }
+        local += cnt[i];
+      }
+      int total;
+      int off = block_exclusive_scan(local, scan_lds, total);
+#pragma unroll
+      for (int i = 0; i < kScanItems; ++i) {
+        const int hc = tid * kScanItems + i;
+        if (hc < nhome) homestart[hc] = off;
+        off += cnt[i];
+      }
+      if (tid == 0) homestart[nhome] = total;
+    }
+    const int P = hstart[nhalo];
+    if (P > B.pmax) {
+      // The halo does not fit: k_brick_global computes this brick.
+      if (tid == 0) B.ovf_list[atomicAdd(B.ovf_count, 1)] = bi;
+      __syncthreads();
+      continue;
+    }
+    // 3. Stage the halo particles (consecutive indices within a row of cells
+    // are consecutive slots, so the loads coalesce).
+    for (int f = tid; f < P; f += kBT) {
+      const int h = upper_index(hstart, nhalo + 1, f);
+      const int gs = hgb[h] + (f - hstart[h]);
+      hx[f] = A.x[gs];
+      if (kDPD) hv[f] = A.v[gs];
+      hpid[f] = A.pid[gs];
+    }
+    __syncthreads();
+    const int nhp = homestart[nhome];
+
     double acc[kNRed];
 #pragma unroll
     for (int k = 0; k < kNRed; ++k) acc[k] = 0.0;
 
-    if (s < tile.y) {
-      const DevReplica &R = A.rep[tile.z];
-      const float4 xs = A.x[s];
-      const float4 vs = A.v[s];
-      const int tw = __float_as_int(xs.w);
-      const int ti = tw & 0xffff;
-      const int pflags = (unsigned)tw >> 16;
-      const int c = A.cell[s];
-      const int mypid = A.pid[s];
-      int lx, ly, lz;
-      cell_coords(c - R.cell_base, R, lx, ly, lz);
+    for (int f = tid; f < nhp; f += kBT) {
+      const int hc = upper_index(homestart, nhome + 1, f);
+      const int kk = f - homestart[hc];
+      const int a = hc / (nb1 * nb2), b = (hc / nb2) % nb1, cc = hc % nb2;
+      const int ha = a + sp0, hb = b + sp1, hcz = cc + sp2;
+      const int h = (ha * hd1 + hb) * hd2 + hcz;
+      const int s = hgb[h] + kk;       // global slot
+      const int p = hstart[h] + kk;    // LDS index
+      const int lx = br.y + a, ly = br.z + b, lz = br.w + cc;
+      const int c = R.cell_base + (lx * R.cdim[1] + ly) * R.cdim[2] + lz;
+      const float4 xs = hx[p];
+      const float4 vs = kDPD ? hv[p] : A.v[s];
+      const int mypid = hpid[p];
+      const int ti = __float_as_int(xs.w) & 0xffff;
       const DevType T = A.types[ti];
-
-      // engine_kinetic_energy (engine.cpp:1698-1719): FP32 term, FP64 sum.
       {
         float e = 0.5 * T.mass * (vs.x * vs.x + vs.y * vs.y + vs.z * vs.z);
-        acc[2] = (double)e;
+        acc[2] += (double)e;
       }
-
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+      // space_prepare sets f[3] to the self density W(0, cutoff).
+      float nd = A.nd_norm;
       float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-
       if (!passthrough) {
-        // Single-body persistent random force (gaussian_force,
-        // MxForce.cpp:370-385), refreshed when (time + id) % steps == 0.
-        if (A.random) {
-          pfs = A.pf[s];
-          if (T.random) {
-            if (((uint64_t)t1 + (uint64_t)mypid) % T.rnd_steps == 0) {
-              u32x4 ctr = {{(uint32_t)mypid, t1, kStreamSingle, 0u}};
-              u32x4 r = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
-              float g = gauss_from(r.v[0], r.v[1]);
-              float mag = T.rnd_mean + T.rnd_std * g;
-              float theta = 6.2831853f * u01_closed_open(r.v[2]);
-              float phi = acosf(1.0f - 2.0f * u01_closed_open(r.v[3]));
-              pfs.x = mag * sinf(phi) * cosf(theta);
-              pfs.y = mag * sinf(phi) * sinf(theta);
-              pfs.z = mag * cosf(phi);
+        if (A.random) single_body(A, s, mypid, T, t1, fx, fy, fz, pfs);
+        {
+          const LdsNbr nb{hv, hpid};
+          const unsigned dmax = (unsigned)R.dmaxdist;
+          // Shift of the pair for offset o in the reference orientation.
+          auto shift_of = [&](const DevOffset &off, float &sx, float &sy, float &sz) {
+            constexpr int W = 2 * kSpanMax + 1;
+            if (off.side > 0) {
+              sx = hs[(0 * kHaloAxis + ha) * W + off.ox + sp0];
+              sy = hs[(1 * kHaloAxis + hb) * W + off.oy + sp1];
+              sz = hs[(2 * kHaloAxis + hcz) * W + off.oz + sp2];
+            } else {
+              sx = hs[(0 * kHaloAxis + ha + off.ox) * W - off.ox + sp0];
+              sy = hs[(1 * kHaloAxis + hb + off.oy) * W - off.oy + sp1];
+              sz = hs[(2 * kHaloAxis + hcz + off.oz) * W - off.oz + sp2];
             }
-            fx += pfs.x;
-            fy += pfs.y;
-            fz += pfs.z;
-          }
-        }
-
-        for_each_neighbour(A, R, s, lx, ly, lz, xs,
-                           [&](int q, bool p_is_i, bool self, float dx, float dy, float dz, float r2,
-                               const float4 &xq) {
-                             const int tj = __float_as_int(xq.w) & 0xffff;
-                             pair_force<MODE>(A, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, vs, fx,
-                                              fy, fz, acc, t1);
-                           });
-
-        // Bonds (bond_eval, bond.cpp:99-295), gathered per particle.  Both
-        // ends evaluate the bond in its (i, j) orientation; the i end keeps
-        // the energy and writes the dissociation flag.
-        if (MODE & kModeBonds) {
-          const int b0 = A.boff[mypid], b1 = A.boff[mypid + 1];
-          for (int k = b0; k < b1; ++k) {
-            const int ent = A.bent[k];
-            const int bid = ent >> 1;
-            const bool p_is_i = (ent & 1) == 0;
-            if (!A.bact_in[bid]) {
-              if (p_is_i) A.bact_out[bid] = 0;
-              continue;
-            }
-            const int si = p_is_i ? s : A.slot_of[A.bi[bid]];
-            const int sj = p_is_i ? A.slot_of[A.bj[bid]] : s;
-            const float4 xi = A.x[si], xj = A.x[sj];
-            int lix, liy, liz, ljx, ljy, ljz;
-            cell_coords(A.cell[si] - R.cell_base, R, lix, liy, liz);
-            cell_coords(A.cell[sj] - R.cell_base, R, ljx, ljy, ljz);
-            int sh[3] = {lix - ljx, liy - ljy, liz - ljz};
-            for (int d = 0; d < 3; ++d) sh[d] = sh[d] > 1 ? -1 : (sh[d] < -1 ? 1 : sh[d]);
-            const float pix = (float)((double)xi.x + R.hd[0] * sh[0]);
-            const float piy = (float)((double)xi.y + R.hd[1] * sh[1]);
-            const float piz = (float)((double)xi.z + R.hd[2] * sh[2]);
-            const float dx = pix - xj.x, dy = piy - xj.y, dz = piz - xj.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            const DevPot P = A.bpots[A.bpot[bid]];
-            const float lo = P.a * P.a, hi = P.b * P.b;
-            if (r2 < lo || r2 > hi) r2 = fmaxf(lo, fminf(hi, r2));
-            float ee, eff;
-            table_eval(P, A.coef, r2, ee, eff);
-            if ((double)eff >= A.bdis[bid]) {
-              if (p_is_i) A.bact_out[bid] = 0;
-              continue;
-            }
-            if (p_is_i) A.bact_out[bid] = 1;
-            const float wx = eff * dx, wy = eff * dy, wz = eff * dz;
-            if (p_is_i) {
-              fx -= wx;
-              fy -= wy;
-              fz -= wz;
-              acc[1] += (double)ee;
-              if (MODE & kModeVirial) {
-                acc[3] -= dx * wx;
-                acc[4] -= dx * wy;
-                acc[5] -= dx * wz;
-                acc[6] -= dy * wx;
-                acc[7] -= dy * wy;
-                acc[8] -= dy * wz;
-                acc[9] -= dz * wx;
-                acc[10] -= dz * wy;
-                acc[11] -= dz * wz;
+          };
+          auto process = [&](int n) {
+            for (int e = 0; e < n; ++e) {
+              const unsigned ent = nlist[e * kBT + tid];
+              const int q = (int)(ent >> 8);
+              const DevOffset off = hoff[ent & 255u];
+              const float4 xq = hx[q];
+              float dx, dy, dz;
+              if (off.side == 0) {
+                dx = xs.x - xq.x;
+                dy = xs.y - xq.y;
+                dz = xs.z - xq.z;
+              } else {
+                float sx, sy, sz;
+                shift_of(off, sx, sy, sz);
+                if (off.side > 0) {
+                  const float pix = xs.x - sx, piy = xs.y - sy, piz = xs.z - sz;
+                  dx = pix - xq.x;
+                  dy = piy - xq.y;
+                  dz = piz - xq.z;
+                } else {
+                  const float qix = xq.x - sx, qiy = xq.y - sy, qiz = xq.z - sz;
+                  dx = qix - xs.x;
+                  dy = qiy - xs.y;
+                  dz = qiz - xs.z;
+                }
               }
-            } else {
-              fx += wx;
-              fy += wy;
-              fz += wz;
+              const float r2 = dx * dx + dy * dy + dz * dz;
+              const int tj = __float_as_int(xq.w) & 0xffff;
+              if (pair_force<MODE>(A, nb, ti, tj, off.side >= 0, mypid, q, off.side == 0, dx, dy, dz, r2, vs, fx,
+                                   fy, fz, acc, t1))
+                nd += w_cubic(r2, A.cutoff, A.nd_norm);
             }
+          };
+          // Gather in-cutoff neighbours into the per-thread list, process
+          // the list when it is full or the scan is done (one call site, so
+          // the pair code is instantiated once).
+          int o = 0, q = -1;
+          for (;;) {
+            int nn = 0;
+            while (o < R.n_off) {
+              const DevOffset off = hoff[o];
+              const int h2 = h + (off.ox * hd1 + off.oy) * hd2 + off.oz;
+              const int qb = hstart[h2], qe = hstart[h2 + 1];
+              if (q < 0) q = qb;
+              bool full = false;
+              if (off.side == 0) {
+                for (; q < qe; ++q) {
+                  if (q == p) continue;
+                  const float4 xq = hx[q];
+                  const float dx = xs.x - xq.x, dy = xs.y - xq.y, dz = xs.z - xq.z;
+                  const float r2 = dx * dx + dy * dy + dz * dz;
+                  if (r2 > A.cut2_self) continue;
+                  nlist[nn * kBT + tid] = ((unsigned)q << 8) | (unsigned)o;
+                  if (++nn == B.cap) {
+                    ++q;
+                    full = true;
+                    break;
+                  }
+                }
+              } else {
+                float sx, sy, sz;
+                shift_of(off, sx, sy, sz);
+                const float nshift = sqrtf(sx * sx + sy * sy + sz * sz);
+                const unsigned dnshift = (unsigned)(int)(R.dscale * nshift);
+                const unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, off.nx, off.ny, off.nz);
+                const float pix = xs.x - sx, piy = xs.y - sy, piz = xs.z - sz;
+                for (; q < qe; ++q) {
+                  const float4 xq = hx[q];
+                  float dx, dy, dz;
+                  if (off.side > 0) {
+                    dx = pix - xq.x;
+                    dy = piy - xq.y;
+                    dz = piz - xq.z;
+                  } else {
+                    const float qix = xq.x - sx, qiy = xq.y - sy, qiz = xq.z - sz;
+                    dx = qix - xs.x;
+                    dy = qiy - xs.y;
+                    dz = qiz - xs.z;
+                  }
+                  const float r2 = dx * dx + dy * dy + dz * dz;
+                  if (r2 > A.cut2_pair) continue;
+                  const unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, off.nx, off.ny, off.nz);
+                  const bool pass = off.side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax);
+                  if (!pass) continue;
+                  nlist[nn * kBT + tid] = ((unsigned)q << 8) | (unsigned)o;
+                  if (++nn == B.cap) {
+                    ++q;
+                    full = true;
+                    break;
+                  }
+                }
+              }
+              if (full) break;
+              ++o;
+              q = -1;
+            }
+            process(nn);
+            if (o >= R.n_off) break;
           }
         }
+        if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
       }
-
-      // Advance (cell_advance_forward_euler, engine_advance.cpp:126-239).
-      float X[3] = {xs.x, xs.y, xs.z};
-      float V[3] = {vs.x, vs.y, vs.z};
-      const float F[3] = {fx, fy, fz};
-      int newc = c;
-      const bool all_frozen = (pflags & kFrozenX) && (pflags & kFrozenY) && (pflags & kFrozenZ);
-      if (!passthrough && !all_frozen) {
-        const float mask[3] = {(pflags & kFrozenX) ? 0.0f : 1.0f, (pflags & kFrozenY) ? 0.0f : 1.0f,
-                               (pflags & kFrozenZ) ? 0.0f : 1.0f};
-        int delta[3];
-        bool toofast = false;
-        if (T.dynamics == kDynNewtonian) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            float vn = (float)((double)mask[k] * ((double)V[k] + (double)(A.dt * F[k]) * T.imass));
-            if (vn * vn <= R.maxv2[k]) {
-              V[k] = vn;
-            } else {
-              float neg = vn / fabsf(vn);
-              V[k] = neg * R.maxv[k];
-            }
-            X[k] += A.dt * V[k];
-            delta[k] = (X[k] >= R.h[k] ? 1 : 0) - (X[k] < 0.0f ? 1 : 0);
-            toofast = toofast || (X[k] >= R.h2[k] || X[k] <= -R.h[k]);
-          }
-        } else {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            float d = (float)((double)mask[k] * ((double)(A.dt * F[k]) * T.imass));
-            float step;
-            if (d * d <= R.maxx2[k]) {
-              step = d;
-            } else {
-              float neg = d / fabsf(d);
-              step = neg * R.maxx[k];
-            }
-            X[k] += step;
-            delta[k] = (X[k] >= R.h[k] ? 1 : 0) - (X[k] < 0.0f ? 1 : 0);
-            toofast = toofast || (X[k] >= R.h2[k] || X[k] <= -R.h[k]);
-          }
-        }
-        if (toofast) {
-          if (atomicCAS(&A.err[0], 0, (int)kErrToofast) == 0) A.err[1] = mypid;
-        }
-        if (delta[0] | delta[1] | delta[2]) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) X[k] -= delta[k] * R.h[k];
-          const int nx = (lx + delta[0] + R.cdim[0]) % R.cdim[0];
-          const int ny = (ly + delta[1] + R.cdim[1]) % R.cdim[1];
-          const int nz = (lz + delta[2] + R.cdim[2]) % R.cdim[2];
-          newc = R.cell_base + (nx * R.cdim[1] + ny) * R.cdim[2] + nz;
-        }
-      }
-      A.x_out[s] = make_float4(X[0], X[1], X[2], xs.w);
-      A.v_out[s] = make_float4(V[0], V[1], V[2], 0.0f);
-      A.cell_out[s] = newc != c ? (int)((unsigned)newc | 0x80000000u) : newc;
-      atomicAdd(&A.cnt[newc], 1);
-      if (newc != c) atomicAdd(&A.inc[newc], 1);
-      A.f_out[s] = make_float4(fx, fy, fz, 0.0f);
-      if (A.random) A.pf_out[s] = pfs;
+      advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd);
     }
 
-    // Deterministic block reduction of the FP64 partials for this tile.
+    // Deterministic block reduction of the FP64 partials for this brick.
 #pragma unroll
     for (int k = 0; k < kNRed; ++k) {
       if (!(MODE & kModeVirial) && k >= 3) break;
@@ -665,10 +1016,11 @@ __global__ __launch_bounds__(kTile) void k_step(StepArgs A) {
       if (lane == 0) red[k][wave] = w;
     }
     __syncthreads();
-    if (threadIdx.x < kNRed) {
+    if (tid < kNRed) {
       double sum = 0.0;
-      for (int w = 0; w < kTile / 64; ++w) sum += ((MODE & kModeVirial) || threadIdx.x < 3) ? red[threadIdx.x][w] : 0.0;
-      A.partials[(size_t)t * kNRed + threadIdx.x] = sum;
+      if ((MODE & kModeVirial) || tid < 3)
+        for (int w = 0; w < kBT / 64; ++w) sum += red[tid][w];
+      A.partials[(size_t)bi * kNRed + tid] = sum;
     }
     __syncthreads();
   }
@@ -713,28 +1065,6 @@ struct RebuildArgs {
   int with_pf;
 };
 
-constexpr int kScanBlock = 256, kScanItems = 4, kScanChunk = kScanBlock * kScanItems;
-
-__device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total) {
-  // Inclusive scan over the block with wave shuffles, then exclusive.
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int x = v;
-  for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) lds[wave] = x;
-  __syncthreads();
-  int wsum = 0, tot = 0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-    if (w < wave) wsum += lds[w];
-    tot += lds[w];
-  }
-  __syncthreads();
-  total = tot;
-  return wsum + x - v;
-}
-
 // Exclusive scan of the per-cell counts (first pass); the counts are reset
 // to zero for the next step once read.
 __global__ __launch_bounds__(kScanBlock) void k_scan_block(int *__restrict__ cnt, int *__restrict__ out,
@@ -765,7 +1095,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_block(int *__restrict__ cnt
 // step was skipped because of an earlier error, and records the error state:
 // err[3] = this step was skipped, err[2] = error state for the next step.
 __global__ __launch_bounds__(kScanBlock) void k_scan_top(int *__restrict__ bsum, int nb, int *__restrict__ out,
-                                                        int n, unsigned *time, int *err) {
+                                                        int n, unsigned *time, int *err, int *ovf_count) {
   __shared__ int lds[kScanBlock / 64];
   int carry = 0;
   for (int base = 0; base < nb; base += kScanBlock) {
@@ -779,6 +1109,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_top(int *__restrict__ bsum,
   }
   if (threadIdx.x == 0) {
     out[n] = carry;
+    *ovf_count = 0;
     const int skipped = err[2];
     err[3] = skipped;
     if (!skipped) *time += 1u;
@@ -880,12 +1211,116 @@ __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ par
   }
 }
 
+// Bricks whose halo overflowed LDS: the same computation with neighbours
+// read from global memory, one thread per particle of the brick.
 template <int MODE>
-void launch_step_mode(const StepArgs &A, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_step<MODE>, dim3(grid), dim3(kTile), 0, st, A);
+__global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict__ Ap, BrickArgs B) {
+  const StepArgs &A = *Ap;
+  __shared__ int scan_lds[kBT / 64];
+  __shared__ int homestart[kScanItems * kBT + 1];
+  __shared__ int homegb[kScanItems * kBT];
+  __shared__ double red[kNRed][kBT / 64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int passthrough = A.err[2];
+  const unsigned t1 = *A.time + 1u;
+  const int nov = *B.ovf_count;
+  for (int i = blockIdx.x; i < nov; i += gridDim.x) {
+    const int bi = B.ovf_list[i];
+    const int4 br = B.bricks[bi];
+    const DevReplica &R = A.rep[br.x];
+    const int nb0 = min(B.bdim[0], R.cdim[0] - br.y);
+    const int nb1 = min(B.bdim[1], R.cdim[1] - br.z);
+    const int nb2 = min(B.bdim[2], R.cdim[2] - br.w);
+    const int nhome = nb0 * nb1 * nb2;
+    {
+      int cnt[kScanItems];
+      int local = 0;
+#pragma unroll
+      for (int k = 0; k < kScanItems; ++k) {
+        const int hc = tid * kScanItems + k;
+        cnt[k] = 0;
+        if (hc < nhome) {
+          const int a = hc / (nb1 * nb2), b = (hc / nb2) % nb1, cc = hc % nb2;
+          const int c = R.cell_base + ((br.y + a) * R.cdim[1] + (br.z + b)) * R.cdim[2] + (br.w + cc);
+          homegb[hc] = A.cstart[c];
+          cnt[k] = A.cstart[c + 1] - A.cstart[c];
+        }
+        local += cnt[k];
+      }
+      int total;
+      int off = block_exclusive_scan(local, scan_lds, total);
+#pragma unroll
+      for (int k = 0; k < kScanItems; ++k) {
+        const int hc = tid * kScanItems + k;
+        if (hc < nhome) homestart[hc] = off;
+        off += cnt[k];
+      }
+      if (tid == 0) homestart[nhome] = total;
+    }
+    __syncthreads();
+    const int nhp = homestart[nhome];
+    double acc[kNRed];
+#pragma unroll
+    for (int k = 0; k < kNRed; ++k) acc[k] = 0.0;
+    for (int f = tid; f < nhp; f += kBT) {
+      const int hc = upper_index(homestart, nhome + 1, f);
+      const int s = homegb[hc] + (f - homestart[hc]);
+      const int a = hc / (nb1 * nb2), b = (hc / nb2) % nb1, cc = hc % nb2;
+      const int lx = br.y + a, ly = br.z + b, lz = br.w + cc;
+      const int c = R.cell_base + (lx * R.cdim[1] + ly) * R.cdim[2] + lz;
+      const float4 xs = A.x[s];
+      const float4 vs = A.v[s];
+      const int mypid = A.pid[s];
+      const int ti = __float_as_int(xs.w) & 0xffff;
+      const DevType T = A.types[ti];
+      {
+        float e = 0.5 * T.mass * (vs.x * vs.x + vs.y * vs.y + vs.z * vs.z);
+        acc[2] += (double)e;
+      }
+      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+      // space_prepare sets f[3] to the self density W(0, cutoff).
+      float nd = A.nd_norm;
+      float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (!passthrough) {
+        if (A.random) single_body(A, s, mypid, T, t1, fx, fy, fz, pfs);
+        const GlobalNbr nb{A.v, A.pid};
+        for_each_neighbour(A, R, s, lx, ly, lz, xs,
+                           [&](int q, bool p_is_i, bool self, float dx, float dy, float dz, float r2,
+                               const float4 &xq) {
+                             const int tj = __float_as_int(xq.w) & 0xffff;
+                             if (pair_force<MODE>(A, nb, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, vs, fx,
+                                                  fy, fz, acc, t1))
+                               nd += w_cubic(r2, A.cutoff, A.nd_norm);
+                           });
+        if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
+      }
+      advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd);
+    }
+#pragma unroll
+    for (int k = 0; k < kNRed; ++k) {
+      if (!(MODE & kModeVirial) && k >= 3) break;
+      double w = wave_sum(acc[k]);
+      if (lane == 0) red[k][wave] = w;
+    }
+    __syncthreads();
+    if (tid < kNRed) {
+      double sum = 0.0;
+      if ((MODE & kModeVirial) || tid < 3)
+        for (int w = 0; w < kBT / 64; ++w) sum += red[tid][w];
+      A.partials[(size_t)bi * kNRed + tid] = sum;
+    }
+    __syncthreads();
+  }
 }
 
-using StepLauncher = void (*)(const StepArgs &, int, hipStream_t);
+template <int MODE>
+void launch_step_mode(const StepArgs *A, const BrickArgs &B, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL(k_brick<MODE>, dim3(B.nbricks), dim3(kBT), lds, st, A, B);
+  hipLaunchKernelGGL(k_brick_global<MODE>, dim3(64), dim3(kBT), 0, st, A, B);
+}
+
+using StepLauncher = void (*)(const StepArgs *, const BrickArgs &, size_t, hipStream_t);
 
 // One instantiation per combination of table / DPD / virial / bonds.
 const StepLauncher kStepLaunchers[16] = {
@@ -1141,22 +1576,26 @@ int Engine::set_random_force(int type, float mean, float std, float duration) {
   t.rnd_steps = steps < 1 ? 1u : (unsigned)steps;
   any_random_ = true;
   potentials_dirty_ = true;
+  args_dirty_ = true;
   return kErrOk;
 }
 
 int Engine::set_seed(uint64_t seed) {
   seed_ = seed;
+  args_dirty_ = true;
   return kErrOk;
 }
 
 int Engine::set_dpd_counter_noise(int on) {
   dpd_noise_ = on ? 1 : 0;
+  args_dirty_ = true;
   return kErrOk;
 }
 
 int Engine::set_max_dist_fraction(float f) {
   if (!(f > 0)) return fail(kErrRange, "max distance fraction must be positive");
   max_frac_ = f;
+  args_dirty_ = true;
   if (d_) return build_geometry();
   return kErrOk;
 }
@@ -1228,6 +1667,7 @@ int Engine::upload_potentials() {
   rc |= hipStreamSynchronize(d->stream);
   if (rc) return check_device(rc, "potential upload");
   potentials_dirty_ = false;
+  args_dirty_ = true;
   return kErrOk;
 }
 
@@ -1297,6 +1737,9 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   }
   N_ = N;
   ntiles_ = (int)tiles.size();
+  mean_cell_count_.assign(reps_.size(), 0.0);
+  for (size_t r = 0; r < reps_.size(); ++r)
+    mean_cell_count_[r] = reps_[r].ncells ? (double)reps_[r].nparts / reps_[r].ncells : 0.0;
 
   DeviceState *d = d_;
   // (Re)allocate particle buffers.
@@ -1311,7 +1754,6 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   if (int st = dalloc(d, &d->slot_of, N)) return check_device(st, "hipMalloc");
   if (int st = dalloc(d, &d->ncount, N)) return check_device(st, "hipMalloc");
   if (int st = dalloc(d, &d->tiles, tiles.size())) return check_device(st, "hipMalloc");
-  if (int st = dalloc(d, &d->partials, tiles.size() * kNRed)) return check_device(st, "hipMalloc");
   if (int st = dalloc(d, &d->rep_tiles, rep_tiles.size())) return check_device(st, "hipMalloc");
   hipStream_t s = d->stream;
   int rc = 0;
@@ -1327,10 +1769,111 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   rc |= hipMemcpyAsync(d->slot_of, slot_of.data(), N * sizeof(int), hipMemcpyHostToDevice, s);
   rc |= hipMemcpyAsync(d->tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s);
   rc |= hipMemcpyAsync(d->rep_tiles, rep_tiles.data(), rep_tiles.size() * sizeof(int2), hipMemcpyHostToDevice, s);
-  rc |= hipMemsetAsync(d->partials, 0, tiles.size() * kNRed * sizeof(double), s);
   rc |= hipStreamSynchronize(s);
   if (rc) return check_device(rc, "particle upload");
   pid_prev_valid_ = false;
+  parity_ = 0;
+  args_dirty_ = true;
+  return set_brick(0, 0, 0, 0);
+}
+
+// Capacity of the LDS particle buffer for a brick shape: room for the
+// expected halo population plus margin, within the LDS budget.
+static int brick_pmax(const std::vector<HostReplica> &reps, const std::vector<double> &mean, const int bd[3],
+                      int cap, bool dpd, int &hmax, size_t budget) {
+  hmax = 0;
+  double worst = 0;
+  for (size_t r = 0; r < reps.size(); ++r) {
+    int hc = 1;
+    for (int k = 0; k < 3; ++k) hc *= std::min(bd[k], reps[r].cdim[k]) + 2 * reps[r].span[k];
+    hmax = std::max(hmax, hc);
+    worst = std::max(worst, mean[r] * hc);
+  }
+  int want = (int)std::ceil(1.35 * worst + 6.0 * std::sqrt(worst) + 64.0);
+  want = (want + 63) / 64 * 64;
+  int fit = 0;
+  for (int pm = 64; pm <= 16384; pm += 64)
+    if (brick_lds_bytes(pm, hmax, cap, dpd) <= budget) fit = pm;
+  return std::min(want, fit);
+}
+
+// Chooses (or sets) the brick shape of the force kernel and builds the brick
+// list.  bx = 0 picks the largest shape whose expected halo fits in LDS.
+int Engine::set_brick(int bx, int by, int bz, int cap) {
+  if (!d_ || N_ == 0) return fail(kErrNull, "no particles loaded");
+  const size_t budget = 65536;
+  const bool dpd = any_dpd_ || std::any_of(pots_.begin(), pots_.end(), [](const Potential &p) {
+                     return p.kind == kPotKindDPD;
+                   });
+  int c = cap > 0 ? cap : cap_;
+  int bd[3] = {bx, by, bz};
+  int hmax = 0, pm = 0;
+  for (const auto &r : reps_)
+    for (int k = 0; k < 3; ++k)
+      if (r.span[k] > kSpanMax) return fail(kErrCutoff, "cutoff spans more than 2 cells");
+  if (bx <= 0) {
+    static const int shapes[][3] = {{4, 4, 4}, {2, 4, 4}, {2, 2, 4}, {2, 2, 2}, {1, 2, 2}, {1, 1, 2}, {1, 1, 1}};
+    bool ok = false;
+    for (const auto &sh : shapes) {
+      for (int k = 0; k < 3; ++k) bd[k] = sh[k];
+      pm = brick_pmax(reps_, mean_cell_count_, bd, c, dpd, hmax, budget);
+      double worst = 0;
+      for (size_t r = 0; r < reps_.size(); ++r) {
+        int hc = 1;
+        for (int k = 0; k < 3; ++k) hc *= std::min(bd[k], reps_[r].cdim[k]) + 2 * reps_[r].span[k];
+        worst = std::max(worst, mean_cell_count_[r] * hc);
+      }
+      if (pm >= 1.2 * worst && hmax <= kScanItems * kBT) {
+        ok = true;
+        break;
+      }
+    }
+    if (!ok) {
+      for (int k = 0; k < 3; ++k) bd[k] = 1;
+      pm = brick_pmax(reps_, mean_cell_count_, bd, c, dpd, hmax, budget);
+    }
+  } else {
+    if (bx < 1 || by < 1 || bz < 1) return fail(kErrRange, "brick dimensions must be positive");
+    pm = brick_pmax(reps_, mean_cell_count_, bd, c, dpd, hmax, budget);
+  }
+  for (const auto &r : reps_)
+    for (int k = 0; k < 3; ++k)
+      if (std::min(bd[k], r.cdim[k]) + 2 * r.span[k] > kHaloAxis) return fail(kErrRange, "brick too large");
+  if (hmax > kScanItems * kBT) return fail(kErrRange, "brick halo has more than 1024 cells");
+  if (pm < 64) return fail(kErrRange, "brick does not fit in LDS");
+  std::vector<int4> bricks;
+  std::vector<int2> rep_bricks(reps_.size());
+  for (size_t r = 0; r < reps_.size(); ++r) {
+    const HostReplica &R = reps_[r];
+    rep_bricks[r].x = (int)bricks.size();
+    for (int x = 0; x < R.cdim[0]; x += bd[0])
+      for (int y = 0; y < R.cdim[1]; y += bd[1])
+        for (int z = 0; z < R.cdim[2]; z += bd[2]) bricks.push_back(make_int4((int)r, x, y, z));
+    rep_bricks[r].y = (int)bricks.size();
+  }
+  DeviceState *d = d_;
+  if (int st = dalloc(d, &d->bricks, bricks.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->rep_bricks, rep_bricks.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->partials, bricks.size() * kNRed)) return check_device(st, "hipMalloc");
+  int rc = 0;
+  rc |= hipMemcpyAsync(d->bricks, bricks.data(), bricks.size() * sizeof(int4), hipMemcpyHostToDevice, d->stream);
+  rc |= hipMemcpyAsync(d->rep_bricks, rep_bricks.data(), rep_bricks.size() * sizeof(int2), hipMemcpyHostToDevice,
+                       d->stream);
+  rc |= hipMemsetAsync(d->partials, 0, bricks.size() * kNRed * sizeof(double), d->stream);
+  rc |= hipStreamSynchronize(d->stream);
+  if (rc) return check_device(rc, "brick upload");
+  for (int k = 0; k < 3; ++k) bdim_[k] = bd[k];
+  cap_ = c;
+  pmax_ = pm;
+  hmax_ = hmax;
+  nbricks_ = (int)bricks.size();
+  brick_dpd_ = dpd;
+  if (int st = dalloc(d, &d->ovf_list, bricks.size())) return check_device(st, "hipMalloc");
+  if (!d->ovf_count) {
+    if (int st = dalloc(d, &d->ovf_count, 1)) return check_device(st, "hipMalloc");
+    if (int st = hipMemset(d->ovf_count, 0, sizeof(int))) return check_device(st, "hipMemset");
+  }
+  args_dirty_ = true;
   return kErrOk;
 }
 
@@ -1398,27 +1941,29 @@ int Engine::set_bonds(int M, const int *bi, const int *bj, const Potential *cons
   rc |= hipStreamSynchronize(s);
   if (rc) return check_device(rc, "bond upload");
   M_ = M;
-  bact_cur_ = 0;
+  // Both flag buffers start active; the current one follows the parity.
   potentials_dirty_ = true;
+  args_dirty_ = true;
   return kErrOk;
 }
 
-int Engine::launch_step() {
-  DeviceState *d = d_;
-  const int a = 0, b = 1;  // buffer 0 is the step's input, 1 its output
-  StepArgs A;
+// Kernel arguments of the force kernel for the step parity par (the
+// ping-pong buffers of ids, cell offsets and bond flags alternate).
+void Engine::fill_step_args(void *out, int par) const {
+  const DeviceState *d = d_;
+  StepArgs &A = *reinterpret_cast<StepArgs *>(out);
   std::memset(&A, 0, sizeof(A));
-  A.x = d->x[a];
-  A.v = d->v[a];
-  A.pf = d->pf[a];
-  A.pid = d->pid[a];
-  A.cell = d->cell[a];
-  A.cstart = d->cstart[a];
-  A.x_out = d->x[b];
-  A.v_out = d->v[b];
-  A.pf_out = d->pf[b];
+  A.x = d->x[0];
+  A.v = d->v[0];
+  A.pf = d->pf[0];
+  A.pid = d->pid[par];
+  A.cell = d->cell[0];
+  A.cstart = d->cstart[par];
+  A.x_out = d->x[1];
+  A.v_out = d->v[1];
+  A.pf_out = d->pf[1];
   A.f_out = d->f;
-  A.cell_out = d->cell[b];
+  A.cell_out = d->cell[1];
   A.rep = d->rep;
   A.offs = d->offs;
   A.shifts = d->shifts;
@@ -1436,8 +1981,8 @@ int Engine::launch_step() {
   A.bj = d->bj;
   A.bpot = d->bpot;
   A.bdis = d->bdis;
-  A.bact_in = M_ > 0 ? d->bact[bact_cur_] : nullptr;
-  A.bact_out = M_ > 0 ? d->bact[bact_cur_ ^ 1] : nullptr;
+  A.bact_in = M_ > 0 ? d->bact[par] : nullptr;
+  A.bact_out = M_ > 0 ? d->bact[par ^ 1] : nullptr;
   A.slot_of = d->slot_of;
   A.dt = (float)dt_;
   const float cf = (float)cutoff_;
@@ -1453,56 +1998,86 @@ int Engine::launch_step() {
   A.partials = d->partials;
   A.cnt = d->cnt;
   A.inc = d->inc;
+  A.ncount = d->ncount;
+  A.random = any_random_ ? 1 : 0;
+  A.cutoff = (float)cutoff_;
+  A.nd_norm = (float)(1.0 / (M_PI * cutoff_ * cutoff_ * cutoff_));
+}
 
+// Uploads both parities of the force-kernel arguments (device resident, so
+// the kernel reads them through one pointer).
+int Engine::refresh_args() {
+  DeviceState *d = d_;
+  if (!d->dargs)
+    if (int st = dalloc(d, &d->dargs, 2 * sizeof(StepArgs))) return check_device(st, "hipMalloc");
+  StepArgs h[2];
+  fill_step_args(&h[0], 0);
+  fill_step_args(&h[1], 1);
+  int rc = hipMemcpyAsync(d->dargs, h, sizeof(h), hipMemcpyHostToDevice, d->stream);
+  rc |= hipStreamSynchronize(d->stream);
+  if (rc) return check_device(rc, "argument upload");
+  args_dirty_ = false;
+  return kErrOk;
+}
+
+int Engine::launch_step() {
+  DeviceState *d = d_;
+  const int par = parity_;
   int mode = 0;
   if (any_table_) mode |= kModeTable;
   if (any_dpd_) mode |= kModeDPD;
   if (virial_) mode |= kModeVirial;
   if (M_ > 0) mode |= kModeBonds;
-  A.random = any_random_ ? 1 : 0;
   hipStream_t s = d->stream;
   const bool rec = timing_ && timed_steps_ < (int)d->kev.size() / 2;
   if (rec) hipEventRecord(d->kev[2 * timed_steps_], s);
-  kStepLaunchers[mode](A, ntiles_, s);
+  BrickArgs Bk;
+  Bk.bricks = d->bricks;
+  Bk.nbricks = nbricks_;
+  for (int k = 0; k < 3; ++k) Bk.bdim[k] = bdim_[k];
+  Bk.pmax = pmax_;
+  Bk.cap = cap_;
+  Bk.hmax = hmax_;
+  Bk.ovf_list = d->ovf_list;
+  Bk.ovf_count = d->ovf_count;
+  const size_t lds = brick_lds_bytes(pmax_, hmax_, cap_, (mode & kModeDPD) != 0);
+  kStepLaunchers[mode](reinterpret_cast<const StepArgs *>(d->dargs) + par, Bk, lds, s);
   if (rec) hipEventRecord(d->kev[2 * timed_steps_ + 1], s);
   if (rec) ++timed_steps_;
 
+  // Cell-list rebuild: x, v, pf, cell go from buffer 1 back to buffer 0;
+  // ids and cell offsets go from parity par to par ^ 1.
   RebuildArgs B;
-  B.cstart = d->cstart[a];
-  B.cstart_new = d->cstart[b];
-  B.cell_out = d->cell[b];
+  B.cstart = d->cstart[par];
+  B.cstart_new = d->cstart[par ^ 1];
+  B.cell_out = d->cell[1];
   B.inc = d->inc;
   B.cursor = d->cursor;
   B.nslots = N_;
   B.ncells = ncells_;
-  B.x_in = d->x[b];
-  B.v_in = d->v[b];
-  B.pf_in = d->pf[b];
-  B.pid_in = d->pid[a];
-  B.x = d->x[a];
-  B.v = d->v[a];
-  B.pf = d->pf[a];
-  B.pid = d->pid[b];
-  B.cell = d->cell[a];
+  B.x_in = d->x[1];
+  B.v_in = d->v[1];
+  B.pf_in = d->pf[1];
+  B.pid_in = d->pid[par];
+  B.x = d->x[0];
+  B.v = d->v[0];
+  B.pf = d->pf[0];
+  B.pid = d->pid[par ^ 1];
+  B.cell = d->cell[0];
   B.slot_of = d->slot_of;
   B.with_pf = any_random_ ? 1 : 0;
   const int cg = (ncells_ + 255) / 256;
   const int pg = (N_ + 255) / 256;
   const int nb = (ncells_ + kScanChunk - 1) / kScanChunk;
-  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, d->cnt, d->cstart[b], d->bsum, ncells_);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, d->bsum, nb, d->cstart[b], ncells_, d->time,
-                     d->err);
-  hipLaunchKernelGGL(k_scan_add, dim3(cg), dim3(256), 0, s, d->cstart[b], d->bsum, ncells_);
+  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, d->cnt, d->cstart[par ^ 1], d->bsum, ncells_);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, d->bsum, nb, d->cstart[par ^ 1], ncells_,
+                     d->time, d->err, d->ovf_count);
+  hipLaunchKernelGGL(k_scan_add, dim3(cg), dim3(256), 0, s, d->cstart[par ^ 1], d->bsum, ncells_);
   hipLaunchKernelGGL(k_place, dim3(pg), dim3(256), 0, s, B);
   hipLaunchKernelGGL(k_order_incoming, dim3(cg), dim3(256), 0, s, B);
-  hipLaunchKernelGGL(k_finalize, dim3(reps_.size()), dim3(256), 0, s, d->partials, d->rep_tiles,
+  hipLaunchKernelGGL(k_finalize, dim3(reps_.size()), dim3(256), 0, s, d->partials, d->rep_bricks,
                      (int)reps_.size(), d->obs, d->err);
-  // The placement wrote the new order into buffer a, except pid and cstart
-  // which went to buffer b.
-  std::swap(d->pid[0], d->pid[1]);
-  std::swap(d->cstart[0], d->cstart[1]);
-  // cell[a] holds the new cells; x/v/pf[a] hold the new state.
-  if (M_ > 0) bact_cur_ ^= 1;
+  parity_ ^= 1;
   pid_prev_valid_ = true;
   return (int)hipGetLastError();
 }
@@ -1512,6 +2087,12 @@ int Engine::step(int nsteps) {
   if (err_ != 0) return err_;
   if (potentials_dirty_)
     if (int rc = upload_potentials()) return rc;
+  // Potentials bound after the load can change the LDS layout (velocities
+  // are staged for DPD only).
+  if (brick_dpd_ != any_dpd_)
+    if (int rc = set_brick(bdim_[0], bdim_[1], bdim_[2], cap_)) return rc;
+  if (args_dirty_)
+    if (int rc = refresh_args()) return rc;
   if (timing_) {
     // Two events per step, for up to 4096 steps per call.
     const size_t want = 2 * (size_t)std::min(nsteps, 4096);
@@ -1567,7 +2148,7 @@ int Engine::set_virial(int on) {
   return kErrOk;
 }
 
-int Engine::get_state(float *xlocal, int *cell, float *v, float *f, float *pf) {
+int Engine::get_state(float *xlocal, int *cell, float *v, float *f, float *pf, float *nd) {
   if (!d_ || N_ == 0) return fail(kErrNull, "no particles loaded");
   DeviceState *d = d_;
   std::vector<float4> hx(N_), hv(N_), hf(N_), hpf(N_);
@@ -1578,8 +2159,8 @@ int Engine::get_state(float *xlocal, int *cell, float *v, float *f, float *pf) {
   rc |= hipMemcpyAsync(hv.data(), d->v[0], N_ * sizeof(float4), hipMemcpyDeviceToHost, s);
   rc |= hipMemcpyAsync(hf.data(), d->f, N_ * sizeof(float4), hipMemcpyDeviceToHost, s);
   rc |= hipMemcpyAsync(hpf.data(), d->pf[0], N_ * sizeof(float4), hipMemcpyDeviceToHost, s);
-  rc |= hipMemcpyAsync(hpid.data(), d->pid[0], N_ * sizeof(int), hipMemcpyDeviceToHost, s);
-  rc |= hipMemcpyAsync(hpidf.data(), d->pid[1], N_ * sizeof(int), hipMemcpyDeviceToHost, s);
+  rc |= hipMemcpyAsync(hpid.data(), d->pid[parity_], N_ * sizeof(int), hipMemcpyDeviceToHost, s);
+  rc |= hipMemcpyAsync(hpidf.data(), d->pid[parity_ ^ 1], N_ * sizeof(int), hipMemcpyDeviceToHost, s);
   rc |= hipMemcpyAsync(hcell.data(), d->cell[0], N_ * sizeof(int), hipMemcpyDeviceToHost, s);
   rc |= hipStreamSynchronize(s);
   if (rc) return check_device(rc, "state download");
@@ -1602,14 +2183,18 @@ int Engine::get_state(float *xlocal, int *cell, float *v, float *f, float *pf) {
     }
     if (cell) cell[i] = hcell[k];
   }
-  if (f) {
-    // Forces are stored in the slot order of the step that computed them.
+  if (f || nd) {
+    // Forces (and the number density in w) are stored in the slot order of
+    // the step that computed them.
     const std::vector<int> &order = pid_prev_valid_ ? hpidf : hpid;
     for (int k = 0; k < N_; ++k) {
       const size_t i = order[k];
-      f[3 * i] = hf[k].x;
-      f[3 * i + 1] = hf[k].y;
-      f[3 * i + 2] = hf[k].z;
+      if (f) {
+        f[3 * i] = hf[k].x;
+        f[3 * i + 1] = hf[k].y;
+        f[3 * i + 2] = hf[k].z;
+      }
+      if (nd) nd[i] = hf[k].w;
     }
   }
   return kErrOk;
@@ -1618,7 +2203,7 @@ int Engine::get_state(float *xlocal, int *cell, float *v, float *f, float *pf) {
 int Engine::get_global(double *x, double *v) {
   std::vector<float> xl(3 * (size_t)N_), vv(3 * (size_t)N_);
   std::vector<int> cell(N_);
-  if (int rc = get_state(xl.data(), cell.data(), vv.data(), nullptr, nullptr)) return rc;
+  if (int rc = get_state(xl.data(), cell.data(), vv.data(), nullptr, nullptr, nullptr)) return rc;
   for (int i = 0; i < N_; ++i) {
     const int c = cell[i];
     const HostReplica *R = nullptr;
@@ -1642,9 +2227,9 @@ int Engine::neighbour_counts(int *counts) {
   StepArgs A;
   std::memset(&A, 0, sizeof(A));
   A.x = d->x[0];
-  A.pid = d->pid[0];
+  A.pid = d->pid[parity_];
   A.cell = d->cell[0];
-  A.cstart = d->cstart[0];
+  A.cstart = d->cstart[parity_];
   A.rep = d->rep;
   A.offs = d->offs;
   A.shifts = d->shifts;
@@ -1682,7 +2267,7 @@ int Engine::observables(int replica, double *out) {
 
 int Engine::bond_active(uint8_t *active) {
   if (M_ == 0) return kErrOk;
-  int rc = hipMemcpyAsync(active, d_->bact[bact_cur_], M_, hipMemcpyDeviceToHost, d_->stream);
+  int rc = hipMemcpyAsync(active, d_->bact[parity_], M_, hipMemcpyDeviceToHost, d_->stream);
   rc |= hipStreamSynchronize(d_->stream);
   return check_device(rc, "bond flags");
 }

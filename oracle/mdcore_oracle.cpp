@@ -30,8 +30,6 @@
 //   gaussian_force (persistent random force) ........ MxForce.cpp:370-385
 //
 // Deviations, each stated where it happens:
-//   * the number density f[3] (smoothing_kernel.hpp, x86 rsqrt) is not
-//     computed: it never feeds a force;
 //   * gaussian_force draws from Philox4x32-10 instead of carbon's CRandom
 //     (absent from the reference tree), keyed by (particle id, time);
 //   * the optional counter-based DPD noise mode is this project's own; the
@@ -56,6 +54,8 @@
 #include <set>
 #include <thread>
 #include <vector>
+
+#include <immintrin.h>
 
 #include "../mechanica_amd/csrc/mxg_philox.h"
 
@@ -402,7 +402,7 @@ inline bool dpd_eval(Oracle &o, const Pot &p, Part *pi, Part *pj, const float dx
 }
 
 // potential_eval_super_ex: overlap randomisation, then DPD or table.
-inline void eval_pair(Oracle &o, Cell &cell, int pot, Part *pi, Part *pj, float dx[3], float r2,
+inline void eval_pair(Oracle &o, Cell &cell, int pot, Part *pi, Part *pj, float dx[3], float r2, float nd,
                       float *epot) {
   const Pot &p = o.pots[pot];
   if (r2 < p.a * p.a) {
@@ -418,7 +418,11 @@ inline void eval_pair(Oracle &o, Cell &cell, int pot, Part *pi, Part *pj, float 
   if (p.kind == kKindDPD) {
     // the reference draws (and discards) one more gaussian here
     (void)cell.gauss(cell.gen);
-    if (dpd_eval(o, p, pi, pj, dx, r2)) *epot += 0.0f;
+    if (dpd_eval(o, p, pi, pj, dx, r2)) {
+      pi->f[3] += nd;
+      pj->f[3] += nd;
+      *epot += 0.0f;
+    }
   } else {
     float e, f;
     if (eval_ex(p, pi->radius, pj->radius, r2, &e, &f)) {
@@ -427,9 +431,35 @@ inline void eval_pair(Oracle &o, Cell &cell, int pot, Part *pi, Part *pj, float 
         pi->f[k] -= w;
         pj->f[k] += w;
       }
+      pi->f[3] += nd;
+      pj->f[3] += nd;
       *epot += e;
     }
   }
+}
+
+// Number-density kernel W(r^2, h): cubic spline of r / h with r from the
+// approximate x86 reciprocal square root (smoothing_kernel.hpp:19-54).
+inline float qsqrt(float f) {
+  __m128 t = _mm_set_ss(f);
+  t = _mm_rsqrt_ss(t);
+  return 1.0 / _mm_cvtss_f32(t);
+}
+
+inline float w_cubic(float r2, float h) {
+  float r = qsqrt(r2);
+  float x = r / h;
+  float y;
+  if (x < 1.f) {
+    float x2 = x * x;
+    y = 1.f - (3.f / 2.f) * x2 + (3.f / 4.f) * x2 * x;
+  } else if (x >= 1.f && x < 2.f) {
+    float arg = 2.f - x;
+    y = (1.f / 4.f) * arg * arg * arg;
+  } else {
+    y = 0.f;
+  }
+  return y / (M_PI * h * h * h);
 }
 
 inline float fr2(const float *x1, const float *x2, float *dx) {
@@ -576,9 +606,14 @@ void do_pair(Oracle &o, int ci_, int cj_) {
       if (r2 > cutoff2) continue;
       part_i->ncount++;
       part_j->ncount++;
+      float nd = w_cubic(r2, cutoff);
       int pot = o.potmat[part_i->type * o.types.size() + part_j->type];
-      if (pot < 0) continue;
-      eval_pair(o, *cell_i, pot, part_i, part_j, dx, r2, &epot);
+      if (pot < 0) {
+        part_i->f[3] += nd;
+        part_j->f[3] += nd;
+        continue;
+      }
+      eval_pair(o, *cell_i, pot, part_i, part_j, dx, r2, nd, &epot);
     }
   }
   cell_i->epot += epot;
@@ -627,9 +662,14 @@ void do_self(Oracle &o, int cid) {
       if (r2 > cutoff2) continue;
       part_i->ncount++;
       part_j->ncount++;
+      float nd = w_cubic(r2, (float)o.cutoff);
       int pot = o.potmat[part_i->type * o.types.size() + part_j->type];
-      if (pot < 0) continue;
-      eval_pair(o, c, pot, part_i, part_j, dx, r2, &epot);
+      if (pot < 0) {
+        part_i->f[3] += nd;
+        part_j->f[3] += nd;
+        continue;
+      }
+      eval_pair(o, c, pot, part_i, part_j, dx, r2, nd, &epot);
     }
   }
   c.epot += epot;
@@ -829,11 +869,12 @@ void force(Oracle &o) {
       total += e;
     }
   o.ekin = total;
+  const float self_nd = w_cubic(0, (float)o.cutoff);
   for (auto &c : o.cells) {
     c.epot = 0.0;
     for (auto &p : c.parts) {
       p.f[0] = p.f[1] = p.f[2] = 0.0f;
-      p.f[3] = 0.0f;
+      p.f[3] = self_nd;
       p.ncount = 0;
     }
   }
@@ -1260,6 +1301,13 @@ int oracle_get_state(void *h, float *xl, int *cell, float *v, float *f, int *nco
       if (cell) cell[i] = c;
       if (ncount) ncount[i] = p.ncount;
     }
+  return 0;
+}
+
+int oracle_number_density(void *h, float *nd) {
+  Oracle *o = (Oracle *)h;
+  for (const Cell &c : o->cells)
+    for (const Part &p : c.parts) nd[p.id] = p.f[3];
   return 0;
 }
 

@@ -51,6 +51,7 @@ _SIG = {
     "oracle_bond_active": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8)]),
     "oracle_forces64": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "oracle_error": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "oracle_number_density": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
 }
 
 _lib = None
@@ -175,6 +176,11 @@ class Oracle:
         pf = np.zeros((N, 3), np.float32)
         lib().oracle_get_state(self._h, _fp(xl), _ip(cell), _fp(v), _fp(f), _ip(nc), _fp(pf))
         return {"x": xl, "cell": cell, "v": v, "f": f, "ncount": nc, "pf": pf}
+
+    def number_density(self):
+        nd = np.zeros(self.N, np.float32)
+        lib().oracle_number_density(self._h, _fp(nd))
+        return nd
 
     def global_positions(self):
         x = np.zeros((self.N, 3), np.float64)

@@ -79,6 +79,10 @@ def test_forces_match_oracle(m, oracle_mod, wi):
     e_gpu = obs["epot_nonbond"] + obs["epot_bond"]
     assert abs(e_gpu - e64) <= 1e-6 * max(abs(e64), 1.0)
     assert abs(obs["ekin"] - ref_obs["ekin"]) <= 1e-12 * max(abs(ref_obs["ekin"]), 1.0)
+    # Number density: the oracle keeps the reference's approximate x86 rsqrt
+    # (relative error <= 1.5 * 2^-12 per pair), the GPU uses the exact root.
+    nd_g, nd_r = u.number_density(), o.number_density()
+    assert np.abs(nd_g - nd_r).max() <= 1e-3 * np.abs(nd_r).max()
 
 
 @pytest.mark.parametrize("wi", [0, 1, 2])
