@@ -176,6 +176,7 @@ struct StepArgs {
   double *partials;
   int *ncount;  // neighbour-count mode only
   int random;   // some type carries a persistent random force
+  int npots;    // pair potentials
   float cutoff;       // number-density kernel width
   float nd_norm;      // 1 / (pi cutoff^3)
   int *cnt;     // particles per cell of the next step
@@ -330,15 +331,23 @@ struct LdsNbr {
 // Returns whether the pair adds to the number density (no potential, or the
 // potential was evaluated in range: potential_eval_super_ex adds it only on
 // success, dpd_eval.hpp:135-160).
+// Per-type-pair tables read by the pair code (staged in LDS by the kernels).
+struct PairTabs {
+  const DevPot *pots;
+  const int *potmat;
+  const float *radius;
+};
+
 template <int MODE, typename NB>
-__device__ __forceinline__ bool pair_force(const StepArgs &A, const NB &nb, int ti, int tj, bool p_is_i,
+__device__ __forceinline__ bool pair_force(const StepArgs &A, const PairTabs &tb, const NB &nb, int ti, int tj,
+                                           bool p_is_i,
                                            int mypid, int q,
                                            bool self_pair, float dxx, float dxy, float dxz, float r2,
                                            const float4 &vs, float &fx, float &fy, float &fz, double *acc,
-                                           unsigned t1) {
-  int pidx = A.potmat[ti * A.ntypes + tj];
+                                           unsigned t1, uint32_t tk) {
+  int pidx = tb.potmat[ti * A.ntypes + tj];
   if (pidx < 0) return true;
-  const DevPot P = A.pots[pidx];
+  const DevPot P = tb.pots[pidx];
   if (r2 < P.a2) {
     int qpid = nb.pid(q);
     uint32_t ki, kj;
@@ -357,7 +366,7 @@ __device__ __forceinline__ bool pair_force(const StepArgs &A, const NB &nb, int 
     dxz = sgn * d.z;
     r2 = P.a * P.a;
   }
-  const float ri = A.types[ti].radius, rj = A.types[tj].radius;
+  const float ri = tb.radius[ti], rj = tb.radius[tj];
   if ((MODE & kModeDPD) && P.kind == kPotKindDPD) {
     bool shifted = P.flags & kPotFlagShifted;
     float cutoff = shifted ? (P.b + ri + rj) : P.b;
@@ -384,10 +393,9 @@ __device__ __forceinline__ bool pair_force(const StepArgs &A, const NB &nb, int 
     float fd = -P.dpd_gamma * omega_d * ev;
     float fr;
     if (A.dpd_noise) {
-      int qpid = nb.pid(q);
-      u32x4 ctr = {{(uint32_t)min(mypid, qpid), (uint32_t)max(mypid, qpid), t1, kStreamDPD}};
-      u32x4 rr = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
-      fr = P.dpd_sigma * omega_c * unit_uniform(rr.v[0]) * A.dpd_delta;
+      const int qpid = nb.pid(q);
+      fr = P.dpd_sigma * omega_c * pair_uniform((uint32_t)min(mypid, qpid), (uint32_t)max(mypid, qpid), tk) *
+           A.dpd_delta;
     } else {
       fr = P.dpd_sigma * omega_c * A.dpd_delta;
     }
@@ -706,6 +714,21 @@ constexpr int kBT = 256;        // threads per brick block
 constexpr int kHaloAxis = 64;   // max halo cells per axis
 constexpr int kSpanMax = 2;     // max cell span
 constexpr int kOffMax = 125;    // (2 * kSpanMax + 1)^3
+constexpr int kMaxTypes = 16;   // particle types per engine
+constexpr int kMaxPots = 64;    // pair potentials per engine
+constexpr size_t kPairTabBytes = kMaxPots * sizeof(DevPot) + kMaxTypes * kMaxTypes * 4 + kMaxTypes * 4;
+
+// Copies the per-type-pair tables into LDS at p (kPairTabBytes).
+__device__ __forceinline__ PairTabs stage_pair_tabs(const StepArgs &A, unsigned char *p, int npots) {
+  DevPot *pots = reinterpret_cast<DevPot *>(p);
+  int *potmat = reinterpret_cast<int *>(pots + kMaxPots);
+  float *radius = reinterpret_cast<float *>(potmat + kMaxTypes * kMaxTypes);
+  const int nt = A.ntypes;
+  for (int i = threadIdx.x; i < npots; i += blockDim.x) pots[i] = A.pots[i];
+  for (int i = threadIdx.x; i < nt * nt; i += blockDim.x) potmat[i] = A.potmat[i];
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) radius[i] = A.types[i].radius;
+  return PairTabs{pots, potmat, radius};
+}
 
 __host__ __device__ inline size_t brick_lds_bytes(int pmax, int hmax, int cap, bool dpd) {
   size_t b = 0;
@@ -716,6 +739,7 @@ __host__ __device__ inline size_t brick_lds_bytes(int pmax, int hmax, int cap, b
   b += 3 * kHaloAxis * (2 * kSpanMax + 1) * 4;  // shifts
   b += (size_t)kOffMax * sizeof(DevOffset);
   b += (size_t)cap * kBT * 4;          // neighbour lists
+  b += kPairTabBytes;                  // potentials, type pairs, radii
   return b;
 }
 
@@ -753,6 +777,8 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
   const int lane = tid & 63, wave = tid >> 6;
   const int passthrough = A.err[2];
   const unsigned t1 = *A.time + 1u;
+  const uint32_t tk = step_key(A.seed_lo, t1);
+  const PairTabs tb = stage_pair_tabs(A, reinterpret_cast<unsigned char *>(nlist + B.cap * kBT), A.npots);
 
   for (int bi = blockIdx.x; bi < B.nbricks; bi += gridDim.x) {
     const int4 br = B.bricks[bi];
@@ -931,8 +957,8 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
               }
               const float r2 = dx * dx + dy * dy + dz * dz;
               const int tj = __float_as_int(xq.w) & 0xffff;
-              if (pair_force<MODE>(A, nb, ti, tj, off.side >= 0, mypid, q, off.side == 0, dx, dy, dz, r2, vs, fx,
-                                   fy, fz, acc, t1))
+              if (pair_force<MODE>(A, tb, nb, ti, tj, off.side >= 0, mypid, q, off.side == 0, dx, dy, dz, r2, vs,
+                                   fx, fy, fz, acc, t1, tk))
                 nd += w_cubic(r2, A.cutoff, A.nd_norm);
             }
           };
@@ -1220,10 +1246,14 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
   __shared__ int homestart[kScanItems * kBT + 1];
   __shared__ int homegb[kScanItems * kBT];
   __shared__ double red[kNRed][kBT / 64];
+  extern __shared__ __align__(16) unsigned char gsmem[];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int passthrough = A.err[2];
   const unsigned t1 = *A.time + 1u;
+  const uint32_t tk = step_key(A.seed_lo, t1);
+  const PairTabs tb = stage_pair_tabs(A, gsmem, A.npots);
+  __syncthreads();
   const int nov = *B.ovf_count;
   for (int i = blockIdx.x; i < nov; i += gridDim.x) {
     const int bi = B.ovf_list[i];
@@ -1289,8 +1319,8 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
                            [&](int q, bool p_is_i, bool self, float dx, float dy, float dz, float r2,
                                const float4 &xq) {
                              const int tj = __float_as_int(xq.w) & 0xffff;
-                             if (pair_force<MODE>(A, nb, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, vs, fx,
-                                                  fy, fz, acc, t1))
+                             if (pair_force<MODE>(A, tb, nb, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, vs,
+                                                  fx, fy, fz, acc, t1, tk))
                                nd += w_cubic(r2, A.cutoff, A.nd_norm);
                            });
         if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
@@ -1317,7 +1347,7 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
 template <int MODE>
 void launch_step_mode(const StepArgs *A, const BrickArgs &B, size_t lds, hipStream_t st) {
   hipLaunchKernelGGL(k_brick<MODE>, dim3(B.nbricks), dim3(kBT), lds, st, A, B);
-  hipLaunchKernelGGL(k_brick_global<MODE>, dim3(64), dim3(kBT), 0, st, A, B);
+  hipLaunchKernelGGL(k_brick_global<MODE>, dim3(64), dim3(kBT), kPairTabBytes, st, A, B);
 }
 
 using StepLauncher = void (*)(const StepArgs *, const BrickArgs &, size_t, hipStream_t);
@@ -1534,6 +1564,7 @@ int Engine::build_geometry() {
 
 int Engine::add_type(double mass, float radius, int dynamics) {
   if (!(mass > 0)) return fail(kErrRange, "mass must be positive");
+  if ((int)types_.size() >= kMaxTypes) return fail(kErrRange, "at most 16 particle types per engine");
   if (dynamics != kDynNewtonian && dynamics != kDynOverdamped) return fail(kErrRange, "unknown dynamics");
   TypeInfo t;
   t.mass = mass;
@@ -1555,6 +1586,7 @@ int Engine::add_pot(const Potential &p, int i, int j) {
   int nt = (int)types_.size();
   if (i < 0 || i >= nt || j < 0 || j >= nt) return fail(kErrRange, "type index out of range");
   if (p.flags & kPotFlagBound) return fail(kErrPotential, "cluster-bound potentials are not supported");
+  if ((int)pots_.size() >= kMaxPots) return fail(kErrPotential, "at most 64 pair potentials per engine");
   if (p.kind == kPotKindTable && (p.n < 1 || p.c.size() < (size_t)(p.n + 1) * kPotChunk))
     return fail(kErrPotential, "potential has no table");
   pots_.push_back(p);
@@ -1974,6 +2006,7 @@ void Engine::fill_step_args(void *out, int par) const {
   A.coef = d->coef;
   A.potmat = d->potmat;
   A.ntypes = (int)types_.size();
+  A.npots = (int)pots_.size();
   A.types = d->types;
   A.boff = d->boff;
   A.bent = d->bent;

@@ -53,6 +53,25 @@ MXG_HD float u01_closed_open(uint32_t u) {
   return (float)(u >> 8) * (1.0f / 16777216.0f);
 }
 
+// 32-bit integer hash (the output permutation of O'Neill's PCG, "RXS M XS"
+// on one LCG step).  Chained, it gives the per-pair DPD noise: one draw per
+// pair and step at a fraction of a Philox call.
+MXG_HD uint32_t mix32(uint32_t v) {
+  uint32_t state = v * 747796405u + 2891336453u;
+  uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+// Key of one time step for the pair noise.
+MXG_HD uint32_t step_key(uint32_t seed, uint32_t time) { return mix32(time ^ mix32(seed ^ 0x64706472u)); }
+
+// Noise of the pair (a, b), a < b, at the step whose key is k: uniform on
+// [-sqrt(3), sqrt(3)), zero mean and unit variance.
+MXG_HD float pair_uniform(uint32_t a, uint32_t b, uint32_t k) {
+  const uint32_t h = mix32(a ^ mix32(b ^ k));
+  return 1.7320508f * (2.0f * ((float)(h >> 8) * (1.0f / 16777216.0f)) - 1.0f);
+}
+
 // Stream identifiers keep the different uses of the generator independent.
 enum : uint32_t {
   kStreamOverlap = 0x6F766C70u,   // direction for pairs closer than a

@@ -363,12 +363,10 @@ inline bool eval_ex(const Pot &p, float ri, float rj, float r2, float *e, float 
   return true;
 }
 
-// Pair noise of the counter-based DPD mode: uniform on [-sqrt(3), sqrt(3))
-// (zero mean, unit variance), keyed by the pair and the time.
-float pair_noise(uint32_t a, uint32_t b, uint64_t time, uint32_t klo, uint32_t khi) {
-  mxg::u32x4 ctr = {{a, b, (uint32_t)time, mxg::kStreamDPD}};
-  mxg::u32x4 r = mxg::philox4x32_10(ctr, klo, khi);
-  return 1.7320508f * (2.0f * mxg::u01_closed_open(r.v[0]) - 1.0f);
+// Pair noise of the counter-based DPD mode (zero mean, unit variance), keyed
+// by the pair and the time step.
+float pair_noise(uint32_t a, uint32_t b, uint64_t time, uint32_t klo, uint32_t) {
+  return mxg::pair_uniform(a, b, mxg::step_key(klo, (uint32_t)time));
 }
 
 // dpd_eval: forces on the pair along e = dx / r.  Returns false out of range.
