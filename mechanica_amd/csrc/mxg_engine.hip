@@ -218,9 +218,9 @@ __device__ __forceinline__ float3 overlap_dir(uint32_t pi, uint32_t pj, uint32_t
 
 // potential_eval_ex (potential_eval.hpp:106-161).
 __device__ __forceinline__ bool table_eval_ex(const DevPot &P, const float *__restrict__ coef, float ri,
-                                              float rj, float r2, float &e, float &f) {
+                                              float rj, float r_in, float &e, float &f) {
   const float eps = 1.1920928955078125e-07f;
-  float r = sqrtf(r2);
+  float r = r_in;  // sqrt(r^2), correctly rounded
   float ro = r < eps ? eps : r;
   r = r < P.a ? P.a : r;
   if (P.flags & kPotFlagScaled)
@@ -282,8 +282,8 @@ __device__ __forceinline__ unsigned sort_key(float bias, float dscale, float x, 
 
 // Cubic-spline number-density kernel W(r^2, h) (smoothing_kernel.hpp:28-50)
 // with the exact square root (the reference uses an approximate x86 rsqrt).
-__device__ __forceinline__ float w_cubic(float r2, float h, float inv_norm) {
-  const float x = __fdividef(sqrtf(r2), h);
+__device__ __forceinline__ float w_cubic(float r, float h, float inv_norm) {
+  const float x = __fdividef(r, h);
   float y;
   if (x < 1.f) {
     const float x2 = x * x;
@@ -319,11 +319,19 @@ struct GlobalNbr {
   __device__ __forceinline__ int pid(int q) const { return pp[q]; }
 };
 
+// In LDS the velocity's w holds the particle id (DPD bricks stage
+// velocities); without staged velocities the id comes from global memory
+// through the halo map (only pairs closer than the potential's lower bound
+// need it).
 struct LdsNbr {
   const float4 *vv;
-  const int *pp;
+  bool staged_v;
+  const int *hstart;
+  const int *hgb;
+  int nhalo;
+  const int *__restrict__ gpid;
   __device__ __forceinline__ float4 v(int q) const { return vv[q]; }
-  __device__ __forceinline__ int pid(int q) const { return pp[q]; }
+  __device__ int pid(int q) const;
 };
 
 // Accumulates one pair into the owner's force.  dx is in the reference
@@ -342,7 +350,7 @@ template <int MODE, typename NB>
 __device__ __forceinline__ bool pair_force(const StepArgs &A, const PairTabs &tb, const NB &nb, int ti, int tj,
                                            bool p_is_i,
                                            int mypid, int q,
-                                           bool self_pair, float dxx, float dxy, float dxz, float r2,
+                                           bool self_pair, float dxx, float dxy, float dxz, float r2, float r,
                                            const float4 &vs, float &fx, float &fy, float &fz, double *acc,
                                            unsigned t1, uint32_t tk) {
   int pidx = tb.potmat[ti * A.ntypes + tj];
@@ -365,13 +373,13 @@ __device__ __forceinline__ bool pair_force(const StepArgs &A, const PairTabs &tb
     dxy = sgn * d.y;
     dxz = sgn * d.z;
     r2 = P.a * P.a;
+    r = sqrtf(r2);
   }
   const float ri = tb.radius[ti], rj = tb.radius[tj];
   if ((MODE & kModeDPD) && P.kind == kPotKindDPD) {
     bool shifted = P.flags & kPotFlagShifted;
     float cutoff = shifted ? (P.b + ri + rj) : P.b;
     if (r2 > cutoff * cutoff) return false;
-    float r = sqrtf(r2);
     const float ir = __fdividef(1.0f, r);
     float ex = dxx * ir, ey = dxy * ir, ez = dxz * ir;
     float4 vq = nb.v(q);
@@ -425,7 +433,7 @@ __device__ __forceinline__ bool pair_force(const StepArgs &A, const PairTabs &tb
   }
   if (MODE & kModeTable) {
     float e, f;
-    if (!table_eval_ex(P, A.coef, ri, rj, r2, e, f)) return false;
+    if (!table_eval_ex(P, A.coef, ri, rj, r, e, f)) return false;
     float wx = f * dxx, wy = f * dxy, wz = f * dxz;
     if (p_is_i) {
       fx -= wx;
@@ -686,7 +694,7 @@ __device__ __forceinline__ void advance_store(const StepArgs &A, const DevReplic
     }
   }
   A.x_out[s] = make_float4(X[0], X[1], X[2], xs.w);
-  A.v_out[s] = make_float4(V[0], V[1], V[2], 0.0f);
+  A.v_out[s] = make_float4(V[0], V[1], V[2], vs.w);  // w carries the particle id
   A.cell_out[s] = newc != c ? (int)((unsigned)newc | 0x80000000u) : newc;
   atomicAdd(&A.cnt[newc], 1);
   if (newc != c) atomicAdd(&A.inc[newc], 1);
@@ -706,6 +714,8 @@ struct BrickArgs {
   int pmax;     // LDS particle capacity
   int cap;      // neighbour-list entries per thread
   int hmax;     // halo-cell capacity
+  int hdmax;    // halo cells per axis (capacity)
+  int noff;     // neighbour offsets (capacity)
   int *ovf_list;   // bricks whose halo overflowed the LDS capacity
   int *ovf_count;
 };
@@ -715,7 +725,7 @@ constexpr int kHaloAxis = 64;   // max halo cells per axis
 constexpr int kSpanMax = 2;     // max cell span
 constexpr int kOffMax = 125;    // (2 * kSpanMax + 1)^3
 constexpr int kMaxTypes = 16;   // particle types per engine
-constexpr int kMaxPots = 64;    // pair potentials per engine
+constexpr int kMaxPots = 32;    // pair potentials per engine
 constexpr size_t kPairTabBytes = kMaxPots * sizeof(DevPot) + kMaxTypes * kMaxTypes * 4 + kMaxTypes * 4;
 
 // Copies the per-type-pair tables into LDS at p (kPairTabBytes).
@@ -730,14 +740,13 @@ __device__ __forceinline__ PairTabs stage_pair_tabs(const StepArgs &A, unsigned 
   return PairTabs{pots, potmat, radius};
 }
 
-__host__ __device__ inline size_t brick_lds_bytes(int pmax, int hmax, int cap, bool dpd) {
+__host__ __device__ inline size_t brick_lds_bytes(int pmax, int hmax, int hdmax, int noff, int cap, bool dpd) {
   size_t b = 0;
-  b += (size_t)pmax * 16;              // positions
-  if (dpd) b += (size_t)pmax * 16;     // velocities
-  b += (size_t)pmax * 4;               // ids
+  b += (size_t)pmax * 16;              // positions (w: type and flags)
+  if (dpd) b += (size_t)pmax * 16;     // velocities (w: particle id)
+  b += (size_t)noff * sizeof(DevOffset);
   b += (size_t)(hmax + 1) * 4 * 3;     // halo starts, global starts, home starts
-  b += 3 * kHaloAxis * (2 * kSpanMax + 1) * 4;  // shifts
-  b += (size_t)kOffMax * sizeof(DevOffset);
+  b += (size_t)3 * hdmax * (2 * kSpanMax + 1) * 4;  // shifts
   b += (size_t)cap * kBT * 4;          // neighbour lists
   b += kPairTabBytes;                  // potentials, type pairs, radii
   return b;
@@ -756,6 +765,12 @@ __device__ __forceinline__ int upper_index(const int *a, int n, int f) {
   return lo;
 }
 
+__device__ int LdsNbr::pid(int q) const {
+  if (staged_v) return __float_as_int(vv[q].w);
+  const int h = upper_index(hstart, nhalo + 1, q);
+  return gpid[hgb[h] + (q - hstart[h])];
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, BrickArgs B) {
   const StepArgs &A = *Ap;
@@ -763,13 +778,12 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
   constexpr bool kDPD = (MODE & kModeDPD) != 0;
   float4 *hx = reinterpret_cast<float4 *>(smem);
   float4 *hv = hx + B.pmax;
-  int *hpid = reinterpret_cast<int *>(hv + (kDPD ? B.pmax : 0));
-  int *hstart = hpid + B.pmax;
+  DevOffset *hoff = reinterpret_cast<DevOffset *>(hv + (kDPD ? B.pmax : 0));
+  int *hstart = reinterpret_cast<int *>(hoff + B.noff);
   int *hgb = hstart + (B.hmax + 1);
   int *homestart = hgb + (B.hmax + 1);
   float *hs = reinterpret_cast<float *>(homestart + (B.hmax + 1));
-  DevOffset *hoff = reinterpret_cast<DevOffset *>(hs + 3 * kHaloAxis * (2 * kSpanMax + 1));
-  unsigned *nlist = reinterpret_cast<unsigned *>(hoff + kOffMax);
+  unsigned *nlist = reinterpret_cast<unsigned *>(hs + 3 * B.hdmax * (2 * kSpanMax + 1));
   __shared__ int scan_lds[kBT / 64];
   __shared__ double red[kNRed][kBT / 64];
 
@@ -779,6 +793,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
   const unsigned t1 = *A.time + 1u;
   const uint32_t tk = step_key(A.seed_lo, t1);
   const PairTabs tb = stage_pair_tabs(A, reinterpret_cast<unsigned char *>(nlist + B.cap * kBT), A.npots);
+  (void)kOffMax;
 
   for (int bi = blockIdx.x; bi < B.nbricks; bi += gridDim.x) {
     const int4 br = B.bricks[bi];
@@ -824,17 +839,17 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
     // Shift table of the brick and the offset list.
     for (int i = tid; i < hd0 * (2 * sp0 + 1); i += kBT) {
       const int hl = i / (2 * sp0 + 1), d = i % (2 * sp0 + 1);
-      hs[(0 * kHaloAxis + hl) * (2 * kSpanMax + 1) + d] =
+      hs[(0 * B.hdmax + hl) * (2 * kSpanMax + 1) + d] =
           A.shifts[R.shift_base[0] + wrap(br.y - sp0 + hl, R.cdim[0]) * (2 * sp0 + 1) + d];
     }
     for (int i = tid; i < hd1 * (2 * sp1 + 1); i += kBT) {
       const int hl = i / (2 * sp1 + 1), d = i % (2 * sp1 + 1);
-      hs[(1 * kHaloAxis + hl) * (2 * kSpanMax + 1) + d] =
+      hs[(1 * B.hdmax + hl) * (2 * kSpanMax + 1) + d] =
           A.shifts[R.shift_base[1] + wrap(br.z - sp1 + hl, R.cdim[1]) * (2 * sp1 + 1) + d];
     }
     for (int i = tid; i < hd2 * (2 * sp2 + 1); i += kBT) {
       const int hl = i / (2 * sp2 + 1), d = i % (2 * sp2 + 1);
-      hs[(2 * kHaloAxis + hl) * (2 * kSpanMax + 1) + d] =
+      hs[(2 * B.hdmax + hl) * (2 * kSpanMax + 1) + d] =
           A.shifts[R.shift_base[2] + wrap(br.w - sp2 + hl, R.cdim[2]) * (2 * sp2 + 1) + d];
     }
     for (int i = tid; i < R.n_off; i += kBT) hoff[i] = A.offs[R.off_base + i];
@@ -879,7 +894,6 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
       const int gs = hgb[h] + (f - hstart[h]);
       hx[f] = A.x[gs];
       if (kDPD) hv[f] = A.v[gs];
-      hpid[f] = A.pid[gs];
     }
     __syncthreads();
     const int nhp = homestart[nhome];
@@ -900,7 +914,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
       const int c = R.cell_base + (lx * R.cdim[1] + ly) * R.cdim[2] + lz;
       const float4 xs = hx[p];
       const float4 vs = kDPD ? hv[p] : A.v[s];
-      const int mypid = hpid[p];
+      const int mypid = __float_as_int(vs.w);
       const int ti = __float_as_int(xs.w) & 0xffff;
       const DevType T = A.types[ti];
       {
@@ -914,19 +928,19 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
       if (!passthrough) {
         if (A.random) single_body(A, s, mypid, T, t1, fx, fy, fz, pfs);
         {
-          const LdsNbr nb{hv, hpid};
+          const LdsNbr nb{hv, kDPD, hstart, hgb, nhalo, A.pid};
           const unsigned dmax = (unsigned)R.dmaxdist;
           // Shift of the pair for offset o in the reference orientation.
           auto shift_of = [&](const DevOffset &off, float &sx, float &sy, float &sz) {
             constexpr int W = 2 * kSpanMax + 1;
             if (off.side > 0) {
-              sx = hs[(0 * kHaloAxis + ha) * W + off.ox + sp0];
-              sy = hs[(1 * kHaloAxis + hb) * W + off.oy + sp1];
-              sz = hs[(2 * kHaloAxis + hcz) * W + off.oz + sp2];
+              sx = hs[(0 * B.hdmax + ha) * W + off.ox + sp0];
+              sy = hs[(1 * B.hdmax + hb) * W + off.oy + sp1];
+              sz = hs[(2 * B.hdmax + hcz) * W + off.oz + sp2];
             } else {
-              sx = hs[(0 * kHaloAxis + ha + off.ox) * W - off.ox + sp0];
-              sy = hs[(1 * kHaloAxis + hb + off.oy) * W - off.oy + sp1];
-              sz = hs[(2 * kHaloAxis + hcz + off.oz) * W - off.oz + sp2];
+              sx = hs[(0 * B.hdmax + ha + off.ox) * W - off.ox + sp0];
+              sy = hs[(1 * B.hdmax + hb + off.oy) * W - off.oy + sp1];
+              sz = hs[(2 * B.hdmax + hcz + off.oz) * W - off.oz + sp2];
             }
           };
           auto process = [&](int n) {
@@ -956,10 +970,11 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
                 }
               }
               const float r2 = dx * dx + dy * dy + dz * dz;
+              const float r = sqrtf(r2);
               const int tj = __float_as_int(xq.w) & 0xffff;
-              if (pair_force<MODE>(A, tb, nb, ti, tj, off.side >= 0, mypid, q, off.side == 0, dx, dy, dz, r2, vs,
-                                   fx, fy, fz, acc, t1, tk))
-                nd += w_cubic(r2, A.cutoff, A.nd_norm);
+              if (pair_force<MODE>(A, tb, nb, ti, tj, off.side >= 0, mypid, q, off.side == 0, dx, dy, dz, r2, r,
+                                   vs, fx, fy, fz, acc, t1, tk))
+                nd += w_cubic(r, A.cutoff, A.nd_norm);
             }
           };
           // Gather in-cutoff neighbours into the per-thread list, process
@@ -1319,9 +1334,10 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
                            [&](int q, bool p_is_i, bool self, float dx, float dy, float dz, float r2,
                                const float4 &xq) {
                              const int tj = __float_as_int(xq.w) & 0xffff;
-                             if (pair_force<MODE>(A, tb, nb, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, vs,
+                             const float r = sqrtf(r2);
+                             if (pair_force<MODE>(A, tb, nb, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, r, vs,
                                                   fx, fy, fz, acc, t1, tk))
-                               nd += w_cubic(r2, A.cutoff, A.nd_norm);
+                               nd += w_cubic(r, A.cutoff, A.nd_norm);
                            });
         if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
       }
@@ -1586,7 +1602,7 @@ int Engine::add_pot(const Potential &p, int i, int j) {
   int nt = (int)types_.size();
   if (i < 0 || i >= nt || j < 0 || j >= nt) return fail(kErrRange, "type index out of range");
   if (p.flags & kPotFlagBound) return fail(kErrPotential, "cluster-bound potentials are not supported");
-  if ((int)pots_.size() >= kMaxPots) return fail(kErrPotential, "at most 64 pair potentials per engine");
+  if ((int)pots_.size() >= kMaxPots) return fail(kErrPotential, "at most 32 pair potentials per engine");
   if (p.kind == kPotKindTable && (p.n < 1 || p.c.size() < (size_t)(p.n + 1) * kPotChunk))
     return fail(kErrPotential, "potential has no table");
   pots_.push_back(p);
@@ -1748,8 +1764,10 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   for (int i = 0; i < N; ++i) {
     const int k = fill[cellof[i]]++;
     xs[k] = xl[i];
-    vs[k] = v ? make_float4((float)v[3 * (size_t)i], (float)v[3 * (size_t)i + 1], (float)v[3 * (size_t)i + 2], 0.f)
-              : make_float4(0.f, 0.f, 0.f, 0.f);
+    float idbits;
+    std::memcpy(&idbits, &i, sizeof(idbits));  // v.w carries the particle id
+    vs[k] = v ? make_float4((float)v[3 * (size_t)i], (float)v[3 * (size_t)i + 1], (float)v[3 * (size_t)i + 2], idbits)
+              : make_float4(0.f, 0.f, 0.f, idbits);
     pid[k] = i;
     cell[k] = cellof[i];
     slot_of[i] = k;
@@ -1809,24 +1827,36 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   return set_brick(0, 0, 0, 0);
 }
 
-// Capacity of the LDS particle buffer for a brick shape: room for the
-// expected halo population plus margin, within the LDS budget.
-static int brick_pmax(const std::vector<HostReplica> &reps, const std::vector<double> &mean, const int bd[3],
-                      int cap, bool dpd, int &hmax, size_t budget) {
-  hmax = 0;
-  double worst = 0;
+// Capacities of the LDS layout for a brick shape: halo cells, halo cells per
+// axis, neighbour offsets, and the particle buffer (the expected halo
+// population plus margin, within the LDS budget).
+struct BrickFit {
+  int pmax = 0, hmax = 0, hdmax = 0, noff = 0;
+  double worst = 0;  // expected particles in the fullest halo
+};
+
+static BrickFit brick_fit(const std::vector<HostReplica> &reps, const std::vector<double> &mean, const int bd[3],
+                          int cap, bool dpd, size_t budget) {
+  BrickFit F;
   for (size_t r = 0; r < reps.size(); ++r) {
     int hc = 1;
-    for (int k = 0; k < 3; ++k) hc *= std::min(bd[k], reps[r].cdim[k]) + 2 * reps[r].span[k];
-    hmax = std::max(hmax, hc);
-    worst = std::max(worst, mean[r] * hc);
+    for (int k = 0; k < 3; ++k) {
+      const int hd = std::min(bd[k], reps[r].cdim[k]) + 2 * reps[r].span[k];
+      F.hdmax = std::max(F.hdmax, hd);
+      hc *= hd;
+    }
+    const int sp = std::max(reps[r].span[0], std::max(reps[r].span[1], reps[r].span[2]));
+    F.noff = std::max(F.noff, (2 * sp + 1) * (2 * sp + 1) * (2 * sp + 1));
+    F.hmax = std::max(F.hmax, hc);
+    F.worst = std::max(F.worst, mean[r] * hc);
   }
-  int want = (int)std::ceil(1.35 * worst + 6.0 * std::sqrt(worst) + 64.0);
-  want = (want + 63) / 64 * 64;
+  int want = (int)std::ceil(1.2 * F.worst + 4.0 * std::sqrt(F.worst) + 32.0);
+  want = (want + 31) / 32 * 32;
   int fit = 0;
-  for (int pm = 64; pm <= 16384; pm += 64)
-    if (brick_lds_bytes(pm, hmax, cap, dpd) <= budget) fit = pm;
-  return std::min(want, fit);
+  for (int pm = 32; pm <= 16384; pm += 32)
+    if (brick_lds_bytes(pm, F.hmax, F.hdmax, F.noff, cap, dpd) <= budget) fit = pm;
+  F.pmax = std::min(want, fit);
+  return F;
 }
 
 // Chooses (or sets) the brick shape of the force kernel and builds the brick
@@ -1840,6 +1870,7 @@ int Engine::set_brick(int bx, int by, int bz, int cap) {
   int c = cap > 0 ? cap : cap_;
   int bd[3] = {bx, by, bz};
   int hmax = 0, pm = 0;
+  BrickFit fitv;
   for (const auto &r : reps_)
     for (int k = 0; k < 3; ++k)
       if (r.span[k] > kSpanMax) return fail(kErrCutoff, "cutoff spans more than 2 cells");
@@ -1848,29 +1879,23 @@ int Engine::set_brick(int bx, int by, int bz, int cap) {
     bool ok = false;
     for (const auto &sh : shapes) {
       for (int k = 0; k < 3; ++k) bd[k] = sh[k];
-      pm = brick_pmax(reps_, mean_cell_count_, bd, c, dpd, hmax, budget);
-      double worst = 0;
-      for (size_t r = 0; r < reps_.size(); ++r) {
-        int hc = 1;
-        for (int k = 0; k < 3; ++k) hc *= std::min(bd[k], reps_[r].cdim[k]) + 2 * reps_[r].span[k];
-        worst = std::max(worst, mean_cell_count_[r] * hc);
-      }
-      if (pm >= 1.2 * worst && hmax <= kScanItems * kBT) {
+      fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, budget);
+      if (fitv.pmax >= 1.1 * fitv.worst && fitv.hmax <= kScanItems * kBT) {
         ok = true;
         break;
       }
     }
     if (!ok) {
       for (int k = 0; k < 3; ++k) bd[k] = 1;
-      pm = brick_pmax(reps_, mean_cell_count_, bd, c, dpd, hmax, budget);
+      fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, budget);
     }
   } else {
     if (bx < 1 || by < 1 || bz < 1) return fail(kErrRange, "brick dimensions must be positive");
-    pm = brick_pmax(reps_, mean_cell_count_, bd, c, dpd, hmax, budget);
+    fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, budget);
   }
-  for (const auto &r : reps_)
-    for (int k = 0; k < 3; ++k)
-      if (std::min(bd[k], r.cdim[k]) + 2 * r.span[k] > kHaloAxis) return fail(kErrRange, "brick too large");
+  hmax = fitv.hmax;
+  pm = fitv.pmax;
+  if (fitv.hdmax > kHaloAxis) return fail(kErrRange, "brick too large");
   if (hmax > kScanItems * kBT) return fail(kErrRange, "brick halo has more than 1024 cells");
   if (pm < 64) return fail(kErrRange, "brick does not fit in LDS");
   std::vector<int4> bricks;
@@ -1898,6 +1923,8 @@ int Engine::set_brick(int bx, int by, int bz, int cap) {
   cap_ = c;
   pmax_ = pm;
   hmax_ = hmax;
+  hdmax_ = fitv.hdmax;
+  noff_ = fitv.noff;
   nbricks_ = (int)bricks.size();
   brick_dpd_ = dpd;
   if (int st = dalloc(d, &d->ovf_list, bricks.size())) return check_device(st, "hipMalloc");
@@ -2071,9 +2098,11 @@ int Engine::launch_step() {
   Bk.pmax = pmax_;
   Bk.cap = cap_;
   Bk.hmax = hmax_;
+  Bk.hdmax = hdmax_;
+  Bk.noff = noff_;
   Bk.ovf_list = d->ovf_list;
   Bk.ovf_count = d->ovf_count;
-  const size_t lds = brick_lds_bytes(pmax_, hmax_, cap_, (mode & kModeDPD) != 0);
+  const size_t lds = brick_lds_bytes(pmax_, hmax_, hdmax_, noff_, cap_, (mode & kModeDPD) != 0);
   kStepLaunchers[mode](reinterpret_cast<const StepArgs *>(d->dargs) + par, Bk, lds, s);
   if (rec) hipEventRecord(d->kev[2 * timed_steps_ + 1], s);
   if (rec) ++timed_steps_;
