@@ -38,6 +38,15 @@
 
 namespace mxg {
 
+// Pointers into device memory carry the global address space in device code,
+// so loads through argument structs held in memory are global (or scalar)
+// loads rather than flat ones; host code sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define G __attribute__((address_space(1)))
+#else
+#define G
+#endif
+
 namespace {
 
 constexpr int kNRed = 12;  // epot_nonbond, epot_bond, ekin, virial[9]
@@ -135,52 +144,52 @@ struct DeviceState {
 namespace {
 
 struct StepArgs {
-  const float4 *__restrict__ x;
-  const float4 *__restrict__ v;
-  const float4 *__restrict__ pf;
-  const int *__restrict__ pid;
-  const int *__restrict__ cell;
-  const int *__restrict__ cstart;
-  float4 *__restrict__ x_out;
-  float4 *__restrict__ v_out;
-  float4 *__restrict__ pf_out;
-  float4 *__restrict__ f_out;
-  int *__restrict__ cell_out;
-  const DevReplica *__restrict__ rep;
-  const DevOffset *__restrict__ offs;
-  const float *__restrict__ shifts;
-  const int4 *__restrict__ tiles;
+  const float4 G *__restrict__ x;
+  const float4 G *__restrict__ v;
+  const float4 G *__restrict__ pf;
+  const int G *__restrict__ pid;
+  const int G *__restrict__ cell;
+  const int G *__restrict__ cstart;
+  float4 G *__restrict__ x_out;
+  float4 G *__restrict__ v_out;
+  float4 G *__restrict__ pf_out;
+  float4 G *__restrict__ f_out;
+  int G *__restrict__ cell_out;
+  const DevReplica G *__restrict__ rep;
+  const DevOffset G *__restrict__ offs;
+  const float G *__restrict__ shifts;
+  const int4 G *__restrict__ tiles;
   int ntiles;
-  const DevPot *__restrict__ pots;
-  const DevPot *__restrict__ bpots;
-  const float *__restrict__ coef;
-  const int *__restrict__ potmat;
+  const DevPot G *__restrict__ pots;
+  const DevPot G *__restrict__ bpots;
+  const float G *__restrict__ coef;
+  const int G *__restrict__ potmat;
   int ntypes;
-  const DevType *__restrict__ types;
-  const int *__restrict__ boff;
-  const int *__restrict__ bent;
-  const int *__restrict__ bi;
-  const int *__restrict__ bj;
-  const int *__restrict__ bpot;
-  const double *__restrict__ bdis;
-  const unsigned char *__restrict__ bact_in;
-  unsigned char *__restrict__ bact_out;
-  const int *__restrict__ slot_of;
+  const DevType G *__restrict__ types;
+  const int G *__restrict__ boff;
+  const int G *__restrict__ bent;
+  const int G *__restrict__ bi;
+  const int G *__restrict__ bj;
+  const int G *__restrict__ bpot;
+  const double G *__restrict__ bdis;
+  const unsigned char G *__restrict__ bact_in;
+  unsigned char G *__restrict__ bact_out;
+  const int G *__restrict__ slot_of;
   float dt;
   float cut2_pair, cut2_self;
   float dpd_delta;
   uint32_t seed_lo, seed_hi;
   int dpd_noise;
-  const unsigned *time;
-  int *err;
-  double *partials;
-  int *ncount;  // neighbour-count mode only
+  const unsigned G *time;
+  int G *err;
+  double G *partials;
+  int G *ncount;  // neighbour-count mode only
   int random;   // some type carries a persistent random force
   int npots;    // pair potentials
   float cutoff;       // number-density kernel width
   float nd_norm;      // 1 / (pi cutoff^3)
-  int *cnt;     // particles per cell of the next step
-  int *inc;     // particles entering each cell
+  int G *cnt;     // particles per cell of the next step
+  int G *inc;     // particles entering each cell
 };
 
 enum : int {
@@ -797,7 +806,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
 
   for (int bi = blockIdx.x; bi < B.nbricks; bi += gridDim.x) {
     const int4 br = B.bricks[bi];
-    const DevReplica &R = A.rep[br.x];
+    const DevReplica R = A.rep[br.x];  // by value: the fields stay in registers
     const int sp0 = R.span[0], sp1 = R.span[1], sp2 = R.span[2];
     const int nb0 = min(B.bdim[0], R.cdim[0] - br.y);
     const int nb1 = min(B.bdim[1], R.cdim[1] - br.z);
@@ -913,7 +922,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
       const int lx = br.y + a, ly = br.z + b, lz = br.w + cc;
       const int c = R.cell_base + (lx * R.cdim[1] + ly) * R.cdim[2] + lz;
       const float4 xs = hx[p];
-      const float4 vs = kDPD ? hv[p] : A.v[s];
+      const float4 vs = kDPD ? float4(hv[p]) : float4(A.v[s]);
       const int mypid = __float_as_int(vs.w);
       const int ti = __float_as_int(xs.w) & 0xffff;
       const DevType T = A.types[ti];
@@ -1074,7 +1083,7 @@ __global__ __launch_bounds__(kTile) void k_ncount(StepArgs A) {
     const int4 tile = A.tiles[t];
     const int s = tile.x + threadIdx.x;
     if (s >= tile.y) continue;
-    const DevReplica &R = A.rep[tile.z];
+    const DevReplica R = A.rep[tile.z];
     const float4 xs = A.x[s];
     int lx, ly, lz;
     cell_coords(A.cell[s] - R.cell_base, R, lx, ly, lz);
@@ -1273,7 +1282,7 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
   for (int i = blockIdx.x; i < nov; i += gridDim.x) {
     const int bi = B.ovf_list[i];
     const int4 br = B.bricks[bi];
-    const DevReplica &R = A.rep[br.x];
+    const DevReplica R = A.rep[br.x];  // by value: the fields stay in registers
     const int nb0 = min(B.bdim[0], R.cdim[0] - br.y);
     const int nb1 = min(B.bdim[1], R.cdim[1] - br.z);
     const int nb2 = min(B.bdim[2], R.cdim[2] - br.w);
@@ -2012,56 +2021,56 @@ void Engine::fill_step_args(void *out, int par) const {
   const DeviceState *d = d_;
   StepArgs &A = *reinterpret_cast<StepArgs *>(out);
   std::memset(&A, 0, sizeof(A));
-  A.x = d->x[0];
-  A.v = d->v[0];
-  A.pf = d->pf[0];
-  A.pid = d->pid[par];
-  A.cell = d->cell[0];
-  A.cstart = d->cstart[par];
-  A.x_out = d->x[1];
-  A.v_out = d->v[1];
-  A.pf_out = d->pf[1];
-  A.f_out = d->f;
-  A.cell_out = d->cell[1];
-  A.rep = d->rep;
-  A.offs = d->offs;
-  A.shifts = d->shifts;
-  A.tiles = d->tiles;
-  A.ntiles = ntiles_;
-  A.pots = d->pots;
-  A.bpots = d->bpots;
-  A.coef = d->coef;
-  A.potmat = d->potmat;
-  A.ntypes = (int)types_.size();
-  A.npots = (int)pots_.size();
-  A.types = d->types;
-  A.boff = d->boff;
-  A.bent = d->bent;
-  A.bi = d->bi;
-  A.bj = d->bj;
-  A.bpot = d->bpot;
-  A.bdis = d->bdis;
-  A.bact_in = M_ > 0 ? d->bact[par] : nullptr;
-  A.bact_out = M_ > 0 ? d->bact[par ^ 1] : nullptr;
-  A.slot_of = d->slot_of;
-  A.dt = (float)dt_;
+  A.x = (decltype(A.x))(d->x[0]);
+  A.v = (decltype(A.v))(d->v[0]);
+  A.pf = (decltype(A.pf))(d->pf[0]);
+  A.pid = (decltype(A.pid))(d->pid[par]);
+  A.cell = (decltype(A.cell))(d->cell[0]);
+  A.cstart = (decltype(A.cstart))(d->cstart[par]);
+  A.x_out = (decltype(A.x_out))(d->x[1]);
+  A.v_out = (decltype(A.v_out))(d->v[1]);
+  A.pf_out = (decltype(A.pf_out))(d->pf[1]);
+  A.f_out = (decltype(A.f_out))(d->f);
+  A.cell_out = (decltype(A.cell_out))(d->cell[1]);
+  A.rep = (decltype(A.rep))(d->rep);
+  A.offs = (decltype(A.offs))(d->offs);
+  A.shifts = (decltype(A.shifts))(d->shifts);
+  A.tiles = (decltype(A.tiles))(d->tiles);
+  A.ntiles = (decltype(A.ntiles))(ntiles_);
+  A.pots = (decltype(A.pots))(d->pots);
+  A.bpots = (decltype(A.bpots))(d->bpots);
+  A.coef = (decltype(A.coef))(d->coef);
+  A.potmat = (decltype(A.potmat))(d->potmat);
+  A.ntypes = (decltype(A.ntypes))((int)types_.size());
+  A.npots = (decltype(A.npots))((int)pots_.size());
+  A.types = (decltype(A.types))(d->types);
+  A.boff = (decltype(A.boff))(d->boff);
+  A.bent = (decltype(A.bent))(d->bent);
+  A.bi = (decltype(A.bi))(d->bi);
+  A.bj = (decltype(A.bj))(d->bj);
+  A.bpot = (decltype(A.bpot))(d->bpot);
+  A.bdis = (decltype(A.bdis))(d->bdis);
+  A.bact_in = (decltype(A.bact_in))(M_ > 0 ? d->bact[par] : nullptr);
+  A.bact_out = (decltype(A.bact_out))(M_ > 0 ? d->bact[par ^ 1] : nullptr);
+  A.slot_of = (decltype(A.slot_of))(d->slot_of);
+  A.dt = (decltype(A.dt))((float)dt_);
   const float cf = (float)cutoff_;
-  A.cut2_pair = cf * cf;
-  A.cut2_self = (float)(cutoff_ * cutoff_);
+  A.cut2_pair = (decltype(A.cut2_pair))(cf * cf);
+  A.cut2_self = (decltype(A.cut2_self))((float)(cutoff_ * cutoff_));
   // dpd_eval's delta = 1/sqrt(dt) (dpd_eval.hpp:23).
-  A.dpd_delta = 1.f / std::sqrt(dt_);
-  A.seed_lo = (uint32_t)seed_;
-  A.seed_hi = (uint32_t)(seed_ >> 32);
-  A.dpd_noise = dpd_noise_;
-  A.time = d->time;
-  A.err = d->err;
-  A.partials = d->partials;
-  A.cnt = d->cnt;
-  A.inc = d->inc;
-  A.ncount = d->ncount;
-  A.random = any_random_ ? 1 : 0;
-  A.cutoff = (float)cutoff_;
-  A.nd_norm = (float)(1.0 / (M_PI * cutoff_ * cutoff_ * cutoff_));
+  A.dpd_delta = (decltype(A.dpd_delta))(1.f / std::sqrt(dt_));
+  A.seed_lo = (decltype(A.seed_lo))((uint32_t)seed_);
+  A.seed_hi = (decltype(A.seed_hi))((uint32_t)(seed_ >> 32));
+  A.dpd_noise = (decltype(A.dpd_noise))(dpd_noise_);
+  A.time = (decltype(A.time))(d->time);
+  A.err = (decltype(A.err))(d->err);
+  A.partials = (decltype(A.partials))(d->partials);
+  A.cnt = (decltype(A.cnt))(d->cnt);
+  A.inc = (decltype(A.inc))(d->inc);
+  A.ncount = (decltype(A.ncount))(d->ncount);
+  A.random = (decltype(A.random))(any_random_ ? 1 : 0);
+  A.cutoff = (decltype(A.cutoff))((float)cutoff_);
+  A.nd_norm = (decltype(A.nd_norm))((float)(1.0 / (M_PI * cutoff_ * cutoff_ * cutoff_)));
 }
 
 // Uploads both parities of the force-kernel arguments (device resident, so
@@ -2287,20 +2296,7 @@ int Engine::neighbour_counts(int *counts) {
     if (int rc = upload_potentials()) return rc;
   DeviceState *d = d_;
   StepArgs A;
-  std::memset(&A, 0, sizeof(A));
-  A.x = d->x[0];
-  A.pid = d->pid[parity_];
-  A.cell = d->cell[0];
-  A.cstart = d->cstart[parity_];
-  A.rep = d->rep;
-  A.offs = d->offs;
-  A.shifts = d->shifts;
-  A.tiles = d->tiles;
-  A.ntiles = ntiles_;
-  const float cf = (float)cutoff_;
-  A.cut2_pair = cf * cf;
-  A.cut2_self = (float)(cutoff_ * cutoff_);
-  A.ncount = d->ncount;
+  fill_step_args(&A, parity_);
   hipLaunchKernelGGL(k_ncount, dim3(ntiles_), dim3(kTile), 0, d->stream, A);
   int rc = hipMemcpyAsync(counts, d->ncount, N_ * sizeof(int), hipMemcpyDeviceToHost, d->stream);
   rc |= hipStreamSynchronize(d->stream);
