@@ -139,6 +139,17 @@ int mxg_engine_set_max_dist_fraction(mxg_engine *e, float fraction);
  * working counterpart). */
 int mxg_engine_set_virial(mxg_engine *e, int on);
 
+/* Work unit of the force kernel: bricks of bx*by*bz cells staged with their
+ * halo in LDS, with a per-thread neighbour list of cap entries; bx = 0 picks
+ * the largest brick that fits; pmax > 0 caps the LDS particle capacity
+ * (bricks above it run from global memory).  No reference counterpart. */
+int mxg_engine_set_brick(mxg_engine *e, int bx, int by, int bz, int cap, int pmax);
+int mxg_engine_get_brick(mxg_engine *e, int dims[3], int *cap, int *pmax);
+
+/* Replay steps from captured HIP graphs (two steps per graph); on by
+ * default.  No reference counterpart (mdcore drives pthread runners). */
+int mxg_engine_set_graphs(mxg_engine *e, int on);
+
 /* engine_load (engine.h:612-613): x, v hold 3N doubles; particle ids are
  * 0..N-1 in argument order; replica and flags may be NULL. */
 int mxg_engine_load(mxg_engine *e, int N, const double *x, const double *v, const int *type,

@@ -218,6 +218,20 @@ class Universe:
     def set_max_dist_fraction(self, fraction):
         self._check(_capi.lib().mxg_engine_set_max_dist_fraction(self._h, float(fraction)))
 
+    def set_brick(self, bx=0, by=0, bz=0, cap=0, pmax=0):
+        """Brick shape of the force kernel (0 = automatic)."""
+        self._check(_capi.lib().mxg_engine_set_brick(self._h, int(bx), int(by), int(bz), int(cap), int(pmax)))
+
+    def brick(self):
+        d = np.zeros(3, np.int32)
+        cap, pmax = C.c_int(), C.c_int()
+        self._check(_capi.lib().mxg_engine_get_brick(self._h, _ip(d), C.byref(cap), C.byref(pmax)))
+        return tuple(int(x) for x in d), cap.value, pmax.value
+
+    def set_graphs(self, on=True):
+        """Replay pairs of steps from captured HIP graphs (default on)."""
+        self._check(_capi.lib().mxg_engine_set_graphs(self._h, int(bool(on))))
+
     def set_virial(self, on=True):
         self._check(_capi.lib().mxg_engine_set_virial(self._h, int(bool(on))))
 

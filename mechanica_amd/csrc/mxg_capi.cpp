@@ -163,6 +163,22 @@ int mxg_engine_set_max_dist_fraction(mxg_engine *e, float fraction) {
   return e->e.set_max_dist_fraction(fraction);
 }
 
+int mxg_engine_set_brick(mxg_engine *e, int bx, int by, int bz, int cap, int pmax) {
+  MXG_CHECK(e);
+  return e->e.set_brick(bx, by, bz, cap, pmax);
+}
+
+int mxg_engine_get_brick(mxg_engine *e, int dims[3], int *cap, int *pmax) {
+  if (!e || !dims || !cap || !pmax) return MXG_ERR_NULL;
+  e->e.brick(dims, cap, pmax);
+  return MXG_ERR_OK;
+}
+
+int mxg_engine_set_graphs(mxg_engine *e, int on) {
+  MXG_CHECK(e);
+  return e->e.set_graphs(on);
+}
+
 int mxg_engine_set_virial(mxg_engine *e, int on) {
   MXG_CHECK(e);
   return e->e.set_virial(on);

@@ -86,6 +86,9 @@ class Engine {
 
   int fail(int code, const std::string &msg);
 
+  // Replays captured HIP graphs of two steps (on by default).
+  int set_graphs(int on);
+
   // Adds the virial tensor to the per-step reductions.
   int set_virial(int on);
 
@@ -97,8 +100,9 @@ class Engine {
 
   // Shape of the force kernel's work unit: a brick of bx*by*bz cells staged
   // with its halo in LDS, and the per-thread neighbour-list length.
-  // set_brick(0, 0, 0, 0) picks the largest brick whose halo fits.
-  int set_brick(int bx, int by, int bz, int cap);
+  // set_brick(0, 0, 0, 0, 0) picks the largest brick whose halo fits; pmax
+  // (> 0) lowers the LDS particle capacity.
+  int set_brick(int bx, int by, int bz, int cap, int pmax);
   void brick(int dims[3], int *cap, int *pmax) const {
     for (int k = 0; k < 3; ++k) dims[k] = bdim_[k];
     *cap = cap_;
@@ -111,6 +115,7 @@ class Engine {
   int build_geometry();
   int upload_potentials();
   int launch_step();
+  int ensure_graph(int parity);
   void fill_step_args(void *out, int parity) const;
   int refresh_args();
   int check_device(int hip_status, const char *what);
@@ -136,7 +141,7 @@ class Engine {
 
   int N_ = 0, M_ = 0, ncells_ = 0, ntiles_ = 0;
   int bdim_[3] = {4, 4, 4};
-  int cap_ = 12, pmax_ = 0, hmax_ = 0, hdmax_ = 0, noff_ = 0, nbricks_ = 0;
+  int cap_ = 12, pmax_ = 0, pmax_req_ = 0, hmax_ = 0, hdmax_ = 0, noff_ = 0, nbricks_ = 0;
   bool brick_dpd_ = false;
   std::vector<double> mean_cell_count_;
   int block_ = 256;
@@ -152,6 +157,9 @@ class Engine {
   bool pid_prev_valid_ = false;  // pid[1] holds the order forces were computed in
   int parity_ = 0;               // which ping-pong ids/offsets/bond flags are current
   bool args_dirty_ = true;        // device copy of the kernel arguments is stale
+  bool use_graphs_ = true;
+  uint64_t config_gen_ = 1;       // bumped when the captured launches change
+  uint64_t graph_gen_[2] = {0, 0};
   DeviceState *d_ = nullptr;
 };
 

@@ -123,6 +123,7 @@ struct DeviceState {
   int *ovf_list = nullptr;
   int *ovf_count = nullptr;
   unsigned char *dargs = nullptr;  // two StepArgs, one per step parity
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // two captured steps per parity
   DevPot *pots = nullptr;
   DevPot *bpots = nullptr;
   float *coef = nullptr;
@@ -1400,6 +1401,8 @@ Engine::~Engine() {
       if (e) hipEventDestroy(e);
     for (auto &e : d_->kev)
       if (e) hipEventDestroy(e);
+    for (auto &g : d_->gexec)
+      if (g) hipGraphExecDestroy(g);
     if (d_->stream) hipStreamDestroy(d_->stream);
     delete d_;
   }
@@ -1833,7 +1836,7 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   pid_prev_valid_ = false;
   parity_ = 0;
   args_dirty_ = true;
-  return set_brick(0, 0, 0, 0);
+  return set_brick(0, 0, 0, 0, 0);
 }
 
 // Capacities of the LDS layout for a brick shape: halo cells, halo cells per
@@ -1870,7 +1873,7 @@ static BrickFit brick_fit(const std::vector<HostReplica> &reps, const std::vecto
 
 // Chooses (or sets) the brick shape of the force kernel and builds the brick
 // list.  bx = 0 picks the largest shape whose expected halo fits in LDS.
-int Engine::set_brick(int bx, int by, int bz, int cap) {
+int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
   if (!d_ || N_ == 0) return fail(kErrNull, "no particles loaded");
   const size_t budget = 65536;
   const bool dpd = any_dpd_ || std::any_of(pots_.begin(), pots_.end(), [](const Potential &p) {
@@ -1904,6 +1907,8 @@ int Engine::set_brick(int bx, int by, int bz, int cap) {
   }
   hmax = fitv.hmax;
   pm = fitv.pmax;
+  // An explicit capacity below the fit sends fuller bricks to the global path.
+  if (pmax > 0) pm = std::min(pm, (pmax + 31) / 32 * 32);
   if (fitv.hdmax > kHaloAxis) return fail(kErrRange, "brick too large");
   if (hmax > kScanItems * kBT) return fail(kErrRange, "brick halo has more than 1024 cells");
   if (pm < 64) return fail(kErrRange, "brick does not fit in LDS");
@@ -1931,6 +1936,7 @@ int Engine::set_brick(int bx, int by, int bz, int cap) {
   for (int k = 0; k < 3; ++k) bdim_[k] = bd[k];
   cap_ = c;
   pmax_ = pm;
+  pmax_req_ = pmax;
   hmax_ = hmax;
   hdmax_ = fitv.hdmax;
   noff_ = fitv.noff;
@@ -2086,6 +2092,7 @@ int Engine::refresh_args() {
   rc |= hipStreamSynchronize(d->stream);
   if (rc) return check_device(rc, "argument upload");
   args_dirty_ = false;
+  ++config_gen_;  // captured graphs hold the old arguments
   return kErrOk;
 }
 
@@ -2153,6 +2160,42 @@ int Engine::launch_step() {
   return (int)hipGetLastError();
 }
 
+// Captures two steps starting at parity par into a graph (kernel nodes only).
+int Engine::ensure_graph(int par) {
+  DeviceState *d = d_;
+  if (d->gexec[par] && graph_gen_[par] == config_gen_) return kErrOk;
+  if (d->gexec[par]) {
+    hipGraphExecDestroy(d->gexec[par]);
+    d->gexec[par] = nullptr;
+  }
+  const int saved = parity_;
+  parity_ = par;
+  hipGraph_t g = nullptr;
+  if (int st = hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal))
+    return check_device(st, "hipStreamBeginCapture");
+  int st1 = launch_step();
+  int st2 = launch_step();
+  int st3 = hipStreamEndCapture(d->stream, &g);
+  parity_ = saved;
+  if (st1 || st2 || st3) {
+    if (g) hipGraphDestroy(g);
+    return check_device(st1 ? st1 : (st2 ? st2 : st3), "graph capture");
+  }
+  int st4 = hipGraphInstantiate(&d->gexec[par], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (st4) {
+    d->gexec[par] = nullptr;
+    return check_device(st4, "hipGraphInstantiate");
+  }
+  graph_gen_[par] = config_gen_;
+  return kErrOk;
+}
+
+int Engine::set_graphs(int on) {
+  use_graphs_ = on != 0;
+  return kErrOk;
+}
+
 int Engine::step(int nsteps) {
   if (!d_ || N_ == 0) return fail(kErrNull, "no particles loaded");
   if (err_ != 0) return err_;
@@ -2161,7 +2204,7 @@ int Engine::step(int nsteps) {
   // Potentials bound after the load can change the LDS layout (velocities
   // are staged for DPD only).
   if (brick_dpd_ != any_dpd_)
-    if (int rc = set_brick(bdim_[0], bdim_[1], bdim_[2], cap_)) return rc;
+    if (int rc = set_brick(bdim_[0], bdim_[1], bdim_[2], cap_, pmax_req_)) return rc;
   if (args_dirty_)
     if (int rc = refresh_args()) return rc;
   if (timing_) {
@@ -2175,7 +2218,17 @@ int Engine::step(int nsteps) {
     hipEventRecord(d_->ev[0], d_->stream);
   }
   timed_steps_ = 0;
-  for (int k = 0; k < nsteps; ++k) {
+  int k = 0;
+  if (use_graphs_ && !timing_) {
+    // Pairs of steps replay a captured graph (the ping-pong buffers return
+    // to the same parity after two steps).
+    for (; k + 2 <= nsteps; k += 2) {
+      if (int rc = ensure_graph(parity_)) return rc;
+      if (int st = hipGraphLaunch(d_->gexec[parity_], d_->stream)) return check_device(st, "hipGraphLaunch");
+      pid_prev_valid_ = true;
+    }
+  }
+  for (; k < nsteps; ++k) {
     if (int st = launch_step()) return check_device(st, "kernel launch");
   }
   if (timing_) hipEventRecord(d_->ev[2], d_->stream);
@@ -2216,6 +2269,7 @@ int Engine::set_timing(int on) {
 
 int Engine::set_virial(int on) {
   virial_ = on != 0;
+  ++config_gen_;
   return kErrOk;
 }
 

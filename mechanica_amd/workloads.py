@@ -166,6 +166,17 @@ def cell_sort_params(kT=0.1, dt=0.01, mass=1.0):
     }
 
 
+def cell_sort(N=4000, seed=5, kT=0.1, dt=0.01):
+    """Two-type cell sorting: Morse adhesion plus the single-body random force
+    on every type, overdamped."""
+    w = cell_sort_replica(N, seed, kT=kT, dt=dt)
+    w.update(cell_sort_params(kT=kT, dt=dt))
+    w["name"] = "cell_sort"
+    w["v"] = np.zeros_like(w["x"])
+    w["bonds"] = None
+    return w
+
+
 def ensemble_sizes(R=256, lo=500, hi=20000, seed=4):
     rng = _rng(seed)
     return np.exp(rng.uniform(math.log(lo), math.log(hi), size=R)).astype(np.int64)
@@ -200,6 +211,12 @@ def build_universe(m, w, seed=None):
         pots.append(p)
     if seed is not None:
         u.set_seed(seed)
+    if w.get("random"):
+        rf = w["random"]
+        for t in range(len(w["types"])):
+            u.add_random_force(t, rf["mean"], rf["std"], rf["duration"])
+    if w.get("dpd_noise"):
+        u.set_dpd_noise(1)
     u.load(w["x"], w["v"], w["type"])
     if w.get("bonds"):
         b = w["bonds"]

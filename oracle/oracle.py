@@ -230,6 +230,12 @@ def from_workload(w, m, threads=1, seed=None):
         keep.append(p)
     if seed is not None:
         o.set_seed(seed)
+    if w.get("random"):
+        rf = w["random"]
+        for t in range(len(w["types"])):
+            o.add_random_force(t, rf["mean"], rf["std"], rf["duration"])
+    if w.get("dpd_noise"):
+        o.set_dpd_noise(1)
     o.load(w["x"], w["v"], w["type"])
     if w.get("bonds"):
         b = w["bonds"]

@@ -1,0 +1,188 @@
+"""GPU tests of the engine beyond single-step parity: stochastic forces
+against the oracle, the DPD thermostat against its fluctuation-dissipation
+temperature, graph replay and brick layouts against the eager default, and
+the batched ensemble against independent engines.
+
+Bars (stated per test):
+  * random and DPD forces: the same per-particle bar as the deterministic
+    forces (1e-5 relative), since both sides draw the same counter-based
+    numbers;
+  * graph replay, brick shape, global fallback, ensemble: bitwise equal
+    positions, velocities, forces and cells; energies within 1e-12 relative
+    (their per-brick partial sums are added in a different grouping when the
+    brick list changes).
+"""
+import numpy as np
+import pytest
+
+from mechanica_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-5
+
+
+def rel_force_error(fg, fr):
+    n = np.linalg.norm(fr, axis=1)
+    rms = np.sqrt((n ** 2).mean())
+    return (np.linalg.norm(fg - fr, axis=1) / np.maximum(n, 0.1 * rms)).max()
+
+
+def bitwise_same(a, b, keys=("x", "v", "f", "cell")):
+    for k in keys:
+        assert np.array_equal(np.asarray(a[k]).view(np.uint32), np.asarray(b[k]).view(np.uint32)), k
+
+
+def relay(u, o, steps):
+    """Each step the oracle advances with the GPU's forces; states agree
+    bitwise."""
+    for k in range(steps):
+        u.step(1)
+        g = u.state()
+        o.step_with_forces(g["f"])
+        r = o.state()
+        assert np.array_equal(g["cell"], r["cell"]), "step %d cell" % k
+        assert np.array_equal(g["x"].view(np.uint32), r["x"].view(np.uint32)), "step %d x" % k
+        assert np.array_equal(g["v"].view(np.uint32), r["v"].view(np.uint32)), "step %d v" % k
+
+
+def force_parity(m, oracle_mod, w, seed):
+    u = W.build_universe(m, w, seed=seed)
+    o = oracle_mod.from_workload(w, m, seed=seed)
+    u.step(1)
+    o.force()
+    sg, so = u.state(), o.state()
+    rel = rel_force_error(sg["f"].astype(np.float64), so["f"].astype(np.float64))
+    print("rel(gpu,ref32)=%.3e" % rel)
+    assert rel <= FORCE_TOL
+    return sg, so
+
+
+def test_dpd_random_force_parity(m, oracle_mod):
+    """DPD with the random term on, counter-based pair noise on both sides."""
+    w = W.dpd_fluid(N=4000, sigma=3.0, seed=41)
+    w["dpd_noise"] = True
+    print("case: dpd sigma=3 N=4000", flush=True)
+    force_parity(m, oracle_mod, w, 9)
+    relay(W.build_universe(m, w, seed=9), oracle_mod.from_workload(w, m, seed=9), 5)
+
+
+def test_single_body_random_force_parity(m, oracle_mod):
+    """Cell sorting: Morse plus the Philox single-body force, overdamped."""
+    w = W.cell_sort(N=3000, seed=42)
+    print("case: cell_sort N=3000", flush=True)
+    sg, so = force_parity(m, oracle_mod, w, 17)
+    assert np.array_equal(sg["pf"].view(np.uint32), so["pf"].view(np.uint32)), "persistent force"
+    relay(W.build_universe(m, w, seed=17), oracle_mod.from_workload(w, m, seed=17), 5)
+
+
+def test_dpd_temperature():
+    """Fluctuation-dissipation: sigma^2 = 2 gamma kT, so sigma = 3 and
+    gamma = 4.5 hold the fluid at kT = 1.  Euler integration at dt = 0.01
+    heats it by about 1% (friction rate ~1.9 per unit time); the bar is 4%."""
+    import mechanica_amd as m
+    w = W.dpd_fluid(N=24000, sigma=3.0, seed=43)
+    w["dpd_noise"] = True
+    print("case: dpd temperature N=24000", flush=True)
+    u = W.build_universe(m, w, seed=3)
+    u.step(300)
+    T = []
+    for _ in range(20):
+        u.step(10)
+        T.append(2.0 * u.observables()["ekin"] / (3.0 * len(w["x"])))
+    T = float(np.mean(T))
+    x, v = u.unload()
+    p = v.sum(axis=0)
+    print("kT=%.4f  |P|=%.3e" % (T, np.linalg.norm(p)))
+    assert abs(T - 1.0) < 0.04
+    # Pair forces are antisymmetric, so total momentum stays at its start
+    # value up to float rounding.
+    p0 = w["v"].sum(axis=0)
+    assert np.linalg.norm(p - p0) < 1e-2 * np.sqrt(len(w["x"]))
+
+
+def run_steps(m, w, seed, steps, graphs=False, brick=None, read_every=None):
+    u = W.build_universe(m, w, seed=seed)
+    if w.get("name") == "dpd_fluid":
+        u.set_dpd_noise(1)
+    if brick is not None:
+        u.set_brick(*brick)
+    u.set_graphs(graphs)
+    done = 0
+    while done < steps:
+        k = min(read_every or steps, steps - done)
+        u.step(k)
+        done += k
+        if read_every:
+            u.state()  # eager work and a synchronisation between replays
+    s = u.state()
+    return s, u.observables(), u
+
+
+def test_graph_replay_bitwise(m):
+    w = W.dpd_fluid(N=6000, sigma=3.0, seed=44)
+    print("case: graphs dpd N=6000", flush=True)
+    a, oa, _ = run_steps(m, w, 5, 23, graphs=False)
+    b, ob, _ = run_steps(m, w, 5, 23, graphs=True, read_every=5)
+    bitwise_same(a, b)
+    assert oa["ekin"] == ob["ekin"]
+
+
+@pytest.mark.parametrize("brick", [(1, 1, 1, 0, 0), (2, 2, 2, 0, 0), (4, 2, 1, 8, 0), (2, 2, 2, 0, 64)])
+def test_brick_layout_bitwise(m, brick):
+    """Brick shapes, list capacities and the global fallback (pmax = 64
+    sends nearly every brick there) give the same per-particle results."""
+    w = W.argon_lj(N=3000, seed=45, min_dist=0.275)
+    print("case: brick", brick, flush=True)
+    a, oa, _ = run_steps(m, w, 6, 5)
+    b, ob, ub = run_steps(m, w, 6, 5, brick=brick)
+    dims, cap, pmax = ub.brick()
+    print("brick", dims, cap, pmax)
+    bitwise_same(a, b)
+    for k in ("epot_nonbond", "ekin"):
+        assert abs(oa[k] - ob[k]) <= 1e-12 * max(abs(oa[k]), 1.0)
+
+
+def test_ensemble_matches_independent(m):
+    """Three argon replicas of different sizes in one engine, against three
+    engines of one replica each (same brick shape)."""
+    sizes = [1500, 2600, 4100]
+    ws = [W.argon_lj(N=n, seed=50 + i, min_dist=0.275) for i, n in enumerate(sizes)]
+    print("case: ensemble", sizes, flush=True)
+    brick = (2, 2, 2, 0, 0)
+    singles = []
+    for w in ws:
+        s, o, _ = run_steps(m, w, 8, 6, brick=brick)
+        singles.append((s, o))
+    w0 = ws[0]
+    u = m.Universe(cutoff=w0["cutoff"], dt=w0["dt"], replicas=[(w["dim"], w["cells"]) for w in ws])
+    for t in w0["types"]:
+        u.add_type(t["mass"], t["radius"], t["dynamics"])
+    pots = []
+    for spec in w0["pots"]:
+        p = W.make_potential(m, spec)
+        u.bind(p, spec[2], spec[3])
+        pots.append(p)
+    u.set_seed(8)
+    x = np.concatenate([w["x"] for w in ws])
+    v = np.concatenate([w["v"] for w in ws])
+    t = np.concatenate([w["type"] for w in ws])
+    rep = np.concatenate([np.full(len(w["x"]), i, np.int32) for i, w in enumerate(ws)])
+    u.load(x, v, t, replica=rep)
+    u.set_brick(*brick)
+    u.step(6)
+    s = u.state()
+    base = 0
+    cbase = 0  # cells are numbered across the ensemble, replica by replica
+    for i, (si, oi) in enumerate(singles):
+        n = sizes[i]
+        part = {k: s[k][base:base + n] for k in ("x", "v", "f", "cell")}
+        part["cell"] = part["cell"] - cbase
+        bitwise_same(part, si)
+        cbase += int(np.prod(ws[i]["cells"]))
+        ob = u.observables(i)
+        for k in ("epot_nonbond", "ekin"):
+            assert abs(ob[k] - oi[k]) <= 1e-12 * max(abs(oi[k]), 1.0), (i, k)
+        base += n
+    tot = u.observables(-1)
+    assert abs(tot["ekin"] - sum(o["ekin"] for _, o in singles)) <= 1e-12 * tot["ekin"]
