@@ -206,6 +206,56 @@ const char *mxg_engine_error_message(const mxg_engine *e);
 int mxg_engine_set_timing(mxg_engine *e, int on);
 int mxg_engine_timing(mxg_engine *e, double *force_ms, double *step_ms);
 
+/* ------------------------------------------------------------------------
+ * Adjoint: a differentiable overdamped Morse system and the discrete adjoint
+ * of its trajectory.  No reference counterpart: the forward map restates
+ * engine_step's overdamped update (engine_advance.cpp:126-239) with the
+ * Morse pair term of potential_eval_ex (potential_eval.hpp:106-161, shift
+ * rule of potential_create_morse, MxPotential.cpp) evaluated analytically,
+ * so that its parameter derivatives are exact.  Buffers marked (device) are
+ * device pointers; work runs on `stream` (a hipStream_t, 0 = default).
+ * ------------------------------------------------------------------------ */
+typedef struct mxg_adjoint mxg_adjoint;
+
+/* N particles of ntypes (<= 4) types in a periodic box dim with a cells grid
+ * (>= 3 per axis, cells no narrower than cutoff); mass and radius per type;
+ * type per particle (host arrays).  max_dist_fraction: the overdamped
+ * displacement clamp as a fraction of the cell width. */
+int mxg_adjoint_create(mxg_adjoint **out, int N, const double dim[3], const int cells[3], double cutoff, double dt,
+                       float max_dist_fraction, int ntypes, const double *mass, const double *radius,
+                       const int *type);
+void mxg_adjoint_free(mxg_adjoint *a);
+const char *mxg_adjoint_error_message(const mxg_adjoint *a);
+
+/* theta (host): ntypes*(ntypes+1)/2 rows of (d, a, r0), one per type pair
+ * ti <= tj in row-major upper-triangle order; [smin, smax] is the Morse
+ * domain (potential_create_morse's min and max).
+ * Forward: nsteps steps from x0 (device, N*3 FP64, rounded to FP32) to xT
+ * (device, N*3 FP64); ckpt (device, ceil(nsteps/every)*N*4 FP32) receives
+ * the state every `every` steps. */
+int mxg_adjoint_forward(mxg_adjoint *a, const double *theta, double smin, double smax, const double *x0, double *xT,
+                        float *ckpt, int nsteps, int every, void *stream);
+
+/* Backward: from dL/dxT (device, N*3) and the forward's checkpoints, the
+ * segments are recomputed and swept in reverse; writes dL/dx0 (device,
+ * N*3) and dL/dtheta (host, 3 per type pair), accumulated in FP64.
+ * Synchronises the stream. */
+int mxg_adjoint_backward(mxg_adjoint *a, const double *theta, double smin, double smax, const float *ckpt, int nsteps,
+                         int every, const double *gxT, double *gx0, double *gtheta, void *stream);
+
+/* Forces (device, N*3 FP32) at x (device, N*3 FP64) and the total Morse
+ * energy (host, FP64 sum).  Synchronises the stream. */
+int mxg_adjoint_forces(mxg_adjoint *a, const double *theta, double smin, double smax, const double *x, float *f,
+                       double *epot, void *stream);
+
+/* Soft pair-distance histogram per type pair (device, npairs*nbins FP64):
+ * each pair with r < rmax (<= cutoff) adds the biweight (1 - t^2)^2,
+ * |t| < 1, t = r/dr - (k + 1/2), to bins k; and its gradient, dL/dx
+ * (device, N*3) from dL/dhist (device). */
+int mxg_adjoint_rdf(mxg_adjoint *a, const double *x, int nbins, double rmax, double *hist, void *stream);
+int mxg_adjoint_rdf_backward(mxg_adjoint *a, const double *x, int nbins, double rmax, const double *ghist,
+                             double *gx, void *stream);
+
 /* Number of visible HIP devices (0 when none). */
 int mxg_device_count(void);
 

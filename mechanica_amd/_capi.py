@@ -75,6 +75,19 @@ SIGNATURES = {
     "mxg_engine_error_message": (C.c_char_p, [C.c_void_p]),
     "mxg_engine_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "mxg_engine_timing": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    "mxg_adjoint_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, c_double_p, c_int_p, C.c_double, C.c_double,
+                                     C.c_float, C.c_int, c_double_p, c_double_p, c_int_p]),
+    "mxg_adjoint_free": (None, [C.c_void_p]),
+    "mxg_adjoint_error_message": (C.c_char_p, [C.c_void_p]),
+    "mxg_adjoint_forward": (C.c_int, [C.c_void_p, c_double_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "mxg_adjoint_backward": (C.c_int, [C.c_void_p, c_double_p, C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_int,
+                                       C.c_void_p, C.c_void_p, c_double_p, C.c_void_p]),
+    "mxg_adjoint_forces": (C.c_int, [C.c_void_p, c_double_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                     c_double_p, C.c_void_p]),
+    "mxg_adjoint_rdf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]),
+    "mxg_adjoint_rdf_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
     "mxg_device_count": (C.c_int, []),
 }
 

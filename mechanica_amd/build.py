@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmxg.so")
 
-SOURCES = ["mx_potential.cpp", "mxg_engine.hip", "mxg_capi.cpp"]
+SOURCES = ["mx_potential.cpp", "mxg_engine.hip", "mxg_adjoint.hip", "mxg_capi.cpp"]
 
 # -ffp-contract=off: the reference never fuses multiply-adds, and bit-exact
 # cell assignment depends on the same rounding.  Packed FP32 instructions

@@ -1,0 +1,53 @@
+// Differentiable overdamped Morse system: the forward trajectory of
+// engine_step's overdamped path with the Morse pair force evaluated
+// analytically, and its discrete adjoint.
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace mxg {
+
+constexpr int kAdjMaxTypes = 4;
+constexpr int kAdjMaxPairs = kAdjMaxTypes * (kAdjMaxTypes + 1) / 2;
+constexpr int kAdjMaxBins = 256;
+
+struct AdjointDev;
+
+class Adjoint {
+ public:
+  ~Adjoint();
+  int create(int N, const double dim[3], const int cells[3], double cutoff, double dt, float max_dist_fraction,
+             int ntypes, const double *mass, const double *radius, const int *type);
+  // theta: one (d, a, r0) row per type pair (ti <= tj, row-major upper
+  // triangle); [smin, smax] is the Morse potential's domain.
+  int forward(const double *theta, double smin, double smax, const double *x0, double *xT, float *ckpt, int nsteps,
+              int every, void *stream);
+  int backward(const double *theta, double smin, double smax, const float *ckpt, int nsteps, int every,
+               const double *gxT, double *gx0, double *gtheta, void *stream);
+  int forces(const double *theta, double smin, double smax, const double *x, float *f, double *epot, void *stream);
+  int rdf(const double *x, int nbins, double rmax, double *hist, void *stream);
+  int rdf_backward(const double *x, int nbins, double rmax, const double *ghist, double *gx, void *stream);
+  const char *message() const { return msg_.c_str(); }
+  int npairs() const { return ntypes_ * (ntypes_ + 1) / 2; }
+
+ private:
+  int fail(int code, const std::string &m) {
+    msg_ = m;
+    return code;
+  }
+  int device(int st, const char *what);
+  int ensure_workspace(int every);
+  int set_args(const double *theta, double smin, double smax);
+  void bin(const void *x, int slot, void *stream);
+
+  AdjointDev *d_ = nullptr;
+  int N_ = 0, ntypes_ = 0, ncells_ = 0, ws_cap_ = 0, nblocks_ = 0;
+  int cdim_[3] = {0, 0, 0};
+  double dim_[3] = {0, 0, 0}, cutoff_ = 0, dt_ = 0;
+  float maxfrac_ = 0.05f;
+  std::vector<double> mass_, radius_;
+  std::string msg_;
+};
+
+}  // namespace mxg

@@ -8,6 +8,7 @@
 
 #include "../../include/mxg.h"
 #include "mx_potential.h"
+#include "mxg_adjoint.h"
 #include "mxg_engine.h"
 
 struct mxg_potential {
@@ -16,6 +17,10 @@ struct mxg_potential {
 
 struct mxg_engine {
   mxg::Engine e;
+};
+
+struct mxg_adjoint {
+  mxg::Adjoint a;
 };
 
 namespace {
@@ -265,6 +270,51 @@ int mxg_engine_timing(mxg_engine *e, double *force_ms, double *step_ms) {
   if (force_ms) *force_ms = e->e.force_kernel_ms();
   if (step_ms) *step_ms = e->e.step_ms();
   return MXG_ERR_OK;
+}
+
+int mxg_adjoint_create(mxg_adjoint **out, int N, const double dim[3], const int cells[3], double cutoff, double dt,
+                       float max_dist_fraction, int ntypes, const double *mass, const double *radius,
+                       const int *type) {
+  if (!out) return MXG_ERR_NULL;
+  *out = nullptr;
+  auto *h = new (std::nothrow) mxg_adjoint();
+  if (!h) return MXG_ERR_MALLOC;
+  int rc = h->a.create(N, dim, cells, cutoff, dt, max_dist_fraction, ntypes, mass, radius, type);
+  *out = h;  // kept on failure so the message can be read; free it
+  return rc;
+}
+
+void mxg_adjoint_free(mxg_adjoint *a) { delete a; }
+
+const char *mxg_adjoint_error_message(const mxg_adjoint *a) { return a ? a->a.message() : "null adjoint"; }
+
+int mxg_adjoint_forward(mxg_adjoint *a, const double *theta, double smin, double smax, const double *x0, double *xT,
+                        float *ckpt, int nsteps, int every, void *stream) {
+  if (!a) return MXG_ERR_NULL;
+  return a->a.forward(theta, smin, smax, x0, xT, ckpt, nsteps, every, stream);
+}
+
+int mxg_adjoint_backward(mxg_adjoint *a, const double *theta, double smin, double smax, const float *ckpt, int nsteps,
+                         int every, const double *gxT, double *gx0, double *gtheta, void *stream) {
+  if (!a) return MXG_ERR_NULL;
+  return a->a.backward(theta, smin, smax, ckpt, nsteps, every, gxT, gx0, gtheta, stream);
+}
+
+int mxg_adjoint_forces(mxg_adjoint *a, const double *theta, double smin, double smax, const double *x, float *f,
+                       double *epot, void *stream) {
+  if (!a) return MXG_ERR_NULL;
+  return a->a.forces(theta, smin, smax, x, f, epot, stream);
+}
+
+int mxg_adjoint_rdf(mxg_adjoint *a, const double *x, int nbins, double rmax, double *hist, void *stream) {
+  if (!a) return MXG_ERR_NULL;
+  return a->a.rdf(x, nbins, rmax, hist, stream);
+}
+
+int mxg_adjoint_rdf_backward(mxg_adjoint *a, const double *x, int nbins, double rmax, const double *ghist,
+                             double *gx, void *stream) {
+  if (!a) return MXG_ERR_NULL;
+  return a->a.rdf_backward(x, nbins, rmax, ghist, gx, stream);
 }
 
 int mxg_device_count(void) {

@@ -177,6 +177,82 @@ def cell_sort(N=4000, seed=5, kT=0.1, dt=0.01):
     return w
 
 
+def adjoint_morse(N=65536, L=40.0, seed=6, min_dist=None):
+    """Two-type overdamped Morse system for the adjoint: uniform start in a
+    periodic box, Morse d = 3 (AA, BB) and 1 (AB), a = 6, r0 = 1, cutoff
+    2.5, dt 0.01; theta rows (d, a, r0) for AA, AB, BB."""
+    rng = _rng(seed)
+    if min_dist:
+        x = _min_dist_fill(rng, N, L, min_dist)
+    else:
+        x = rng.random((N, 3)) * L
+    t = np.zeros(N, np.int32)
+    t[N // 2:] = 1
+    rng.shuffle(t)
+    theta = np.array([[3.0, 6.0, 1.0], [1.0, 6.0, 1.0], [3.0, 6.0, 1.0]])
+    return {"name": "adjoint_morse", "dim": [L, L, L], "cutoff": 2.5, "dt": 0.01, "x": x, "type": t,
+            "mass": (1.0, 1.0), "radius": (1.0, 1.0), "theta": theta, "smin": 1e-4, "smax": 2.5}
+
+
+def rdf_target(w, nbins=64, rmax=2.5, seed=7):
+    """A seeded target g(r): the soft g(r) of an independent uniform
+    configuration of the same size and composition, on the host."""
+    rng = _rng(seed)
+    L = np.asarray(w["dim"], np.float64)
+    N = len(w["x"])
+    x = rng.random((N, 3)) * L
+    t = np.asarray(w["type"])
+    nt = int(t.max()) + 1
+    np_ = nt * (nt + 1) // 2
+    dr = rmax / nbins
+    H = np.zeros((np_, nbins))
+    cells = np.maximum(3, np.floor(L / rmax)).astype(int)
+    c = np.minimum((x / (L / cells)).astype(int), cells - 1)
+    lin = (c[:, 0] * cells[1] + c[:, 1]) * cells[2] + c[:, 2]
+    order = np.argsort(lin, kind="stable")
+    bounds = np.searchsorted(lin[order], np.arange(np.prod(cells) + 1))
+    pidx = {}
+    k = 0
+    for a in range(nt):
+        for b in range(a, nt):
+            pidx[(a, b)] = pidx[(b, a)] = k
+            k += 1
+    for cx in range(cells[0]):
+        for cy in range(cells[1]):
+            for cz in range(cells[2]):
+                me = order[bounds[(cx * cells[1] + cy) * cells[2] + cz]:bounds[(cx * cells[1] + cy) * cells[2] + cz + 1]]
+                if len(me) == 0:
+                    continue
+                nb = []
+                for ox in (-1, 0, 1):
+                    for oy in (-1, 0, 1):
+                        for oz in (-1, 0, 1):
+                            q = ((cx + ox) % cells[0] * cells[1] + (cy + oy) % cells[1]) * cells[2] + (cz + oz) % cells[2]
+                            nb.append(order[bounds[q]:bounds[q + 1]])
+                nb = np.unique(np.concatenate(nb))
+                d = x[me][:, None, :] - x[nb][None, :, :]
+                d -= L * np.round(d / L)
+                r = np.sqrt((d * d).sum(-1))
+                ii, jj = np.nonzero((r < rmax) & (me[:, None] < nb[None, :]))
+                rr = r[ii, jj]
+                pp = np.array([pidx[(int(t[me[a]]), int(t[nb[b]]))] for a, b in zip(ii, jj)], dtype=int)
+                qv = rr / dr - 0.5
+                k0 = np.floor(qv).astype(int)
+                for off in (0, 1):
+                    kk = k0 + off
+                    tt = qv - kk
+                    ok = (kk >= 0) & (kk < nbins) & (np.abs(tt) < 1)
+                    np.add.at(H, (pp[ok], kk[ok]), (1 - tt[ok] ** 2) ** 2)
+    n = np.bincount(t, minlength=nt).astype(np.float64)
+    pairs = []
+    for a in range(nt):
+        for b in range(a, nt):
+            pairs.append(n[a] * (n[a] - 1) / 2 if a == b else n[a] * n[b])
+    r = (np.arange(nbins) + 0.5) * dr
+    ideal = np.asarray(pairs)[:, None] * 4 * np.pi * r[None, :] ** 2 * dr / np.prod(L)
+    return H / ideal
+
+
 def ensemble_sizes(R=256, lo=500, hi=20000, seed=4):
     rng = _rng(seed)
     return np.exp(rng.uniform(math.log(lo), math.log(hi), size=R)).astype(np.int64)

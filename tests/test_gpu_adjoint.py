@@ -1,0 +1,161 @@
+"""GPU adjoint against the FP64 reference (oracle/adjoint_ref.py).
+
+Bars:
+  * forces of the FP32 forward: per particle within 1e-5 of the FP64
+    reference at the same (FP32-rounded) positions, floor 0.1 rms|F|;
+    energy within 2e-6 relative;
+  * trajectory: final positions within 1e-4 (minimum image) after 20 steps;
+  * gradients dL/dx0 and dL/dtheta: relative L2 error <= 1e-3 against the
+    reference's autograd gradient (the FP32 forward's rounding, amplified
+    by the stiff map, is the error);
+  * 16 random directions (8 in theta, 8 in x0): the GPU's directional
+    derivative within 2e-3 of FP64 central differences of the reference;
+  * checkpoint interval: gradients bitwise identical for every = 1..25;
+  * soft histogram within 1e-6 relative, its gradient within 1e-4.
+"""
+import numpy as np
+import pytest
+import torch
+
+from mechanica_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+NBINS, RMAX = 32, 2.5
+
+
+def setup(N=1000, L=12.6, seed=5, min_dist=0.97):
+    import mechanica_amd.adjoint as A
+    from oracle.adjoint_ref import RefSystem
+    w = W.adjoint_morse(N=N, L=L, seed=seed, min_dist=min_dist)
+    kw = dict(mass=w["mass"], radius=w["radius"], smin=w["smin"], smax=w["smax"])
+    g = A.MorseSystem(w["type"], w["dim"], w["cutoff"], w["dt"], **kw)
+    r = RefSystem(w["type"], w["dim"], w["cutoff"], w["dt"], **kw)
+    # The GPU forward starts from FP32-rounded positions; so does the reference.
+    w["x32"] = w["x"].astype(np.float32).astype(np.float64)
+    return A, g, r, w
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_forces_and_energy():
+    A, g, r, w = setup()
+    print("case: adjoint forces N=1000", flush=True)
+    th = torch.as_tensor(w["theta"])
+    f, e = g.forces(torch.as_tensor(w["x32"]), th)
+    fr = r.forces(torch.as_tensor(w["x32"]), th).numpy()
+    er = float(r.energy(torch.as_tensor(w["x32"]), th))
+    f = f.cpu().numpy().astype(np.float64)
+    n = np.linalg.norm(fr, axis=1)
+    floor = np.maximum(n, 0.1 * np.sqrt((n ** 2).mean()))
+    rel = (np.linalg.norm(f - fr, axis=1) / floor).max()
+    print("force rel %.3e  energy %.9g vs %.9g" % (rel, e, er))
+    assert rel <= 1e-5
+    assert abs(e - er) <= 2e-6 * abs(er)
+
+
+def test_trajectory_matches_reference():
+    A, g, r, w = setup()
+    print("case: adjoint trajectory N=1000 T=20", flush=True)
+    th = torch.as_tensor(w["theta"])
+    xT, _ = g.run(torch.as_tensor(w["x32"]), th, 20, 10)
+    with torch.no_grad():
+        xr = r.trajectory(torch.as_tensor(w["x32"]), th, 20).numpy()
+    d = xT.cpu().numpy() - xr
+    L = np.asarray(w["dim"])
+    d -= L * np.round(d / L)
+    print("max |dx| %.3e" % np.abs(d).max())
+    assert np.abs(d).max() <= 1e-4
+
+
+def gradients(A, g, r, w, nsteps, every=10):
+    target = torch.as_tensor(W.rdf_target(w, nbins=NBINS, rmax=RMAX))
+    x0 = torch.as_tensor(w["x32"]).cuda().requires_grad_(True)
+    th = torch.as_tensor(w["theta"]).clone().requires_grad_(True)
+    loss = A.rdf_loss(g, x0, th, target.cuda(), nsteps, every, NBINS, RMAX)
+    loss.backward()
+    gx, gt = x0.grad.cpu().numpy(), th.grad.numpy()
+    xr = torch.as_tensor(w["x32"]).clone().requires_grad_(True)
+    tr = torch.as_tensor(w["theta"]).clone().requires_grad_(True)
+    lr = r.loss(xr, tr, target, nsteps, NBINS, RMAX)
+    rx, rt = torch.autograd.grad(lr, (xr, tr))
+    return float(loss), gx, gt, float(lr), rx.numpy(), rt.numpy(), target
+
+
+@pytest.mark.parametrize("min_dist,nsteps", [(0.97, 20), (None, 10)])
+def test_gradients_match_reference(min_dist, nsteps):
+    """min_dist None: uniform start with overlapping pairs (clamped steps)."""
+    A, g, r, w = setup(min_dist=min_dist)
+    print("case: adjoint gradients min_dist=%s T=%d" % (min_dist, nsteps), flush=True)
+    lg, gx, gt, lr, rx, rt, _ = gradients(A, g, r, w, nsteps)
+    ex, et = rel_l2(gx, rx), rel_l2(gt, rt)
+    print("loss %.12g vs %.12g  dx0 rel %.3e  dtheta rel %.3e" % (lg, lr, ex, et))
+    assert abs(lg - lr) <= 1e-4 * abs(lr)
+    assert ex <= 1e-3
+    assert et <= 1e-3
+
+
+def test_directional_finite_differences():
+    A, g, r, w = setup()
+    nsteps = 20
+    print("case: adjoint FD directions T=%d" % nsteps, flush=True)
+    _, gx, gt, _, _, _, target = gradients(A, g, r, w, nsteps)
+    rng = np.random.default_rng(11)
+    x0 = torch.as_tensor(w["x32"])
+    th = torch.as_tensor(w["theta"])
+    eps = 1e-7
+    worst = 0.0
+    with torch.no_grad():
+        for k in range(16):
+            if k < 8:
+                v = rng.normal(size=th.shape)
+                vt = torch.as_tensor(v)
+                fd = (float(r.loss(x0, th + eps * vt, target, nsteps, NBINS, RMAX)) -
+                      float(r.loss(x0, th - eps * vt, target, nsteps, NBINS, RMAX))) / (2 * eps)
+                an = float((gt * v).sum())
+            else:
+                v = rng.normal(size=x0.shape)
+                vx = torch.as_tensor(v)
+                fd = (float(r.loss(x0 + eps * vx, th, target, nsteps, NBINS, RMAX)) -
+                      float(r.loss(x0 - eps * vx, th, target, nsteps, NBINS, RMAX))) / (2 * eps)
+                an = float((gx * v).sum())
+            err = abs(an - fd) / max(abs(fd), 1e-12)
+            worst = max(worst, err)
+            print("dir %2d  gpu %.9g  fd %.9g  rel %.2e" % (k, an, fd, err))
+    assert worst <= 2e-3
+
+
+def test_checkpoint_interval_bitwise():
+    A, g, r, w = setup(N=600, L=10.5)
+    print("case: adjoint checkpoints", flush=True)
+    th = torch.as_tensor(w["theta"])
+    x0 = torch.as_tensor(w["x32"]).cuda()
+    gxT = torch.as_tensor(np.random.default_rng(2).normal(size=(600, 3))).cuda()
+    outs = []
+    for every in (1, 4, 10, 25):
+        _, ck = g.run(x0, th, 20, every)
+        gx, gt = g.adjoint(th, ck, 20, every, gxT)
+        outs.append((gx.cpu().numpy(), gt.numpy()))
+    for gx, gt in outs[1:]:
+        assert np.array_equal(gx, outs[0][0])
+        assert np.array_equal(gt, outs[0][1])
+
+
+def test_soft_histogram():
+    A, g, r, w = setup()
+    print("case: soft histogram", flush=True)
+    x = torch.as_tensor(w["x32"])
+    hg = g.histogram(x, NBINS, RMAX).cpu().numpy()
+    xr = x.clone().requires_grad_(True)
+    hr = r.histogram(xr, NBINS, RMAX)
+    gh = torch.as_tensor(np.random.default_rng(3).normal(size=hr.shape))
+    (hr * gh).sum().backward()
+    gg = g.histogram_backward(x, NBINS, RMAX, gh).cpu().numpy()
+    e1 = np.abs(hg - hr.detach().numpy()).max() / np.abs(hr.detach().numpy()).max()
+    e2 = rel_l2(gg, xr.grad.numpy())
+    print("hist rel %.3e  grad rel %.3e" % (e1, e2))
+    assert e1 <= 1e-6
+    assert e2 <= 1e-4
