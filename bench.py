@@ -1,18 +1,23 @@
 """Benchmark: particle-steps/s of the force-and-advance step on one MI355X.
 
-Workload (the single-GPU configuration the metric is quoted on): a DPD fluid
-of N = 1,048,576 particles at number density 3 in a periodic cube of side
-70.44 with 70^3 cells, alpha = 25, gamma = 4.5, sigma = 3 with the
+Default workload (the single-GPU configuration the metric is quoted on): a
+DPD fluid of N = 1,048,576 particles at number density 3 in a periodic cube
+of side 70.44 with 70^3 cells, alpha = 25, gamma = 4.5, sigma = 3 with the
 counter-based pair noise, cutoff 1, dt = 0.01, FP32 state and FP64
 reductions.  A step is one engine_step: non-bonded DPD forces, advance, cell
 migration and the cell-list rebuild, all on the GPU with the state resident
 in HBM.
 
+Other workloads (--workload): lj (argon LJ at density 10, 1M particles),
+lattice (64^3 bonded Morse lattice, overdamped), ensemble (256 cell-sorting
+replicas in one engine, compared with one universe of the same total size)
+and adjoint (65,536-particle Morse trajectory of 50 steps, forward and
+backward, compared with the plain step on the same system).
+
 Printed: one JSON line with the throughput, the roofline of the dominant
-kernel (the fused force/advance kernel, timed with HIP events on the
-engine's stream) and the CPU baseline (this repository's restatement of the
-reference's CPU step, multi-threaded, timed on a bounded sample of the same
-workload on the host cores).
+kernel (timed with HIP events on the stream it runs on) and the CPU baseline
+(this repository's restatement of the reference's CPU step, multi-threaded,
+timed on a bounded sample of the same workload on the host cores).
 """
 import argparse
 import json
@@ -24,34 +29,62 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per particle-step for Newtonian / DPD particles:
-# read x, cell, type, v (32 B), write and read f (24 B), write x, cell, v
-# (28 B) = 84 B.  The fused kernel keeps f in registers, so it moves less.
-BYTES_PER_PARTICLE_STEP = 84
+# Algorithmic bytes per particle-step: read x, cell, type (20 B) and v
+# (12 B) for Newtonian or DPD particles, write and read f (24 B), write x
+# and cell (16 B) and v (12 B): 84 B; overdamped particles have no v: 60 B.
+# Bonds add 12 B each (two indices and a potential id).
+BYTES_NEWTONIAN = 84
+BYTES_OVERDAMPED = 60
+BYTES_PER_BOND = 12
+
+# HBM traffic per force-kernel launch measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE on the default workload (DPD, 1M particles),
+# scaled by the copy-kernel calibration (bench.py --copy-calibration under
+# the same counters); the summary and its commands are kept in
+# profiles/k_brick_dpd1m_counters.md.  None until measured.
+MEASURED_TRAFFIC = {"dpd": None}
+
+METRIC = "particle-steps/s (1M LJ/DPD)"
 
 
-def build_workload(args):
-    import mechanica_amd as m
-    from mechanica_amd import workloads as W
-    if args.workload == "dpd":
-        w = W.dpd_fluid(N=args.n, sigma=3.0, seed=2)
-    elif args.workload == "lj":
-        w = W.argon_lj(N=args.n, seed=1)
-    else:
-        raise SystemExit("unknown workload " + args.workload)
-    u = W.build_universe(m, w, seed=1234)
-    if args.workload == "dpd":
-        u.set_dpd_noise(1)
-    return m, w, u
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, seconds, threads):
+def timed_steps(u, steps, warmup, graphs=True):
+    u.set_graphs(graphs)
+    u.step(warmup)
+    u.sync()
+    t0 = time.perf_counter()
+    u.step(steps)
+    u.sync()
+    return time.perf_counter() - t0
+
+
+def kernel_ms(u, steps):
+    """Mean force-kernel and step time (ms) from HIP events on the engine's
+    stream."""
+    u.set_timing(True)
+    u.step(min(steps, 100))
+    f, s = u.timing()
+    u.set_timing(False)
+    return f, s
+
+
+def roofline(bytes_alg, ms, kernel, traffic=None):
+    achieved = bytes_alg / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel, "kernel_ms": ms,
+            "bytes_per_launch": bytes_alg}
+
+
+def cpu_baseline_oracle(w, seconds, threads, dpd_noise=False, seed=1234):
     """Times the oracle (CPU restatement of the reference step) with
-    `threads` runners on the same workload, for about `seconds`."""
+    `threads` runners on workload w for about `seconds`."""
     import mechanica_amd as m
     from oracle import oracle as O
-    o = O.from_workload(w, m, threads=threads, seed=1234)
-    if w["name"] == "dpd_fluid":
+    o = O.from_workload(w, m, threads=threads, seed=seed)
+    if dpd_noise:
         o.set_dpd_noise(1)
     o.step(1)  # warm-up
     N = len(w["x"])
@@ -62,16 +95,197 @@ def cpu_baseline(w, seconds, threads):
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
-    total_steps = steps + 1
-    total_t = dt + one
-    return {
-        "value": N * total_steps / total_t,
-        "unit": "particle-steps/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": "%s N=%d, %d timed steps after 1 warm-up step, %d threads, %.1f s" % (
-            w["name"], N, total_steps, threads, total_t),
-    }
+    total_steps, total_t = steps + 1, dt + one
+    return {"value": N * total_steps / total_t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "%s N=%d, %d timed steps after 1 warm-up step, %d threads, %.1f s" % (
+                w["name"], N, total_steps, threads, total_t)}
+
+
+def line(args, value, wall, dtype, config, roof, cpu, metric=METRIC, **extra):
+    out = {"metric": metric, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config,
+           "roofline": roof, "cpu_baseline": cpu}
+    if cpu:
+        out["speedup_vs_cpu"] = value / cpu["value"]
+    out.update(extra)
+    return out
+
+
+# ------------------------------------------------------------------ workloads
+
+def bench_single(args, threads):
+    import mechanica_amd as m
+    from mechanica_amd import workloads as W
+    if args.workload == "dpd":
+        w = W.dpd_fluid(N=args.n, sigma=3.0, seed=2)
+        w["dpd_noise"] = True
+        cfg = "dpd_fluid N=%d density=3 alpha=25 gamma=4.5 sigma=3 (counter RNG) cutoff=1 dt=0.01 cells=%dx%dx%d"
+        per, bonds = BYTES_NEWTONIAN, 0
+    elif args.workload == "lj":
+        w = W.argon_lj(N=args.n, seed=1)
+        cfg = "argon_lj N=%d density=10 LJ 12-6 on [0.275,1] tol=1e-3 cutoff=1 dt=0.01 cells=%dx%dx%d"
+        per, bonds = BYTES_NEWTONIAN, 0
+    else:  # lattice
+        w = W.lattice(n=64, seed=3)
+        cfg = ("lattice N=%d (64^3) Morse(d=1,a=6,r0=1) cutoff=2 + 774144 harmonic bonds k=100 r0=1 "
+               "dissociation 5, overdamped dt=0.005 cells=%dx%dx%d")
+        per, bonds = BYTES_OVERDAMPED, len(w["bonds"]["i"])
+    u = W.build_universe(m, w, seed=1234)
+    N = len(w["x"])
+    log("workload %s N=%d built" % (w["name"], N))
+    tune = u.autotune(steps_per=2) if args.autotune else None
+    wall = timed_steps(u, args.steps, args.warmup)
+    value = N * args.steps / wall
+    log("timed %d steps in %.3f s" % (args.steps, wall))
+    force_ms, step_ms = kernel_ms(u, args.steps)
+    bytes_alg = per * N + BYTES_PER_BOND * bonds
+    roof = roofline(bytes_alg, force_ms, "k_brick (LDS brick force + advance) with its global-memory fallback",
+                    MEASURED_TRAFFIC.get(args.workload))
+    roof["step_ms_events"] = step_ms
+    cpu = None if args.no_cpu_baseline else cpu_baseline_oracle(w, args.cpu_seconds, threads,
+                                                                dpd_noise=bool(w.get("dpd_noise")))
+    config = {"workload": cfg % ((N,) + tuple(w["cells"])), "particles": N, "cells": w["cells"],
+              "brick": u.brick()}
+    extra = {"autotune": tune} if tune else {}
+    return line(args, value, wall, "f32", config, roof, cpu, **extra)
+
+
+def bench_ensemble(args, threads):
+    import mechanica_amd as m
+    from mechanica_amd import workloads as W
+    e = W.ensemble(R=args.replicas)
+    sizes = [len(r["x"]) for r in e["replicas"]]
+    N = sum(sizes)
+    u = W.build_ensemble(m, e, seed=1234)
+    log("ensemble of %d replicas, %d particles built" % (len(sizes), N))
+    wall = timed_steps(u, args.steps, args.warmup)
+    value = N * args.steps / wall
+    force_ms, step_ms = kernel_ms(u, args.steps)
+    del u
+    # The plain kernel: one cell-sorting universe of the same total size.
+    w = W.cell_sort(N=N, seed=5)
+    p = W.build_universe(m, w, seed=1234)
+    pwall = timed_steps(p, args.steps, args.warmup)
+    pforce, _ = kernel_ms(p, args.steps)
+    plain = {"value": N * args.steps / pwall, "workload": "cell_sort N=%d cells=%dx%dx%d (one universe)" % (
+        (N,) + tuple(w["cells"])), "force_kernel_ms": pforce}
+    del p
+    roof = roofline(BYTES_OVERDAMPED * N, force_ms, "k_brick over all replicas (one launch per step)")
+    roof["step_ms_events"] = step_ms
+    cpu = None
+    if not args.no_cpu_baseline:
+        # A bounded sample: the oracle on replicas taken in order until
+        # about the requested time is spent, rate over the sample.
+        import numpy as np
+        from oracle import oracle as O
+        done_n, done_t, used = 0, 0.0, 0
+        for r in e["replicas"]:
+            wr = dict(r)
+            wr.update({k: e[k] for k in ("cutoff", "dt", "types", "pots", "random")})
+            wr.update({"name": "cell_sort", "bonds": None, "v": np.zeros_like(r["x"])})
+            o = O.from_workload(wr, m, threads=threads, seed=1234)
+            t0 = time.perf_counter()
+            o.step(5)
+            done_t += time.perf_counter() - t0
+            done_n += 5 * len(wr["x"])
+            used += 1
+            if done_t > args.cpu_seconds:
+                break
+        cpu = {"value": done_n / done_t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": "first %d replicas, 5 steps each, %d threads, %.1f s" % (used, threads, done_t)}
+    config = {"workload": "ensemble of %d two-type cell-sorting replicas, N_i log-uniform in [500, 20000], "
+                          "total %d, density 0.8, Morse d=3/1 a=6 r0=1 cutoff 2.5, overdamped Langevin kT=0.1, "
+                          "dt=0.01, hipGraph-captured steps" % (len(sizes), N), "particles": N,
+              "replicas": len(sizes)}
+    return line(args, value, wall, "f32", config, roof, cpu,
+                metric="batched particle-steps/s vs plain kernel", plain_kernel=plain,
+                batched_over_plain=value / plain["value"])
+
+
+def bench_adjoint(args, threads):
+    import numpy as np
+    import torch
+    import mechanica_amd as m
+    import mechanica_amd.adjoint as A
+    from mechanica_amd import workloads as W
+    w = W.adjoint_morse(N=65536, L=40.0, seed=6)
+    N, T, every = len(w["x"]), 50, 10
+    kw = dict(mass=w["mass"], radius=w["radius"], smin=w["smin"], smax=w["smax"])
+    s = A.MorseSystem(w["type"], w["dim"], w["cutoff"], w["dt"], **kw)
+    target = torch.as_tensor(W.rdf_target(w)).cuda()
+    x0 = torch.as_tensor(w["x"]).cuda()
+    th0 = torch.as_tensor(w["theta"])
+
+    def one():
+        x = x0.clone().requires_grad_(True)
+        th = th0.clone().requires_grad_(True)
+        loss = A.rdf_loss(s, x, th, target, T, every)
+        loss.backward()
+        return loss
+
+    reps = max(1, args.steps // T)
+    for _ in range(max(1, args.warmup // T)):
+        one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss = one()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = N * T * reps / wall
+    # Forward alone.
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.run(x0, th0, T, every)
+    torch.cuda.synchronize()
+    fwd = N * T * reps / (time.perf_counter() - t0)
+    # The plain engine step on the same system (tabulated Morse).
+    u = m.Universe(dim=w["dim"], cells=[16, 16, 16], cutoff=2.5, dt=0.01)
+    for _ in range(2):
+        u.add_type(1.0, 1.0, 1)
+    keep = []
+    for (a, b), row in zip(((0, 0), (0, 1), (1, 1)), w["theta"]):
+        p = m.Potential.morse(d=row[0], a=row[1], r0=row[2], min=1e-4, max=2.5, tol=1e-3)
+        u.bind(p, a, b)
+        keep.append(p)
+    u.load(w["x"], None, w["type"])
+    pwall = timed_steps(u, args.steps, args.warmup)
+    plain = {"value": N * args.steps / pwall, "workload": "same system through the engine step (tabulated "
+             "Morse tol 1e-3, overdamped)"}
+    step_ms = 1e3 * wall / (reps * T)
+    roof = roofline(BYTES_OVERDAMPED * N * 3, step_ms,
+                    "adjoint step: forward + recomputed forward + k_backward per trajectory step")
+    cpu = None
+    if not args.no_cpu_baseline:
+        wc = {"name": "adjoint_morse", "dim": w["dim"], "cells": [16, 16, 16], "cutoff": 2.5, "dt": 0.01,
+              "types": [{"mass": 1.0, "radius": 1.0, "dynamics": 1}] * 2, "x": w["x"],
+              "v": np.zeros_like(w["x"]), "type": w["type"], "bonds": None,
+              "pots": [("morse", (row[0], row[1], row[2], 1e-4, 2.5, 1e-3), a, b)
+                       for (a, b), row in zip(((0, 0), (0, 1), (1, 1)), w["theta"])]}
+        cpu = cpu_baseline_oracle(wc, args.cpu_seconds, threads)
+        cpu["sample"] += " (forward step only: the reference has no adjoint)"
+    config = {"workload": "adjoint: N=65536 two-type Morse, box 40, uniform start, 50 overdamped steps dt=0.01, "
+                          "loss = MSE of the soft 64-bin g(r) on [0,2.5] vs a seeded target, checkpoints every 10, "
+                          "gradients w.r.t. x0 and (d,a,r0) per type pair", "particles": N, "steps_per_trajectory": T,
+              "trajectories": reps}
+    return line(args, value, wall, "f32 forward, f64 adjoint accumulation", config, roof, cpu,
+                metric="adjoint particle-steps/s (forward+backward) vs plain kernel", plain_kernel=plain,
+                forward_only=fwd, adjoint_over_plain=value / plain["value"], loss=float(loss))
+
+
+def copy_calibration(args):
+    """A plain device copy of known size for scaling FETCH_SIZE and
+    WRITE_SIZE: 256 MiB read and 256 MiB written per launch."""
+    import torch
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}), flush=True)
 
 
 def main():
@@ -79,82 +293,30 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="dpd")
+    ap.add_argument("--workload", default="dpd", choices=["dpd", "lj", "lattice", "ensemble", "adjoint"])
     ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-
-    m, w, u = build_workload(args)
-    N = len(w["x"])
-    print("workload %s N=%d built" % (w["name"], N), file=sys.stderr, flush=True)
-
-    u.step(args.warmup)
-    u.sync()
-    t0 = time.perf_counter()
-    u.step(args.steps)
-    u.sync()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    value = N * args.steps / wall
-    print("timed %d steps in %.3f s" % (args.steps, wall), file=sys.stderr, flush=True)
-
-    # Kernel durations with HIP events on the engine's stream.
-    u.set_timing(True)
-    ksteps = min(args.steps, 100)
-    u.step(ksteps)
-    force_ms, step_ms = u.timing()
-    u.set_timing(False)
-    bytes_alg = BYTES_PER_PARTICLE_STEP * N
-    achieved = bytes_alg / (force_ms * 1e-3) / 1e9
-
+    if args.copy_calibration:
+        copy_calibration(args)
+        return
     threads = min(16, os.cpu_count() or 1)
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(w, args.cpu_seconds, threads)
-
-    line = {
-        "metric": "particle-steps/s (1M LJ/DPD)",
-        "value": value,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": ("dpd_fluid N=%d density=3 alpha=25 gamma=4.5 sigma=3 (counter RNG) cutoff=1 dt=0.01 "
-                         "cells=%dx%dx%d" % (N, *w["cells"])) if w["name"] == "dpd_fluid" else
-                        ("argon_lj N=%d density=10 LJ(0.275,1.0) tol=1e-3 cutoff=1 dt=0.01" % N),
-            "particles": N,
-            "cells": w["cells"],
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_step (fused non-bonded force + advance)",
-            "kernel_ms": force_ms,
-            "step_ms_events": step_ms,
-            "bytes_per_launch": bytes_alg,
-        },
-        "cpu_baseline": cpu,
-    }
-    if cpu:
-        line["speedup_vs_cpu"] = value / cpu["value"]
-    print(json.dumps(line), flush=True)
+    if args.workload in ("dpd", "lj", "lattice"):
+        out = bench_single(args, threads)
+    elif args.workload == "ensemble":
+        out = bench_ensemble(args, threads)
+    else:
+        out = bench_adjoint(args, threads)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -146,6 +146,15 @@ int mxg_engine_set_virial(mxg_engine *e, int on);
 int mxg_engine_set_brick(mxg_engine *e, int bx, int by, int bz, int cap, int pmax);
 int mxg_engine_get_brick(mxg_engine *e, int dims[3], int *cap, int *pmax);
 
+/* Autotuner: times the force kernel for each brick shape and list capacity
+ * that fits (steps_per real steps each: the layouts give identical
+ * per-particle results, so the trajectory is unchanged) and keeps the
+ * fastest; *taken = steps advanced.  mxg_engine_tune_log returns the number
+ * of entries and copies up to max of them: layout (bx, by, bz, cap) and the
+ * mean force-kernel time in ms.  No reference counterpart. */
+int mxg_engine_autotune(mxg_engine *e, int steps_per, int *taken);
+int mxg_engine_tune_log(mxg_engine *e, int max, int *layout, double *force_ms);
+
 /* Replay steps from captured HIP graphs (two steps per graph); on by
  * default.  No reference counterpart (mdcore drives pthread runners). */
 int mxg_engine_set_graphs(mxg_engine *e, int on);

@@ -228,6 +228,20 @@ class Universe:
         self._check(_capi.lib().mxg_engine_get_brick(self._h, _ip(d), C.byref(cap), C.byref(pmax)))
         return tuple(int(x) for x in d), cap.value, pmax.value
 
+    def autotune(self, steps_per=5):
+        """Picks the fastest brick layout using real steps (the trajectory
+        is the same for every layout).  Returns the steps taken, the chosen
+        layout and the timings."""
+        L = _capi.lib()
+        taken = C.c_int()
+        self._check(L.mxg_engine_autotune(self._h, int(steps_per), C.byref(taken)))
+        n = L.mxg_engine_tune_log(self._h, 0, None, None)
+        lay = np.zeros((max(n, 1), 4), np.int32)
+        ms = np.zeros(max(n, 1), np.float64)
+        L.mxg_engine_tune_log(self._h, n, _ip(lay), _dp(ms))
+        log = [(tuple(int(v) for v in lay[k]), float(ms[k])) for k in range(n)]
+        return {"steps": taken.value, "brick": self.brick(), "log": log}
+
     def set_graphs(self, on=True):
         """Replay pairs of steps from captured HIP graphs (default on)."""
         self._check(_capi.lib().mxg_engine_set_graphs(self._h, int(bool(on))))

@@ -179,6 +179,27 @@ int mxg_engine_get_brick(mxg_engine *e, int dims[3], int *cap, int *pmax) {
   return MXG_ERR_OK;
 }
 
+int mxg_engine_autotune(mxg_engine *e, int steps_per, int *taken) {
+  MXG_CHECK(e);
+  return e->e.autotune(steps_per, taken);
+}
+
+int mxg_engine_tune_log(mxg_engine *e, int max, int *layout, double *force_ms) {
+  if (!e) return MXG_ERR_NULL;
+  const auto &log = e->e.tune_log();
+  const int n = (int)log.size();
+  for (int k = 0; k < n && k < max; ++k) {
+    if (layout) {
+      layout[4 * k + 0] = log[k].bx;
+      layout[4 * k + 1] = log[k].by;
+      layout[4 * k + 2] = log[k].bz;
+      layout[4 * k + 3] = log[k].cap;
+    }
+    if (force_ms) force_ms[k] = log[k].force_ms;
+  }
+  return n;
+}
+
 int mxg_engine_set_graphs(mxg_engine *e, int on) {
   MXG_CHECK(e);
   return e->e.set_graphs(on);

@@ -84,7 +84,8 @@ class Engine {
   int error() const { return err_; }
   const std::string &message() const { return msg_; }
 
-  int fail(int code, const std::string &msg);
+  int fail(int code, const std::string &msg);    // sticky: the engine stops
+  int reject(int code, const std::string &msg);  // a refused request only
 
   // Replays captured HIP graphs of two steps (on by default).
   int set_graphs(int on);
@@ -103,6 +104,14 @@ class Engine {
   // set_brick(0, 0, 0, 0, 0) picks the largest brick whose halo fits; pmax
   // (> 0) lowers the LDS particle capacity.
   int set_brick(int bx, int by, int bz, int cap, int pmax);
+  // Tries brick shapes and list capacities, steps_per steps each, keeps the
+  // fastest; *taken = steps advanced.
+  int autotune(int steps_per, int *taken);
+  struct TuneEntry {
+    int bx, by, bz, cap;
+    double force_ms;
+  };
+  const std::vector<TuneEntry> &tune_log() const { return tune_log_; }
   void brick(int dims[3], int *cap, int *pmax) const {
     for (int k = 0; k < 3; ++k) dims[k] = bdim_[k];
     *cap = cap_;
@@ -149,6 +158,7 @@ class Engine {
   bool potentials_dirty_ = true;
   uint64_t time_ = 0;
   int err_ = 0;
+  std::vector<TuneEntry> tune_log_;
   std::string msg_;
   bool timing_ = false;
   int timed_steps_ = 0;

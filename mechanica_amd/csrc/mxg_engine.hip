@@ -1,16 +1,19 @@
 // MI355X (gfx950) engine for mdcore's force-and-advance step.
 //
-// One step is five launches on the engine's stream:
-//   k_step      per particle slot: non-bonded gather over the neighbour
-//               cells, bonds, single-body force, then the Euler/overdamped
-//               advance and the cell-migration rule; per-tile FP64 partial
-//               sums of energy, kinetic energy and virial
-//   k_count     per cell: how many particles land in it (pull over the 27
-//               cells they can come from)
-//   k_scan_*    exclusive scan of the counts into the new cell offsets
-//   k_scatter   per cell: gather its particles in a fixed source order
-//               (stable, no atomics) into the slot arrays of the next step
-//   k_finalize  per replica: sum the tile partials in tile order
+// One step is eight launches on the engine's stream (capturable as a HIP
+// graph):
+//   k_brick         per brick of cells: stage the brick and its halo in LDS,
+//                   then per particle the non-bonded gather over the
+//                   neighbour cells (candidate scan into a per-thread list,
+//                   then the pair phase), bonds, the single-body force, the
+//                   Euler/overdamped advance and the cell-migration rule;
+//                   integer per-cell counts and FP64 per-brick partial sums
+//   k_brick_global  the same for bricks whose halo overflowed LDS, with
+//                   neighbours read from global memory
+//   k_scan_*        exclusive scan of the counts into the new cell offsets
+//   k_place         stayers keep their rank in the cell, movers take a cursor
+//   k_order_incoming  movers of each cell ordered by particle id
+//   k_finalize      per replica: sum the brick partials in brick order
 //
 // Every pair is evaluated from both sides (no Newton-3 write-back), so each
 // particle's force is written once by the thread that owns it and the result
@@ -1414,6 +1417,11 @@ int Engine::fail(int code, const std::string &msg) {
   return code;
 }
 
+int Engine::reject(int code, const std::string &msg) {
+  msg_ = msg;
+  return code;
+}
+
 int Engine::check_device(int st, const char *what) {
   if (st != hipSuccess) return fail(kErrDevice, std::string(what) + ": " + hipGetErrorString((hipError_t)st));
   return kErrOk;
@@ -1885,7 +1893,7 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
   BrickFit fitv;
   for (const auto &r : reps_)
     for (int k = 0; k < 3; ++k)
-      if (r.span[k] > kSpanMax) return fail(kErrCutoff, "cutoff spans more than 2 cells");
+      if (r.span[k] > kSpanMax) return reject(kErrCutoff, "cutoff spans more than 2 cells");
   if (bx <= 0) {
     static const int shapes[][3] = {{4, 4, 4}, {2, 4, 4}, {2, 2, 4}, {2, 2, 2}, {1, 2, 2}, {1, 1, 2}, {1, 1, 1}};
     bool ok = false;
@@ -1902,16 +1910,16 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
       fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, budget);
     }
   } else {
-    if (bx < 1 || by < 1 || bz < 1) return fail(kErrRange, "brick dimensions must be positive");
+    if (bx < 1 || by < 1 || bz < 1) return reject(kErrRange, "brick dimensions must be positive");
     fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, budget);
   }
   hmax = fitv.hmax;
   pm = fitv.pmax;
   // An explicit capacity below the fit sends fuller bricks to the global path.
   if (pmax > 0) pm = std::min(pm, (pmax + 31) / 32 * 32);
-  if (fitv.hdmax > kHaloAxis) return fail(kErrRange, "brick too large");
-  if (hmax > kScanItems * kBT) return fail(kErrRange, "brick halo has more than 1024 cells");
-  if (pm < 64) return fail(kErrRange, "brick does not fit in LDS");
+  if (fitv.hdmax > kHaloAxis) return reject(kErrRange, "brick too large");
+  if (hmax > kScanItems * kBT) return reject(kErrRange, "brick halo has more than 1024 cells");
+  if (pm < 64) return reject(kErrRange, "brick does not fit in LDS");
   std::vector<int4> bricks;
   std::vector<int2> rep_bricks(reps_.size());
   for (size_t r = 0; r < reps_.size(); ++r) {
@@ -2189,6 +2197,48 @@ int Engine::ensure_graph(int par) {
   }
   graph_gen_[par] = config_gen_;
   return kErrOk;
+}
+
+// Times the force kernel over a list of brick shapes and neighbour-list
+// capacities, steps_per steps each, and keeps the fastest.  The steps are
+// real steps of the simulation: every layout gives the same per-particle
+// results (each particle's neighbours are visited in the same order), so
+// tuning does not change the trajectory.
+int Engine::autotune(int steps_per, int *taken) {
+  if (!d_ || N_ == 0) return fail(kErrNull, "no particles loaded");
+  if (steps_per < 1) return reject(kErrRange, "autotune needs at least one step per candidate");
+  static const int shapes[][3] = {{4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
+  static const int caps[] = {8, 12, 16};
+  const bool was_timing = timing_;
+  int best[4] = {bdim_[0], bdim_[1], bdim_[2], cap_};
+  double best_ms = 1e30;
+  int steps = 0;
+  timing_ = true;
+  tune_log_.clear();
+  for (const auto &sh : shapes) {
+    for (int c : caps) {
+      if (set_brick(sh[0], sh[1], sh[2], c, pmax_req_) != kErrOk) continue;  // does not fit: skip
+      int rc = step(steps_per);
+      steps += steps_per;
+      if (rc != kErrOk) {
+        timing_ = was_timing;
+        if (taken) *taken = steps;
+        return rc;
+      }
+      tune_log_.push_back({sh[0], sh[1], sh[2], c, force_ms_});
+      if (force_ms_ > 0 && force_ms_ < best_ms) {
+        best_ms = force_ms_;
+        best[0] = sh[0];
+        best[1] = sh[1];
+        best[2] = sh[2];
+        best[3] = c;
+      }
+    }
+  }
+  timing_ = was_timing;
+  if (taken) *taken = steps;
+  msg_.clear();
+  return set_brick(best[0], best[1], best[2], best[3], pmax_req_);
 }
 
 int Engine::set_graphs(int on) {

@@ -258,6 +258,39 @@ def ensemble_sizes(R=256, lo=500, hi=20000, seed=4):
     return np.exp(rng.uniform(math.log(lo), math.log(hi), size=R)).astype(np.int64)
 
 
+def ensemble(R=256, seed=4, kT=0.1, dt=0.01, lo=500, hi=20000):
+    """R two-type cell-sorting replicas, sizes log-uniform in [lo, hi]."""
+    sizes = ensemble_sizes(R, lo=lo, hi=hi, seed=seed)
+    reps = [cell_sort_replica(int(n), seed=1000 + i, kT=kT, dt=dt) for i, n in enumerate(sizes)]
+    e = {"name": "ensemble", "replicas": reps}
+    e.update(cell_sort_params(kT=kT, dt=dt))
+    return e
+
+
+def build_ensemble(m, e, seed=None):
+    """One engine holding every replica of ensemble e."""
+    u = m.Universe(cutoff=e["cutoff"], dt=e["dt"], replicas=[(r["dim"], r["cells"]) for r in e["replicas"]])
+    for t in e["types"]:
+        u.add_type(t["mass"], t["radius"], t["dynamics"])
+    pots = []
+    for spec in e["pots"]:
+        p = make_potential(m, spec)
+        u.bind(p, spec[2], spec[3])
+        pots.append(p)
+    if seed is not None:
+        u.set_seed(seed)
+    rf = e.get("random")
+    if rf:
+        for t in range(len(e["types"])):
+            u.add_random_force(t, rf["mean"], rf["std"], rf["duration"])
+    x = np.concatenate([r["x"] for r in e["replicas"]])
+    t = np.concatenate([r["type"] for r in e["replicas"]])
+    rep = np.concatenate([np.full(len(r["x"]), i, np.int32) for i, r in enumerate(e["replicas"])])
+    u.load(x, None, t, replica=rep)
+    u._keep = pots
+    return u
+
+
 def make_potential(m, spec):
     """Builds a Potential from a (kind, args) spec with the given module."""
     kind, args = spec[0], spec[1]

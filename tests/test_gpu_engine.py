@@ -186,3 +186,19 @@ def test_ensemble_matches_independent(m):
         base += n
     tot = u.observables(-1)
     assert abs(tot["ekin"] - sum(o["ekin"] for _, o in singles)) <= 1e-12 * tot["ekin"]
+
+
+def test_autotune_keeps_trajectory(m):
+    """The tuner's timing steps are real steps; the trajectory equals an
+    untuned run of the same length."""
+    w = W.argon_lj(N=3000, seed=46, min_dist=0.275)
+    print("case: autotune argon N=3000", flush=True)
+    u = W.build_universe(m, w, seed=6)
+    res = u.autotune(steps_per=1)
+    print("autotune", res["brick"], res["steps"], res["log"][:3])
+    assert res["steps"] == len(res["log"]) > 0
+    total = res["steps"] + 3
+    u.step(3)
+    a = u.state()
+    b, _, _ = run_steps(m, w, 6, total)
+    bitwise_same(a, b)
