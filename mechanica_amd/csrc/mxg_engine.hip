@@ -1661,7 +1661,7 @@ int Engine::set_dpd_counter_noise(int on) {
 }
 
 int Engine::set_max_dist_fraction(float f) {
-  if (!(f > 0)) return fail(kErrRange, "max distance fraction must be positive");
+  if (!(f > 0)) return reject(kErrRange, "max distance fraction must be positive");
   max_frac_ = f;
   args_dirty_ = true;
   if (d_) return build_geometry();
@@ -1669,7 +1669,7 @@ int Engine::set_max_dist_fraction(float f) {
 }
 
 int Engine::set_block_size(int b) {
-  if (b != kTile) return fail(kErrRange, "the force kernel is compiled for 256 threads per block");
+  if (b != kTile) return reject(kErrRange, "the force kernel is compiled for 256 threads per block");
   block_ = b;
   return kErrOk;
 }

@@ -51,9 +51,10 @@ class RefSystem:
                 k += 1
 
     def min_image(self, d):
+        dim = self.dim.to(d.dtype)
         with torch.no_grad():
-            k = torch.round(d / self.dim)
-        return d - k * self.dim
+            k = torch.round(d / dim)
+        return d - k * dim
 
     def pairs(self, x, rmax=None):
         """Index pairs i < j with r^2 <= rmax^2 (default: the cutoff)."""
@@ -76,7 +77,7 @@ class RefSystem:
         r = torch.sqrt((d * d).sum(-1))
         p = self.pair_of[self.types[i], self.types[j]]
         D, A, r0 = theta[p, 0], theta[p, 1], theta[p, 2]
-        R = self.radius[self.types[i]] + self.radius[self.types[j]]
+        R = (self.radius[self.types[i]] + self.radius[self.types[j]]).to(x.dtype)
         s = torch.clamp(r, min=self.smin) - R + r0 + 1.0
         with torch.no_grad():
             ok = s <= self.smax
@@ -91,15 +92,16 @@ class RefSystem:
     def forces(self, x, theta):
         i, j, d, r, u, D, A = self.pair_terms(x, theta)
         ep = 2.0 * A * D * (u - u * u)
-        phi = -ep / torch.clamp(r, min=FLT_EPSILON)
+        phi = -ep / torch.clamp(r, min=FLT_EPSILON)  # also runs in float32 to size FP32 rounding
         fp = phi[:, None] * d
         F = torch.zeros_like(x)
         return F.index_add(0, i, fp).index_add(0, j, -fp)
 
     def step(self, x, theta):
         F = self.forces(x, theta)
-        dx = self.dt * F * self.imass[self.types][:, None]
-        dx = torch.maximum(torch.minimum(dx, self.maxx), -self.maxx)
+        maxx = self.maxx.to(x.dtype)
+        dx = (self.dt * F) * self.imass[self.types][:, None].to(x.dtype)
+        dx = torch.maximum(torch.minimum(dx, maxx), -maxx)
         return x + dx
 
     def trajectory(self, x0, theta, nsteps):

@@ -72,7 +72,10 @@ def test_single_body_random_force_parity(m, oracle_mod):
     w = W.cell_sort(N=3000, seed=42)
     print("case: cell_sort N=3000", flush=True)
     sg, so = force_parity(m, oracle_mod, w, 17)
-    assert np.array_equal(sg["pf"].view(np.uint32), so["pf"].view(np.uint32)), "persistent force"
+    # The random vector uses sinf/cosf/logf: the device's and the host's
+    # libm differ by an ulp or two.
+    pf_err = np.abs(sg["pf"] - so["pf"]).max() / np.abs(so["pf"]).max()
+    assert pf_err <= 1e-6, pf_err
     relay(W.build_universe(m, w, seed=17), oracle_mod.from_workload(w, m, seed=17), 5)
 
 
@@ -85,6 +88,9 @@ def test_dpd_temperature():
     w["dpd_noise"] = True
     print("case: dpd temperature N=24000", flush=True)
     u = W.build_universe(m, w, seed=3)
+    # No velocity clamp (0.05 h / dt = 5 by default, a few events per run at
+    # kT = 1, each removing momentum): momentum is then conserved.
+    u.set_max_dist_fraction(0.5)
     u.step(300)
     T = []
     for _ in range(20):
@@ -144,10 +150,12 @@ def test_brick_layout_bitwise(m, brick):
 
 
 def test_ensemble_matches_independent(m):
-    """Three argon replicas of different sizes in one engine, against three
+    """Three DPD replicas of different sizes in one engine, against three
     engines of one replica each (same brick shape)."""
     sizes = [1500, 2600, 4100]
-    ws = [W.argon_lj(N=n, seed=50 + i, min_dist=0.275) for i, n in enumerate(sizes)]
+    # DPD without the random term: no random stream is involved (those are
+    # keyed by the global particle id, which differs between the two runs).
+    ws = [W.dpd_fluid(N=n, sigma=0.0, seed=50 + i) for i, n in enumerate(sizes)]
     print("case: ensemble", sizes, flush=True)
     brick = (2, 2, 2, 0, 0)
     singles = []
