@@ -134,7 +134,7 @@ def bench_single(args, threads):
     u = W.build_universe(m, w, seed=1234)
     N = len(w["x"])
     log("workload %s N=%d built" % (w["name"], N))
-    tune = u.autotune(steps_per=2) if args.autotune else None
+    tune = None if args.no_autotune else u.autotune(steps_per=2)
     wall = timed_steps(u, args.steps, args.warmup)
     value = N * args.steps / wall
     log("timed %d steps in %.3f s" % (args.steps, wall))
@@ -159,17 +159,20 @@ def bench_ensemble(args, threads):
     N = sum(sizes)
     u = W.build_ensemble(m, e, seed=1234)
     log("ensemble of %d replicas, %d particles built" % (len(sizes), N))
+    tune = None if args.no_autotune else u.autotune(steps_per=2)
     wall = timed_steps(u, args.steps, args.warmup)
     value = N * args.steps / wall
     force_ms, step_ms = kernel_ms(u, args.steps)
+    ebrick = u.brick()
     del u
     # The plain kernel: one cell-sorting universe of the same total size.
     w = W.cell_sort(N=N, seed=5)
     p = W.build_universe(m, w, seed=1234)
+    ptune = None if args.no_autotune else p.autotune(steps_per=2)
     pwall = timed_steps(p, args.steps, args.warmup)
     pforce, _ = kernel_ms(p, args.steps)
     plain = {"value": N * args.steps / pwall, "workload": "cell_sort N=%d cells=%dx%dx%d (one universe)" % (
-        (N,) + tuple(w["cells"])), "force_kernel_ms": pforce}
+        (N,) + tuple(w["cells"])), "force_kernel_ms": pforce, "brick": p.brick()}
     del p
     roof = roofline(BYTES_OVERDAMPED * N, force_ms, "k_brick over all replicas (one launch per step)")
     roof["step_ms_events"] = step_ms
@@ -198,9 +201,10 @@ def bench_ensemble(args, threads):
                           "total %d, density 0.8, Morse d=3/1 a=6 r0=1 cutoff 2.5, overdamped Langevin kT=0.1, "
                           "dt=0.01, hipGraph-captured steps" % (len(sizes), N), "particles": N,
               "replicas": len(sizes)}
+    config["brick"] = ebrick
     return line(args, value, wall, "f32", config, roof, cpu,
                 metric="batched particle-steps/s vs plain kernel", plain_kernel=plain,
-                batched_over_plain=value / plain["value"])
+                batched_over_plain=value / plain["value"], autotune=tune and tune["brick"])
 
 
 def bench_adjoint(args, threads):
@@ -251,9 +255,11 @@ def bench_adjoint(args, threads):
         u.bind(p, a, b)
         keep.append(p)
     u.load(w["x"], None, w["type"])
+    if not args.no_autotune:
+        u.autotune(steps_per=2)
     pwall = timed_steps(u, args.steps, args.warmup)
     plain = {"value": N * args.steps / pwall, "workload": "same system through the engine step (tabulated "
-             "Morse tol 1e-3, overdamped)"}
+             "Morse tol 1e-3, overdamped)", "brick": u.brick()}
     step_ms = 1e3 * wall / (reps * T)
     roof = roofline(BYTES_OVERDAMPED * N * 3, step_ms,
                     "adjoint step: forward + recomputed forward + k_backward per trajectory step")
@@ -296,7 +302,8 @@ def main():
     ap.add_argument("--workload", default="dpd", choices=["dpd", "lj", "lattice", "ensemble", "adjoint"])
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--replicas", type=int, default=256)
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default brick layout instead of timing the candidates before warm-up")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")

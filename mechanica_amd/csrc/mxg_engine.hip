@@ -734,6 +734,7 @@ struct BrickArgs {
 };
 
 constexpr int kBT = 256;        // threads per brick block
+constexpr int kChunk = 4;       // candidates loaded together in the scan
 constexpr int kHaloAxis = 64;   // max halo cells per axis
 constexpr int kSpanMax = 2;     // max cell span
 constexpr int kOffMax = 125;    // (2 * kSpanMax + 1)^3
@@ -1002,18 +1003,26 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
               const int qb = hstart[h2], qe = hstart[h2 + 1];
               if (q < 0) q = qb;
               bool full = false;
+              // Candidates go in chunks of kChunk: their LDS loads are issued
+              // together, and the list is flushed before a chunk that might
+              // not fit, so the visiting order is unchanged.
               if (off.side == 0) {
-                for (; q < qe; ++q) {
-                  if (q == p) continue;
-                  const float4 xq = hx[q];
-                  const float dx = xs.x - xq.x, dy = xs.y - xq.y, dz = xs.z - xq.z;
-                  const float r2 = dx * dx + dy * dy + dz * dz;
-                  if (r2 > A.cut2_self) continue;
-                  nlist[nn * kBT + tid] = ((unsigned)q << 8) | (unsigned)o;
-                  if (++nn == B.cap) {
-                    ++q;
+                for (; q < qe; q += kChunk) {
+                  if (nn > B.cap - kChunk) {
                     full = true;
                     break;
+                  }
+                  float4 xq[kChunk];
+#pragma unroll
+                  for (int u = 0; u < kChunk; ++u) xq[u] = hx[min(q + u, qe - 1)];
+#pragma unroll
+                  for (int u = 0; u < kChunk; ++u) {
+                    const float dx = xs.x - xq[u].x, dy = xs.y - xq[u].y, dz = xs.z - xq[u].z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    if (q + u < qe && q + u != p && r2 <= A.cut2_self) {
+                      nlist[nn * kBT + tid] = ((unsigned)(q + u) << 8) | (unsigned)o;
+                      ++nn;
+                    }
                   }
                 }
               } else {
@@ -1023,29 +1032,37 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
                 const unsigned dnshift = (unsigned)(int)(R.dscale * nshift);
                 const unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, off.nx, off.ny, off.nz);
                 const float pix = xs.x - sx, piy = xs.y - sy, piz = xs.z - sz;
-                for (; q < qe; ++q) {
-                  const float4 xq = hx[q];
-                  float dx, dy, dz;
-                  if (off.side > 0) {
-                    dx = pix - xq.x;
-                    dy = piy - xq.y;
-                    dz = piz - xq.z;
-                  } else {
-                    const float qix = xq.x - sx, qiy = xq.y - sy, qiz = xq.z - sz;
-                    dx = qix - xs.x;
-                    dy = qiy - xs.y;
-                    dz = qiz - xs.z;
-                  }
-                  const float r2 = dx * dx + dy * dy + dz * dz;
-                  if (r2 > A.cut2_pair) continue;
-                  const unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, off.nx, off.ny, off.nz);
-                  const bool pass = off.side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax);
-                  if (!pass) continue;
-                  nlist[nn * kBT + tid] = ((unsigned)q << 8) | (unsigned)o;
-                  if (++nn == B.cap) {
-                    ++q;
+                for (; q < qe; q += kChunk) {
+                  if (nn > B.cap - kChunk) {
                     full = true;
                     break;
+                  }
+                  float4 xq[kChunk];
+#pragma unroll
+                  for (int u = 0; u < kChunk; ++u) xq[u] = hx[min(q + u, qe - 1)];
+#pragma unroll
+                  for (int u = 0; u < kChunk; ++u) {
+                    float dx, dy, dz;
+                    if (off.side > 0) {
+                      dx = pix - xq[u].x;
+                      dy = piy - xq[u].y;
+                      dz = piz - xq[u].z;
+                    } else {
+                      const float qix = xq[u].x - sx, qiy = xq[u].y - sy, qiz = xq[u].z - sz;
+                      dx = qix - xs.x;
+                      dy = qiy - xs.y;
+                      dz = qiz - xs.z;
+                    }
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    if (q + u < qe && r2 <= A.cut2_pair) {
+                      const unsigned kq =
+                          sort_key(R.bias, R.dscale, xq[u].x, xq[u].y, xq[u].z, off.nx, off.ny, off.nz);
+                      const bool pass = off.side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax);
+                      if (pass) {
+                        nlist[nn * kBT + tid] = ((unsigned)(q + u) << 8) | (unsigned)o;
+                        ++nn;
+                      }
+                    }
                   }
                 }
               }
@@ -1888,6 +1905,7 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
                      return p.kind == kPotKindDPD;
                    });
   int c = cap > 0 ? cap : cap_;
+  if (c < kChunk || c > 64) return reject(kErrRange, "neighbour-list capacity must lie in [4, 64]");
   int bd[3] = {bx, by, bz};
   int hmax = 0, pm = 0;
   BrickFit fitv;
