@@ -134,7 +134,10 @@ def bench_single(args, threads):
     u = W.build_universe(m, w, seed=1234)
     N = len(w["x"])
     log("workload %s N=%d built" % (w["name"], N))
-    tune = None if args.no_autotune else u.autotune(steps_per=2)
+    tune = None
+    if not args.no_autotune:
+        u.step(args.warmup)  # tune on a settled state, not the initial one
+        tune = u.autotune(steps_per=2)
     wall = timed_steps(u, args.steps, args.warmup)
     value = N * args.steps / wall
     log("timed %d steps in %.3f s" % (args.steps, wall))

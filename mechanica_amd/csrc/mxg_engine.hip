@@ -196,6 +196,34 @@ struct StepArgs {
   int G *inc;     // particles entering each cell
 };
 
+// Loop-invariant scalars of the step, read once per block into registers:
+// reading them through the argument pointer inside the neighbour loops makes
+// each read a scalar load whose wait also drains the outstanding LDS loads.
+struct LoopConsts {
+  const float G *coef;
+  int ntypes;
+  float cut2_pair, cut2_self;
+  float dpd_delta;
+  uint32_t seed_lo, seed_hi;
+  int dpd_noise;
+  float cutoff, nd_norm;
+};
+
+__device__ __forceinline__ LoopConsts loop_consts(const StepArgs &A) {
+  LoopConsts C;
+  C.coef = A.coef;
+  C.ntypes = A.ntypes;
+  C.cut2_pair = A.cut2_pair;
+  C.cut2_self = A.cut2_self;
+  C.dpd_delta = A.dpd_delta;
+  C.seed_lo = A.seed_lo;
+  C.seed_hi = A.seed_hi;
+  C.dpd_noise = A.dpd_noise;
+  C.cutoff = A.cutoff;
+  C.nd_norm = A.nd_norm;
+  return C;
+}
+
 enum : int {
   kModeTable = 1,
   kModeDPD = 2,
@@ -360,7 +388,7 @@ struct PairTabs {
 };
 
 template <int MODE, typename NB>
-__device__ __forceinline__ bool pair_force(const StepArgs &A, const PairTabs &tb, const NB &nb, int ti, int tj,
+__device__ __forceinline__ bool pair_force(const LoopConsts &A, const PairTabs &tb, const NB &nb, int ti, int tj,
                                            bool p_is_i,
                                            int mypid, int q,
                                            bool self_pair, float dxx, float dxy, float dxz, float r2, float r,
@@ -478,8 +506,8 @@ __device__ __forceinline__ bool pair_force(const StepArgs &A, const PairTabs &tb
 // runner_dosort.cpp:126-137; window test runner_dopair.cpp:170,182) so the
 // pair set is exactly the reference's.  fn(q, p_is_i, self, dx, r2).
 template <typename Fn>
-__device__ __forceinline__ void for_each_neighbour(const StepArgs &A, const DevReplica &R, int s, int lx,
-                                                   int ly, int lz, float4 xs, Fn &&fn) {
+__device__ __forceinline__ void for_each_neighbour(const StepArgs &A, const LoopConsts &C, const DevReplica &R,
+                                                   int s, int lx, int ly, int lz, float4 xs, Fn &&fn) {
   const int cd0 = R.cdim[0], cd1 = R.cdim[1], cd2 = R.cdim[2];
   const int w0 = 2 * R.span[0] + 1, w1 = 2 * R.span[1] + 1, w2 = 2 * R.span[2] + 1;
   for (int o = 0; o < R.n_off; ++o) {
@@ -493,7 +521,7 @@ __device__ __forceinline__ void for_each_neighbour(const StepArgs &A, const DevR
         float4 xq = A.x[q];
         float dx = xs.x - xq.x, dy = xs.y - xq.y, dz = xs.z - xq.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 > A.cut2_self) continue;
+        if (r2 > C.cut2_self) continue;
         fn(q, true, true, dx, dy, dz, r2, xq);
       }
       continue;
@@ -519,7 +547,7 @@ __device__ __forceinline__ void for_each_neighbour(const StepArgs &A, const DevR
         float4 xq = A.x[q];
         float dx = pix - xq.x, dy = piy - xq.y, dz = piz - xq.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 > A.cut2_pair) continue;
+        if (r2 > C.cut2_pair) continue;
         unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, off.nx, off.ny, off.nz);
         if (!(kq + dnshift - kp < dmax)) continue;
         fn(q, true, false, dx, dy, dz, r2, xq);
@@ -530,7 +558,7 @@ __device__ __forceinline__ void for_each_neighbour(const StepArgs &A, const DevR
         float qix = xq.x - sx, qiy = xq.y - sy, qiz = xq.z - sz;
         float dx = qix - xs.x, dy = qiy - xs.y, dz = qiz - xs.z;
         float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 > A.cut2_pair) continue;
+        if (r2 > C.cut2_pair) continue;
         unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, off.nx, off.ny, off.nz);
         if (!(kp + dnshift - kq < dmax)) continue;
         fn(q, false, false, dx, dy, dz, r2, xq);
@@ -807,6 +835,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
   const unsigned t1 = *A.time + 1u;
   const uint32_t tk = step_key(A.seed_lo, t1);
   const PairTabs tb = stage_pair_tabs(A, reinterpret_cast<unsigned char *>(nlist + B.cap * kBT), A.npots);
+  const LoopConsts C = loop_consts(A);
   (void)kOffMax;
 
   for (int bi = blockIdx.x; bi < B.nbricks; bi += gridDim.x) {
@@ -963,32 +992,20 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
               const int q = (int)(ent >> 8);
               const DevOffset off = hoff[ent & 255u];
               const float4 xq = hx[q];
-              float dx, dy, dz;
-              if (off.side == 0) {
-                dx = xs.x - xq.x;
-                dy = xs.y - xq.y;
-                dz = xs.z - xq.z;
-              } else {
-                float sx, sy, sz;
-                shift_of(off, sx, sy, sz);
-                if (off.side > 0) {
-                  const float pix = xs.x - sx, piy = xs.y - sy, piz = xs.z - sz;
-                  dx = pix - xq.x;
-                  dy = piy - xq.y;
-                  dz = piz - xq.z;
-                } else {
-                  const float qix = xq.x - sx, qiy = xq.y - sy, qiz = xq.z - sz;
-                  dx = qix - xs.x;
-                  dy = qiy - xs.y;
-                  dz = qiz - xs.z;
-                }
-              }
+              // Self pairs: x_i - x_j; others (a - shift) - b in the
+              // reference orientation (x - 0 is exact, so one form serves).
+              float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+              if (off.side != 0) shift_of(off, sx, sy, sz);
+              const bool fwd = off.side >= 0;
+              const float ax = fwd ? xs.x : xq.x, ay = fwd ? xs.y : xq.y, az = fwd ? xs.z : xq.z;
+              const float bx = fwd ? xq.x : xs.x, by = fwd ? xq.y : xs.y, bz = fwd ? xq.z : xs.z;
+              const float dx = (ax - sx) - bx, dy = (ay - sy) - by, dz = (az - sz) - bz;
               const float r2 = dx * dx + dy * dy + dz * dz;
               const float r = sqrtf(r2);
               const int tj = __float_as_int(xq.w) & 0xffff;
-              if (pair_force<MODE>(A, tb, nb, ti, tj, off.side >= 0, mypid, q, off.side == 0, dx, dy, dz, r2, r,
+              if (pair_force<MODE>(C, tb, nb, ti, tj, off.side >= 0, mypid, q, off.side == 0, dx, dy, dz, r2, r,
                                    vs, fx, fy, fz, acc, t1, tk))
-                nd += w_cubic(r, A.cutoff, A.nd_norm);
+                nd += w_cubic(r, C.cutoff, C.nd_norm);
             }
           };
           // Gather in-cutoff neighbours into the per-thread list, process
@@ -1019,7 +1036,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
                   for (int u = 0; u < kChunk; ++u) {
                     const float dx = xs.x - xq[u].x, dy = xs.y - xq[u].y, dz = xs.z - xq[u].z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    if (q + u < qe && q + u != p && r2 <= A.cut2_self) {
+                    if (q + u < qe && q + u != p && r2 <= C.cut2_self) {
                       nlist[nn * kBT + tid] = ((unsigned)(q + u) << 8) | (unsigned)o;
                       ++nn;
                     }
@@ -1031,7 +1048,6 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
                 const float nshift = sqrtf(sx * sx + sy * sy + sz * sz);
                 const unsigned dnshift = (unsigned)(int)(R.dscale * nshift);
                 const unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, off.nx, off.ny, off.nz);
-                const float pix = xs.x - sx, piy = xs.y - sy, piz = xs.z - sz;
                 for (; q < qe; q += kChunk) {
                   if (nn > B.cap - kChunk) {
                     full = true;
@@ -1042,19 +1058,14 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
                   for (int u = 0; u < kChunk; ++u) xq[u] = hx[min(q + u, qe - 1)];
 #pragma unroll
                   for (int u = 0; u < kChunk; ++u) {
-                    float dx, dy, dz;
-                    if (off.side > 0) {
-                      dx = pix - xq[u].x;
-                      dy = piy - xq[u].y;
-                      dz = piz - xq[u].z;
-                    } else {
-                      const float qix = xq[u].x - sx, qiy = xq[u].y - sy, qiz = xq[u].z - sz;
-                      dx = qix - xs.x;
-                      dy = qiy - xs.y;
-                      dz = qiz - xs.z;
-                    }
+                    // (a - shift) - b with (a, b) = (i, j) in the reference
+                    // orientation: selects instead of a per-lane branch.
+                    const bool fwd = off.side > 0;
+                    const float ax = fwd ? xs.x : xq[u].x, ay = fwd ? xs.y : xq[u].y, az = fwd ? xs.z : xq[u].z;
+                    const float bx = fwd ? xq[u].x : xs.x, by = fwd ? xq[u].y : xs.y, bz = fwd ? xq[u].z : xs.z;
+                    const float dx = (ax - sx) - bx, dy = (ay - sy) - by, dz = (az - sz) - bz;
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    if (q + u < qe && r2 <= A.cut2_pair) {
+                    if (q + u < qe && r2 <= C.cut2_pair) {
                       const unsigned kq =
                           sort_key(R.bias, R.dscale, xq[u].x, xq[u].y, xq[u].z, off.nx, off.ny, off.nz);
                       const bool pass = off.side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax);
@@ -1109,7 +1120,8 @@ __global__ __launch_bounds__(kTile) void k_ncount(StepArgs A) {
     int lx, ly, lz;
     cell_coords(A.cell[s] - R.cell_base, R, lx, ly, lz);
     int n = 0;
-    for_each_neighbour(A, R, s, lx, ly, lz, xs,
+    const LoopConsts C = loop_consts(A);
+    for_each_neighbour(A, C, R, s, lx, ly, lz, xs,
                        [&](int, bool, bool, float, float, float, float, const float4 &) { ++n; });
     A.ncount[A.pid[s]] = n;
   }
@@ -1298,6 +1310,7 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
   const unsigned t1 = *A.time + 1u;
   const uint32_t tk = step_key(A.seed_lo, t1);
   const PairTabs tb = stage_pair_tabs(A, gsmem, A.npots);
+  const LoopConsts C = loop_consts(A);
   __syncthreads();
   const int nov = *B.ovf_count;
   for (int i = blockIdx.x; i < nov; i += gridDim.x) {
@@ -1360,14 +1373,14 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
       if (!passthrough) {
         if (A.random) single_body(A, s, mypid, T, t1, fx, fy, fz, pfs);
         const GlobalNbr nb{A.v, A.pid};
-        for_each_neighbour(A, R, s, lx, ly, lz, xs,
+        for_each_neighbour(A, C, R, s, lx, ly, lz, xs,
                            [&](int q, bool p_is_i, bool self, float dx, float dy, float dz, float r2,
                                const float4 &xq) {
                              const int tj = __float_as_int(xq.w) & 0xffff;
                              const float r = sqrtf(r2);
-                             if (pair_force<MODE>(A, tb, nb, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, r, vs,
+                             if (pair_force<MODE>(C, tb, nb, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, r, vs,
                                                   fx, fy, fz, acc, t1, tk))
-                               nd += w_cubic(r, A.cutoff, A.nd_norm);
+                               nd += w_cubic(r, C.cutoff, C.nd_norm);
                            });
         if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
       }
