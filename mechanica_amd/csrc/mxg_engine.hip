@@ -813,6 +813,18 @@ __device__ int LdsNbr::pid(int q) const {
   return gpid[hgb[h] + (q - hstart[h])];
 }
 
+// Workgroups go to the eight XCDs round-robin (workgroup g to XCD g % 8),
+// and each XCD has its own L2.  Giving XCD k a contiguous run of the brick
+// list (neighbouring bricks share most of their halo) keeps halo re-reads in
+// that XCD's L2.  A bijection on [0, n); the tail past a multiple of 8 maps
+// to itself.
+__device__ __forceinline__ int xcd_brick(int g, int n) {
+  constexpr int kXcd = 8;
+  const int full = n - n % kXcd;
+  if (g >= full) return g;
+  return (g % kXcd) * (full / kXcd) + g / kXcd;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, BrickArgs B) {
   const StepArgs &A = *Ap;
@@ -838,7 +850,8 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
   const LoopConsts C = loop_consts(A);
   (void)kOffMax;
 
-  for (int bi = blockIdx.x; bi < B.nbricks; bi += gridDim.x) {
+  for (int g = blockIdx.x; g < B.nbricks; g += gridDim.x) {
+    const int bi = xcd_brick(g, B.nbricks);
     const int4 br = B.bricks[bi];
     const DevReplica R = A.rep[br.x];  // by value: the fields stay in registers
     const int sp0 = R.span[0], sp1 = R.span[1], sp2 = R.span[2];
