@@ -192,7 +192,7 @@ struct StepArgs {
   int npots;    // pair potentials
   float cutoff;       // number-density kernel width
   float nd_norm;      // 1 / (pi cutoff^3)
-  int G *cnt;     // particles per cell of the next step
+  int G *cnt;     // particles staying in each cell (written by its brick; the scan adds inc)
   int G *inc;     // particles entering each cell
 };
 
@@ -677,7 +677,7 @@ __device__ __forceinline__ void bond_forces(const StepArgs &A, const DevReplica 
 // Advance of one particle (cell_advance_forward_euler,
 // engine_advance.cpp:126-239) and the store of its next-step state, the
 // migration flag and the per-cell counts.
-__device__ __forceinline__ void advance_store(const StepArgs &A, const DevReplica &R, int s, int c, int lx, int ly,
+__device__ __forceinline__ bool advance_store(const StepArgs &A, const DevReplica &R, int s, int c, int lx, int ly,
                                               int lz, const float4 &xs, const float4 &vs, const DevType &T,
                                               int passthrough, float fx, float fy, float fz, const float4 &pfs,
                                               int mypid, float nd) {
@@ -737,10 +737,12 @@ __device__ __forceinline__ void advance_store(const StepArgs &A, const DevReplic
   A.x_out[s] = make_float4(X[0], X[1], X[2], xs.w);
   A.v_out[s] = make_float4(V[0], V[1], V[2], vs.w);  // w carries the particle id
   A.cell_out[s] = newc != c ? (int)((unsigned)newc | 0x80000000u) : newc;
-  atomicAdd(&A.cnt[newc], 1);
+  // Movers are counted into their new cell here; the caller counts stayers
+  // per home cell in LDS and writes the cell's count once.
   if (newc != c) atomicAdd(&A.inc[newc], 1);
   A.f_out[s] = make_float4(fx, fy, fz, nd);
   if (A.random) A.pf_out[s] = pfs;
+  return newc == c;
 }
 
 // Bricks of cells, the work unit of the force kernel: a block stages the
@@ -840,6 +842,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
   unsigned *nlist = reinterpret_cast<unsigned *>(hs + 3 * B.hdmax * (2 * kSpanMax + 1));
   __shared__ int scan_lds[kBT / 64];
   __shared__ double red[kNRed][kBT / 64];
+  __shared__ int home_stay[kScanItems * kBT];  // stayers per home cell
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -923,6 +926,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
           const int a = hc / (nb1 * nb2), b = (hc / nb2) % nb1, cc = hc % nb2;
           const int h = ((a + sp0) * hd1 + (b + sp1)) * hd2 + (cc + sp2);
           cnt[i] = hstart[h + 1] - hstart[h];
+          home_stay[hc] = 0;
         }
         local += cnt[i];
       }
@@ -1100,7 +1104,8 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
         }
         if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
       }
-      advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd);
+      if (advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd))
+        atomicAdd(&home_stay[hc], 1);
     }
 
     // Deterministic block reduction of the FP64 partials for this brick.
@@ -1116,6 +1121,11 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
       if ((MODE & kModeVirial) || tid < 3)
         for (int w = 0; w < kBT / 64; ++w) sum += red[tid][w];
       A.partials[(size_t)bi * kNRed + tid] = sum;
+    }
+    // Stayer counts of the home cells (this brick is their only writer).
+    for (int hc = tid; hc < nhome; hc += kBT) {
+      const int a = hc / (nb1 * nb2), b = (hc / nb2) % nb1, cc = hc % nb2;
+      A.cnt[R.cell_base + ((br.y + a) * R.cdim[1] + (br.z + b)) * R.cdim[2] + (br.w + cc)] = home_stay[hc];
     }
     __syncthreads();
   }
@@ -1163,15 +1173,15 @@ struct RebuildArgs {
 
 // Exclusive scan of the per-cell counts (first pass); the counts are reset
 // to zero for the next step once read.
-__global__ __launch_bounds__(kScanBlock) void k_scan_block(int *__restrict__ cnt, int *__restrict__ out,
-                                                          int *__restrict__ bsum, int n) {
+__global__ __launch_bounds__(kScanBlock) void k_scan_block(int *__restrict__ cnt, const int *__restrict__ inc,
+                                                          int *__restrict__ out, int *__restrict__ bsum, int n) {
   __shared__ int lds[kScanBlock / 64];
   const int base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
   int vals[kScanItems];
   int local = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    vals[i] = (base + i < n) ? cnt[base + i] : 0;
+    vals[i] = (base + i < n) ? cnt[base + i] + inc[base + i] : 0;  // stayers + arrivals
     local += vals[i];
   }
 #pragma unroll
@@ -1315,6 +1325,7 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
   __shared__ int scan_lds[kBT / 64];
   __shared__ int homestart[kScanItems * kBT + 1];
   __shared__ int homegb[kScanItems * kBT];
+  __shared__ int home_stay[kScanItems * kBT];  // stayers per home cell
   __shared__ double red[kNRed][kBT / 64];
   extern __shared__ __align__(16) unsigned char gsmem[];
   const int tid = threadIdx.x;
@@ -1346,6 +1357,7 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
           const int c = R.cell_base + ((br.y + a) * R.cdim[1] + (br.z + b)) * R.cdim[2] + (br.w + cc);
           homegb[hc] = A.cstart[c];
           cnt[k] = A.cstart[c + 1] - A.cstart[c];
+          home_stay[hc] = 0;
         }
         local += cnt[k];
       }
@@ -1397,7 +1409,8 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
                            });
         if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
       }
-      advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd);
+      if (advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd))
+        atomicAdd(&home_stay[hc], 1);
     }
 #pragma unroll
     for (int k = 0; k < kNRed; ++k) {
@@ -1411,6 +1424,10 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
       if ((MODE & kModeVirial) || tid < 3)
         for (int w = 0; w < kBT / 64; ++w) sum += red[tid][w];
       A.partials[(size_t)bi * kNRed + tid] = sum;
+    }
+    for (int hc = tid; hc < nhome; hc += kBT) {
+      const int a = hc / (nb1 * nb2), b = (hc / nb2) % nb1, cc = hc % nb2;
+      A.cnt[R.cell_base + ((br.y + a) * R.cdim[1] + (br.z + b)) * R.cdim[2] + (br.w + cc)] = home_stay[hc];
     }
     __syncthreads();
   }
@@ -2199,7 +2216,8 @@ int Engine::launch_step() {
   const int cg = (ncells_ + 255) / 256;
   const int pg = (N_ + 255) / 256;
   const int nb = (ncells_ + kScanChunk - 1) / kScanChunk;
-  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, d->cnt, d->cstart[par ^ 1], d->bsum, ncells_);
+  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, d->cnt, d->inc, d->cstart[par ^ 1], d->bsum,
+                     ncells_);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, d->bsum, nb, d->cstart[par ^ 1], ncells_,
                      d->time, d->err, d->ovf_count);
   hipLaunchKernelGGL(k_scan_add, dim3(cg), dim3(256), 0, s, d->cstart[par ^ 1], d->bsum, ncells_);
