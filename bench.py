@@ -135,9 +135,11 @@ def bench_single(args, threads):
     N = len(w["x"])
     log("workload %s N=%d built" % (w["name"], N))
     tune = None
-    if not args.no_autotune:
+    if args.brick:
+        u.set_brick(*[int(v) for v in args.brick.split(",")])
+    elif not args.no_autotune:
         u.step(args.warmup)  # tune on a settled state, not the initial one
-        tune = u.autotune(steps_per=2)
+        tune = u.autotune(steps_per=3)
     wall = timed_steps(u, args.steps, args.warmup)
     value = N * args.steps / wall
     log("timed %d steps in %.3f s" % (args.steps, wall))
@@ -306,6 +308,7 @@ def main():
     ap.add_argument("--workload", default="dpd", choices=["dpd", "lj", "lattice", "ensemble", "adjoint"])
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--brick", default=None, help="bx,by,bz,cap: fixed force-kernel layout (implies --no-autotune)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default brick layout instead of timing the candidates before warm-up")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -150,6 +150,7 @@ class Engine {
 
   int N_ = 0, M_ = 0, ncells_ = 0, ntiles_ = 0;
   int bdim_[3] = {4, 4, 4};
+  size_t brick_tabs_ = 0;
   int cap_ = 12, pmax_ = 0, pmax_req_ = 0, hmax_ = 0, hdmax_ = 0, noff_ = 0, nbricks_ = 0;
   bool brick_dpd_ = false;
   std::vector<double> mean_cell_count_;

@@ -770,29 +770,36 @@ constexpr int kSpanMax = 2;     // max cell span
 constexpr int kOffMax = 125;    // (2 * kSpanMax + 1)^3
 constexpr int kMaxTypes = 16;   // particle types per engine
 constexpr int kMaxPots = 32;    // pair potentials per engine
-constexpr size_t kPairTabBytes = kMaxPots * sizeof(DevPot) + kMaxTypes * kMaxTypes * 4 + kMaxTypes * 4;
+// LDS bytes of the per-type-pair tables: the potentials in use, the type-pair
+// matrix and the radii (sized to what the engine holds, so small systems
+// keep LDS for particles).
+__host__ __device__ inline size_t pair_tab_bytes(int npots, int ntypes) {
+  const size_t b = (size_t)npots * sizeof(DevPot) + (size_t)ntypes * ntypes * 4 + (size_t)ntypes * 4;
+  return (b + 15) / 16 * 16;
+}
 
-// Copies the per-type-pair tables into LDS at p (kPairTabBytes).
+// Copies the per-type-pair tables into LDS at p (pair_tab_bytes).
 __device__ __forceinline__ PairTabs stage_pair_tabs(const StepArgs &A, unsigned char *p, int npots) {
-  DevPot *pots = reinterpret_cast<DevPot *>(p);
-  int *potmat = reinterpret_cast<int *>(pots + kMaxPots);
-  float *radius = reinterpret_cast<float *>(potmat + kMaxTypes * kMaxTypes);
   const int nt = A.ntypes;
+  DevPot *pots = reinterpret_cast<DevPot *>(p);
+  int *potmat = reinterpret_cast<int *>(pots + npots);
+  float *radius = reinterpret_cast<float *>(potmat + nt * nt);
   for (int i = threadIdx.x; i < npots; i += blockDim.x) pots[i] = A.pots[i];
   for (int i = threadIdx.x; i < nt * nt; i += blockDim.x) potmat[i] = A.potmat[i];
   for (int i = threadIdx.x; i < nt; i += blockDim.x) radius[i] = A.types[i].radius;
   return PairTabs{pots, potmat, radius};
 }
 
-__host__ __device__ inline size_t brick_lds_bytes(int pmax, int hmax, int hdmax, int noff, int cap, bool dpd) {
+__host__ __device__ inline size_t brick_lds_bytes(int pmax, int hmax, int hdmax, int noff, int cap, bool dpd,
+                                                  size_t tabs) {
   size_t b = 0;
   b += (size_t)pmax * 16;              // positions (w: type and flags)
   if (dpd) b += (size_t)pmax * 16;     // velocities (w: particle id)
   b += (size_t)noff * sizeof(DevOffset);
-  b += (size_t)(hmax + 1) * 4 * 3;     // halo starts, global starts, home starts
+  b += (size_t)(hmax + 1) * 4 * 4;     // halo starts, global starts, home starts, home stayers
   b += (size_t)3 * hdmax * (2 * kSpanMax + 1) * 4;  // shifts
   b += (size_t)cap * kBT * 4;          // neighbour lists
-  b += kPairTabBytes;                  // potentials, type pairs, radii
+  b += tabs;                           // potentials, type pairs, radii
   return b;
 }
 
@@ -838,11 +845,11 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
   int *hstart = reinterpret_cast<int *>(hoff + B.noff);
   int *hgb = hstart + (B.hmax + 1);
   int *homestart = hgb + (B.hmax + 1);
-  float *hs = reinterpret_cast<float *>(homestart + (B.hmax + 1));
+  int *home_stay = homestart + (B.hmax + 1);  // stayers per home cell
+  float *hs = reinterpret_cast<float *>(home_stay + (B.hmax + 1));
   unsigned *nlist = reinterpret_cast<unsigned *>(hs + 3 * B.hdmax * (2 * kSpanMax + 1));
   __shared__ int scan_lds[kBT / 64];
   __shared__ double red[kNRed][kBT / 64];
-  __shared__ int home_stay[kScanItems * kBT];  // stayers per home cell
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -1434,12 +1441,12 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
 }
 
 template <int MODE>
-void launch_step_mode(const StepArgs *A, const BrickArgs &B, size_t lds, hipStream_t st) {
+void launch_step_mode(const StepArgs *A, const BrickArgs &B, size_t lds, size_t tabs, hipStream_t st) {
   hipLaunchKernelGGL(k_brick<MODE>, dim3(B.nbricks), dim3(kBT), lds, st, A, B);
-  hipLaunchKernelGGL(k_brick_global<MODE>, dim3(64), dim3(kBT), kPairTabBytes, st, A, B);
+  hipLaunchKernelGGL(k_brick_global<MODE>, dim3(64), dim3(kBT), tabs, st, A, B);
 }
 
-using StepLauncher = void (*)(const StepArgs *, const BrickArgs &, size_t, hipStream_t);
+using StepLauncher = void (*)(const StepArgs *, const BrickArgs &, size_t, size_t, hipStream_t);
 
 // One instantiation per combination of table / DPD / virial / bonds.
 const StepLauncher kStepLaunchers[16] = {
@@ -1916,7 +1923,7 @@ struct BrickFit {
 };
 
 static BrickFit brick_fit(const std::vector<HostReplica> &reps, const std::vector<double> &mean, const int bd[3],
-                          int cap, bool dpd, size_t budget) {
+                          int cap, bool dpd, size_t tabs, size_t budget) {
   BrickFit F;
   for (size_t r = 0; r < reps.size(); ++r) {
     int hc = 1;
@@ -1934,7 +1941,7 @@ static BrickFit brick_fit(const std::vector<HostReplica> &reps, const std::vecto
   want = (want + 31) / 32 * 32;
   int fit = 0;
   for (int pm = 32; pm <= 16384; pm += 32)
-    if (brick_lds_bytes(pm, F.hmax, F.hdmax, F.noff, cap, dpd) <= budget) fit = pm;
+    if (brick_lds_bytes(pm, F.hmax, F.hdmax, F.noff, cap, dpd, tabs) <= budget) fit = pm;
   F.pmax = std::min(want, fit);
   return F;
 }
@@ -1949,6 +1956,7 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
                    });
   int c = cap > 0 ? cap : cap_;
   if (c < kChunk || c > 64) return reject(kErrRange, "neighbour-list capacity must lie in [4, 64]");
+  const size_t tabs = pair_tab_bytes((int)pots_.size(), (int)types_.size());
   int bd[3] = {bx, by, bz};
   int hmax = 0, pm = 0;
   BrickFit fitv;
@@ -1960,7 +1968,7 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
     bool ok = false;
     for (const auto &sh : shapes) {
       for (int k = 0; k < 3; ++k) bd[k] = sh[k];
-      fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, budget);
+      fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, tabs, budget);
       if (fitv.pmax >= 1.1 * fitv.worst && fitv.hmax <= kScanItems * kBT) {
         ok = true;
         break;
@@ -1968,11 +1976,11 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
     }
     if (!ok) {
       for (int k = 0; k < 3; ++k) bd[k] = 1;
-      fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, budget);
+      fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, tabs, budget);
     }
   } else {
     if (bx < 1 || by < 1 || bz < 1) return reject(kErrRange, "brick dimensions must be positive");
-    fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, budget);
+    fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, tabs, budget);
   }
   hmax = fitv.hmax;
   pm = fitv.pmax;
@@ -2011,6 +2019,7 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
   noff_ = fitv.noff;
   nbricks_ = (int)bricks.size();
   brick_dpd_ = dpd;
+  brick_tabs_ = tabs;
   if (int st = dalloc(d, &d->ovf_list, bricks.size())) return check_device(st, "hipMalloc");
   if (!d->ovf_count) {
     if (int st = dalloc(d, &d->ovf_count, 1)) return check_device(st, "hipMalloc");
@@ -2187,8 +2196,9 @@ int Engine::launch_step() {
   Bk.noff = noff_;
   Bk.ovf_list = d->ovf_list;
   Bk.ovf_count = d->ovf_count;
-  const size_t lds = brick_lds_bytes(pmax_, hmax_, hdmax_, noff_, cap_, (mode & kModeDPD) != 0);
-  kStepLaunchers[mode](reinterpret_cast<const StepArgs *>(d->dargs) + par, Bk, lds, s);
+  const size_t tabs = pair_tab_bytes((int)pots_.size(), (int)types_.size());
+  const size_t lds = brick_lds_bytes(pmax_, hmax_, hdmax_, noff_, cap_, (mode & kModeDPD) != 0, tabs);
+  kStepLaunchers[mode](reinterpret_cast<const StepArgs *>(d->dargs) + par, Bk, lds, tabs, s);
   if (rec) hipEventRecord(d->kev[2 * timed_steps_ + 1], s);
   if (rec) ++timed_steps_;
 
@@ -2280,8 +2290,12 @@ int Engine::autotune(int steps_per, int *taken) {
   for (const auto &sh : shapes) {
     for (int c : caps) {
       if (set_brick(sh[0], sh[1], sh[2], c, pmax_req_) != kErrOk) continue;  // does not fit: skip
-      int rc = step(steps_per);
-      steps += steps_per;
+      // One untimed step after the change of layout, then the timed ones.
+      timing_ = false;
+      int rc = step(1);
+      timing_ = true;
+      if (rc == kErrOk) rc = step(steps_per);
+      steps += steps_per + 1;
       if (rc != kErrOk) {
         timing_ = was_timing;
         if (taken) *taken = steps;
@@ -2315,7 +2329,7 @@ int Engine::step(int nsteps) {
     if (int rc = upload_potentials()) return rc;
   // Potentials bound after the load can change the LDS layout (velocities
   // are staged for DPD only).
-  if (brick_dpd_ != any_dpd_)
+  if (brick_dpd_ != any_dpd_ || brick_tabs_ != pair_tab_bytes((int)pots_.size(), (int)types_.size()))
     if (int rc = set_brick(bdim_[0], bdim_[1], bdim_[2], cap_, pmax_req_)) return rc;
   if (args_dirty_)
     if (int rc = refresh_args()) return rc;

@@ -204,7 +204,7 @@ def test_autotune_keeps_trajectory(m):
     u = W.build_universe(m, w, seed=6)
     res = u.autotune(steps_per=1)
     print("autotune", res["brick"], res["steps"], res["log"][:3])
-    assert res["steps"] == len(res["log"]) > 0
+    assert len(res["log"]) > 0 and res["steps"] == 2 * len(res["log"])  # one untimed step each
     total = res["steps"] + 3
     u.step(3)
     a = u.state()
