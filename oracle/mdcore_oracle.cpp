@@ -27,6 +27,7 @@
 //   bond_eval (periodic shift, clamp, dissociation).. bond.cpp:99-295
 //   cell_advance_forward_euler ...................... engine_advance.cpp:126-239
 //   engine_advance_forward_euler .................... engine_advance.cpp:248-422
+//   engine_advance_runge_kutta_4 .................... engine_advance.cpp:443-698
 //   gaussian_force (persistent random force) ........ MxForce.cpp:370-385
 //
 // Deviations, each stated where it happens:
@@ -120,6 +121,8 @@ struct Part {
   float v[4];
   float x[4];
   float pf[4];  // persistent random force
+  // Runge-Kutta stage state (MxParticle p0, v0, xk, vk: MxParticle.h:150-153)
+  float p0[3], v0[3], xk[4][3], vk[4][3];
   double imass;
   float radius;
   double mass;
@@ -169,6 +172,8 @@ struct Oracle {
   int dpd_counter_noise = 0;
   int err = 0;
   int err_pid = -1;
+  int integrator = 0;     // 0 forward Euler, 1 Runge-Kutta 4 (EngineIntegrator)
+  bool update_pf = true;  // INTEGRATOR_UPDATE_PERSISTENTFORCE
 
   std::vector<Cell> cells;
   std::vector<Task> tasks;
@@ -620,7 +625,7 @@ void do_pair(Oracle &o, int ci_, int cj_) {
 void single_body(Oracle &o, Part *p) {
   const Type &t = o.types[p->type];
   if (!t.has_random) return;
-  if ((o.time + (uint64_t)p->id) % t.rnd_steps == 0) {
+  if (o.update_pf && (o.time + (uint64_t)p->id) % t.rnd_steps == 0) {
     // MxRandomVector(mean, std) with Philox draws.
     mxg::u32x4 ctr = {{(uint32_t)p->id, (uint32_t)o.time, mxg::kStreamSingle, 0u}};
     mxg::u32x4 r = mxg::philox4x32_10(ctr, o.seed_lo, o.seed_hi);
@@ -887,7 +892,10 @@ void force(Oracle &o) {
       for (int k = 0; k < 3; k++)
         delta[k] = std::isgreaterequal(p.x[k], h[k]) - std::isless(p.x[k], 0.0);
       if (delta[0] || delta[1] || delta[2]) {
-        for (int k = 0; k < 3; k++) p.x[k] -= delta[k] * h[k];
+        for (int k = 0; k < 3; k++) {
+          p.x[k] -= delta[k] * h[k];
+          p.p0[k] -= delta[k] * h[k];
+        }
         int dest = o.cellid((c.loc[0] + delta[0] + o.cdim[0]) % o.cdim[0],
                             (c.loc[1] + delta[1] + o.cdim[1]) % o.cdim[1],
                             (c.loc[2] + delta[2] + o.cdim[2]) % o.cdim[2]);
@@ -927,9 +935,13 @@ AdvanceConst advance_const(const Oracle &o) {
 
 void advance(Oracle &o);
 
+int step_rk4(Oracle &o);
+
 int step(Oracle &o) {
   if (o.err != 0) return o.err;
+  if (o.integrator == 1) return step_rk4(o);
   o.time += 1;
+  o.update_pf = true;
   force(o);
   double epot = 0;
   for (auto &c : o.cells) epot += c.epot;
@@ -1115,10 +1127,130 @@ Pot make_pot(uint32_t kind, uint32_t flags, const float alpha[4], float a, float
   return p;
 }
 
+// engine_advance_runge_kutta_4 (engine_advance.cpp:443-698), the
+// non-Verlet branch: four engine_force calls; between them each particle's
+// position is set from its start position p0 and the stage slope, checked
+// with CHECK_TOOFAST (:424-431), and the next engine_force's
+// space_shuffle_local moves it (and p0) to a neighbour cell if it left its
+// own (space.cpp:333-393).  No velocity or displacement clamp, no frozen
+// mask.  The arithmetic is Magnum Vector3 (float) arithmetic: scalar factors
+// are converted to float first (0.5 * dt, dt / 6., the double imass).
+int step_rk4(Oracle &o) {
+  o.time += 1;
+  const float dt = o.dt;
+  const float half = (float)(0.5 * dt);
+  const float sixth = (float)(dt / 6.);
+  float h[3], h2[3];
+  for (int k = 0; k < 3; ++k) {
+    h[k] = (float)o.h[k];
+    h2[k] = (float)(2. * o.h[k]);
+  }
+  auto toofast = [&](Part &p) {
+    for (int k = 0; k < 3; ++k)
+      if (p.x[k] >= h2[k] || p.x[k] <= -h[k]) {
+        if (o.err == 0) {
+          o.err = kErrToofast;
+          o.err_pid = p.id;
+        }
+        return true;
+      }
+    return false;
+  };
+  // K1 at the current positions (refreshes persistent forces).
+  o.update_pf = true;
+  force(o);
+  o.update_pf = false;
+  for (auto &c : o.cells)
+    for (auto &p : c.parts) {
+      const float imf = (float)p.imass;
+      const bool newton = o.types[p.type].dynamics == kNewtonian;
+      for (int k = 0; k < 3; ++k) {
+        if (newton) {
+          p.vk[0][k] = p.f[k] * imf;
+          p.xk[0][k] = p.v[k];
+        } else {
+          p.xk[0][k] = p.f[k] * imf;
+        }
+        p.p0[k] = p.x[k];
+        p.v0[k] = p.v[k];
+        p.x[k] = p.p0[k] + half * p.xk[0][k];
+      }
+      if (toofast(p)) return o.err;
+    }
+  // K2 and K3.
+  for (int st = 1; st <= 2; ++st) {
+    force(o);
+    const float scale = st == 1 ? half : dt;  // position for the next stage
+    for (auto &c : o.cells)
+      for (auto &p : c.parts) {
+        const float imf = (float)p.imass;
+        const bool newton = o.types[p.type].dynamics == kNewtonian;
+        for (int k = 0; k < 3; ++k) {
+          if (newton) {
+            p.vk[st][k] = p.f[k] * imf;
+            p.xk[st][k] = p.v0[k] + half * p.vk[st - 1][k];
+          } else {
+            p.xk[st][k] = p.f[k] * imf;
+          }
+          p.x[k] = p.p0[k] + scale * p.xk[st][k];
+        }
+        if (toofast(p)) return o.err;
+      }
+  }
+  // K4 and the update.
+  force(o);
+  double epot = 0;
+  for (auto &c : o.cells) epot += c.epot;
+  o.epot_nb = epot;
+  for (int cid = 0; cid < o.ncells; ++cid) {
+    Cell &c = o.cells[cid];
+    size_t pid = 0;
+    while (pid < c.parts.size()) {
+      Part *p = &c.parts[pid];
+      const float imf = (float)p->imass;
+      const bool newton = o.types[p->type].dynamics == kNewtonian;
+      int delta[3];
+      bool fast = false;
+      for (int k = 0; k < 3; ++k) {
+        if (newton) {
+          p->vk[3][k] = imf * p->f[k];
+          p->xk[3][k] = p->v0[k] + dt * p->vk[2][k];
+          p->v[k] = p->v0[k] + sixth * (p->vk[0][k] + 2.0f * p->vk[1][k] + 2.0f * p->vk[2][k] + p->vk[3][k]);
+        } else {
+          p->xk[3][k] = imf * p->f[k];
+        }
+        p->x[k] = p->p0[k] + sixth * (p->xk[0][k] + 2.0f * p->xk[1][k] + 2.0f * p->xk[2][k] + p->xk[3][k]);
+        delta[k] = std::isgreaterequal(p->x[k], h[k]) - std::isless(p->x[k], 0.0);
+        fast = fast || (p->x[k] >= h2[k] || p->x[k] <= -h[k]);
+      }
+      if (fast && o.err == 0) {
+        o.err = kErrToofast;
+        o.err_pid = p->id;
+      }
+      if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
+        for (int k = 0; k < 3; k++) p->x[k] -= delta[k] * h[k];
+        int dest = o.cellid((c.loc[0] + delta[0] + o.cdim[0]) % o.cdim[0],
+                            (c.loc[1] + delta[1] + o.cdim[1]) % o.cdim[1],
+                            (c.loc[2] + delta[2] + o.cdim[2]) % o.cdim[2]);
+        o.cells[dest].incoming.push_back(*p);
+        size_t last = c.parts.size() - 1;
+        if (pid < last) c.parts[pid] = c.parts[last];
+        c.parts.pop_back();
+      } else {
+        pid += 1;
+      }
+    }
+  }
+  welcome_all(o);
+  return o.err;
+}
+
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // C interface for the tests and the CPU-baseline leg of bench.py.
+
 
 extern "C" {
 
@@ -1243,6 +1375,8 @@ int oracle_set_bonds(void *h, int M, const int *bi, const int *bj, const int *po
   return 0;
 }
 
+void oracle_set_integrator(void *h, int kind) { ((Oracle *)h)->integrator = kind; }
+
 int oracle_step(void *h, int nsteps) {
   Oracle *o = (Oracle *)h;
   int rc = 0;
@@ -1258,6 +1392,7 @@ int oracle_step(void *h, int nsteps) {
 int oracle_force(void *h) {
   Oracle *o = (Oracle *)h;
   o->time += 1;
+  o->update_pf = true;
   force(*o);
   double epot = 0;
   for (auto &c : o->cells) epot += c.epot;

@@ -40,6 +40,7 @@ _SIG = {
     "oracle_set_bonds": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "oracle_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "oracle_set_integrator": (None, [C.c_void_p, C.c_int]),
     "oracle_force": (C.c_int, [C.c_void_p]),
     "oracle_step_with_forces": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "oracle_get_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_float),
@@ -156,6 +157,10 @@ class Oracle:
         lib().oracle_set_bonds(self._h, M, _ip(i), _ip(j), _ip(p), _dp(d))
         self.M = M
 
+    def set_integrator(self, kind):
+        """0: forward Euler, 1: Runge-Kutta 4 (engine_advance_runge_kutta_4)."""
+        lib().oracle_set_integrator(self._h, int(kind))
+
     def step(self, n=1):
         return lib().oracle_step(self._h, int(n))
 
@@ -236,6 +241,8 @@ def from_workload(w, m, threads=1, seed=None):
             o.add_random_force(t, rf["mean"], rf["std"], rf["duration"])
     if w.get("dpd_noise"):
         o.set_dpd_noise(1)
+    if w.get("integrator"):
+        o.set_integrator(w["integrator"])
     o.load(w["x"], w["v"], w["type"])
     if w.get("bonds"):
         b = w["bonds"]
