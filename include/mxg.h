@@ -146,6 +146,14 @@ int mxg_engine_set_virial(mxg_engine *e, int on);
 int mxg_engine_set_brick(mxg_engine *e, int bx, int by, int bz, int cap, int pmax);
 int mxg_engine_get_brick(mxg_engine *e, int dims[3], int *cap, int *pmax);
 
+/* Integrator of the step: MXG_INTEGRATOR_EULER (engine_advance_forward_euler,
+ * engine_advance.cpp:248-422) or MXG_INTEGRATOR_RK4 (engine_advance_runge_kutta_4,
+ * engine_advance.cpp:443-698); replaces setting engine.integrator
+ * (EngineIntegrator, MxSimulator.cpp:218-231). */
+#define MXG_INTEGRATOR_EULER 0
+#define MXG_INTEGRATOR_RK4 1
+int mxg_engine_set_integrator(mxg_engine *e, int kind);
+
 /* Autotuner: times the force kernel for each brick shape and list capacity
  * that fits (steps_per real steps each: the layouts give identical
  * per-particle results, so the trajectory is unchanged) and keeps the

@@ -153,6 +153,10 @@ class Potential:
         return bool(ok), e.value, f.value
 
 
+FORWARD_EULER = 0
+RUNGE_KUTTA_4 = 1
+
+
 class Universe:
     """One engine: a periodic universe (or a batch of them) on the GPU."""
 
@@ -241,6 +245,10 @@ class Universe:
         L.mxg_engine_tune_log(self._h, n, _ip(lay), _dp(ms))
         log = [(tuple(int(v) for v in lay[k]), float(ms[k])) for k in range(n)]
         return {"steps": taken.value, "brick": self.brick(), "log": log}
+
+    def set_integrator(self, kind):
+        """FORWARD_EULER (0) or RUNGE_KUTTA_4 (1), as Mechanica's integrator."""
+        self._check(_capi.lib().mxg_engine_set_integrator(self._h, int(kind)))
 
     def set_graphs(self, on=True):
         """Replay pairs of steps from captured HIP graphs (default on)."""

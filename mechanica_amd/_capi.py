@@ -56,6 +56,7 @@ SIGNATURES = {
     "mxg_engine_set_max_dist_fraction": (C.c_int, [C.c_void_p, C.c_float]),
     "mxg_engine_set_virial": (C.c_int, [C.c_void_p, C.c_int]),
     "mxg_engine_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "mxg_engine_set_integrator": (C.c_int, [C.c_void_p, C.c_int]),
     "mxg_engine_set_brick": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mxg_engine_get_brick": (C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p]),
     "mxg_engine_autotune": (C.c_int, [C.c_void_p, C.c_int, c_int_p]),

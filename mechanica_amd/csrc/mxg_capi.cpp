@@ -200,6 +200,11 @@ int mxg_engine_tune_log(mxg_engine *e, int max, int *layout, double *force_ms) {
   return n;
 }
 
+int mxg_engine_set_integrator(mxg_engine *e, int kind) {
+  MXG_CHECK(e);
+  return e->e.set_integrator(kind);
+}
+
 int mxg_engine_set_graphs(mxg_engine *e, int on) {
   MXG_CHECK(e);
   return e->e.set_graphs(on);

@@ -27,6 +27,11 @@ enum : int {
   kErrToofast = -29,
 };
 
+enum : int {
+  kIntegratorEuler = 0,
+  kIntegratorRK4 = 1,
+};
+
 enum : int { kDynNewtonian = 0, kDynOverdamped = 1 };
 
 // Particle flags understood by the advance (MxParticle.h PARTICLE_FROZEN_*).
@@ -89,6 +94,9 @@ class Engine {
 
   // Replays captured HIP graphs of two steps (on by default).
   int set_graphs(int on);
+  // 0: forward Euler (default), 1: Runge-Kutta 4.
+  int set_integrator(int kind);
+  int integrator() const { return integrator_; }
 
   // Adds the virial tensor to the per-step reductions.
   int set_virial(int on);
@@ -124,6 +132,8 @@ class Engine {
   int build_geometry();
   int upload_potentials();
   int launch_step();
+  int launch_substep(int rk_stage, int tick);
+  int ensure_rk_buffers();
   int ensure_graph(int parity);
   void fill_step_args(void *out, int parity) const;
   int refresh_args();
@@ -151,6 +161,7 @@ class Engine {
   int N_ = 0, M_ = 0, ncells_ = 0, ntiles_ = 0;
   int bdim_[3] = {4, 4, 4};
   size_t brick_tabs_ = 0;
+  int integrator_ = 0;
   int cap_ = 12, pmax_ = 0, pmax_req_ = 0, hmax_ = 0, hdmax_ = 0, noff_ = 0, nbricks_ = 0;
   bool brick_dpd_ = false;
   std::vector<double> mean_cell_count_;

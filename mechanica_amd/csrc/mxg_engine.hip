@@ -105,6 +105,11 @@ struct DeviceState {
   float4 *x[2] = {nullptr, nullptr};
   float4 *v[2] = {nullptr, nullptr};
   float4 *pf[2] = {nullptr, nullptr};
+  // Runge-Kutta 4 state: start position, slope sums, previous velocity slope
+  float4 *rk_p0[2] = {nullptr, nullptr};
+  float4 *rk_ax[2] = {nullptr, nullptr};
+  float4 *rk_av[2] = {nullptr, nullptr};
+  float4 *rk_vk[2] = {nullptr, nullptr};
   int *pid[2] = {nullptr, nullptr};
   int *cell[2] = {nullptr, nullptr};
   int *cstart[2] = {nullptr, nullptr};
@@ -158,6 +163,17 @@ struct StepArgs {
   float4 G *__restrict__ v_out;
   float4 G *__restrict__ pf_out;
   float4 G *__restrict__ f_out;
+  // Runge-Kutta 4 state per slot (MxParticle p0, xk, vk): the stage's start
+  // position, the running sums of position and velocity slopes, and the
+  // previous stage's velocity slope; read from buffer 0, written to 1.
+  const float4 G *__restrict__ rk_p0;
+  const float4 G *__restrict__ rk_ax;
+  const float4 G *__restrict__ rk_av;
+  const float4 G *__restrict__ rk_vk;
+  float4 G *__restrict__ rk_p0_out;
+  float4 G *__restrict__ rk_ax_out;
+  float4 G *__restrict__ rk_av_out;
+  float4 G *__restrict__ rk_vk_out;
   int G *__restrict__ cell_out;
   const DevReplica G *__restrict__ rep;
   const DevOffset G *__restrict__ offs;
@@ -591,11 +607,13 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
 
 // Single-body persistent random force (gaussian_force, MxForce.cpp:370-385),
 // refreshed when (time + id) % steps == 0.
+// refresh: INTEGRATOR_UPDATE_PERSISTENTFORCE (set for the Euler step and the
+// first Runge-Kutta stage only).
 __device__ __forceinline__ void single_body(const StepArgs &A, int s, int mypid, const DevType &T, unsigned t1,
-                                            float &fx, float &fy, float &fz, float4 &pfs) {
+                                            bool refresh, float &fx, float &fy, float &fz, float4 &pfs) {
   pfs = A.pf[s];
   if (!T.random) return;
-  if (((uint64_t)t1 + (uint64_t)mypid) % T.rnd_steps == 0) {
+  if (refresh && ((uint64_t)t1 + (uint64_t)mypid) % T.rnd_steps == 0) {
     u32x4 ctr = {{(uint32_t)mypid, t1, kStreamSingle, 0u}};
     u32x4 r = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
     float g = gauss_from(r.v[0], r.v[1]);
@@ -745,6 +763,106 @@ __device__ __forceinline__ bool advance_store(const StepArgs &A, const DevReplic
   return newc == c;
 }
 
+// One Runge-Kutta 4 stage of engine_advance_runge_kutta_4
+// (engine_advance.cpp:443-698) for slot s, with this stage's force.  Stages
+// 0-2 set the position for the next force evaluation, p0 + c dt xk (c = 1/2,
+// 1/2, 1), check it (CHECK_TOOFAST, :424-431) and move it with p0 to the
+// neighbour cell it entered, as the next engine_force's
+// space_shuffle_local does (space.cpp:333-393); stage 3 forms the update
+// p0 + dt/6 (xk0 + 2 xk1 + 2 xk2 + xk3) (likewise v) and migrates.  Float
+// arithmetic as the reference's Magnum Vector3 (scalars converted to float:
+// 0.5 dt, dt / 6., imass); no clamps, no frozen mask.  Returns whether the
+// particle stayed in its cell.
+__device__ __forceinline__ bool rk_store(const StepArgs &A, const DevReplica &R, int stage, int s, int c, int lx,
+                                         int ly, int lz, const float4 &xs, const float4 &vs, const DevType &T,
+                                         int passthrough, float fx, float fy, float fz, const float4 &pfs,
+                                         int mypid, float nd) {
+  const float dt = A.dt;
+  const float half = (float)(0.5 * (double)dt);
+  const float sixth = (float)((double)dt / 6.);
+  const float imf = (float)T.imass;
+  const bool newton = T.dynamics == kDynNewtonian;
+  const float F[3] = {fx, fy, fz};
+  const float X[3] = {xs.x, xs.y, xs.z};
+  const float V[3] = {vs.x, vs.y, vs.z};  // v0: velocities change only at the last stage
+  float P0[3], AX[3], AV[3], VK[3], Xn[3], Vn[3] = {V[0], V[1], V[2]};
+  if (stage == 0) {
+    P0[0] = X[0];
+    P0[1] = X[1];
+    P0[2] = X[2];
+  } else {
+    const float4 p0 = float4(A.rk_p0[s]);
+    P0[0] = p0.x;
+    P0[1] = p0.y;
+    P0[2] = p0.z;
+  }
+  const float4 axv = stage ? float4(A.rk_ax[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 avv = stage ? float4(A.rk_av[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 vkv = stage ? float4(A.rk_vk[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float ax0[3] = {axv.x, axv.y, axv.z}, av0[3] = {avv.x, avv.y, avv.z}, vkp[3] = {vkv.x, vkv.y, vkv.z};
+  int newc = c;
+  if (!passthrough) {
+    int delta[3];
+    bool toofast = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float vk = newton ? (stage == 3 ? imf * F[k] : F[k] * imf) : 0.0f;
+      float xk;
+      if (newton)
+        xk = stage == 0 ? V[k] : V[k] + (stage == 3 ? dt : half) * vkp[k];
+      else
+        xk = stage == 3 ? imf * F[k] : F[k] * imf;
+      if (stage < 3) {
+        AX[k] = stage == 0 ? xk : ax0[k] + 2.0f * xk;
+        AV[k] = stage == 0 ? vk : av0[k] + 2.0f * vk;
+        VK[k] = vk;
+        Xn[k] = P0[k] + (stage == 2 ? dt : half) * xk;
+      } else {
+        AX[k] = AV[k] = VK[k] = 0.0f;
+        if (newton) Vn[k] = V[k] + sixth * (av0[k] + vk);
+        Xn[k] = P0[k] + sixth * (ax0[k] + xk);
+      }
+      delta[k] = (Xn[k] >= R.h[k] ? 1 : 0) - (Xn[k] < 0.0f ? 1 : 0);
+      toofast = toofast || (Xn[k] >= R.h2[k] || Xn[k] <= -R.h[k]);
+    }
+    if (toofast) {
+      if (atomicCAS(&A.err[0], 0, (int)kErrToofast) == 0) A.err[1] = mypid;
+    }
+    if (delta[0] | delta[1] | delta[2]) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        Xn[k] -= delta[k] * R.h[k];
+        P0[k] -= delta[k] * R.h[k];
+      }
+      const int nx = (lx + delta[0] + R.cdim[0]) % R.cdim[0];
+      const int ny = (ly + delta[1] + R.cdim[1]) % R.cdim[1];
+      const int nz = (lz + delta[2] + R.cdim[2]) % R.cdim[2];
+      newc = R.cell_base + (nx * R.cdim[1] + ny) * R.cdim[2] + nz;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Xn[k] = X[k];
+      AX[k] = ax0[k];
+      AV[k] = av0[k];
+      VK[k] = vkp[k];
+    }
+  }
+  A.x_out[s] = make_float4(Xn[0], Xn[1], Xn[2], xs.w);
+  A.v_out[s] = make_float4(Vn[0], Vn[1], Vn[2], vs.w);
+  A.cell_out[s] = newc != c ? (int)((unsigned)newc | 0x80000000u) : newc;
+  if (newc != c) atomicAdd(&A.inc[newc], 1);
+  A.f_out[s] = make_float4(fx, fy, fz, nd);
+  if (A.random) A.pf_out[s] = pfs;
+  if (stage < 3) {
+    A.rk_p0_out[s] = make_float4(P0[0], P0[1], P0[2], 0.f);
+    A.rk_ax_out[s] = make_float4(AX[0], AX[1], AX[2], 0.f);
+    A.rk_av_out[s] = make_float4(AV[0], AV[1], AV[2], 0.f);
+    A.rk_vk_out[s] = make_float4(VK[0], VK[1], VK[2], 0.f);
+  }
+  return newc == c;
+}
+
 // Bricks of cells, the work unit of the force kernel: a block stages the
 // brick and a halo of span cells around it (positions, velocities, ids) in
 // LDS with coalesced row loads, then each thread owns one particle of the
@@ -761,6 +879,7 @@ struct BrickArgs {
   int noff;     // neighbour offsets (capacity)
   int *ovf_list;   // bricks whose halo overflowed the LDS capacity
   int *ovf_count;
+  int rk_stage;    // -1: forward Euler step; 0-3: Runge-Kutta 4 stage
 };
 
 constexpr int kBT = 256;        // threads per brick block
@@ -993,7 +1112,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
       float nd = A.nd_norm;
       float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (!passthrough) {
-        if (A.random) single_body(A, s, mypid, T, t1, fx, fy, fz, pfs);
+        if (A.random) single_body(A, s, mypid, T, t1, B.rk_stage <= 0, fx, fy, fz, pfs);
         {
           const LdsNbr nb{hv, kDPD, hstart, hgb, nhalo, A.pid};
           const unsigned dmax = (unsigned)R.dmaxdist;
@@ -1111,8 +1230,11 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
         }
         if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
       }
-      if (advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd))
-        atomicAdd(&home_stay[hc], 1);
+      const bool stayed =
+          B.rk_stage < 0 ? advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd)
+                         : rk_store(A, R, B.rk_stage, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs,
+                                    mypid, nd);
+      if (stayed) atomicAdd(&home_stay[hc], 1);
     }
 
     // Deterministic block reduction of the FP64 partials for this brick.
@@ -1176,6 +1298,10 @@ struct RebuildArgs {
   int *__restrict__ cell;
   int *__restrict__ slot_of;
   int with_pf;
+  // Runge-Kutta 4 state, carried along between stages (buffer 1 -> 0)
+  const float4 *__restrict__ rk_in[4];
+  float4 *__restrict__ rk[4];
+  int with_rk;
 };
 
 // Exclusive scan of the per-cell counts (first pass); the counts are reset
@@ -1208,7 +1334,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_block(int *__restrict__ cnt
 // step was skipped because of an earlier error, and records the error state:
 // err[3] = this step was skipped, err[2] = error state for the next step.
 __global__ __launch_bounds__(kScanBlock) void k_scan_top(int *__restrict__ bsum, int nb, int *__restrict__ out,
-                                                        int n, unsigned *time, int *err, int *ovf_count) {
+                                                        int n, unsigned *time, int *err, int *ovf_count, int tick) {
   __shared__ int lds[kScanBlock / 64];
   int carry = 0;
   for (int base = 0; base < nb; base += kScanBlock) {
@@ -1225,7 +1351,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_top(int *__restrict__ bsum,
     *ovf_count = 0;
     const int skipped = err[2];
     err[3] = skipped;
-    if (!skipped) *time += 1u;
+    if (!skipped && tick) *time += 1u;  // once per step (the last Runge-Kutta stage)
     err[2] = err[0];
   }
 }
@@ -1256,6 +1382,10 @@ __global__ __launch_bounds__(256) void k_place(RebuildArgs A) {
   A.x[dst] = A.x_in[q];
   A.v[dst] = A.v_in[q];
   if (A.with_pf) A.pf[dst] = A.pf_in[q];
+  if (A.with_rk) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) A.rk[k][dst] = A.rk_in[k][q];
+  }
   const int p = A.pid_in[q];
   A.pid[dst] = p;
   A.cell[dst] = c1;
@@ -1403,7 +1533,7 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
       float nd = A.nd_norm;
       float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (!passthrough) {
-        if (A.random) single_body(A, s, mypid, T, t1, fx, fy, fz, pfs);
+        if (A.random) single_body(A, s, mypid, T, t1, B.rk_stage <= 0, fx, fy, fz, pfs);
         const GlobalNbr nb{A.v, A.pid};
         for_each_neighbour(A, C, R, s, lx, ly, lz, xs,
                            [&](int q, bool p_is_i, bool self, float dx, float dy, float dz, float r2,
@@ -1416,8 +1546,11 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
                            });
         if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
       }
-      if (advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd))
-        atomicAdd(&home_stay[hc], 1);
+      const bool stayed =
+          B.rk_stage < 0 ? advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd)
+                         : rk_store(A, R, B.rk_stage, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs,
+                                    mypid, nd);
+      if (stayed) atomicAdd(&home_stay[hc], 1);
     }
 #pragma unroll
     for (int k = 0; k < kNRed; ++k) {
@@ -1911,6 +2044,7 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   pid_prev_valid_ = false;
   parity_ = 0;
   args_dirty_ = true;
+  if (int rc = ensure_rk_buffers()) return rc;
   return set_brick(0, 0, 0, 0, 0);
 }
 
@@ -2116,6 +2250,14 @@ void Engine::fill_step_args(void *out, int par) const {
   A.pf_out = (decltype(A.pf_out))(d->pf[1]);
   A.f_out = (decltype(A.f_out))(d->f);
   A.cell_out = (decltype(A.cell_out))(d->cell[1]);
+  A.rk_p0 = (decltype(A.rk_p0))(d->rk_p0[0]);
+  A.rk_ax = (decltype(A.rk_ax))(d->rk_ax[0]);
+  A.rk_av = (decltype(A.rk_av))(d->rk_av[0]);
+  A.rk_vk = (decltype(A.rk_vk))(d->rk_vk[0]);
+  A.rk_p0_out = (decltype(A.rk_p0_out))(d->rk_p0[1]);
+  A.rk_ax_out = (decltype(A.rk_ax_out))(d->rk_ax[1]);
+  A.rk_av_out = (decltype(A.rk_av_out))(d->rk_av[1]);
+  A.rk_vk_out = (decltype(A.rk_vk_out))(d->rk_vk[1]);
   A.rep = (decltype(A.rep))(d->rep);
   A.offs = (decltype(A.offs))(d->offs);
   A.shifts = (decltype(A.shifts))(d->shifts);
@@ -2175,6 +2317,18 @@ int Engine::refresh_args() {
 }
 
 int Engine::launch_step() {
+  if (integrator_ == kIntegratorRK4) {
+    // engine_advance_runge_kutta_4: four force evaluations, each followed
+    // by the cell-list rebuild (the reference's space_shuffle_local in the
+    // next engine_force); the time advances once.
+    for (int stage = 0; stage < 4; ++stage)
+      if (int st = launch_substep(stage, stage == 3)) return st;
+    return 0;
+  }
+  return launch_substep(-1, 1);
+}
+
+int Engine::launch_substep(int rk_stage, int tick) {
   DeviceState *d = d_;
   const int par = parity_;
   int mode = 0;
@@ -2196,6 +2350,7 @@ int Engine::launch_step() {
   Bk.noff = noff_;
   Bk.ovf_list = d->ovf_list;
   Bk.ovf_count = d->ovf_count;
+  Bk.rk_stage = rk_stage;
   const size_t tabs = pair_tab_bytes((int)pots_.size(), (int)types_.size());
   const size_t lds = brick_lds_bytes(pmax_, hmax_, hdmax_, noff_, cap_, (mode & kModeDPD) != 0, tabs);
   kStepLaunchers[mode](reinterpret_cast<const StepArgs *>(d->dargs) + par, Bk, lds, tabs, s);
@@ -2223,13 +2378,19 @@ int Engine::launch_step() {
   B.cell = d->cell[0];
   B.slot_of = d->slot_of;
   B.with_pf = any_random_ ? 1 : 0;
+  B.with_rk = rk_stage >= 0 && rk_stage < 3 ? 1 : 0;
+  for (int k = 0; k < 4; ++k) {
+    float4 *const *buf = k == 0 ? d->rk_p0 : k == 1 ? d->rk_ax : k == 2 ? d->rk_av : d->rk_vk;
+    B.rk_in[k] = buf[1];
+    B.rk[k] = buf[0];
+  }
   const int cg = (ncells_ + 255) / 256;
   const int pg = (N_ + 255) / 256;
   const int nb = (ncells_ + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, d->cnt, d->inc, d->cstart[par ^ 1], d->bsum,
                      ncells_);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, d->bsum, nb, d->cstart[par ^ 1], ncells_,
-                     d->time, d->err, d->ovf_count);
+                     d->time, d->err, d->ovf_count, tick);
   hipLaunchKernelGGL(k_scan_add, dim3(cg), dim3(256), 0, s, d->cstart[par ^ 1], d->bsum, ncells_);
   hipLaunchKernelGGL(k_place, dim3(pg), dim3(256), 0, s, B);
   hipLaunchKernelGGL(k_order_incoming, dim3(cg), dim3(256), 0, s, B);
@@ -2317,6 +2478,29 @@ int Engine::autotune(int steps_per, int *taken) {
   return set_brick(best[0], best[1], best[2], best[3], pmax_req_);
 }
 
+// engine.integrator (EngineIntegrator FORWARD_EULER / RUNGE_KUTTA_4, chosen
+// by the simulator's integrator argument, MxSimulator.cpp:218-231).
+int Engine::set_integrator(int kind) {
+  if (kind != kIntegratorEuler && kind != kIntegratorRK4) return reject(kErrRange, "unknown integrator");
+  integrator_ = kind;
+  ++config_gen_;
+  if (d_ && N_ > 0) return ensure_rk_buffers();
+  return kErrOk;
+}
+
+int Engine::ensure_rk_buffers() {
+  if (integrator_ != kIntegratorRK4 || !d_ || N_ == 0) return kErrOk;
+  DeviceState *d = d_;
+  for (int b = 0; b < 2; ++b) {
+    if (int st = dalloc(d, &d->rk_p0[b], N_)) return check_device(st, "hipMalloc");
+    if (int st = dalloc(d, &d->rk_ax[b], N_)) return check_device(st, "hipMalloc");
+    if (int st = dalloc(d, &d->rk_av[b], N_)) return check_device(st, "hipMalloc");
+    if (int st = dalloc(d, &d->rk_vk[b], N_)) return check_device(st, "hipMalloc");
+  }
+  args_dirty_ = true;
+  return kErrOk;
+}
+
 int Engine::set_graphs(int on) {
   use_graphs_ = on != 0;
   return kErrOk;
@@ -2334,8 +2518,9 @@ int Engine::step(int nsteps) {
   if (args_dirty_)
     if (int rc = refresh_args()) return rc;
   if (timing_) {
-    // Two events per step, for up to 4096 steps per call.
-    const size_t want = 2 * (size_t)std::min(nsteps, 4096);
+    // Two events per force launch, for up to 4096 launches per call.
+    const int per = integrator_ == kIntegratorRK4 ? 4 : 1;
+    const size_t want = 2 * (size_t)std::min(nsteps * per, 4096);
     while (d_->kev.size() < want) {
       hipEvent_t e;
       if (int rc = check_device(hipEventCreate(&e), "hipEventCreate")) return rc;

@@ -326,6 +326,8 @@ def build_universe(m, w, seed=None):
             u.add_random_force(t, rf["mean"], rf["std"], rf["duration"])
     if w.get("dpd_noise"):
         u.set_dpd_noise(1)
+    if w.get("integrator"):
+        u.set_integrator(w["integrator"])
     u.load(w["x"], w["v"], w["type"])
     if w.get("bonds"):
         b = w["bonds"]
