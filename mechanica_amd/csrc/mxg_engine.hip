@@ -1417,6 +1417,14 @@ __global__ __launch_bounds__(256) void k_order_incoming(RebuildArgs A) {
           A.pf[a] = A.pf[b];
           A.pf[b] = tf;
         }
+        if (A.with_rk) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float4 t = A.rk[r][a];
+            A.rk[r][a] = A.rk[r][b];
+            A.rk[r][b] = t;
+          }
+        }
       }
     }
     for (int k = base; k < base + n_in; ++k) A.slot_of[A.pid[k]] = k;
