@@ -21,9 +21,14 @@ def min_image(d, L):
 
 
 def rk4_cases():
+    # The lattice at its Euler step (dt 0.005) is too stiff for the
+    # unclamped RK4 step: the oracle itself stops with toofast (-29) in the
+    # first step.  dt 0.002 is stable.
+    lat = W.lattice(n=10, seed=62)
+    lat["dt"] = 0.002
     return [
         W.argon_lj(N=3000, seed=61, min_dist=0.3),
-        W.lattice(n=10, seed=62),
+        lat,
         W.dpd_fluid(N=4000, sigma=0.0, seed=63),
     ]
 
@@ -72,6 +77,20 @@ def test_rk4_two_body_fp64(m):
     xr, vr = rk4_ref(x0, v0, 0, 0.01, 50.0, 1.0, 2.0, 25)
     assert np.abs(xg - xr).max() < 1e-4
     assert np.abs(vg - vr).max() < 2.5e-4
+
+
+def test_rk4_toofast_like_oracle(m, oracle_mod):
+    """The stiff lattice at dt 0.005: both stop with toofast in step one."""
+    w = W.lattice(n=10, seed=62)
+    w["integrator"] = 1
+    print("case: rk4 lattice toofast", flush=True)
+    u = W.build_universe(m, w, seed=3)
+    o = oracle_mod.from_workload(w, m, seed=3)
+    assert o.step(1) == -29
+    with pytest.raises(m.EngineError) as e:
+        u.step(1)
+    assert e.value.code == -29
+    assert u.error == -29
 
 
 def test_rk4_graph_replay_bitwise(m):
