@@ -123,6 +123,19 @@ int mxg_engine_addpot(mxg_engine *e, const mxg_potential *p, int i, int j);
  * direction, refreshed every ceil(duration/dt) steps. */
 int mxg_engine_add_random_force(mxg_engine *e, int type, float mean, float std, float duration);
 
+/* The other single-body forces of MxForce.cpp (one per type; a later one
+ * replaces the type's force, as engine_add_singlebody_force does):
+ * berenderson_create(tau) (MxForce.cpp:386-393, force :330-337: f += v/tau
+ * (E_target/E_kin(type) - 1) with E_kin the type's kinetic energy at the
+ * start of the step and E_target set by mxg_engine_set_target_energy,
+ * MxParticleType::target_energy), the constant force (:339-344, f += F) and
+ * friction_create(coef, mean, std, durration) (:408-420, force :352-367:
+ * f += -coef |v|^2 v + a persistent random vector). */
+int mxg_engine_add_force_berendsen(mxg_engine *e, int type, float tau);
+int mxg_engine_add_force_constant(mxg_engine *e, int type, const float force[3]);
+int mxg_engine_add_force_friction(mxg_engine *e, int type, float coef, float mean, float std, float duration);
+int mxg_engine_set_target_energy(mxg_engine *e, int type, double energy);
+
 /* Seed of the counter-based generator (no reference counterpart: the
  * reference uses a global Mersenne twister). */
 int mxg_engine_set_seed(mxg_engine *e, uint64_t seed);

@@ -213,6 +213,23 @@ class Universe:
         self._check(_capi.lib().mxg_engine_add_random_force(self._h, int(type_id), float(mean), float(std),
                                                             float(duration)))
 
+    def add_berendsen(self, type_id, tau):
+        """forces.berenderson_tstat(tau) bound to a type."""
+        self._check(_capi.lib().mxg_engine_add_force_berendsen(self._h, int(type_id), float(tau)))
+
+    def add_constant_force(self, type_id, force):
+        f = np.ascontiguousarray(force, dtype=np.float32).reshape(3)
+        self._check(_capi.lib().mxg_engine_add_force_constant(self._h, int(type_id), _fp(f)))
+
+    def add_friction(self, type_id, coef, mean=0.0, std=0.0, duration=0.01):
+        """forces.friction(coef, mean, std, durration) bound to a type."""
+        self._check(_capi.lib().mxg_engine_add_force_friction(self._h, int(type_id), float(coef), float(mean),
+                                                              float(std), float(duration)))
+
+    def set_target_energy(self, type_id, energy):
+        """MxParticleType.target_energy (the Berendsen target)."""
+        self._check(_capi.lib().mxg_engine_set_target_energy(self._h, int(type_id), float(energy)))
+
     def set_seed(self, seed):
         self._check(_capi.lib().mxg_engine_set_seed(self._h, int(seed) & 0xFFFFFFFFFFFFFFFF))
 

@@ -51,6 +51,10 @@ SIGNATURES = {
     "mxg_engine_addtype": (C.c_int, [C.c_void_p, C.c_double, C.c_float, C.c_int]),
     "mxg_engine_addpot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
     "mxg_engine_add_random_force": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "mxg_engine_add_force_berendsen": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
+    "mxg_engine_add_force_constant": (C.c_int, [C.c_void_p, C.c_int, c_float_p]),
+    "mxg_engine_add_force_friction": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "mxg_engine_set_target_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "mxg_engine_set_seed": (C.c_int, [C.c_void_p, C.c_uint64]),
     "mxg_engine_set_dpd_noise": (C.c_int, [C.c_void_p, C.c_int]),
     "mxg_engine_set_max_dist_fraction": (C.c_int, [C.c_void_p, C.c_float]),

@@ -153,6 +153,28 @@ int mxg_engine_add_random_force(mxg_engine *e, int type, float mean, float std, 
   return e->e.set_random_force(type, mean, std, duration);
 }
 
+int mxg_engine_add_force_berendsen(mxg_engine *e, int type, float tau) {
+  MXG_CHECK(e);
+  return e->e.set_single_body(type, 2, &tau);
+}
+
+int mxg_engine_add_force_constant(mxg_engine *e, int type, const float force[3]) {
+  MXG_CHECK(e);
+  if (!force) return MXG_ERR_NULL;
+  return e->e.set_single_body(type, 3, force);
+}
+
+int mxg_engine_add_force_friction(mxg_engine *e, int type, float coef, float mean, float std, float duration) {
+  MXG_CHECK(e);
+  const float p[4] = {mean, std, duration, coef};
+  return e->e.set_single_body(type, 4, p);
+}
+
+int mxg_engine_set_target_energy(mxg_engine *e, int type, double energy) {
+  MXG_CHECK(e);
+  return e->e.set_target_energy(type, energy);
+}
+
 int mxg_engine_set_seed(mxg_engine *e, uint64_t seed) {
   MXG_CHECK(e);
   return e->e.set_seed(seed);

@@ -65,6 +65,11 @@ class Engine {
   int add_type(double mass, float radius, int dynamics);
   int add_pot(const Potential &p, int i, int j);
   int set_random_force(int type, float mean, float std, float duration);
+  // Single-body force of a type (kind: 1 gaussian p = mean, std, duration;
+  // 2 berendsen p = tau; 3 constant p = F; 4 friction p = mean, std,
+  // duration, coef).
+  int set_single_body(int type, int kind, const float *p);
+  int set_target_energy(int type, double e);
   int set_seed(uint64_t seed);
   int set_dpd_counter_noise(int on);
   int set_max_dist_fraction(float f);
@@ -149,9 +154,12 @@ class Engine {
     double mass, imass;
     float radius;
     int dynamics;
-    bool random = false;
+    int sb = 0;  // single-body force kind (0 none, 1 gaussian, 2 berendsen, 3 constant, 4 friction)
     float rnd_mean = 0, rnd_std = 0;
     unsigned rnd_steps = 1;
+    float sb_coef = 0;
+    float cf[3] = {0, 0, 0};
+    double target_energy = 0;
   };
   std::vector<TypeInfo> types_;
   std::vector<Potential> pots_;
@@ -166,7 +174,7 @@ class Engine {
   bool brick_dpd_ = false;
   std::vector<double> mean_cell_count_;
   int block_ = 256;
-  bool any_dpd_ = false, any_table_ = false, any_random_ = false;
+  bool any_dpd_ = false, any_table_ = false, any_random_ = false, any_sb_ = false, any_berendsen_ = false;
   bool potentials_dirty_ = true;
   uint64_t time_ = 0;
   int err_ = 0;

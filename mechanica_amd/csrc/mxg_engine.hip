@@ -88,13 +88,20 @@ struct DevPot {
   int pad[2];
 };
 
+// Single-body force of a type (engine's psbs[type], one per type):
+// MxForce.cpp:330-385.
+enum : int { kSbNone = 0, kSbGaussian = 1, kSbBerendsen = 2, kSbConstant = 3, kSbFriction = 4 };
+
 struct DevType {
   double mass, imass;
+  double target_energy;  // MxParticleType::target_energy (Berendsen)
   float radius;
   int dynamics;
-  int random;
+  int sb;                // kSb*
   float rnd_mean, rnd_std;
   unsigned rnd_steps;
+  float sb_coef;         // Berendsen 1/tau, friction coefficient
+  float cf[3];           // constant force
 };
 
 }  // namespace
@@ -141,6 +148,8 @@ struct DeviceState {
   double *bdis = nullptr;
   unsigned char *bact[2] = {nullptr, nullptr};
   double *partials = nullptr;
+  double *type_ke = nullptr;       // kinetic energy per type (Berendsen)
+  double *type_ke_part = nullptr;  // per-block partial sums of it
   double *obs = nullptr;
   unsigned *time = nullptr;
   int *err = nullptr;  // [0] error code, [1] pid, [2] error state at step start
@@ -200,6 +209,8 @@ struct StepArgs {
   float dpd_delta;
   uint32_t seed_lo, seed_hi;
   int dpd_noise;
+  const double G *type_ke;  // kinetic energy per type at the start of the step (Berendsen)
+  int with_pf;               // a type carries a persistent random force
   const unsigned G *time;
   int G *err;
   double G *partials;
@@ -605,28 +616,54 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
   return wsum + x - v;
 }
 
-// Single-body persistent random force (gaussian_force, MxForce.cpp:370-385),
-// refreshed when (time + id) % steps == 0.
+// The type's single-body force, added in runner_doself before the pair
+// loop (runner_dopair.cpp:519-523), MxForce.cpp:330-385:
+//   gaussian  f += pf, pf = MxRandomVector(mean, std) refreshed when
+//             (time + id) % steps == 0;
+//   berendsen f += itau (E_target / E_kin(type) - 1) v;
+//   constant  f += F;
+//   friction  f += -coef |v|^2 v + pf (pf refreshed as gaussian's).
 // refresh: INTEGRATOR_UPDATE_PERSISTENTFORCE (set for the Euler step and the
-// first Runge-Kutta stage only).
-__device__ __forceinline__ void single_body(const StepArgs &A, int s, int mypid, const DevType &T, unsigned t1,
-                                            bool refresh, float &fx, float &fy, float &fz, float4 &pfs) {
-  pfs = A.pf[s];
-  if (!T.random) return;
-  if (refresh && ((uint64_t)t1 + (uint64_t)mypid) % T.rnd_steps == 0) {
-    u32x4 ctr = {{(uint32_t)mypid, t1, kStreamSingle, 0u}};
-    u32x4 r = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
-    float g = gauss_from(r.v[0], r.v[1]);
-    float mag = T.rnd_mean + T.rnd_std * g;
-    float theta = 6.2831853f * u01_closed_open(r.v[2]);
-    float phi = acosf(1.0f - 2.0f * u01_closed_open(r.v[3]));
-    pfs.x = mag * sinf(phi) * cosf(theta);
-    pfs.y = mag * sinf(phi) * sinf(theta);
-    pfs.z = mag * cosf(phi);
+// first Runge-Kutta stage only).  The random vector comes from Philox keyed
+// by (id, time).
+__device__ __forceinline__ void single_body(const StepArgs &A, int s, int mypid, int ti, const DevType &T,
+                                            unsigned t1, bool refresh, const float4 &vs, float &fx, float &fy,
+                                            float &fz, float4 &pfs) {
+  pfs = A.with_pf ? float4(A.pf[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  if (T.sb == kSbNone) return;
+  if (T.sb == kSbGaussian || T.sb == kSbFriction) {
+    if (refresh && ((uint64_t)t1 + (uint64_t)mypid) % T.rnd_steps == 0) {
+      u32x4 ctr = {{(uint32_t)mypid, t1, kStreamSingle, 0u}};
+      u32x4 r = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
+      float g = gauss_from(r.v[0], r.v[1]);
+      float mag = T.rnd_mean + T.rnd_std * g;
+      float theta = 6.2831853f * u01_closed_open(r.v[2]);
+      float phi = acosf(1.0f - 2.0f * u01_closed_open(r.v[3]));
+      pfs.x = mag * sinf(phi) * cosf(theta);
+      pfs.y = mag * sinf(phi) * sinf(theta);
+      pfs.z = mag * cosf(phi);
+    }
   }
-  fx += pfs.x;
-  fy += pfs.y;
-  fz += pfs.z;
+  if (T.sb == kSbGaussian) {
+    fx += pfs.x;
+    fy += pfs.y;
+    fz += pfs.z;
+  } else if (T.sb == kSbFriction) {
+    const float v2 = vs.x * vs.x + vs.y * vs.y + vs.z * vs.z;
+    const float scale = (float)(-1. * (double)T.sb_coef * (double)v2);
+    fx += scale * vs.x + pfs.x;
+    fy += scale * vs.y + pfs.y;
+    fz += scale * vs.z + pfs.z;
+  } else if (T.sb == kSbBerendsen) {
+    const float scale = (float)((double)T.sb_coef * ((T.target_energy / A.type_ke[ti]) - 1.0));
+    fx += scale * vs.x;
+    fy += scale * vs.y;
+    fz += scale * vs.z;
+  } else if (T.sb == kSbConstant) {
+    fx += T.cf[0];
+    fy += T.cf[1];
+    fz += T.cf[2];
+  }
 }
 
 // Bonds of one particle (bond_eval, bond.cpp:99-295), gathered per particle.
@@ -759,7 +796,7 @@ __device__ __forceinline__ bool advance_store(const StepArgs &A, const DevReplic
   // per home cell in LDS and writes the cell's count once.
   if (newc != c) atomicAdd(&A.inc[newc], 1);
   A.f_out[s] = make_float4(fx, fy, fz, nd);
-  if (A.random) A.pf_out[s] = pfs;
+  if (A.with_pf) A.pf_out[s] = pfs;
   return newc == c;
 }
 
@@ -853,7 +890,7 @@ __device__ __forceinline__ bool rk_store(const StepArgs &A, const DevReplica &R,
   A.cell_out[s] = newc != c ? (int)((unsigned)newc | 0x80000000u) : newc;
   if (newc != c) atomicAdd(&A.inc[newc], 1);
   A.f_out[s] = make_float4(fx, fy, fz, nd);
-  if (A.random) A.pf_out[s] = pfs;
+  if (A.with_pf) A.pf_out[s] = pfs;
   if (stage < 3) {
     A.rk_p0_out[s] = make_float4(P0[0], P0[1], P0[2], 0.f);
     A.rk_ax_out[s] = make_float4(AX[0], AX[1], AX[2], 0.f);
@@ -1112,7 +1149,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
       float nd = A.nd_norm;
       float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (!passthrough) {
-        if (A.random) single_body(A, s, mypid, T, t1, B.rk_stage <= 0, fx, fy, fz, pfs);
+        if (A.random) single_body(A, s, mypid, ti, T, t1, B.rk_stage <= 0, vs, fx, fy, fz, pfs);
         {
           const LdsNbr nb{hv, kDPD, hstart, hgb, nhalo, A.pid};
           const unsigned dmax = (unsigned)R.dmaxdist;
@@ -1462,6 +1499,45 @@ __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ par
   }
 }
 
+// engine_kinetic_energy's per-type sums (engine.cpp:1698-1719) for the
+// Berendsen force: 1/2 m v^2 per particle in FP32, summed in FP64 per block
+// (a fixed strided subset per thread), then over blocks in block order.
+constexpr int kKeBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_type_ke(const float4 *__restrict__ x, const float4 *__restrict__ v,
+                                                 const DevType *__restrict__ types, int n,
+                                                 double *__restrict__ part) {
+  double acc[kMaxTypes];
+#pragma unroll
+  for (int t = 0; t < kMaxTypes; ++t) acc[t] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int ti = __float_as_int(x[i].w) & 0xffff;
+    const float4 vv = v[i];
+    const float e = 0.5 * types[ti].mass * (vv.x * vv.x + vv.y * vv.y + vv.z * vv.z);
+#pragma unroll
+    for (int t = 0; t < kMaxTypes; ++t) acc[t] += ti == t ? (double)e : 0.0;
+  }
+  __shared__ double red[kMaxTypes][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < kMaxTypes; ++t) {
+    const double w = wave_sum(acc[t]);
+    if (lane == 0) red[t][wave] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxTypes)
+    part[blockIdx.x * kMaxTypes + threadIdx.x] = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] +
+                                                 red[threadIdx.x][3];
+}
+
+__global__ __launch_bounds__(64) void k_type_ke_final(const double *__restrict__ part, double *__restrict__ ke) {
+  const int t = threadIdx.x;
+  if (t >= kMaxTypes) return;
+  double s = 0.0;
+  for (int b = 0; b < kKeBlocks; ++b) s += part[b * kMaxTypes + t];
+  ke[t] = s;
+}
+
 // Bricks whose halo overflowed LDS: the same computation with neighbours
 // read from global memory, one thread per particle of the brick.
 template <int MODE>
@@ -1541,7 +1617,7 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
       float nd = A.nd_norm;
       float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (!passthrough) {
-        if (A.random) single_body(A, s, mypid, T, t1, B.rk_stage <= 0, fx, fy, fz, pfs);
+        if (A.random) single_body(A, s, mypid, ti, T, t1, B.rk_stage <= 0, vs, fx, fy, fz, pfs);
         const GlobalNbr nb{A.v, A.pid};
         for_each_neighbour(A, C, R, s, lx, ly, lz, xs,
                            [&](int q, bool p_is_i, bool self, float dx, float dy, float dz, float r2,
@@ -1841,18 +1917,48 @@ int Engine::add_pot(const Potential &p, int i, int j) {
   return kErrOk;
 }
 
-int Engine::set_random_force(int type, float mean, float std, float duration) {
-  if (type < 0 || type >= (int)types_.size()) return fail(kErrRange, "type index out of range");
+// engine_add_singlebody_force: one force per type, a later one replaces it.
+int Engine::set_single_body(int type, int kind, const float *p) {
+  if (type < 0 || type >= (int)types_.size()) return reject(kErrRange, "type index out of range");
   TypeInfo &t = types_[type];
-  t.random = true;
-  t.rnd_mean = mean;
-  t.rnd_std = std;
-  // durration_steps = ceil(durration / dt) (MxForce.cpp:397-407).
-  double steps = std::ceil(duration / dt_);
-  t.rnd_steps = steps < 1 ? 1u : (unsigned)steps;
-  any_random_ = true;
+  t.sb = kind;
+  if (kind == kSbGaussian || kind == kSbFriction) {
+    // random_create / friction_create: durration_steps = ceil(durration / dt)
+    // (MxForce.cpp:397-420).
+    const float mean = p[0], std = p[1], duration = p[2];
+    t.rnd_mean = mean;
+    t.rnd_std = std;
+    double steps = std::ceil(duration / dt_);
+    t.rnd_steps = steps < 1 ? 1u : (unsigned)steps;
+    t.sb_coef = kind == kSbFriction ? p[3] : 0.f;
+  } else if (kind == kSbBerendsen) {
+    t.sb_coef = 1 / p[0];  // berenderson_create: itau = 1 / tau
+  } else if (kind == kSbConstant) {
+    for (int k = 0; k < 3; ++k) t.cf[k] = p[k];
+  }
+  any_random_ = false;
+  any_sb_ = false;
+  any_berendsen_ = false;
+  for (const auto &u : types_) {
+    any_random_ = any_random_ || u.sb == kSbGaussian || u.sb == kSbFriction;
+    any_sb_ = any_sb_ || u.sb != kSbNone;
+    any_berendsen_ = any_berendsen_ || u.sb == kSbBerendsen;
+  }
   potentials_dirty_ = true;
   args_dirty_ = true;
+  ++config_gen_;
+  return kErrOk;
+}
+
+int Engine::set_random_force(int type, float mean, float std, float duration) {
+  const float p[3] = {mean, std, duration};
+  return set_single_body(type, kSbGaussian, p);
+}
+
+int Engine::set_target_energy(int type, double e) {
+  if (type < 0 || type >= (int)types_.size()) return reject(kErrRange, "type index out of range");
+  types_[type].target_energy = e;
+  potentials_dirty_ = true;
   return kErrOk;
 }
 
@@ -1921,10 +2027,13 @@ int Engine::upload_potentials() {
     dt[i].imass = types_[i].imass;
     dt[i].radius = types_[i].radius;
     dt[i].dynamics = types_[i].dynamics;
-    dt[i].random = types_[i].random ? 1 : 0;
+    dt[i].target_energy = types_[i].target_energy;
+    dt[i].sb = types_[i].sb;
     dt[i].rnd_mean = types_[i].rnd_mean;
     dt[i].rnd_std = types_[i].rnd_std;
     dt[i].rnd_steps = types_[i].rnd_steps;
+    dt[i].sb_coef = types_[i].sb_coef;
+    for (int k = 0; k < 3; ++k) dt[i].cf[k] = types_[i].cf[k];
   }
   // Potentials change at set-up only, so reallocating here is fine.
   auto upload = [&](auto **dst, const auto &vec) -> int {
@@ -2025,6 +2134,10 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
     if (int st = dalloc(d, &d->x[b], N)) return check_device(st, "hipMalloc");
     if (int st = dalloc(d, &d->v[b], N)) return check_device(st, "hipMalloc");
     if (int st = dalloc(d, &d->pf[b], N)) return check_device(st, "hipMalloc");
+    if (b == 0) {
+      if (int st = dalloc(d, &d->type_ke, kMaxTypes)) return check_device(st, "hipMalloc");
+      if (int st = dalloc(d, &d->type_ke_part, (size_t)kKeBlocks * kMaxTypes)) return check_device(st, "hipMalloc");
+    }
     if (int st = dalloc(d, &d->pid[b], N)) return check_device(st, "hipMalloc");
     if (int st = dalloc(d, &d->cell[b], N)) return check_device(st, "hipMalloc");
   }
@@ -2302,7 +2415,9 @@ void Engine::fill_step_args(void *out, int par) const {
   A.cnt = (decltype(A.cnt))(d->cnt);
   A.inc = (decltype(A.inc))(d->inc);
   A.ncount = (decltype(A.ncount))(d->ncount);
-  A.random = (decltype(A.random))(any_random_ ? 1 : 0);
+  A.random = (decltype(A.random))(any_sb_ ? 1 : 0);
+  A.with_pf = (decltype(A.with_pf))(any_random_ ? 1 : 0);
+  A.type_ke = (decltype(A.type_ke))(d->type_ke);
   A.cutoff = (decltype(A.cutoff))((float)cutoff_);
   A.nd_norm = (decltype(A.nd_norm))((float)(1.0 / (M_PI * cutoff_ * cutoff_ * cutoff_)));
 }
@@ -2345,6 +2460,11 @@ int Engine::launch_substep(int rk_stage, int tick) {
   if (virial_) mode |= kModeVirial;
   if (M_ > 0) mode |= kModeBonds;
   hipStream_t s = d->stream;
+  if (any_berendsen_) {
+    hipLaunchKernelGGL(k_type_ke, dim3(kKeBlocks), dim3(256), 0, s, d->x[0], d->v[0], d->types, N_,
+                       d->type_ke_part);
+    hipLaunchKernelGGL(k_type_ke_final, dim3(1), dim3(64), 0, s, d->type_ke_part, d->type_ke);
+  }
   const bool rec = timing_ && timed_steps_ < (int)d->kev.size() / 2;
   if (rec) hipEventRecord(d->kev[2 * timed_steps_], s);
   BrickArgs Bk;

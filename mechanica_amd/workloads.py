@@ -328,6 +328,17 @@ def build_universe(m, w, seed=None):
         u.set_dpd_noise(1)
     if w.get("integrator"):
         u.set_integrator(w["integrator"])
+    # single_body: {type: (kind, params)}, kinds 2 berendsen (tau), 3 constant
+    # (F), 4 friction (mean, std, duration, coef)
+    for t, (kind, params) in (w.get("single_body") or {}).items():
+        if kind == 2:
+            u.add_berendsen(t, params[0])
+        elif kind == 3:
+            u.add_constant_force(t, params)
+        elif kind == 4:
+            u.add_friction(t, params[3], params[0], params[1], params[2])
+    for t, e in (w.get("target_energy") or {}).items():
+        u.set_target_energy(t, e)
     u.load(w["x"], w["v"], w["type"])
     if w.get("bonds"):
         b = w["bonds"]

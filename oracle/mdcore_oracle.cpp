@@ -29,6 +29,8 @@
 //   engine_advance_forward_euler .................... engine_advance.cpp:248-422
 //   engine_advance_runge_kutta_4 .................... engine_advance.cpp:443-698
 //   gaussian_force (persistent random force) ........ MxForce.cpp:370-385
+//   berendsen_force, constant_force, friction_force . MxForce.cpp:330-367
+//   per-type kinetic energy ......................... engine.cpp:1698-1719
 //
 // Deviations, each stated where it happens:
 //   * gaussian_force draws from Philox4x32-10 instead of carbon's CRandom
@@ -111,9 +113,15 @@ struct Type {
   float radius = 1;
   int dynamics = kNewtonian;
   // single-body persistent random force (gaussian_force)
-  bool has_random = false;
+  bool has_random = false;  // persistent random force (gaussian or friction)
   float rnd_mean = 0, rnd_std = 0;
   unsigned rnd_steps = 1;
+  // single-body force (psbs[type]): 0 none, 1 gaussian, 2 berendsen,
+  // 3 constant, 4 friction
+  int sb = 0;
+  float sb_coef = 0;  // berendsen itau, friction coef
+  float cf[3] = {0, 0, 0};
+  double target_energy = 0, kinetic_energy = 0;
 };
 
 struct Part {
@@ -624,8 +632,8 @@ void do_pair(Oracle &o, int ci_, int cj_) {
 
 void single_body(Oracle &o, Part *p) {
   const Type &t = o.types[p->type];
-  if (!t.has_random) return;
-  if (o.update_pf && (o.time + (uint64_t)p->id) % t.rnd_steps == 0) {
+  if (t.sb == 0) return;
+  if (t.has_random && o.update_pf && (o.time + (uint64_t)p->id) % t.rnd_steps == 0) {
     // MxRandomVector(mean, std) with Philox draws.
     mxg::u32x4 ctr = {{(uint32_t)p->id, (uint32_t)o.time, mxg::kStreamSingle, 0u}};
     mxg::u32x4 r = mxg::philox4x32_10(ctr, o.seed_lo, o.seed_hi);
@@ -641,9 +649,26 @@ void single_body(Oracle &o, Part *p) {
     p->pf[1] = mag * std::sin(phi) * std::sin(theta);
     p->pf[2] = mag * std::cos(phi);
   }
-  p->f[0] += p->pf[0];
-  p->f[1] += p->pf[1];
-  p->f[2] += p->pf[2];
+  if (t.sb == 1) {  // gaussian_force
+    p->f[0] += p->pf[0];
+    p->f[1] += p->pf[1];
+    p->f[2] += p->pf[2];
+  } else if (t.sb == 2) {  // berendsen_force
+    float scale = t.sb_coef * ((t.target_energy / t.kinetic_energy) - 1.0);
+    p->f[0] += scale * p->v[0];
+    p->f[1] += scale * p->v[1];
+    p->f[2] += scale * p->v[2];
+  } else if (t.sb == 3) {  // constant_force, scaling constant 1
+    p->f[0] += t.cf[0] * 1.f;
+    p->f[1] += t.cf[1] * 1.f;
+    p->f[2] += t.cf[2] * 1.f;
+  } else if (t.sb == 4) {  // friction_force
+    float v2 = p->v[0] * p->v[0] + p->v[1] * p->v[1] + p->v[2] * p->v[2];
+    float scale = -1. * t.sb_coef * v2;
+    p->f[0] += scale * p->v[0] + p->pf[0];
+    p->f[1] += scale * p->v[1] + p->pf[1];
+    p->f[2] += scale * p->v[2] + p->pf[2];
+  }
 }
 
 void do_self(Oracle &o, int cid) {
@@ -866,9 +891,11 @@ void force(Oracle &o) {
     o.dpd_delta_set = true;
   }
   double total = 0;
+  for (auto &t : o.types) t.kinetic_energy = 0;
   for (auto &c : o.cells)
     for (auto &p : c.parts) {
       float e = 0.5 * o.types[p.type].mass * (p.v[0] * p.v[0] + p.v[1] * p.v[1] + p.v[2] * p.v[2]);
+      o.types[p.type].kinetic_energy += e;
       total += e;
     }
   o.ekin = total;
@@ -1066,11 +1093,23 @@ void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_o
       }
     }
   }
-  // single-body persistent forces as stored
+  // single-body forces (persistent parts as stored)
   for (const Cell &c : o.cells)
-    for (const Part &p : c.parts)
-      if (o.types[p.type].has_random)
-        for (int k = 0; k < 3; ++k) F[3 * (size_t)p.id + k] += p.pf[k];
+    for (const Part &p : c.parts) {
+      const Type &t = o.types[p.type];
+      double *Fp = F + 3 * (size_t)p.id;
+      if (t.sb == 1) {
+        for (int k = 0; k < 3; ++k) Fp[k] += p.pf[k];
+      } else if (t.sb == 2) {
+        const double scale = (double)t.sb_coef * ((t.target_energy / t.kinetic_energy) - 1.0);
+        for (int k = 0; k < 3; ++k) Fp[k] += scale * p.v[k];
+      } else if (t.sb == 3) {
+        for (int k = 0; k < 3; ++k) Fp[k] += t.cf[k];
+      } else if (t.sb == 4) {
+        const double v2 = (double)p.v[0] * p.v[0] + (double)p.v[1] * p.v[1] + (double)p.v[2] * p.v[2];
+        for (int k = 0; k < 3; ++k) Fp[k] += -(double)t.sb_coef * v2 * p.v[k] + p.pf[k];
+      }
+    }
   // bonds
   std::vector<Loc> where = locate(o);
   for (const Bond &b : o.bonds) {
@@ -1311,16 +1350,33 @@ int oracle_add_type(void *h, double mass, float radius, int dynamics) {
   return nt - 1;
 }
 
-int oracle_set_random_force(void *h, int type, float mean, float std, float duration) {
+// kind: 1 gaussian p = (mean, std, duration); 2 berendsen p = (tau);
+// 3 constant p = F; 4 friction p = (mean, std, duration, coef).
+int oracle_set_single_body(void *h, int type, int kind, const float *p) {
   Oracle *o = (Oracle *)h;
   Type &t = o->types[type];
-  t.has_random = true;
-  t.rnd_mean = mean;
-  t.rnd_std = std;
-  t.rnd_steps = (unsigned)std::ceil(duration / o->dt_d);
-  if (t.rnd_steps == 0) t.rnd_steps = 1;
+  t.sb = kind;
+  t.has_random = kind == 1 || kind == 4;
+  if (t.has_random) {
+    t.rnd_mean = p[0];
+    t.rnd_std = p[1];
+    t.rnd_steps = (unsigned)std::ceil(p[2] / o->dt_d);
+    if (t.rnd_steps == 0) t.rnd_steps = 1;
+    if (kind == 4) t.sb_coef = p[3];
+  } else if (kind == 2) {
+    t.sb_coef = 1 / p[0];
+  } else if (kind == 3) {
+    for (int k = 0; k < 3; ++k) t.cf[k] = p[k];
+  }
   return 0;
 }
+
+int oracle_set_random_force(void *h, int type, float mean, float std, float duration) {
+  const float p[3] = {mean, std, duration};
+  return oracle_set_single_body(h, type, 1, p);
+}
+
+void oracle_set_target_energy(void *h, int type, double e) { ((Oracle *)h)->types[type].target_energy = e; }
 
 int oracle_add_pot(void *h, int ti, int tj, uint32_t kind, uint32_t flags, const float alpha[4], float a,
                    float b, float r0_plusone, int n, const float *c, float dpd_alpha, float dpd_gamma,

@@ -41,6 +41,8 @@ _SIG = {
                                    C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "oracle_step": (C.c_int, [C.c_void_p, C.c_int]),
     "oracle_set_integrator": (None, [C.c_void_p, C.c_int]),
+    "oracle_set_single_body": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "oracle_set_target_energy": (None, [C.c_void_p, C.c_int, C.c_double]),
     "oracle_force": (C.c_int, [C.c_void_p]),
     "oracle_step_with_forces": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "oracle_get_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_float),
@@ -125,6 +127,13 @@ class Oracle:
 
     def add_type(self, mass, radius, dynamics):
         return lib().oracle_add_type(self._h, float(mass), float(radius), int(dynamics))
+
+    def set_single_body(self, t, kind, params):
+        p = np.ascontiguousarray(params, np.float32)
+        lib().oracle_set_single_body(self._h, int(t), int(kind), p.ctypes.data_as(C.POINTER(C.c_float)))
+
+    def set_target_energy(self, t, e):
+        lib().oracle_set_target_energy(self._h, int(t), float(e))
 
     def add_random_force(self, t, mean, std, duration):
         lib().oracle_set_random_force(self._h, int(t), float(mean), float(std), float(duration))
@@ -243,6 +252,10 @@ def from_workload(w, m, threads=1, seed=None):
         o.set_dpd_noise(1)
     if w.get("integrator"):
         o.set_integrator(w["integrator"])
+    for t, (kind, params) in (w.get("single_body") or {}).items():
+        o.set_single_body(t, kind, params)
+    for t, e in (w.get("target_energy") or {}).items():
+        o.set_target_energy(t, e)
     o.load(w["x"], w["v"], w["type"])
     if w.get("bonds"):
         b = w["bonds"]
