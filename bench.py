@@ -41,8 +41,8 @@ BYTES_PER_BOND = 12
 # FETCH_SIZE and WRITE_SIZE on the default workload (DPD, 1M particles),
 # scaled by the copy-kernel calibration (bench.py --copy-calibration under
 # the same counters); the summary and its commands are kept in
-# profiles/k_brick_dpd1m_counters.md.  None until measured.
-MEASURED_TRAFFIC = {"dpd": None}
+# profiles/k_brick_dpd1m_counters.md (bytes per launch).
+MEASURED_TRAFFIC = {"dpd": 112.2e6}
 
 METRIC = "particle-steps/s (1M LJ/DPD)"
 
