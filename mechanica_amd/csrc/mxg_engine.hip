@@ -2568,8 +2568,11 @@ int Engine::ensure_graph(int par) {
 int Engine::autotune(int steps_per, int *taken) {
   if (!d_ || N_ == 0) return fail(kErrNull, "no particles loaded");
   if (steps_per < 1) return reject(kErrRange, "autotune needs at least one step per candidate");
-  static const int shapes[][3] = {{4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
-  static const int caps[] = {8, 12, 16};
+  // Shapes whose home-particle count fills the 256-thread block at common
+  // densities, and the power-of-two series for dense or small systems.
+  static const int shapes[][3] = {{6, 4, 4}, {5, 5, 4}, {5, 4, 4}, {4, 4, 4}, {4, 4, 3}, {4, 3, 3}, {3, 3, 3},
+                                  {4, 4, 2}, {4, 2, 2}, {3, 3, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
+  static const int caps[] = {8, 12, 16, 24};
   const bool was_timing = timing_;
   int best[4] = {bdim_[0], bdim_[1], bdim_[2], cap_};
   double best_ms = 1e30;
