@@ -1153,38 +1153,42 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
         {
           const LdsNbr nb{hv, kDPD, hstart, hgb, nhalo, A.pid};
           const unsigned dmax = (unsigned)R.dmaxdist;
-          // Shift of the pair for offset o in the reference orientation.
-          auto shift_of = [&](const DevOffset &off, float &sx, float &sy, float &sz) {
-            constexpr int W = 2 * kSpanMax + 1;
-            if (off.side > 0) {
-              sx = hs[(0 * B.hdmax + ha) * W + off.ox + sp0];
-              sy = hs[(1 * B.hdmax + hb) * W + off.oy + sp1];
-              sz = hs[(2 * B.hdmax + hcz) * W + off.oz + sp2];
-            } else {
-              sx = hs[(0 * B.hdmax + ha + off.ox) * W - off.ox + sp0];
-              sy = hs[(1 * B.hdmax + hb + off.oy) * W - off.oy + sp1];
-              sz = hs[(2 * B.hdmax + hcz + off.oz) * W - off.oz + sp2];
-            }
+          // Shift of the pair for offset (ox, oy, oz) in the reference
+          // orientation, from this particle's cell's rows of the shift table;
+          // the reverse shift (side -1: the neighbour is the reference's i
+          // cell) is the exact negation (checked when the tables are built).
+          constexpr int W = 2 * kSpanMax + 1;
+          const float *hsx = hs + (0 * B.hdmax + ha) * W + sp0, *hsy = hs + (1 * B.hdmax + hb) * W + sp1,
+                      *hsz = hs + (2 * B.hdmax + hcz) * W + sp2;
+          auto shift_of = [&](int ox, int oy, int oz, int side, float &sx, float &sy, float &sz) {
+            const float x = hsx[ox], y = hsy[oy], z = hsz[oz];
+            sx = side > 0 ? x : -x;
+            sy = side > 0 ? y : -y;
+            sz = side > 0 ? z : -z;
           };
           auto process = [&](int n) {
             for (int e = 0; e < n; ++e) {
+              // Entry: LDS index << 12 | (side + 1) << 9 | (ox + 2) << 6 |
+              // (oy + 2) << 3 | (oz + 2).
               const unsigned ent = nlist[e * kBT + tid];
-              const int q = (int)(ent >> 8);
-              const DevOffset off = hoff[ent & 255u];
+              const int q = (int)(ent >> 12);
+              const int side = (int)((ent >> 9) & 3u) - 1;
               const float4 xq = hx[q];
               // Self pairs: x_i - x_j; others (a - shift) - b in the
               // reference orientation (x - 0 is exact, so one form serves).
               float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-              if (off.side != 0) shift_of(off, sx, sy, sz);
-              const bool fwd = off.side >= 0;
+              if (side != 0)
+                shift_of((int)((ent >> 6) & 7u) - 2, (int)((ent >> 3) & 7u) - 2, (int)(ent & 7u) - 2, side, sx, sy,
+                         sz);
+              const bool fwd = side >= 0;
               const float ax = fwd ? xs.x : xq.x, ay = fwd ? xs.y : xq.y, az = fwd ? xs.z : xq.z;
               const float bx = fwd ? xq.x : xs.x, by = fwd ? xq.y : xs.y, bz = fwd ? xq.z : xs.z;
               const float dx = (ax - sx) - bx, dy = (ay - sy) - by, dz = (az - sz) - bz;
               const float r2 = dx * dx + dy * dy + dz * dz;
               const float r = sqrtf(r2);
               const int tj = __float_as_int(xq.w) & 0xffff;
-              if (pair_force<MODE>(C, tb, nb, ti, tj, off.side >= 0, mypid, q, off.side == 0, dx, dy, dz, r2, r,
-                                   vs, fx, fy, fz, acc, t1, tk))
+              if (pair_force<MODE>(C, tb, nb, ti, tj, side >= 0, mypid, q, side == 0, dx, dy, dz, r2, r, vs, fx, fy,
+                                   fz, acc, t1, tk))
                 nd += w_cubic(r, C.cutoff, C.nd_norm);
             }
           };
@@ -1217,14 +1221,16 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
                     const float dx = xs.x - xq[u].x, dy = xs.y - xq[u].y, dz = xs.z - xq[u].z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     if (q + u < qe && q + u != p && r2 <= C.cut2_self) {
-                      nlist[nn * kBT + tid] = ((unsigned)(q + u) << 8) | (unsigned)o;
+                      nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | (1u << 9) | (2u << 6) | (2u << 3) | 2u;
                       ++nn;
                     }
                   }
                 }
               } else {
                 float sx, sy, sz;
-                shift_of(off, sx, sy, sz);
+                shift_of(off.ox, off.oy, off.oz, off.side, sx, sy, sz);
+                const unsigned geo = ((unsigned)(off.side + 1) << 9) | ((unsigned)(off.ox + 2) << 6) |
+                                     ((unsigned)(off.oy + 2) << 3) | (unsigned)(off.oz + 2);
                 const float nshift = sqrtf(sx * sx + sy * sy + sz * sz);
                 const unsigned dnshift = (unsigned)(int)(R.dscale * nshift);
                 const unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, off.nx, off.ny, off.nz);
@@ -1250,7 +1256,7 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
                           sort_key(R.bias, R.dscale, xq[u].x, xq[u].y, xq[u].z, off.nx, off.ny, off.nz);
                       const bool pass = off.side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax);
                       if (pass) {
-                        nlist[nn * kBT + tid] = ((unsigned)(q + u) << 8) | (unsigned)o;
+                        nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
                         ++nn;
                       }
                     }
@@ -1820,7 +1826,14 @@ int Engine::build_geometry() {
             s += R.dim[k];
           shifts.push_back(s);
         }
-      (void)w;
+      // The force kernel uses shift(l + d, -d) = -shift(l, d); check it.
+      const int base = D.shift_base[k], sp = R.span[k];
+      for (int l = 0; l < R.cdim[k]; ++l)
+        for (int d = -sp; d <= sp; ++d) {
+          const int l2 = ((l + d) % R.cdim[k] + R.cdim[k]) % R.cdim[k];
+          if (shifts[base + l2 * w + (-d + sp)] != -shifts[base + l * w + (d + sp)])
+            return fail(kErrSpace, "cell shift table is not antisymmetric");
+        }
     }
     // Neighbour offsets within the span whose cells can hold a pair within
     // the cutoff (space.cpp:837-843), self first.
