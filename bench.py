@@ -131,6 +131,9 @@ def bench_single(args, threads):
         cfg = ("lattice N=%d (64^3) Morse(d=1,a=6,r0=1) cutoff=2 + 774144 harmonic bonds k=100 r0=1 "
                "dissociation 5, overdamped dt=0.005 cells=%dx%dx%d")
         per, bonds = BYTES_OVERDAMPED, len(w["bonds"]["i"])
+    if args.integrator == "rk4":
+        w["integrator"] = 1
+        cfg += " integrator=RK4 (four force stages per step)"
     u = W.build_universe(m, w, seed=1234)
     N = len(w["x"])
     log("workload %s N=%d built" % (w["name"], N))
@@ -307,6 +310,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="dpd", choices=["dpd", "lj", "lattice", "ensemble", "adjoint"])
     ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--integrator", default="euler", choices=["euler", "rk4"],
+                    help="forward Euler (the reference default) or Runge-Kutta 4")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--brick", default=None, help="bx,by,bz,cap: fixed force-kernel layout (implies --no-autotune)")
     ap.add_argument("--no-autotune", action="store_true",
