@@ -74,6 +74,16 @@ mxg_potential *mxg_potential_create_morse(double d, double a, double r0, double 
 mxg_potential *mxg_potential_create_harmonic(double min, double max, double K, double r0, double tol);
 
 /* DPDPotential_New (DissapativeParticleDynamics.cpp:134-148). */
+/* potential_create_harmonic_angle (MxPotential.cpp:555-597): K (theta -
+ * theta0)^2 for angles in [min, max] (clamped to [0, pi]), tabulated in
+ * cos(theta) for angle_eval. */
+mxg_potential *mxg_potential_create_harmonic_angle(double min, double max, double K, double theta0,
+                                                   double tol);
+
+/* potential_create_harmonic_dihedral (MxPotential.cpp:497-517):
+ * K (1 + cos(n phi - delta)), tabulated in cos(phi) for dihedral_eval. */
+mxg_potential *mxg_potential_create_harmonic_dihedral(double K, int n, double delta, double tol);
+
 mxg_potential *mxg_potential_create_dpd(float alpha, float gamma, float sigma, float cutoff, int shifted);
 
 /* Py_DECREF of an MxPotential. */
@@ -94,6 +104,10 @@ int mxg_potential_coefficients(const mxg_potential *p, float *c, int capacity);
 int mxg_potential_eval(const mxg_potential *p, float r2, float *e, float *f);
 
 /* potential_eval_ex (potential_eval.hpp:106-161); returns 1 in range, 0 not. */
+/* potential_eval_r (potential_eval.hpp:180-208): the table at r itself (no
+ * square root), f = dE/dr; r must lie in the potential's domain. */
+int mxg_potential_eval_r(const mxg_potential *p, float r, float *e, float *f);
+
 int mxg_potential_eval_ex(const mxg_potential *p, float ri, float rj, float r2, float *e, float *f);
 
 /* ---- Engine ------------------------------------------------------------ */
@@ -190,6 +204,25 @@ int mxg_engine_load(mxg_engine *e, int N, const double *x, const double *v, cons
 int mxg_engine_set_bonds(mxg_engine *e, int M, const int32_t *i, const int32_t *j,
                          const mxg_potential *const *pots, const double *dissociation_energy);
 
+/* engine_angle_add (engine.h:574, angle.h:50-60): M angles (i, j, k) with
+ * vertex j, each with a potential in cos(theta) such as
+ * mxg_potential_create_harmonic_angle; replaces the previous set.  Evaluated
+ * as angle_eval (angle.cpp:92-354). */
+int mxg_engine_set_angles(mxg_engine *e, int M, const int32_t *i, const int32_t *j, const int32_t *k,
+                          const mxg_potential *const *pots);
+
+/* engine_dihedral_add (engine.h:576, dihedral.h:38-46): M dihedrals
+ * (i, j, k, l) about the axis j-k, each with a potential in cos(phi) such as
+ * mxg_potential_create_harmonic_dihedral; replaces the previous set.
+ * Evaluated as dihedral_eval (dihedral.cpp:90-330). */
+int mxg_engine_set_dihedrals(mxg_engine *e, int M, const int32_t *i, const int32_t *j, const int32_t *k,
+                             const int32_t *l, const mxg_potential *const *pots);
+
+/* engine_exclusion_add (engine.h:578, exclusion.h): M excluded pairs whose
+ * non-bonded table interaction is subtracted again (exclusion_eval,
+ * exclusion.cpp:83-260); replaces the previous set. */
+int mxg_engine_set_exclusions(mxg_engine *e, int M, const int32_t *i, const int32_t *j);
+
 /* engine_step (engine.h:628, engine.cpp:1198-1228), nsteps times. */
 int mxg_engine_step(mxg_engine *e, int nsteps);
 
@@ -217,6 +250,11 @@ int mxg_engine_neighbour_counts(mxg_engine *e, int *counts);
  * the start of the step (engine_kinetic_energy, engine.cpp:1698-1719),
  * out[3..11] virial tensor (row-major), all accumulated in FP64. */
 int mxg_engine_observables(mxg_engine *e, int replica, double out[12]);
+
+/* Energies of the last force evaluation's angles, dihedrals and exclusions
+ * (space epot_angle, epot_dihedral, epot_exclusion; engine_bonded.cpp:
+ * 1533-1538), replica r or the sum over replicas (r < 0), in FP64. */
+int mxg_engine_term_energies(mxg_engine *e, int replica, double out[3]);
 
 /* BOND_ACTIVE of each bond (bond.h:40-43). */
 int mxg_engine_bond_active(mxg_engine *e, uint8_t *active);

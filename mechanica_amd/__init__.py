@@ -90,6 +90,20 @@ class Potential:
         return Potential(_capi.lib().mxg_potential_create_harmonic(min, max, k, r0, tol), "Harmonic")
 
     @staticmethod
+    def harmonic_angle(k, theta0, min=0.0, max=np.pi, tol=1e-4):
+        """potential_create_harmonic_angle: k (theta - theta0)^2 on angles in
+        [min, max], tabulated in cos(theta) (MxPotential.cpp:555-597)."""
+        return Potential(_capi.lib().mxg_potential_create_harmonic_angle(min, max, k, theta0, tol),
+                         "Harmonic Angle")
+
+    @staticmethod
+    def harmonic_dihedral(k, n, delta, tol=1e-4):
+        """potential_create_harmonic_dihedral: k (1 + cos(n phi - delta)),
+        tabulated in cos(phi) (MxPotential.cpp:497-517)."""
+        return Potential(_capi.lib().mxg_potential_create_harmonic_dihedral(k, int(n), delta, tol),
+                         "Harmonic Dihedral")
+
+    @staticmethod
     def dpd(alpha=1.0, gamma=1.0, sigma=1.0, cutoff=1.0, shifted=False):
         return Potential(_capi.lib().mxg_potential_create_dpd(alpha, gamma, sigma, cutoff, int(bool(shifted))),
                          "Dissapative Particle Dynamics")
@@ -140,6 +154,14 @@ class Potential:
         """potential_eval at r^2: (energy, force / r)."""
         e, f = C.c_float(), C.c_float()
         rc = _capi.lib().mxg_potential_eval(self._h, float(r2), C.byref(e), C.byref(f))
+        if rc:
+            raise ValueError("not a table potential")
+        return e.value, f.value
+
+    def eval_r(self, r):
+        """potential_eval_r at r itself: (energy, dE/dr)."""
+        e, f = C.c_float(), C.c_float()
+        rc = _capi.lib().mxg_potential_eval_r(self._h, float(r), C.byref(e), C.byref(f))
         if rc:
             raise ValueError("not a table potential")
         return e.value, f.value
@@ -300,6 +322,44 @@ class Universe:
         self._check(_capi.lib().mxg_engine_set_bonds(self._h, M, _ip(i), _ip(j), arr,
                                                      _dp(d) if d is not None else None))
         self.M = M
+
+    def _term_pots(self, pots, M):
+        if isinstance(pots, Potential):
+            pots = [pots] * M
+        pots = list(pots)
+        if len(pots) != M:
+            raise ValueError("one potential per term")
+        self._pots.extend(set(pots))
+        return (C.c_void_p * max(M, 1))(*[p._h.value for p in pots])
+
+    def set_angles(self, i, j, k, pots):
+        """engine_angle_add for each (i[a], j[a], k[a]), j the vertex;
+        replaces the previous angles."""
+        ids = [np.ascontiguousarray(a, dtype=np.int32) for a in (i, j, k)]
+        M = len(ids[0])
+        arr = self._term_pots(pots, M)
+        self._check(_capi.lib().mxg_engine_set_angles(self._h, M, *[_ip(a) for a in ids], arr))
+
+    def set_dihedrals(self, i, j, k, l, pots):
+        """engine_dihedral_add for each (i, j, k, l) about the axis j-k;
+        replaces the previous dihedrals."""
+        ids = [np.ascontiguousarray(a, dtype=np.int32) for a in (i, j, k, l)]
+        M = len(ids[0])
+        arr = self._term_pots(pots, M)
+        self._check(_capi.lib().mxg_engine_set_dihedrals(self._h, M, *[_ip(a) for a in ids], arr))
+
+    def set_exclusions(self, i, j):
+        """engine_exclusion_add for each pair; replaces the previous set."""
+        i = np.ascontiguousarray(i, dtype=np.int32)
+        j = np.ascontiguousarray(j, dtype=np.int32)
+        self._check(_capi.lib().mxg_engine_set_exclusions(self._h, len(i), _ip(i), _ip(j)))
+
+    def term_energies(self, replica=-1):
+        """Energies of the angles, dihedrals and exclusions of the last force
+        evaluation."""
+        out = np.zeros(3, np.float64)
+        self._check(_capi.lib().mxg_engine_term_energies(self._h, int(replica), _dp(out)))
+        return {"epot_angle": out[0], "epot_dihedral": out[1], "epot_exclusion": out[2]}
 
     # stepping -----------------------------------------------------------
     def step(self, nsteps=1):

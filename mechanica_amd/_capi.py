@@ -37,6 +37,9 @@ SIGNATURES = {
     "mxg_potential_create_lj126": (C.c_void_p, [C.c_double] * 5),
     "mxg_potential_create_morse": (C.c_void_p, [C.c_double] * 6),
     "mxg_potential_create_harmonic": (C.c_void_p, [C.c_double] * 5),
+    "mxg_potential_create_harmonic_angle": (C.c_void_p, [C.c_double] * 5),
+    "mxg_potential_create_harmonic_dihedral": (C.c_void_p, [C.c_double, C.c_int, C.c_double, C.c_double]),
+    "mxg_potential_eval_r": (C.c_int, [C.c_void_p, C.c_float, c_float_p, c_float_p]),
     "mxg_potential_create_dpd": (C.c_void_p, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]),
     "mxg_potential_free": (None, [C.c_void_p]),
     "mxg_potential_error": (C.c_int, []),
@@ -68,6 +71,11 @@ SIGNATURES = {
     "mxg_engine_load": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_int_p, c_int_p, c_u16_p]),
     "mxg_engine_set_bonds": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, C.POINTER(C.c_void_p),
                                        c_double_p]),
+    "mxg_engine_set_angles": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, C.POINTER(C.c_void_p)]),
+    "mxg_engine_set_dihedrals": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, c_int_p,
+                                           C.POINTER(C.c_void_p)]),
+    "mxg_engine_set_exclusions": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p]),
+    "mxg_engine_term_energies": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "mxg_engine_step": (C.c_int, [C.c_void_p, C.c_int]),
     "mxg_engine_sync": (C.c_int, [C.c_void_p]),
     "mxg_engine_unload": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),

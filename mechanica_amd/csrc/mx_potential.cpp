@@ -131,9 +131,11 @@ bool place_edges(const float al[3], float a, float b, int n, float *x) {
 }
 
 // Slopes at the edges from a tridiagonal least-squares system with the end
-// slopes fixed to fpa and fpb.
+// slopes fixed to fpa and fpb (potential_getfp_fixend), or left free when
+// free_ends is set (potential_getfp, MxPotential.cpp:1949-2029: the first and
+// last rows keep their one-sided least-squares terms).
 void edge_slopes(const Fn &f, double fpa, double fpb, int n, const float *x,
-                 double *slope) {
+                 double *slope, bool free_ends = false) {
   const LobattoTable &L = lobatto();
   std::vector<double> fx(n + 1), hx(n), sl(n), sr(n);
   std::vector<double> lower(n + 1), diag(n + 1), upper(n + 1), rhs(n + 1);
@@ -156,9 +158,15 @@ void edge_slopes(const Fn &f, double fpa, double fpb, int n, const float *x,
     sr[i] /= kPotN - 2;
   }
 
-  diag[0] = 1.0;
-  upper[0] = 0.0;
-  rhs[0] = fpa;
+  if (free_ends) {
+    diag[0] = kWl1Wl1 * hx[0];
+    upper[0] = kWl1Wr1 * hx[0];
+    rhs[0] = 2 * (sl[0] - fx[0] * kWl0Wl1 - fx[1] * kWl1Wr0);
+  } else {
+    diag[0] = 1.0;
+    upper[0] = 0.0;
+    rhs[0] = fpa;
+  }
   for (int i = 1; i < n; ++i) {
     lower[i] = kWl1Wr1 * hx[i - 1];
     diag[i] = kWr1Wr1 * hx[i - 1] + kWl1Wl1 * hx[i];
@@ -166,9 +174,15 @@ void edge_slopes(const Fn &f, double fpa, double fpb, int n, const float *x,
     rhs[i] = 2 * (sr[i - 1] - fx[i - 1] * kWl0Wr1 - fx[i] * kWr0Wr1) +
              2 * (sl[i] - fx[i] * kWl0Wl1 - fx[i + 1] * kWl1Wr0);
   }
-  lower[n] = 0.0;
-  diag[n] = 1.0;
-  rhs[n] = fpb;
+  if (free_ends) {
+    lower[n] = kWl1Wr1 * hx[n - 1];
+    diag[n] = kWr1Wr1 * hx[n - 1];
+    rhs[n] = 2 * (sr[n - 1] - fx[n - 1] * kWl0Wr1 - fx[n] * kWr0Wr1);
+  } else {
+    lower[n] = 0.0;
+    diag[n] = 1.0;
+    rhs[n] = fpb;
+  }
 
   // Thomas algorithm.
   for (int i = 1; i <= n; ++i) {
@@ -191,7 +205,10 @@ void fit_intervals(const Fn &f, const Fn &fp, const float *x, int n, float *out,
   double fs[kPotN], phi[7], cee[6];
 
   for (int k = 0; k <= n; ++k) fedge[k] = f(x[k]);
-  edge_slopes(f, fp(x[0]), fp(x[n]), n, x, slope.data());
+  if (fp)
+    edge_slopes(f, fp(x[0]), fp(x[n]), n, x, slope.data());
+  else
+    edge_slopes(f, 0.0, 0.0, n, x, slope.data(), true);
 
   *err = 0.0;
   for (int i = 0; i < n; ++i) {
@@ -424,6 +441,69 @@ int make_harmonic(Potential &p, double a, double b, double K, double r0,
   return kPotErrOk;
 }
 
+// potential_create_harmonic_angle (MxPotential.cpp:555-597): K (acos(c) -
+// theta0)^2 tabulated in c = cos(theta) on [cos(b), cos(a)], the angles
+// clamped to [0, pi]; no derivative is supplied, so the slopes are the
+// free-end least-squares ones.
+int make_harmonic_angle(Potential &p, double a, double b, double K, double theta0, double tol) {
+  Potential q;
+  q.flags = kPotFlagAngle | kPotFlagHarmonic;
+  q.name = "Harmonic Angle";
+  if (a < 0.0) a = 0.0;
+  if (b > M_PI) b = M_PI;
+  const double left = std::cos(b), right = std::cos(a);
+  Fn f = [K, theta0](double r) {
+    r = std::fmin(1.0, std::fmax(-1.0, r));
+    const double theta = std::acos(r);
+    return K * (theta - theta0) * (theta - theta0);
+  };
+  Fn f6p = [](double) { return 0.0; };
+  int rc = build_table(q, f, Fn(), f6p, (float)left, (float)right, (float)tol);
+  if (rc < 0) return rc;
+  p = std::move(q);
+  return kPotErrOk;
+}
+
+// potential_create_harmonic_dihedral (MxPotential.cpp:428-517):
+// K (1 + cos(n acos(c) - delta)) tabulated in c = cos(phi).  With
+// theta = acos(c), cos(n theta) = T_n(c) and sin(n theta) = U_{n-1}(c)
+// sin(theta), the Chebyshev polynomials of the first and second kind; the
+// domain is [-1, 1], pulled in by a relative sqrt(eps) when delta is not a
+// multiple of pi (the sin(theta) factor has unbounded slope at the ends).
+int make_harmonic_dihedral(Potential &p, double K, int n, double delta, double tol) {
+  if (n < 1) return kPotErrBounds;
+  Potential q;
+  q.flags = kPotFlagR | kPotFlagHarmonic | kPotFlagDihedral;
+  q.name = "Harmonic Dihedral";
+  double a = -1.0, b = 1.0;
+  if (std::fmod(delta, M_PI) != 0) {
+    const double e = std::sqrt((double)std::numeric_limits<float>::epsilon());
+    a = -1.0 / (1.0 + e);
+    b = 1.0 / (1.0 + e);
+  }
+  Fn f = [K, n, delta](double r) {
+    // T_k and U_k by their common three-term recurrence.
+    double t0 = 1.0, t1 = r, u0 = 1.0, u1 = 2 * r;
+    for (int k = 2; k <= n; ++k) {
+      const double t2 = 2 * r * t1 - t0, u2 = 2 * r * u1 - u0;
+      t0 = t1;
+      t1 = t2;
+      u0 = u1;
+      u1 = u2;
+    }
+    const double Tn = t1, Un1 = u0;  // T_n and U_{n-1}
+    if (delta == 0.0) return K * (1.0 + Tn);
+    if (delta == M_PI) return K * (1.0 - Tn);
+    if (std::fabs(r) < 1.0) return K * (1.0 + Tn * std::cos(delta) + Un1 * std::sqrt(1.0 - r * r) * std::sin(delta));
+    return K * (1.0 + Tn * std::cos(delta));
+  };
+  Fn f6p = [](double) { return 0.0; };
+  int rc = build_table(q, f, Fn(), f6p, (float)a, (float)b, (float)tol);
+  if (rc < 0) return rc;
+  p = std::move(q);
+  return kPotErrOk;
+}
+
 int make_dpd(Potential &p, float alpha, float gamma, float sigma, float cutoff,
              bool shifted) {
   Potential q;
@@ -452,6 +532,21 @@ void eval_plain(const Potential &p, float r2, float *e, float *f) {
   }
   *e = ee;
   *f = eff * c[1] / r;
+}
+
+void eval_r(const Potential &p, float r, float *e, float *f) {
+  int ind = (int)std::fmax(0.0f, p.alpha[0] + r * (p.alpha[1] + r * p.alpha[2]));
+  ind = std::min(std::max(ind, 0), p.n);
+  const float *c = &p.c[(size_t)ind * kPotChunk];
+  float x = (r - c[0]) * c[1];
+  float ee = c[2] * x + c[3];
+  float eff = c[2];
+  for (int k = 4; k < kPotChunk; ++k) {
+    eff = eff * x + ee;
+    ee = ee * x + c[k];
+  }
+  *e = ee;
+  *f = eff * c[1];
 }
 
 bool eval_ex(const Potential &p, float ri, float rj, float r2, float *e,

@@ -28,7 +28,10 @@ constexpr int kPotIvalsMax = 3048;      // hard cap on the interval count
 enum : uint32_t {
   kPotFlagLJ126 = 1u << 0,
   kPotFlagR2 = 1u << 4,
+  kPotFlagR = 1u << 5,
+  kPotFlagAngle = 1u << 6,
   kPotFlagHarmonic = 1u << 7,
+  kPotFlagDihedral = 1u << 8,
   kPotFlagScaled = 1u << 11,
   kPotFlagShifted = 1u << 12,
   kPotFlagBound = 1u << 13,
@@ -76,11 +79,22 @@ int make_morse(Potential &p, double d, double alpha, double r0, double min,
                double max, double tol);
 int make_harmonic(Potential &p, double a, double b, double K, double r0,
                   double tol);
+// potential_create_harmonic_angle (MxPotential.cpp:555-597): a table in
+// cos(theta) on [cos(b), cos(a)] of K (theta - theta0)^2.
+int make_harmonic_angle(Potential &p, double a, double b, double K,
+                        double theta0, double tol);
+// potential_create_harmonic_dihedral (MxPotential.cpp:428-517): a table in
+// cos(phi) of K (1 + cos(n phi - delta)).
+int make_harmonic_dihedral(Potential &p, double K, int n, double delta,
+                           double tol);
 int make_dpd(Potential &p, float alpha, float gamma, float sigma, float cutoff,
              bool shifted);
 
 // Host evaluation with the kernels' arithmetic (potential_eval.hpp:67-161).
 void eval_plain(const Potential &p, float r2, float *e, float *f);
+// potential_eval_r (potential_eval.hpp:180-208): the table at r itself;
+// f = dE/dr.
+void eval_r(const Potential &p, float r, float *e, float *f);
 bool eval_ex(const Potential &p, float ri, float rj, float r2, float *e,
              float *f);
 

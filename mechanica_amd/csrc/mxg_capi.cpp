@@ -5,6 +5,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/mxg.h"
 #include "mx_potential.h"
@@ -56,6 +57,18 @@ mxg_potential *mxg_potential_create_harmonic(double min, double max, double K, d
   return finish(mxg::make_harmonic(h->p, min, max, K, r0, tol), h);
 }
 
+mxg_potential *mxg_potential_create_harmonic_angle(double min, double max, double K, double theta0, double tol) {
+  auto *h = new (std::nothrow) mxg_potential();
+  if (!h) return finish(mxg::kPotErrMalloc, nullptr);
+  return finish(mxg::make_harmonic_angle(h->p, min, max, K, theta0, tol), h);
+}
+
+mxg_potential *mxg_potential_create_harmonic_dihedral(double K, int n, double delta, double tol) {
+  auto *h = new (std::nothrow) mxg_potential();
+  if (!h) return finish(mxg::kPotErrMalloc, nullptr);
+  return finish(mxg::make_harmonic_dihedral(h->p, K, n, delta, tol), h);
+}
+
 mxg_potential *mxg_potential_create_dpd(float alpha, float gamma, float sigma, float cutoff, int shifted) {
   auto *h = new (std::nothrow) mxg_potential();
   if (!h) return finish(mxg::kPotErrMalloc, nullptr);
@@ -95,6 +108,13 @@ int mxg_potential_eval(const mxg_potential *h, float r2, float *e, float *f) {
   if (!h || !e || !f) return MXG_ERR_NULL;
   if (h->p.kind != mxg::kPotKindTable || h->p.c.empty()) return MXG_ERR_POTENTIAL;
   mxg::eval_plain(h->p, r2, e, f);
+  return MXG_ERR_OK;
+}
+
+int mxg_potential_eval_r(const mxg_potential *h, float r, float *e, float *f) {
+  if (!h || !e || !f) return MXG_ERR_NULL;
+  if (h->p.kind != mxg::kPotKindTable || h->p.c.empty()) return MXG_ERR_POTENTIAL;
+  mxg::eval_r(h->p, r, e, f);
   return MXG_ERR_OK;
 }
 
@@ -250,6 +270,35 @@ int mxg_engine_set_bonds(mxg_engine *e, int M, const int32_t *i, const int32_t *
   std::vector<const mxg::Potential *> pp(M);
   for (int b = 0; b < M; ++b) pp[b] = pots[b] ? &pots[b]->p : nullptr;
   return e->e.set_bonds(M, i, j, pp.data(), dissociation_energy);
+}
+
+int mxg_engine_set_angles(mxg_engine *e, int M, const int32_t *i, const int32_t *j, const int32_t *k,
+                          const mxg_potential *const *pots) {
+  MXG_CHECK(e);
+  if (M > 0 && (!i || !j || !k || !pots)) return MXG_ERR_NULL;
+  std::vector<const mxg::Potential *> pp(M > 0 ? M : 0);
+  for (int a = 0; a < M; ++a) pp[a] = pots[a] ? &pots[a]->p : nullptr;
+  return e->e.set_angles(M, i, j, k, pp.data());
+}
+
+int mxg_engine_set_dihedrals(mxg_engine *e, int M, const int32_t *i, const int32_t *j, const int32_t *k,
+                             const int32_t *l, const mxg_potential *const *pots) {
+  MXG_CHECK(e);
+  if (M > 0 && (!i || !j || !k || !l || !pots)) return MXG_ERR_NULL;
+  std::vector<const mxg::Potential *> pp(M > 0 ? M : 0);
+  for (int a = 0; a < M; ++a) pp[a] = pots[a] ? &pots[a]->p : nullptr;
+  return e->e.set_dihedrals(M, i, j, k, l, pp.data());
+}
+
+int mxg_engine_set_exclusions(mxg_engine *e, int M, const int32_t *i, const int32_t *j) {
+  MXG_CHECK(e);
+  if (M > 0 && (!i || !j)) return MXG_ERR_NULL;
+  return e->e.set_exclusions(M, i, j);
+}
+
+int mxg_engine_term_energies(mxg_engine *e, int replica, double out[3]) {
+  if (!e || !out) return MXG_ERR_NULL;
+  return e->e.term_energies(replica, out);
 }
 
 int mxg_engine_step(mxg_engine *e, int nsteps) {

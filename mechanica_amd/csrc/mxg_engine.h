@@ -77,6 +77,13 @@ class Engine {
            const uint16_t *flags);
   int set_bonds(int M, const int *bi, const int *bj, const Potential *const *pots,
                 const double *dissociation);
+  // Angles (engine_angle_add, angle.h:50-60), dihedrals (engine_dihedral_add,
+  // dihedral.h:38-46) and exclusions (engine_exclusion_add, exclusion.h):
+  // each call replaces the engine's whole set of that kind.
+  int set_angles(int M, const int *i, const int *j, const int *k, const Potential *const *pots);
+  int set_dihedrals(int M, const int *i, const int *j, const int *k, const int *l,
+                    const Potential *const *pots);
+  int set_exclusions(int M, const int *i, const int *j);
   int step(int nsteps);
   int sync();
 
@@ -85,6 +92,10 @@ class Engine {
   int get_global(double *x, double *v);
   int neighbour_counts(int *counts);
   int observables(int replica, double *out12);
+  // Energies of the angles, dihedrals and exclusions of the last force
+  // evaluation (space epot_angle, epot_dihedral, epot_exclusion); replica -1
+  // sums the replicas.
+  int term_energies(int replica, double *out3);
   int bond_active(uint8_t *active);
   int geometry(int replica, int cdim[3], double h[3]);
 
@@ -140,6 +151,7 @@ class Engine {
   int launch_substep(int rk_stage, int tick);
   int ensure_rk_buffers();
   int ensure_graph(int parity);
+  int upload_terms();
   void fill_step_args(void *out, int parity) const;
   int refresh_args();
   int check_device(int hip_status, const char *what);
@@ -165,6 +177,16 @@ class Engine {
   std::vector<Potential> pots_;
   std::vector<int> potmat_;  // ntypes * ntypes
   std::vector<Potential> bpots_;
+  // Angles, dihedrals and exclusions: particle ids (4 per term, -1 unused)
+  // and the index of the term's potential in apots_ or dpots_.
+  struct HostTerm {
+    int id[4];
+    int pot;
+  };
+  std::vector<HostTerm> angles_, dihedrals_, exclusions_;
+  std::vector<Potential> apots_, dpots_;
+  std::vector<int> prep_;  // replica of each particle id
+  int NT_ = 0;             // terms on the device
 
   int N_ = 0, M_ = 0, ncells_ = 0, ntiles_ = 0;
   int bdim_[3] = {4, 4, 4};

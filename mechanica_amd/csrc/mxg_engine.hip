@@ -106,6 +106,21 @@ struct DevType {
 
 }  // namespace
 
+// One angle, dihedral or exclusion.  Each role's force goes to its own
+// entry of the per-particle term list (pos), which the particle sums in its
+// force gather, so every term is evaluated once and the sums stay in a fixed
+// order.
+enum : int { kTermExclusion = 0, kTermAngle = 1, kTermDihedral = 2 };
+
+struct DevTerm {
+  int id[4];   // particle ids: exclusion (i, j), angle (i, j, k), dihedral (i, j, k, l)
+  int pos[4];  // list entry of each role
+  int kind;    // kTerm*
+  int pot;     // angle or dihedral potential
+  int tid;     // index of the term within its kind (random plane of an angle)
+  int pad;
+};
+
 struct DeviceState {
   hipStream_t stream = nullptr;
   // particle state, ping-pong
@@ -147,6 +162,13 @@ struct DeviceState {
   int *boff = nullptr, *bent = nullptr, *bi = nullptr, *bj = nullptr, *bpot = nullptr;
   double *bdis = nullptr;
   unsigned char *bact[2] = {nullptr, nullptr};
+  DevTerm *terms = nullptr;   // grouped by replica, then by kind
+  int *trep = nullptr;        // first term of each replica, nrep + 1
+  int *toff = nullptr;        // per-particle term-list offsets, N + 1
+  float4 *tf = nullptr;       // term-list entries: force on the particle
+  float *te = nullptr;        // energy of each term
+  DevPot *tpots = nullptr;    // angle then dihedral potentials
+  double *obs_terms = nullptr;  // per replica: angle, dihedral, exclusion energy
   double *partials = nullptr;
   double *type_ke = nullptr;       // kinetic energy per type (Berendsen)
   double *type_ke_part = nullptr;  // per-block partial sums of it
@@ -204,6 +226,8 @@ struct StepArgs {
   const unsigned char G *__restrict__ bact_in;
   unsigned char G *__restrict__ bact_out;
   const int G *__restrict__ slot_of;
+  const int G *__restrict__ toff;   // term-list offsets (null without terms)
+  const float4 G *__restrict__ tf;  // term-list entries
   float dt;
   float cut2_pair, cut2_self;
   float dpd_delta;
@@ -672,6 +696,7 @@ __device__ __forceinline__ void single_body(const StepArgs &A, int s, int mypid,
 template <int MODE>
 __device__ __forceinline__ void bond_forces(const StepArgs &A, const DevReplica &R, int s, int mypid, float &fx,
                                             float &fy, float &fz, double *acc) {
+  if (!A.boff) return;
   const int b0 = A.boff[mypid], b1 = A.boff[mypid + 1];
   for (int k = b0; k < b1; ++k) {
     const int ent = A.bent[k];
@@ -727,6 +752,259 @@ __device__ __forceinline__ void bond_forces(const StepArgs &A, const DevReplica 
       fz += wz;
     }
   }
+}
+
+// Forces of the angles, dihedrals and exclusions on one particle: the
+// entries k_terms wrote into its term list.
+__device__ __forceinline__ void term_forces(const StepArgs &A, int mypid, float &fx, float &fy, float &fz) {
+  if (!A.toff) return;
+  const int t0 = A.toff[mypid], t1 = A.toff[mypid + 1];
+  for (int k = t0; k < t1; ++k) {
+    const float4 t = A.tf[k];
+    fx += t.x;
+    fy += t.y;
+    fz += t.z;
+  }
+}
+
+// potential_eval_r (potential_eval.hpp:180-208): the table at r itself, no
+// square root; f is dE/dr.  r has been clamped to [a, b] by the caller.
+__device__ __forceinline__ void table_eval_r(const DevPot &P, const float *__restrict__ coef, float r, float &e,
+                                             float &f) {
+  int ind = (int)fmaxf(0.0f, P.alpha0 + r * (P.alpha1 + r * P.alpha2));
+  ind = min(max(ind, 0), P.n);
+  const float4 *c4 = reinterpret_cast<const float4 *>(coef + P.coff + (size_t)ind * kPotChunk);
+  float4 c0 = c4[0], c1 = c4[1];
+  float x = (r - c0.x) * c0.y;
+  float ee = c0.z * x + c0.w;
+  float eff = c0.z;
+  eff = eff * x + ee;
+  ee = ee * x + c1.x;
+  eff = eff * x + ee;
+  ee = ee * x + c1.y;
+  eff = eff * x + ee;
+  ee = ee * x + c1.z;
+  eff = eff * x + ee;
+  ee = ee * x + c1.w;
+  e = ee;
+  f = eff * c0.y;
+}
+
+__device__ __forceinline__ void cross3(const float a[3], const float b[3], float out[3]) {
+  out[0] = a[1] * b[2] - a[2] * b[1];
+  out[1] = a[2] * b[0] - a[0] * b[2];
+  out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Cosine of the dihedral angle of (xi, xj, xk, xl) and its gradient with
+// respect to xi, xj and xl (the gradient for xk is minus their sum).
+__device__ __forceinline__ void dihedral_geometry(float3 xi, float3 xj, float3 xk, float3 xl, float &c, float gi[3],
+                                                  float gj[3], float gl[3]) {
+  const float b1[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+  const float b2[3] = {xk.x - xj.x, xk.y - xj.y, xk.z - xj.z};
+  const float b3[3] = {xl.x - xk.x, xl.y - xk.y, xl.z - xk.z};
+  float m[3], n[3];
+  cross3(b1, b2, m);
+  cross3(b2, b3, n);
+  const float mm = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+  const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  const float mn = m[0] * n[0] + m[1] * n[1] + m[2] * n[2];
+  const float inv = 1.0f / sqrtf(mm * nn);
+  c = mn * inv;
+  const float cm = c / mm, cn = c / nn;
+  float gm[3], gn[3];
+  for (int d = 0; d < 3; ++d) {
+    gm[d] = n[d] * inv - cm * m[d];
+    gn[d] = m[d] * inv - cn * n[d];
+  }
+  float d1[3], d3[3], t1[3], t2[3];
+  cross3(b2, gm, d1);
+  cross3(gn, b2, d3);
+  cross3(gm, b1, t1);
+  cross3(b3, gn, t2);
+  for (int d = 0; d < 3; ++d) {
+    gi[d] = -d1[d];
+    gj[d] = d1[d] - (t1[d] + t2[d]);
+    gl[d] = d3[d];
+  }
+}
+
+struct TermArgs {
+  const DevTerm *__restrict__ terms;
+  int nterms;
+  const float4 *__restrict__ x;
+  const int *__restrict__ cell;
+  const int *__restrict__ cell_rep;
+  const int *__restrict__ slot_of;
+  const DevReplica *__restrict__ rep;
+  const DevPot *__restrict__ pots;   // pair potentials (exclusions)
+  const DevPot *__restrict__ tpots;  // angle and dihedral potentials
+  const float *__restrict__ coef;
+  const int *__restrict__ potmat;
+  int ntypes;
+  float cut2;
+  const unsigned *__restrict__ time;
+  uint32_t seed_lo, seed_hi;
+  float4 *__restrict__ tf;
+  float *__restrict__ te;
+};
+
+// Position of the particle in slot s relative to the cell (jx, jy, jz): its
+// cell offset folded to -1, 0, +1 as the bonded evaluators do
+// (angle.cpp:148-163), the shift added in double and rounded once.
+__device__ __forceinline__ float3 rel_pos(const TermArgs &T, const DevReplica &R, int s, int jx, int jy, int jz) {
+  int lx, ly, lz;
+  cell_coords(T.cell[s] - R.cell_base, R, lx, ly, lz);
+  int sh[3] = {lx - jx, ly - jy, lz - jz};
+  for (int d = 0; d < 3; ++d) sh[d] = sh[d] > 1 ? -1 : (sh[d] < -1 ? 1 : sh[d]);
+  const float4 p = T.x[s];
+  return make_float3((float)((double)p.x + R.hd[0] * sh[0]), (float)((double)p.y + R.hd[1] * sh[1]),
+                     (float)((double)p.z + R.hd[2] * sh[2]));
+}
+
+__device__ __forceinline__ void put(float4 *tf, int pos, float x, float y, float z) {
+  tf[pos] = make_float4(x, y, z, 0.0f);
+}
+
+// Angles (angle_eval, angle.cpp:92-354), dihedrals (dihedral_eval,
+// dihedral.cpp:90-330) and exclusions (exclusion_eval, exclusion.cpp:83-260),
+// one thread per term, from the positions at the start of the force phase.
+__global__ void __launch_bounds__(256) k_terms(TermArgs T) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T.nterms) return;
+  const DevTerm D = T.terms[t];
+  const int sj = T.slot_of[D.id[1]];
+  const DevReplica R = T.rep[T.cell_rep[T.cell[sj]]];
+  int jx, jy, jz;
+  cell_coords(T.cell[sj] - R.cell_base, R, jx, jy, jz);
+  const float4 pj = T.x[sj];
+  float ee = 0.0f;
+  if (D.kind == kTermExclusion) {
+    // dx = xi - xj + h shift (float difference, double shift), the pair
+    // potential of the two types at r^2, forces and energy subtracted.
+    const int si = T.slot_of[D.id[0]];
+    int ix, iy, iz;
+    cell_coords(T.cell[si] - R.cell_base, R, ix, iy, iz);
+    int sh[3] = {ix - jx, iy - jy, iz - jz};
+    for (int d = 0; d < 3; ++d) sh[d] = sh[d] > 1 ? -1 : (sh[d] < -1 ? 1 : sh[d]);
+    const float4 pi = T.x[si];
+    const float dx = (float)((double)(pi.x - pj.x) + R.hd[0] * sh[0]);
+    const float dy = (float)((double)(pi.y - pj.y) + R.hd[1] * sh[1]);
+    const float dz = (float)((double)(pi.z - pj.z) + R.hd[2] * sh[2]);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const int ti = __float_as_int(pi.w) & 0xffff, tj = __float_as_int(pj.w) & 0xffff;
+    const int pidx = T.potmat[tj * T.ntypes + ti];
+    float wx = 0, wy = 0, wz = 0;
+    if (r2 <= T.cut2 && pidx >= 0) {
+      const DevPot P = T.pots[pidx];
+      if (P.kind == kPotKindTable) {
+        const float r = sqrtf(r2);
+        const int ind = (int)fmaxf(0.0f, P.alpha0 + r * (P.alpha1 + r * P.alpha2));
+        if (ind <= P.n) {
+          float eff;
+          table_eval(P, T.coef, r2, ee, eff);
+          wx = eff * dx;
+          wy = eff * dy;
+          wz = eff * dz;
+        }
+      }
+    }
+    put(T.tf, D.pos[0], wx, wy, wz);
+    put(T.tf, D.pos[1], -wx, -wy, -wz);
+  } else if (D.kind == kTermAngle) {
+    const float3 xi = rel_pos(T, R, T.slot_of[D.id[0]], jx, jy, jz);
+    const float3 xk = rel_pos(T, R, T.slot_of[D.id[2]], jx, jy, jz);
+    const float rji[3] = {xi.x - pj.x, xi.y - pj.y, xi.z - pj.z};
+    const float rjk[3] = {xk.x - pj.x, xk.y - pj.y, xk.z - pj.z};
+    const float dprod = rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2];
+    const float inji = 1.0f / sqrtf(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+    const float injk = 1.0f / sqrtf(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+    float ct = fmaxf(-1.0f, fminf(1.0f, dprod * inji * injk));
+    float dxi[3], dxk[3];
+    if (ct == 0.0f || ct == -1.0f) {
+      // Right or straight angle: the reference draws a random direction
+      // (uniform components on [-1, 1]), removes its component along
+      // xi - xk and uses the unit result for both outer particles; here the
+      // draw is keyed by the angle and the step's time (engine_step counts
+      // the step before its forces, so the counter plus one).
+      mxg::u32x4 ctr = {{(uint32_t)D.tid, *T.time + 1u, mxg::kStreamAngle, 0u}};
+      mxg::u32x4 r = mxg::philox4x32_10(ctr, T.seed_lo, T.seed_hi);
+      float v[3], vik[3] = {xi.x - xk.x, xi.y - xk.y, xi.z - xk.z};
+      for (int d = 0; d < 3; ++d) v[d] = 2.0f * mxg::u01_closed_open(r.v[d]) - 1.0f;
+      const float dv = v[0] * vik[0] + v[1] * vik[1] + v[2] * vik[2];
+      for (int d = 0; d < 3; ++d) v[d] = v[d] - dv * vik[d];
+      const float il = 1.0f / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      for (int d = 0; d < 3; ++d) dxi[d] = dxk[d] = v[d] * il;
+    } else {
+      for (int d = 0; d < 3; ++d) {
+        dxi[d] = (rjk[d] * injk - ct * rji[d] * inji) * inji;
+        dxk[d] = (rji[d] * inji - ct * rjk[d] * injk) * injk;
+      }
+    }
+    const DevPot P = T.tpots[D.pot];
+    if (ct < P.a || ct > P.b) ct = fmaxf(P.a, fminf(P.b, ct));
+    float eff;
+    table_eval_r(P, T.coef, ct, ee, eff);
+    float wi[3], wk[3];
+    for (int d = 0; d < 3; ++d) {
+      wi[d] = eff * dxi[d];
+      wk[d] = eff * dxk[d];
+    }
+    put(T.tf, D.pos[0], -wi[0], -wi[1], -wi[2]);
+    put(T.tf, D.pos[2], -wk[0], -wk[1], -wk[2]);
+    put(T.tf, D.pos[1], wi[0] + wk[0], wi[1] + wk[1], wi[2] + wk[2]);
+  } else {
+    // cos(phi) of the planes (i, j, k) and (j, k, l) from the normals
+    // m = b1 x b2 and n = b2 x b3 (b1 = xj - xi, b2 = xk - xj,
+    // b3 = xl - xk), and its gradient: with g_m = dc/dm and g_n = dc/dn,
+    // dc/db1 = b2 x g_m, dc/db2 = g_m x b1 + b3 x g_n, dc/db3 = g_n x b2.
+    const float3 xi = rel_pos(T, R, T.slot_of[D.id[0]], jx, jy, jz);
+    const float3 xk = rel_pos(T, R, T.slot_of[D.id[2]], jx, jy, jz);
+    const float3 xl = rel_pos(T, R, T.slot_of[D.id[3]], jx, jy, jz);
+    float c, gi[3], gj[3], gl[3];
+    dihedral_geometry(make_float3(xi.x, xi.y, xi.z), make_float3(pj.x, pj.y, pj.z), xk, xl, c, gi, gj, gl);
+    const DevPot P = T.tpots[D.pot];
+    float cp = fmaxf(-1.0f, fminf(1.0f, c));
+    if (cp < P.a || cp > P.b) cp = fmaxf(P.a, fminf(P.b, cp));
+    float eff;
+    table_eval_r(P, T.coef, cp, ee, eff);
+    float wi[3], wj[3], wl[3];
+    for (int d = 0; d < 3; ++d) {
+      wi[d] = eff * gi[d];
+      wj[d] = eff * gj[d];
+      wl[d] = eff * gl[d];
+    }
+    put(T.tf, D.pos[0], -wi[0], -wi[1], -wi[2]);
+    put(T.tf, D.pos[1], -wj[0], -wj[1], -wj[2]);
+    put(T.tf, D.pos[3], -wl[0], -wl[1], -wl[2]);
+    put(T.tf, D.pos[2], wi[0] + wj[0] + wl[0], wi[1] + wj[1] + wl[1], wi[2] + wj[2] + wl[2]);
+  }
+  T.te[t] = ee;
+}
+
+// Energies of the terms per replica and kind in FP64, in a fixed order: one
+// block per replica (the terms are grouped by replica).
+__global__ void __launch_bounds__(256) k_term_energy(const DevTerm *__restrict__ terms, const float *__restrict__ te,
+                                                     const int *__restrict__ trep, double *__restrict__ out) {
+  __shared__ double red[3][256];
+  const int r = blockIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int t = trep[r] + threadIdx.x; t < trep[r + 1]; t += blockDim.x) {
+    const int kind = terms[t].kind;
+    const double e = (double)te[t];
+    acc[0] += kind == kTermAngle ? e : 0.0;
+    acc[1] += kind == kTermDihedral ? e : 0.0;
+    acc[2] += kind == kTermExclusion ? e : 0.0;
+  }
+  for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  // exclusion_eval subtracts its energies (exclusion.cpp:255).
+  if (threadIdx.x < 3) out[r * 3 + threadIdx.x] = threadIdx.x == 2 ? -red[2][0] : red[threadIdx.x][0];
 }
 
 // Advance of one particle (cell_advance_forward_euler,
@@ -1271,7 +1549,10 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
             if (o >= R.n_off) break;
           }
         }
-        if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
+        if (MODE & kModeBonds) {
+          bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
+          term_forces(A, mypid, fx, fy, fz);
+        }
       }
       const bool stayed =
           B.rk_stage < 0 ? advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd)
@@ -1634,7 +1915,10 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
                                                   fx, fy, fz, acc, t1, tk))
                                nd += w_cubic(r, C.cutoff, C.nd_norm);
                            });
-        if (MODE & kModeBonds) bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
+        if (MODE & kModeBonds) {
+          bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
+          term_forces(A, mypid, fx, fy, fz);
+        }
       }
       const bool stayed =
           B.rk_stage < 0 ? advance_store(A, R, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs, mypid, nd)
@@ -2034,6 +2318,9 @@ int Engine::upload_potentials() {
       any_table_ = true;
   }
   for (auto &p : bpots_) dbp.push_back(pack(p));
+  std::vector<DevPot> dtp;
+  for (auto &p : apots_) dtp.push_back(pack(p));
+  for (auto &p : dpots_) dtp.push_back(pack(p));
   std::vector<DevType> dt(types_.size());
   for (size_t i = 0; i < types_.size(); ++i) {
     dt[i].mass = types_[i].mass;
@@ -2059,6 +2346,7 @@ int Engine::upload_potentials() {
   int rc = 0;
   rc |= upload(&d->pots, dp);
   rc |= upload(&d->bpots, dbp);
+  rc |= upload(&d->tpots, dtp);
   rc |= upload(&d->coef, coef);
   rc |= upload(&d->potmat, potmat_);
   rc |= upload(&d->types, dt);
@@ -2136,6 +2424,14 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
     rep_tiles[r] = make_int2(R.tile_begin, R.tile_end);
   }
   N_ = N;
+  prep_.assign(N, 0);
+  if (replica)
+    for (int i = 0; i < N; ++i) prep_[i] = replica[i];
+  // Angles, dihedrals and exclusions name particle ids of the previous load.
+  angles_.clear();
+  dihedrals_.clear();
+  exclusions_.clear();
+  NT_ = 0;
   ntiles_ = (int)tiles.size();
   mean_cell_count_.assign(reps_.size(), 0.0);
   for (size_t r = 0; r < reps_.size(); ++r)
@@ -2367,6 +2663,169 @@ int Engine::set_bonds(int M, const int *bi, const int *bj, const Potential *cons
   return kErrOk;
 }
 
+// Shared checks of a term's particles: valid, distinct ids in one replica.
+static bool term_ids_ok(const int *ids, int n, int N, const std::vector<int> &prep) {
+  for (int a = 0; a < n; ++a) {
+    if (ids[a] < 0 || ids[a] >= N) return false;
+    for (int b = 0; b < a; ++b)
+      if (ids[a] == ids[b]) return false;
+    if (prep[ids[a]] != prep[ids[0]]) return false;
+  }
+  return true;
+}
+
+static int dedup_pot(std::vector<Potential> &list, std::vector<const Potential *> &seen, const Potential *p) {
+  auto it = std::find(seen.begin(), seen.end(), p);
+  if (it != seen.end()) return (int)(it - seen.begin());
+  seen.push_back(p);
+  list.push_back(*p);
+  return (int)list.size() - 1;
+}
+
+// engine_angle_add (engine.cpp): angle (i, j, k) with j the vertex and a
+// potential tabulated in cos(theta).
+int Engine::set_angles(int M, const int *i, const int *j, const int *k, const Potential *const *pots) {
+  if (!d_ || N_ == 0) return fail(kErrNull, "load particles before angles");
+  if (M < 0 || (M > 0 && (!i || !j || !k || !pots))) return reject(kErrNull, "angle arrays missing");
+  std::vector<HostTerm> terms(M);
+  std::vector<Potential> list;
+  std::vector<const Potential *> seen;
+  for (int a = 0; a < M; ++a) {
+    const int ids[3] = {i[a], j[a], k[a]};
+    if (!term_ids_ok(ids, 3, N_, prep_))
+      return reject(kErrRange, "angle " + std::to_string(a) + " has invalid or repeated particle ids");
+    const Potential *p = pots[a];
+    if (!p || p->kind != kPotKindTable || p->n < 1)
+      return reject(kErrPotential, "angle potential must be a table potential");
+    terms[a] = HostTerm{{ids[0], ids[1], ids[2], -1}, dedup_pot(list, seen, p)};
+  }
+  angles_.swap(terms);
+  apots_.swap(list);
+  potentials_dirty_ = true;
+  return upload_terms();
+}
+
+// engine_dihedral_add (engine.cpp): dihedral (i, j, k, l) about the bond
+// j-k, with a potential tabulated in cos(phi).
+int Engine::set_dihedrals(int M, const int *i, const int *j, const int *k, const int *l,
+                          const Potential *const *pots) {
+  if (!d_ || N_ == 0) return fail(kErrNull, "load particles before dihedrals");
+  if (M < 0 || (M > 0 && (!i || !j || !k || !l || !pots))) return reject(kErrNull, "dihedral arrays missing");
+  std::vector<HostTerm> terms(M);
+  std::vector<Potential> list;
+  std::vector<const Potential *> seen;
+  for (int a = 0; a < M; ++a) {
+    const int ids[4] = {i[a], j[a], k[a], l[a]};
+    if (!term_ids_ok(ids, 4, N_, prep_))
+      return reject(kErrRange, "dihedral " + std::to_string(a) + " has invalid or repeated particle ids");
+    const Potential *p = pots[a];
+    if (!p || p->kind != kPotKindTable || p->n < 1)
+      return reject(kErrPotential, "dihedral potential must be a table potential");
+    terms[a] = HostTerm{{ids[0], ids[1], ids[2], ids[3]}, dedup_pot(list, seen, p)};
+  }
+  dihedrals_.swap(terms);
+  dpots_.swap(list);
+  potentials_dirty_ = true;
+  return upload_terms();
+}
+
+// engine_exclusion_add (engine.cpp): the pair's non-bonded interaction is
+// evaluated again and subtracted.
+int Engine::set_exclusions(int M, const int *i, const int *j) {
+  if (!d_ || N_ == 0) return fail(kErrNull, "load particles before exclusions");
+  if (M < 0 || (M > 0 && (!i || !j))) return reject(kErrNull, "exclusion arrays missing");
+  std::vector<HostTerm> terms(M);
+  for (int a = 0; a < M; ++a) {
+    const int ids[2] = {i[a], j[a]};
+    if (!term_ids_ok(ids, 2, N_, prep_))
+      return reject(kErrRange, "exclusion " + std::to_string(a) + " has invalid or repeated particle ids");
+    terms[a] = HostTerm{{ids[0], ids[1], -1, -1}, -1};
+  }
+  exclusions_.swap(terms);
+  return upload_terms();
+}
+
+// Device lists of the terms: grouped by replica and, within a replica, in
+// the reference's evaluation order (exclusions, angles, dihedrals;
+// engine_bonded.cpp:1513-1530); the per-particle term lists hold one entry
+// per role, in term order.
+int Engine::upload_terms() {
+  struct Ref {
+    int rep, kind, idx;
+  };
+  std::vector<Ref> order;
+  const std::vector<HostTerm> *lists[3] = {&exclusions_, &angles_, &dihedrals_};
+  const int kinds[3] = {kTermExclusion, kTermAngle, kTermDihedral};
+  const int roles[3] = {2, 3, 4};
+  for (int g = 0; g < 3; ++g)
+    for (int a = 0; a < (int)lists[g]->size(); ++a) order.push_back(Ref{prep_[(*lists[g])[a].id[1]], g, a});
+  std::stable_sort(order.begin(), order.end(), [](const Ref &x, const Ref &y) { return x.rep < y.rep; });
+  const int NT = (int)order.size();
+  const int R = (int)reps_.size();
+  std::vector<int> toff(N_ + 1, 0), trep(R + 1, 0);
+  for (const Ref &r : order) {
+    const HostTerm &h = (*lists[r.kind])[r.idx];
+    for (int q = 0; q < roles[r.kind]; ++q) toff[h.id[q] + 1]++;
+    trep[r.rep + 1]++;
+  }
+  for (int p = 0; p < N_; ++p) toff[p + 1] += toff[p];
+  for (int r = 0; r < R; ++r) trep[r + 1] += trep[r];
+  std::vector<int> fill(toff.begin(), toff.end() - 1);
+  std::vector<DevTerm> dev(NT);
+  for (int t = 0; t < NT; ++t) {
+    const Ref &r = order[t];
+    const HostTerm &h = (*lists[r.kind])[r.idx];
+    DevTerm &D = dev[t];
+    std::memset(&D, 0, sizeof(D));
+    for (int q = 0; q < 4; ++q) {
+      D.id[q] = q < roles[r.kind] ? h.id[q] : -1;
+      D.pos[q] = q < roles[r.kind] ? fill[h.id[q]]++ : -1;
+    }
+    D.kind = kinds[r.kind];
+    D.pot = r.kind == 1 ? h.pot : (r.kind == 2 ? (int)apots_.size() + h.pot : -1);
+    D.tid = r.idx;
+  }
+  DeviceState *d = d_;
+  const size_t entries = toff[N_];
+  if (int st = dalloc(d, &d->terms, dev.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->trep, trep.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->toff, toff.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->tf, entries)) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->te, dev.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->obs_terms, 3 * (size_t)R)) return check_device(st, "hipMalloc");
+  hipStream_t s = d->stream;
+  int rc = 0;
+  if (NT > 0) {
+    rc |= hipMemcpyAsync(d->terms, dev.data(), dev.size() * sizeof(DevTerm), hipMemcpyHostToDevice, s);
+    rc |= hipMemsetAsync(d->tf, 0, entries * sizeof(float4), s);
+    rc |= hipMemsetAsync(d->te, 0, dev.size() * sizeof(float), s);
+  }
+  rc |= hipMemcpyAsync(d->trep, trep.data(), trep.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->toff, toff.data(), toff.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  rc |= hipMemsetAsync(d->obs_terms, 0, 3 * (size_t)R * sizeof(double), s);
+  rc |= hipStreamSynchronize(s);
+  if (rc) return check_device(rc, "term upload");
+  NT_ = NT;
+  args_dirty_ = true;
+  return kErrOk;
+}
+
+int Engine::term_energies(int replica, double *out) {
+  if (!d_ || !out) return reject(kErrNull, "null argument");
+  const int R = (int)reps_.size();
+  if (replica < -1 || replica >= R) return reject(kErrRange, "replica index out of range");
+  for (int k = 0; k < 3; ++k) out[k] = 0.0;
+  if (NT_ == 0) return kErrOk;
+  std::vector<double> h(3 * (size_t)R);
+  int rc = hipMemcpyAsync(h.data(), d_->obs_terms, h.size() * sizeof(double), hipMemcpyDeviceToHost, d_->stream);
+  rc |= hipStreamSynchronize(d_->stream);
+  if (rc) return check_device(rc, "term energies");
+  for (int r = 0; r < R; ++r)
+    if (replica < 0 || r == replica)
+      for (int k = 0; k < 3; ++k) out[k] += h[3 * (size_t)r + k];
+  return kErrOk;
+}
+
 // Kernel arguments of the force kernel for the step parity par (the
 // ping-pong buffers of ids, cell offsets and bond flags alternate).
 void Engine::fill_step_args(void *out, int par) const {
@@ -2413,6 +2872,8 @@ void Engine::fill_step_args(void *out, int par) const {
   A.bact_in = (decltype(A.bact_in))(M_ > 0 ? d->bact[par] : nullptr);
   A.bact_out = (decltype(A.bact_out))(M_ > 0 ? d->bact[par ^ 1] : nullptr);
   A.slot_of = (decltype(A.slot_of))(d->slot_of);
+  A.toff = (decltype(A.toff))(NT_ > 0 ? d->toff : nullptr);
+  A.tf = (decltype(A.tf))(NT_ > 0 ? d->tf : nullptr);
   A.dt = (decltype(A.dt))((float)dt_);
   const float cf = (float)cutoff_;
   A.cut2_pair = (decltype(A.cut2_pair))(cf * cf);
@@ -2471,12 +2932,36 @@ int Engine::launch_substep(int rk_stage, int tick) {
   if (any_table_) mode |= kModeTable;
   if (any_dpd_) mode |= kModeDPD;
   if (virial_) mode |= kModeVirial;
-  if (M_ > 0) mode |= kModeBonds;
+  if (M_ > 0 || NT_ > 0) mode |= kModeBonds;
   hipStream_t s = d->stream;
   if (any_berendsen_) {
     hipLaunchKernelGGL(k_type_ke, dim3(kKeBlocks), dim3(256), 0, s, d->x[0], d->v[0], d->types, N_,
                        d->type_ke_part);
     hipLaunchKernelGGL(k_type_ke_final, dim3(1), dim3(64), 0, s, d->type_ke_part, d->type_ke);
+  }
+  if (NT_ > 0) {
+    TermArgs T;
+    T.terms = d->terms;
+    T.nterms = NT_;
+    T.x = d->x[0];
+    T.cell = d->cell[0];
+    T.cell_rep = d->cell_rep;
+    T.slot_of = d->slot_of;
+    T.rep = d->rep;
+    T.pots = d->pots;
+    T.tpots = d->tpots;
+    T.coef = d->coef;
+    T.potmat = d->potmat;
+    T.ntypes = (int)types_.size();
+    T.cut2 = (float)(cutoff_ * cutoff_);
+    T.time = d->time;
+    T.seed_lo = (uint32_t)seed_;
+    T.seed_hi = (uint32_t)(seed_ >> 32);
+    T.tf = d->tf;
+    T.te = d->te;
+    hipLaunchKernelGGL(k_terms, dim3((NT_ + 255) / 256), dim3(256), 0, s, T);
+    hipLaunchKernelGGL(k_term_energy, dim3(reps_.size()), dim3(256), 0, s, d->terms, d->te, d->trep,
+                       d->obs_terms);
   }
   const bool rec = timing_ && timed_steps_ < (int)d->kev.size() / 2;
   if (rec) hipEventRecord(d->kev[2 * timed_steps_], s);

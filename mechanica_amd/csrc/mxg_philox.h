@@ -77,6 +77,7 @@ enum : uint32_t {
   kStreamOverlap = 0x6F766C70u,   // direction for pairs closer than a
   kStreamDPD = 0x64706472u,       // DPD pair noise
   kStreamSingle = 0x73696E67u,    // single-body random force
+  kStreamAngle = 0x616E676Cu,     // plane of a right or straight angle
 };
 
 }  // namespace mxg

@@ -17,6 +17,16 @@ parameters so tests can build the same workload at small N.
              lattice of src/lattice.py:449-522; 3 n^2 (n-1) bonds), K = 100,
              r0 = 1, dissociation energy 5, Morse(d=1, a=6, r0=1, max=2)
              between all pairs, cutoff 2, overdamped, dt = 0.005.
+  molecules  four-bead chains (bond 0.3, angles near 110 deg, random
+             dihedrals) with harmonic bonds 1-2, 2-3, 3-4 (K 100, r0 0.3),
+             harmonic angles 1-2-3, 2-3-4 (K 20, theta0 110 deg, defined on
+             theta0 +- 60 deg), a harmonic dihedral 1-2-3-4 (K 1, n 3,
+             delta 0 at tolerance 1e-4 or pi/4 at 1e-2), exclusions of the 1-2 and 1-3 pairs, and LJ
+             12-6 (sigma 0.25, eps 0.5, table on [0.1, 1]) between all
+             particles; molecule centres at least 1.25 apart, cutoff 1,
+             Newtonian, mass 1, dt 0.002 (the bonded set of the reference's
+             flexible-water example, examples/flexible/flexible.cpp:304-336,
+             on a non-Coulomb pair potential).
   cell_sort  two-type Morse cell-sorting universe (examples/cell_sorting.py)
              at density 0.8: AA and BB d = 3, AB d = 1, a = 6, r0 = 1,
              cutoff 2.5, overdamped with a persistent random force whose
@@ -137,6 +147,64 @@ def lattice(n=64, seed=3, jitter=0.05, dissociation=5.0):
         "pots": [("morse", (1.0, 6.0, 1.0, 1e-4, 2.0, 1e-3), 0, 0)],
         "bonds": {"i": bi, "j": bj, "pot": ("harmonic", (0.0, 2.0, 100.0, 1.0, 0.02)),
                   "dissociation": dissociation},
+    }
+
+
+def _unit(v):
+    return v / np.linalg.norm(v)
+
+
+def _chain(rng, bond=0.3, theta0=110.0, spread=15.0):
+    """Four points: bond lengths `bond`, both angles theta0 +- spread degrees,
+    a uniform dihedral; centred on the origin."""
+    u1 = _unit(rng.normal(size=3))
+    pts = [np.zeros(3), bond * u1]
+    prev = u1
+    ref = _unit(np.cross(u1, rng.normal(size=3)))
+    for _ in range(2):
+        th = math.radians(theta0 + rng.uniform(-spread, spread))
+        phi = rng.uniform(0, 2 * math.pi)
+        a = ref
+        b = np.cross(prev, a)
+        # direction making the angle th with -prev at the last point
+        d = -math.cos(th) * prev + math.sin(th) * (math.cos(phi) * a + math.sin(phi) * b)
+        pts.append(pts[-1] + bond * d)
+        ref = _unit(np.cross(d, prev))
+        prev = d
+    P = np.array(pts)
+    return P - P.mean(0)
+
+
+def molecules(nmol=2000, seed=8, delta=0.0, dt=0.002):
+    rng = _rng(seed)
+    L = (nmol * 4.0) ** (1.0 / 3.0)
+    cells = max(3, int(math.floor(L / 1.0)))
+    centres = _min_dist_fill(rng, nmol, L, 1.25)
+    x = np.concatenate([c + _chain(rng) for c in centres]) % L
+    N = 4 * nmol
+    v = rng.normal(0.0, 0.5, size=(N, 3))
+    base = 4 * np.arange(nmol, dtype=np.int32)
+    sig, eps = 0.25, 0.5
+    A, B = 4 * eps * sig ** 12, 4 * eps * sig ** 6
+    th0 = math.radians(110.0)
+    return {
+        "name": "molecules",
+        "dim": [L, L, L], "cells": [cells] * 3, "cutoff": 1.0, "dt": dt,
+        "types": [{"mass": 1.0, "radius": 0.1, "dynamics": 0}],
+        "x": x, "v": v, "type": np.zeros(N, np.int32),
+        "pots": [("lj126", (0.1, 1.0, A, B, 1e-4), 0, 0)],
+        "bonds": {"i": np.concatenate([base, base + 1, base + 2]),
+                  "j": np.concatenate([base + 1, base + 2, base + 3]),
+                  "pot": ("harmonic", (0.0, 0.6, 100.0, 0.3, 1e-4)), "dissociation": 1e30},
+        "angles": {"i": np.concatenate([base, base + 1]), "j": np.concatenate([base + 1, base + 2]),
+                   "k": np.concatenate([base + 2, base + 3]),
+                   "pot": ("harmonic_angle", (th0 - math.radians(60), th0 + math.radians(60), 20.0, th0, 1e-4))},
+        # a delta off the multiples of pi puts sqrt(1 - c^2) into the table,
+        # whose end slopes the builder only meets at a coarser tolerance
+        "dihedrals": {"i": base, "j": base + 1, "k": base + 2, "l": base + 3,
+                      "pot": ("harmonic_dihedral", (1.0, 3, delta, 1e-4 if math.fmod(delta, math.pi) == 0 else 1e-2))},
+        "exclusions": {"i": np.concatenate([base, base + 1, base + 2, base, base + 1]),
+                       "j": np.concatenate([base + 1, base + 2, base + 3, base + 2, base + 3])},
     }
 
 
@@ -302,6 +370,12 @@ def make_potential(m, spec):
     if kind == "harmonic":
         lo, hi, k, r0, tol = args
         return m.Potential.harmonic(k, r0, min=lo, max=hi, tol=tol)
+    if kind == "harmonic_angle":
+        lo, hi, k, th0, tol = args
+        return m.Potential.harmonic_angle(k, th0, min=lo, max=hi, tol=tol)
+    if kind == "harmonic_dihedral":
+        k, n, delta, tol = args
+        return m.Potential.harmonic_dihedral(k, n, delta, tol=tol)
     if kind == "dpd":
         al, g, s, c, sh = args
         return m.Potential.dpd(alpha=al, gamma=g, sigma=s, cutoff=c, shifted=bool(sh))
@@ -345,5 +419,17 @@ def build_universe(m, w, seed=None):
         bp = make_potential(m, b["pot"])
         u.set_bonds(b["i"], b["j"], bp, b["dissociation"])
         pots.append(bp)
+    if w.get("exclusions"):
+        u.set_exclusions(w["exclusions"]["i"], w["exclusions"]["j"])
+    if w.get("angles"):
+        a = w["angles"]
+        ap = make_potential(m, a["pot"])
+        u.set_angles(a["i"], a["j"], a["k"], ap)
+        pots.append(ap)
+    if w.get("dihedrals"):
+        d = w["dihedrals"]
+        dp = make_potential(m, d["pot"])
+        u.set_dihedrals(d["i"], d["j"], d["k"], d["l"], dp)
+        pots.append(dp)
     u._keep = pots
     return u

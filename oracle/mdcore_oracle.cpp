@@ -25,6 +25,10 @@
 //   potential_eval_super_ex / dpd_eval .............. dpd_eval.hpp:21-169
 //   get_potential ................................... potential_eval.hpp:1543
 //   bond_eval (periodic shift, clamp, dissociation).. bond.cpp:99-295
+//   exclusion_eval (pair potential subtracted) ...... exclusion.cpp:83-260
+//   angle_eval (cos theta, potential_eval_r) ........ angle.cpp:92-354
+//   dihedral_eval (cos phi, potential_eval_r) ....... dihedral.cpp:90-330
+//   bonded evaluation order ......................... engine_bonded.cpp:1510-1538
 //   cell_advance_forward_euler ...................... engine_advance.cpp:126-239
 //   engine_advance_forward_euler .................... engine_advance.cpp:248-422
 //   engine_advance_runge_kutta_4 .................... engine_advance.cpp:443-698
@@ -36,7 +40,16 @@
 //   * gaussian_force draws from Philox4x32-10 instead of carbon's CRandom
 //     (absent from the reference tree), keyed by (particle id, time);
 //   * the optional counter-based DPD noise mode is this project's own; the
-//     default DPD mode is the reference's deterministic sigma term.
+//     default DPD mode is the reference's deterministic sigma term;
+//   * a right or straight angle (cos theta exactly 0 or -1) takes its random
+//     plane from Philox4x32-10 keyed by (angle index, time) instead of
+//     CRandom;
+//   * dihedral_eval's gradient of cos(phi) is the reference's generated
+//     expression; here it is derived from the plane normals (same
+//     mathematics, different rounding);
+//   * an exclusion whose distance lies beyond its table (the reference reads
+//     past the table) or whose pair potential is DPD (no table) contributes
+//     nothing.
 //
 // Two precisions: the faithful FP32 path (every float/double step as the
 // reference), and an FP64 evaluation of the same pair set and tables
@@ -166,6 +179,18 @@ struct Bond {
   bool active;
 };
 
+struct Angle {
+  int i, j, k, pot;
+};
+
+struct Dihedral {
+  int i, j, k, l, pot;
+};
+
+struct Exclusion {
+  int i, j;
+};
+
 struct Oracle {
   // geometry
   double origin[3], dim[3], h[3], ih[3];
@@ -191,9 +216,14 @@ struct Oracle {
   std::vector<int> potmat;     // ntypes x ntypes -> index into pots, -1 none
   std::vector<Pot> bpots;      // bond potentials
   std::vector<Bond> bonds;
+  std::vector<Pot> apots, dpots;  // angle and dihedral potentials
+  std::vector<Angle> angles;
+  std::vector<Dihedral> dihedrals;
+  std::vector<Exclusion> exclusions;
   int nparts = 0;
 
   double ekin = 0, epot_nb = 0, epot_bond = 0;
+  double epot_angle = 0, epot_dihedral = 0, epot_exclusion = 0;
   float dpd_delta = 0;
   bool dpd_delta_set = false;
 
@@ -810,6 +840,238 @@ void bond_eval(Oracle &o, const std::vector<Loc> &where) {
 }
 
 // ---------------------------------------------------------------------------
+// Exclusions, angles and dihedrals, in the working precision T: float
+// restates the reference's arithmetic (positions shifted in double and
+// rounded once, everything else in float), double is the FP64 statement of
+// the same terms.
+
+// potential_eval_r (potential_eval.hpp:180-208).
+template <typename T>
+T table_r(const Pot &p, T r, T *f) {
+  int ind = (int)std::fmax((T)0, (T)p.alpha[0] + r * ((T)p.alpha[1] + r * (T)p.alpha[2]));
+  ind = std::min(std::max(ind, 0), p.n);
+  const float *c = &p.c[(size_t)ind * kChunk];
+  T x = (r - (T)c[0]) * (T)c[1];
+  T ee = (T)c[2] * x + (T)c[3];
+  T eff = (T)c[2];
+  for (int k = 4; k < kChunk; ++k) {
+    eff = eff * x + ee;
+    ee = ee * x + (T)c[k];
+  }
+  *f = eff * (T)c[1];
+  return ee;
+}
+
+// Particle b's position relative to particle a's cell: the cell offset
+// folded to -1, 0, +1 (angle.cpp:148-163).
+template <typename T>
+void rel_to(const Oracle &o, const Loc &lb, const Loc &la, T out[3]) {
+  const Part &pb = o.cells[lb.cell].parts[lb.idx];
+  const int *locb = o.cells[lb.cell].loc, *loca = o.cells[la.cell].loc;
+  for (int k = 0; k < 3; ++k) {
+    int shift = locb[k] - loca[k];
+    if (shift > 1)
+      shift = -1;
+    else if (shift < -1)
+      shift = 1;
+    out[k] = (T)(pb.x[k] + shift * o.h[k]);
+  }
+}
+
+inline Part &part_at(Oracle &o, const Loc &l) { return o.cells[l.cell].parts[l.idx]; }
+
+// exclusion_eval: the pair's table potential at r^2 (potential_eval, with
+// the float difference of the positions plus the double shift), its force
+// and energy subtracted.  add(id, k, w) adds w to component k of the
+// force on particle id.
+template <typename T, typename Add>
+double exclusions_eval(const Oracle &o, const std::vector<Loc> &where, Add add) {
+  double epot = 0.0;
+  const int nt = (int)o.types.size();
+  const float cutoff2 = (float)o.cutoff2;
+  for (const Exclusion &x : o.exclusions) {
+    const Loc &li = where[x.i], &lj = where[x.j];
+    const Part &pi = o.cells[li.cell].parts[li.idx];
+    const Part &pj = o.cells[lj.cell].parts[lj.idx];
+    const int pidx = o.potmat[pj.type * nt + pi.type];
+    if (pidx < 0) continue;
+    const Pot &pot = o.pots[pidx];
+    if (pot.kind != kKindTable) continue;
+    const int *loci = o.cells[li.cell].loc, *locj = o.cells[lj.cell].loc;
+    T dx[3], r2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      int shift = loci[k] - locj[k];
+      if (shift > 1)
+        shift = -1;
+      else if (shift < -1)
+        shift = 1;
+      dx[k] = (T)((T)(pi.x[k] - pj.x[k]) + o.h[k] * shift);
+      if (sizeof(T) == 8) dx[k] = ((T)pi.x[k] - (T)pj.x[k]) + o.h[k] * shift;
+      r2 += dx[k] * dx[k];
+    }
+    if ((float)r2 > cutoff2) continue;
+    const T r = std::sqrt(r2);
+    const int ind = (int)std::fmax((T)0, (T)pot.alpha[0] + r * ((T)pot.alpha[1] + r * (T)pot.alpha[2]));
+    if (ind > pot.n) continue;
+    const float *c = &pot.c[(size_t)ind * kChunk];
+    T xx = (r - (T)c[0]) * (T)c[1];
+    T ee = (T)c[2] * xx + (T)c[3], eff = (T)c[2];
+    for (int k = 4; k < kChunk; ++k) {
+      eff = eff * xx + ee;
+      ee = ee * xx + (T)c[k];
+    }
+    eff = eff * (T)c[1] / r;
+    for (int k = 0; k < 3; ++k) {
+      const T w = eff * dx[k];
+      add(pi.id, k, w);
+      add(pj.id, k, -w);
+    }
+    epot += ee;
+  }
+  return -epot;
+}
+
+// angle_eval: cos(theta) at the vertex j from the rays to i and k, its
+// gradients dxi, dxk, clamped into the table's domain, forces
+// -eff dxi on i, -eff dxk on k and their negative sum on j.
+template <typename T, typename Add>
+double angles_eval(const Oracle &o, const std::vector<Loc> &where, Add add) {
+  double epot = 0.0;
+  for (size_t a = 0; a < o.angles.size(); ++a) {
+    const Angle &A = o.angles[a];
+    const Loc &li = where[A.i], &lj = where[A.j], &lk = where[A.k];
+    const Part &pj = o.cells[lj.cell].parts[lj.idx];
+    T xi[3], xk[3], xj[3];
+    rel_to(o, li, lj, xi);
+    rel_to(o, lk, lj, xk);
+    for (int k = 0; k < 3; ++k) xj[k] = pj.x[k];
+    T rji[3], rjk[3];
+    for (int k = 0; k < 3; ++k) {
+      rji[k] = xi[k] - xj[k];
+      rjk[k] = xk[k] - xj[k];
+    }
+    const T dprod = rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2];
+    const T inji = (T)1 / std::sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+    const T injk = (T)1 / std::sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+    T ct = std::fmax((T)-1, std::fmin((T)1, dprod * inji * injk));
+    T dxi[3], dxk[3];
+    if (ct == 0 || ct == -1) {
+      mxg::u32x4 ctr = {{(uint32_t)a, (uint32_t)o.time, mxg::kStreamAngle, 0u}};
+      mxg::u32x4 r = mxg::philox4x32_10(ctr, o.seed_lo, o.seed_hi);
+      T v[3], vik[3];
+      for (int k = 0; k < 3; ++k) {
+        v[k] = (T)(2.0f * mxg::u01_closed_open(r.v[k]) - 1.0f);
+        vik[k] = xi[k] - xk[k];
+      }
+      const T dv = v[0] * vik[0] + v[1] * vik[1] + v[2] * vik[2];
+      for (int k = 0; k < 3; ++k) v[k] = v[k] - dv * vik[k];
+      const T il = (T)1 / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      for (int k = 0; k < 3; ++k) dxi[k] = dxk[k] = v[k] * il;
+    } else {
+      for (int k = 0; k < 3; ++k) {
+        dxi[k] = (rjk[k] * injk - ct * rji[k] * inji) * inji;
+        dxk[k] = (rji[k] * inji - ct * rjk[k] * injk) * injk;
+      }
+    }
+    const Pot &pot = o.apots[A.pot];
+    if (ct < (T)pot.a || ct > (T)pot.b) ct = std::fmax((T)pot.a, std::fmin((T)pot.b, ct));
+    T eff;
+    const T ee = table_r(pot, ct, &eff);
+    for (int k = 0; k < 3; ++k) {
+      const T wi = eff * dxi[k], wk = eff * dxk[k];
+      add(A.i, k, -wi);
+      add(A.k, k, -wk);
+      add(A.j, k, wi + wk);
+    }
+    epot += ee;
+  }
+  return epot;
+}
+
+// dihedral_eval: cos(phi) between the planes (i, j, k) and (j, k, l),
+// phi = 180 degrees for the trans form, and its gradient with respect to
+// the four positions.  With b1 = xj - xi, b2 = xk - xj, b3 = xl - xk and
+// the normals m = b1 x b2, n = b2 x b3, cos(phi) = m.n / (|m| |n|); its
+// derivatives along m and n are g_m = n / (|m||n|) - cos(phi) m / |m|^2 and
+// g_n likewise, so d/db1 = b2 x g_m, d/db2 = g_m x b1 + b3 x g_n,
+// d/db3 = g_n x b2.  Forces -eff dxi, -eff dxj, -eff dxl and their negative
+// sum on k.
+template <typename T, typename Add>
+double dihedrals_eval(const Oracle &o, const std::vector<Loc> &where, Add add) {
+  auto cross = [](const T a[3], const T b[3], T c[3]) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  auto dot = [](const T a[3], const T b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  double epot = 0.0;
+  for (const Dihedral &D : o.dihedrals) {
+    const Loc &li = where[D.i], &lj = where[D.j], &lk = where[D.k], &ll = where[D.l];
+    const Part &pj = o.cells[lj.cell].parts[lj.idx];
+    T xi[3], xj[3], xk[3], xl[3];
+    rel_to(o, li, lj, xi);
+    rel_to(o, lk, lj, xk);
+    rel_to(o, ll, lj, xl);
+    for (int k = 0; k < 3; ++k) xj[k] = pj.x[k];
+    T b1[3], b2[3], b3[3];
+    for (int k = 0; k < 3; ++k) {
+      b1[k] = xj[k] - xi[k];
+      b2[k] = xk[k] - xj[k];
+      b3[k] = xl[k] - xk[k];
+    }
+    T m[3], n[3];
+    cross(b1, b2, m);
+    cross(b2, b3, n);
+    const T mm = dot(m, m), nn = dot(n, n), mn = dot(m, n);
+    const T inv = (T)1 / std::sqrt(mm * nn);
+    const T c = mn * inv;
+    T gm[3], gn[3];
+    for (int k = 0; k < 3; ++k) {
+      gm[k] = n[k] * inv - (c / mm) * m[k];
+      gn[k] = m[k] * inv - (c / nn) * n[k];
+    }
+    T d1[3], d3[3], e1[3], e2[3];
+    cross(b2, gm, d1);
+    cross(gn, b2, d3);
+    cross(gm, b1, e1);
+    cross(b3, gn, e2);
+    T dxi[3], dxj[3], dxl[3];
+    for (int k = 0; k < 3; ++k) {
+      dxi[k] = -d1[k];
+      dxj[k] = d1[k] - (e1[k] + e2[k]);
+      dxl[k] = d3[k];
+    }
+    const Pot &pot = o.dpots[D.pot];
+    T cp = std::fmax((T)-1, std::fmin((T)1, c));
+    if (cp < (T)pot.a || cp > (T)pot.b) cp = std::fmax((T)pot.a, std::fmin((T)pot.b, cp));
+    T eff;
+    const T ee = table_r(pot, cp, &eff);
+    for (int k = 0; k < 3; ++k) {
+      const T wi = eff * dxi[k], wj = eff * dxj[k], wl = eff * dxl[k];
+      add(D.i, k, -wi);
+      add(D.j, k, -wj);
+      add(D.l, k, -wl);
+      add(D.k, k, wi + wj + wl);
+    }
+    epot += ee;
+  }
+  return epot;
+}
+
+// The reference's order: exclusions (stage 0) before the bonds, angles and
+// dihedrals (stage 1) after them, each force added to the particle's float
+// force as it is computed.
+void terms_eval(Oracle &o, const std::vector<Loc> &where, int stage) {
+  auto add = [&](int id, int k, float w) { part_at(o, where[id]).f[k] += w; };
+  if (stage == 0) {
+    if (!o.exclusions.empty()) o.epot_exclusion = exclusions_eval<float>(o, where, add);
+  } else {
+    if (!o.angles.empty()) o.epot_angle = angles_eval<float>(o, where, add);
+    if (!o.dihedrals.empty()) o.epot_dihedral = dihedrals_eval<float>(o, where, add);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Advance (engine_advance.cpp:126-239, 248-422).
 
 struct AdvanceConst {
@@ -943,7 +1205,13 @@ void force(Oracle &o) {
     run_tasks_threaded(o);
   }
   o.epot_bond = 0;
-  if (!o.bonds.empty()) bond_eval(o, locate(o));
+  o.epot_angle = o.epot_dihedral = o.epot_exclusion = 0;
+  const bool terms = !o.exclusions.empty() || !o.angles.empty() || !o.dihedrals.empty();
+  if (o.bonds.empty() && !terms) return;
+  const std::vector<Loc> where = locate(o);
+  terms_eval(o, where, 0);
+  if (!o.bonds.empty()) bond_eval(o, where);
+  terms_eval(o, where, 1);
 }
 
 AdvanceConst advance_const(const Oracle &o) {
@@ -1146,6 +1414,10 @@ void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_o
     }
     epot += ee;
   }
+  auto add64 = [&](int id, int k, double w) { F[3 * (size_t)id + k] += w; };
+  epot += exclusions_eval<double>(o, where, add64);
+  epot += angles_eval<double>(o, where, add64);
+  epot += dihedrals_eval<double>(o, where, add64);
   if (epot_out) *epot_out = epot;
 }
 
@@ -1429,6 +1701,47 @@ int oracle_set_bonds(void *h, int M, const int *bi, const int *bj, const int *po
   o->bonds.clear();
   for (int b = 0; b < M; ++b) o->bonds.push_back(Bond{bi[b], bj[b], pot[b], dissoc[b], true});
   return 0;
+}
+
+int oracle_add_angle_pot(void *h, const float alpha[4], float a, float b, int n, const float *c) {
+  Oracle *o = (Oracle *)h;
+  o->apots.push_back(make_pot(kKindTable, 0, alpha, a, b, 0.0f, n, c, 0, 0, 0));
+  return (int)o->apots.size() - 1;
+}
+
+int oracle_add_dihedral_pot(void *h, const float alpha[4], float a, float b, int n, const float *c) {
+  Oracle *o = (Oracle *)h;
+  o->dpots.push_back(make_pot(kKindTable, 0, alpha, a, b, 0.0f, n, c, 0, 0, 0));
+  return (int)o->dpots.size() - 1;
+}
+
+int oracle_set_angles(void *h, int M, const int *i, const int *j, const int *k, const int *pot) {
+  Oracle *o = (Oracle *)h;
+  o->angles.clear();
+  for (int a = 0; a < M; ++a) o->angles.push_back(Angle{i[a], j[a], k[a], pot[a]});
+  return 0;
+}
+
+int oracle_set_dihedrals(void *h, int M, const int *i, const int *j, const int *k, const int *l, const int *pot) {
+  Oracle *o = (Oracle *)h;
+  o->dihedrals.clear();
+  for (int a = 0; a < M; ++a) o->dihedrals.push_back(Dihedral{i[a], j[a], k[a], l[a], pot[a]});
+  return 0;
+}
+
+int oracle_set_exclusions(void *h, int M, const int *i, const int *j) {
+  Oracle *o = (Oracle *)h;
+  o->exclusions.clear();
+  for (int a = 0; a < M; ++a) o->exclusions.push_back(Exclusion{i[a], j[a]});
+  return 0;
+}
+
+// Energies of the last force evaluation: angles, dihedrals, exclusions.
+void oracle_term_energies(void *h, double *out3) {
+  Oracle *o = (Oracle *)h;
+  out3[0] = o->epot_angle;
+  out3[1] = o->epot_dihedral;
+  out3[2] = o->epot_exclusion;
 }
 
 void oracle_set_integrator(void *h, int kind) { ((Oracle *)h)->integrator = kind; }

@@ -39,6 +39,14 @@ _SIG = {
                               C.POINTER(C.c_int), C.POINTER(C.c_uint16)]),
     "oracle_set_bonds": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "oracle_add_angle_pot": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int,
+                                       C.POINTER(C.c_float)]),
+    "oracle_add_dihedral_pot": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int,
+                                          C.POINTER(C.c_float)]),
+    "oracle_set_angles": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 4),
+    "oracle_set_dihedrals": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 5),
+    "oracle_set_exclusions": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 2),
+    "oracle_term_energies": (None, [C.c_void_p, C.POINTER(C.c_double)]),
     "oracle_step": (C.c_int, [C.c_void_p, C.c_int]),
     "oracle_set_integrator": (None, [C.c_void_p, C.c_int]),
     "oracle_set_single_body": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float)]),
@@ -166,6 +174,30 @@ class Oracle:
         lib().oracle_set_bonds(self._h, M, _ip(i), _ip(j), _ip(p), _dp(d))
         self.M = M
 
+    def set_angles(self, i, j, k, pot):
+        inf, alpha, c = _pot_tables(pot)
+        pidx = lib().oracle_add_angle_pot(self._h, _fp(alpha), inf.a, inf.b, inf.n, _fp(c))
+        ids = [np.ascontiguousarray(a, np.int32) for a in (i, j, k)]
+        p = np.full(len(ids[0]), pidx, np.int32)
+        lib().oracle_set_angles(self._h, len(p), *[_ip(a) for a in ids], _ip(p))
+
+    def set_dihedrals(self, i, j, k, l, pot):
+        inf, alpha, c = _pot_tables(pot)
+        pidx = lib().oracle_add_dihedral_pot(self._h, _fp(alpha), inf.a, inf.b, inf.n, _fp(c))
+        ids = [np.ascontiguousarray(a, np.int32) for a in (i, j, k, l)]
+        p = np.full(len(ids[0]), pidx, np.int32)
+        lib().oracle_set_dihedrals(self._h, len(p), *[_ip(a) for a in ids], _ip(p))
+
+    def set_exclusions(self, i, j):
+        i = np.ascontiguousarray(i, np.int32)
+        j = np.ascontiguousarray(j, np.int32)
+        lib().oracle_set_exclusions(self._h, len(i), _ip(i), _ip(j))
+
+    def term_energies(self):
+        out = np.zeros(3, np.float64)
+        lib().oracle_term_energies(self._h, _dp(out))
+        return {"epot_angle": out[0], "epot_dihedral": out[1], "epot_exclusion": out[2]}
+
     def set_integrator(self, kind):
         """0: forward Euler, 1: Runge-Kutta 4 (engine_advance_runge_kutta_4)."""
         lib().oracle_set_integrator(self._h, int(kind))
@@ -262,5 +294,17 @@ def from_workload(w, m, threads=1, seed=None):
         bp = W.make_potential(m, b["pot"])
         o.set_bonds(b["i"], b["j"], bp, b["dissociation"])
         keep.append(bp)
+    if w.get("exclusions"):
+        o.set_exclusions(w["exclusions"]["i"], w["exclusions"]["j"])
+    if w.get("angles"):
+        a = w["angles"]
+        ap = W.make_potential(m, a["pot"])
+        o.set_angles(a["i"], a["j"], a["k"], ap)
+        keep.append(ap)
+    if w.get("dihedrals"):
+        d = w["dihedrals"]
+        dp = W.make_potential(m, d["pot"])
+        o.set_dihedrals(d["i"], d["j"], d["k"], d["l"], dp)
+        keep.append(dp)
     o._keep = keep
     return o

@@ -1,0 +1,229 @@
+"""Pins the oracle's angles (angle.cpp:92-354), dihedrals
+(dihedral.cpp:90-330) and exclusions (exclusion.cpp:83-260), and the tables
+they evaluate (potential_create_harmonic_angle / _dihedral,
+MxPotential.cpp:428-597):
+
+  * the tables against their closed forms, K (theta - theta0)^2 and
+    K (1 + cos(n phi - delta)), in value and in slope;
+  * the FP64 forces of each term against central differences of the FP64
+    energy (a wrong sign, index or dropped term in a gradient fails this),
+    and the FP32 path against the FP64 one;
+  * the dihedral's cosine against the angle between the two planes computed
+    from numpy's cross products (phi = 180 degrees for the trans form);
+  * an exclusion of an in-cell pair cancels that pair's force and energy
+    exactly, since both use the same table at the same distance;
+  * a right angle takes the random-plane branch: unit gradient direction,
+    equal for both outer particles, and the three forces sum to zero.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mechanica_amd import workloads as W
+
+
+def box(x, types=None, pots=(), L=6.0, cells=3):
+    x = np.asarray(x, np.float64)
+    N = len(x)
+    return {"name": "terms", "dim": [L] * 3, "cells": [cells] * 3, "cutoff": 1.0, "dt": 0.002,
+            "types": [{"mass": 1.0, "radius": 0.1, "dynamics": 0}],
+            "x": x, "v": np.zeros((N, 3)), "type": np.zeros(N, np.int32) if types is None else types,
+            "pots": list(pots), "bonds": None}
+
+
+ANGLE = ("harmonic_angle", (math.radians(50), math.radians(170), 20.0, math.radians(110), 1e-4))
+
+
+def angle_case():
+    # i, j (vertex), k at about 100 degrees, off any cell face
+    return box([[2.3, 2.1, 2.2], [2.55, 2.25, 2.3], [2.6, 2.52, 2.15]])
+
+
+def dihedral_case(delta):
+    w = box([[2.2, 2.1, 2.2], [2.45, 2.25, 2.3], [2.7, 2.2, 2.45], [2.8, 2.45, 2.6]])
+    K, tol = (1.5, 1e-4) if delta == 0.0 else (1.0, 1e-2)
+    w["dihedrals"] = {"i": [0], "j": [1], "k": [2], "l": [3],
+                      "pot": ("harmonic_dihedral", (K, 3, delta, tol))}
+    return w, K
+
+
+def fd_check(m, oracle_mod, w, h=2.0 ** -14, rtol=1e-5):
+    """FP64 forces of the oracle against -dE/dx by central differences of
+    its FP64 energy.  The oracle keeps positions in float, so the positions
+    are rounded to float first and moved by a power of two: every perturbed
+    position is then exact.  A table of many intervals is continuous only
+    to the rounding of its float coefficients, so differences across a knot
+    need a larger step (and bar) than within one polynomial."""
+    w = dict(w)
+    w["x"] = np.asarray(w["x"], np.float32).astype(np.float64)
+    o = oracle_mod.from_workload(w, m)
+    F, _ = o.forces64()
+    x0 = w["x"]
+    G = np.zeros_like(x0)
+    for p in range(len(x0)):
+        for k in range(3):
+            e = []
+            for s in (+1, -1):
+                w2 = dict(w)
+                x = x0.copy()
+                x[p, k] += s * h
+                w2["x"] = x
+                e.append(oracle_mod.from_workload(w2, m).forces64()[1])
+            G[p, k] = -(e[0] - e[1]) / (2 * h)
+    scale = np.abs(G).max()
+    assert scale > 0
+    assert np.abs(F - G).max() <= rtol * scale, (F, G)
+    return o, F
+
+
+def test_angle_table_closed_form(m):
+    K, th0 = 20.0, math.radians(110)
+    p = W.make_potential(m, ANGLE)
+    for th in np.radians(np.linspace(55, 165, 23)):
+        c = math.cos(th)
+        e, f = p.eval_r(np.float32(c))
+        assert abs(e - K * (th - th0) ** 2) <= 1e-3 * max(1.0, K * (th - th0) ** 2)
+        dedc = -2 * K * (th - th0) / math.sin(th)
+        assert abs(f - dedc) <= 2e-3 * max(1.0, abs(dedc))
+
+
+@pytest.mark.parametrize("K,delta,tol", [(1.5, 0.0, 1e-4), (1.5, math.pi, 1e-4), (1.0, math.pi / 4, 1e-2)])
+def test_dihedral_table_closed_form(m, K, delta, tol):
+    """A delta off the multiples of pi adds sqrt(1 - c^2), whose slope is
+    unbounded at the ends: the builder (potential_init's doubling search,
+    capped at 3048 uniform intervals for a zero sixth derivative) meets only
+    coarse tolerances there, so that case is built and checked at 1e-2."""
+    n = 3
+    p = m.Potential.harmonic_dihedral(K, n, delta, tol=tol)
+    for phi in np.radians(np.linspace(5, 175, 35)):
+        c = math.cos(phi)
+        e, f = p.eval_r(np.float32(c))
+        assert abs(e - K * (1 + math.cos(n * phi - delta))) <= 10 * tol * max(1.0, 2 * K)
+        # dE/dc = dE/dphi / (dc/dphi) = K n sin(n phi - delta) / sin(phi)
+        dedc = K * n * math.sin(n * phi - delta) / math.sin(phi)
+        if abs(c) < 0.95:
+            assert abs(f - dedc) <= 50 * tol * max(1.0, abs(dedc))
+
+
+def test_dihedral_table_fails_like_reference(m):
+    """K 1.5, n 3, delta pi/4 at 1e-2 exceeds the interval cap: the factory
+    reports potential_err_ivalsmax (-5), as potential_init does."""
+    with pytest.raises(ValueError, match="-5"):
+        m.Potential.harmonic_dihedral(1.5, 3, math.pi / 4, tol=1e-2)
+
+
+def test_angle_forces_fd(m, oracle_mod):
+    w = angle_case()
+    w["angles"] = {"i": [0], "j": [1], "k": [2], "pot": ANGLE}
+    o, F = fd_check(m, oracle_mod, w)
+    o.force()
+    f = o.state()["f"].astype(np.float64)
+    assert np.abs(f - F).max() <= 1e-5 * np.abs(F).max()
+    assert np.abs(f.sum(0)).max() <= 1e-5 * np.abs(F).max()
+    # energy: the table at cos(theta)
+    x = np.asarray(w["x"])
+    a, b = x[0] - x[1], x[2] - x[1]
+    th = math.acos(a @ b / np.linalg.norm(a) / np.linalg.norm(b))
+    assert abs(o.term_energies()["epot_angle"] - 20.0 * (th - math.radians(110)) ** 2) <= 1e-3
+
+
+@pytest.mark.parametrize("delta", [0.0, math.pi / 4])
+def test_dihedral_forces_fd(m, oracle_mod, delta):
+    w, K = dihedral_case(delta)
+    o, F = fd_check(m, oracle_mod, w, *((2.0 ** -14, 1e-5) if delta == 0.0 else (2.0 ** -9, 2e-3)))
+    o.force()
+    f = o.state()["f"].astype(np.float64)
+    assert np.abs(f - F).max() <= 1e-5 * np.abs(F).max()
+    # cos(phi) from the plane normals: energy K (1 + cos(3 phi - delta))
+    x = np.asarray(w["x"])
+    b1, b2, b3 = x[1] - x[0], x[2] - x[1], x[3] - x[2]
+    n1, n2 = np.cross(b1, b2), np.cross(b2, b3)
+    phi = math.acos(n1 @ n2 / np.linalg.norm(n1) / np.linalg.norm(n2))
+    assert abs(o.term_energies()["epot_dihedral"] - K * (1 + math.cos(3 * phi - delta))) <= 1e-3 if delta == 0 \
+        else 2e-2
+
+
+def test_dihedral_trans_is_pi(m, oracle_mod):
+    """Planar chains: trans (i and l on opposite sides of the j-k axis) has
+    cos(phi) = -1, so with delta = 0, n = 1 the energy K (1 + cos(phi)) is
+    zero; cis has cos(phi) = +1 and the energy 2 K."""
+    for ly, expect in ((2.8, 0.0), (2.0, 2.0)):
+        w = box([[2.0, 2.0, 2.5], [2.3, 2.4, 2.5], [2.7, 2.4, 2.5], [2.95, ly, 2.5]])
+        w["dihedrals"] = {"i": [0], "j": [1], "k": [2], "l": [3],
+                          "pot": ("harmonic_dihedral", (1.0, 1, 0.0, 1e-4))}
+        o = oracle_mod.from_workload(w, m)
+        o.force()
+        assert abs(o.term_energies()["epot_dihedral"] - expect) <= 1e-3
+
+
+def test_exclusion_cancels_pair(m, oracle_mod):
+    lj = ("lj126", (0.1, 1.0, 4 * 0.5 * 0.25 ** 12, 4 * 0.5 * 0.25 ** 6, 1e-4), 0, 0)
+    w = box([[2.2, 2.3, 2.4], [2.45, 2.5, 2.35], [3.5, 3.4, 3.3]], pots=[lj])
+    o = oracle_mod.from_workload(w, m)
+    o.force()
+    f_pair = o.state()["f"].copy()
+    assert np.abs(f_pair[0]).max() > 0.1
+    w["exclusions"] = {"i": [0], "j": [1]}
+    o = oracle_mod.from_workload(w, m)
+    o.force()
+    f = o.state()["f"]
+    # particles 0 and 1 share a cell; 2 is outside their cutoff
+    assert np.array_equal(f[:2], np.zeros((2, 3), np.float32))
+    ob, te = o.observables(), o.term_energies()
+    assert ob["epot_nonbond"] + te["epot_exclusion"] == 0.0
+    assert te["epot_exclusion"] > 0  # the pair sits in the LJ well
+
+
+def test_exclusion_across_cells(m, oracle_mod):
+    """An excluded pair across a periodic corner of the box: the exclusion
+    finds the same image as the pair runner (a wrong cell shift would leave
+    the large pair force standing), so forces and energies cancel to
+    rounding, in FP32 and in FP64."""
+    lj = ("lj126", (0.1, 1.0, 4 * 0.5 * 0.25 ** 12, 4 * 0.5 * 0.25 ** 6, 1e-4), 0, 0)
+    w = box([[5.9, 0.05, 2.0], [0.1, 5.95, 2.1]], pots=[lj])
+    o = oracle_mod.from_workload(w, m)
+    o.force()
+    big = np.abs(o.state()["f"]).max()
+    assert big > 0.1
+    w["exclusions"] = {"i": [0], "j": [1]}
+    o = oracle_mod.from_workload(w, m)
+    o.force()
+    assert np.abs(o.state()["f"]).max() <= 1e-5 * big
+    F, e64 = o.forces64()
+    assert np.abs(F).max() <= 1e-9 * big and abs(e64) <= 1e-9
+    enb = o.observables()["epot_nonbond"]
+    assert abs(enb + o.term_energies()["epot_exclusion"]) <= 1e-5 * abs(enb)
+
+
+def test_right_angle_random_plane(m, oracle_mod):
+    w = box([[2.6, 2.3, 2.2], [2.3, 2.3, 2.2], [2.3, 2.6, 2.2]])
+    w["angles"] = {"i": [0], "j": [1], "k": [2], "pot": ANGLE}
+    o = oracle_mod.from_workload(w, m)
+    o.force()
+    f = o.state()["f"].astype(np.float64)
+    # dxi = dxk = unit vector d: f_i = f_k = -eff d, f_j = 2 eff d
+    assert np.allclose(f[0], f[2], rtol=0, atol=1e-6)
+    assert np.abs(f.sum(0)).max() <= 1e-5 * np.abs(f).max()
+    p = W.make_potential(m, ANGLE)
+    _, eff = p.eval_r(np.float32(0.0))
+    assert abs(np.linalg.norm(f[0]) - abs(eff)) <= 1e-5 * abs(eff)
+
+
+@pytest.mark.parametrize("delta,bar", [(0.0, 5e-5), (math.pi / 4, 5e-4)])
+def test_molecules_fp32_against_fp64(m, oracle_mod, delta, bar):
+    """The FP32 path against FP64 on 200 four-bead molecules.  At delta pi/4
+    the dihedral table's slope grows like 1 / sqrt(1 - c^2), so cos(phi)'s
+    FP32 rounding reaches the force at a few 1e-4."""
+    w = W.molecules(nmol=200, seed=3, delta=delta)
+    o = oracle_mod.from_workload(w, m)
+    o.force()
+    f = o.state()["f"].astype(np.float64)
+    F, e64 = o.forces64()
+    n = np.linalg.norm(F, axis=1)
+    rel = (np.linalg.norm(f - F, axis=1) / np.maximum(n, 0.1 * np.sqrt((n ** 2).mean()))).max()
+    assert rel <= bar
+    ob, te = o.observables(), o.term_energies()
+    total = ob["epot_nonbond"] + ob["epot_bond"] + te["epot_angle"] + te["epot_dihedral"] + te["epot_exclusion"]
+    assert abs(total - e64) <= 1e-5 * abs(e64)
+    assert o.step(10) == 0
