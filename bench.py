@@ -9,7 +9,9 @@ migration and the cell-list rebuild, all on the GPU with the state resident
 in HBM.
 
 Other workloads (--workload): lj (argon LJ at density 10, 1M particles),
-lattice (64^3 bonded Morse lattice, overdamped), ensemble (256 cell-sorting
+lattice (64^3 bonded Morse lattice, overdamped), molecules (64^3 four-bead
+chains with bonds, angles, dihedrals and exclusions in an LJ fluid, 1M
+particles), ensemble (256 cell-sorting
 replicas in one engine, compared with one universe of the same total size)
 and adjoint (65,536-particle Morse trajectory of 50 steps, forward and
 backward, compared with the plain step on the same system).
@@ -36,6 +38,11 @@ HBM_PEAK_GBS = 8000.0
 BYTES_NEWTONIAN = 84
 BYTES_OVERDAMPED = 60
 BYTES_PER_BOND = 12
+# Angles, dihedrals and exclusions reach the force kernel as one 16-byte
+# entry per role in the particle's term list (plus a 4-byte list offset per
+# particle); the term kernel that writes them is profiled on its own.
+BYTES_PER_TERM_ENTRY = 16
+BYTES_TERM_OFFSET = 4
 
 # HBM traffic per force-kernel launch measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE on the default workload (DPD, 1M particles),
@@ -126,6 +133,15 @@ def bench_single(args, threads):
         w = W.argon_lj(N=args.n, seed=1)
         cfg = "argon_lj N=%d density=10 LJ 12-6 on [0.275,1] tol=1e-3 cutoff=1 dt=0.01 cells=%dx%dx%d"
         per, bonds = BYTES_NEWTONIAN, 0
+    elif args.workload == "molecules":
+        side = max(2, int(round((args.n // 4) ** (1.0 / 3.0))))
+        w = W.molecules(n=side, seed=8)
+        cfg = ("molecules N=%d: four-bead chains on a lattice of spacing 1.6, per molecule 3 harmonic bonds "
+               "(k=100 r0=0.3), 2 harmonic angles (k=20 theta0=110deg), 1 harmonic dihedral (k=1 n=3 delta=0), "
+               "5 exclusions; LJ 12-6 sigma=0.25 eps=0.5 on [0.1,1] cutoff=1, dt=0.002 cells=%dx%dx%d")
+        per, bonds = BYTES_NEWTONIAN + BYTES_TERM_OFFSET, len(w["bonds"]["i"])
+        per_terms = BYTES_PER_TERM_ENTRY * (3 * len(w["angles"]["i"]) + 4 * len(w["dihedrals"]["i"])
+                                            + 2 * len(w["exclusions"]["i"]))
     else:  # lattice
         w = W.lattice(n=64, seed=3)
         cfg = ("lattice N=%d (64^3) Morse(d=1,a=6,r0=1) cutoff=2 + 774144 harmonic bonds k=100 r0=1 "
@@ -147,7 +163,7 @@ def bench_single(args, threads):
     value = N * args.steps / wall
     log("timed %d steps in %.3f s" % (args.steps, wall))
     force_ms, step_ms = kernel_ms(u, args.steps)
-    bytes_alg = per * N + BYTES_PER_BOND * bonds
+    bytes_alg = per * N + BYTES_PER_BOND * bonds + (per_terms if args.workload == "molecules" else 0)
     roof = roofline(bytes_alg, force_ms, "k_brick (LDS brick force + advance) with its global-memory fallback",
                     MEASURED_TRAFFIC.get(args.workload))
     roof["step_ms_events"] = step_ms
@@ -308,7 +324,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="dpd", choices=["dpd", "lj", "lattice", "ensemble", "adjoint"])
+    ap.add_argument("--workload", default="dpd",
+                    choices=["dpd", "lj", "lattice", "molecules", "ensemble", "adjoint"])
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--integrator", default="euler", choices=["euler", "rk4"],
                     help="forward Euler (the reference default) or Runge-Kutta 4")
@@ -329,7 +346,7 @@ def main():
         copy_calibration(args)
         return
     threads = min(16, os.cpu_count() or 1)
-    if args.workload in ("dpd", "lj", "lattice"):
+    if args.workload in ("dpd", "lj", "lattice", "molecules"):
         out = bench_single(args, threads)
     elif args.workload == "ensemble":
         out = bench_ensemble(args, threads)

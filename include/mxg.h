@@ -223,6 +223,35 @@ int mxg_engine_set_dihedrals(mxg_engine *e, int M, const int32_t *i, const int32
  * exclusion.cpp:83-260); replaces the previous set. */
 int mxg_engine_set_exclusions(mxg_engine *e, int M, const int32_t *i, const int32_t *j);
 
+/* Boundary kinds (BoundaryConditionKind, MxBoundaryConditions.hpp:15-23) and
+ * sides (left, right: x low, high; front, back: y; bottom, top: z). */
+#define MXG_BOUNDARY_VELOCITY (1 << 0)
+#define MXG_BOUNDARY_PERIODIC (1 << 1)
+#define MXG_BOUNDARY_FREESLIP (1 << 2)
+#define MXG_BOUNDARY_POTENTIAL (1 << 3)
+#define MXG_SIDE_LEFT 0
+#define MXG_SIDE_RIGHT 1
+#define MXG_SIDE_FRONT 2
+#define MXG_SIDE_BACK 3
+#define MXG_SIDE_BOTTOM 4
+#define MXG_SIDE_TOP 5
+
+/* MxBoundaryConditions_Init (MxBoundaryConditions.cpp:496-700): the kind of
+ * each of the six sides, with velocity[3 * side] the velocity of a
+ * velocity wall (no-slip is a velocity wall at rest), restore[side] its
+ * restore factor and radius[side] the radius the wall presents to
+ * potentials (NULL: 0, 1, 0).  Both sides of an axis are periodic or
+ * neither.  Walls act in the advance (apply_update_pos_vel,
+ * boundary_eval.hpp:30-119) and, with a potential, in the force phase
+ * (boundary_eval, boundary_eval.hpp:162-221). */
+int mxg_engine_set_boundaries(mxg_engine *e, const int kind[6], const float *velocity, const float *restore,
+                              const float *radius);
+
+/* MxBoundaryCondition::set_potential (MxBoundaryConditions.cpp:771-790): the
+ * potential between a side and particles of a type (a table or DPD
+ * potential); NULL removes it. */
+int mxg_engine_set_boundary_potential(mxg_engine *e, int side, int type, const mxg_potential *p);
+
 /* engine_step (engine.h:628, engine.cpp:1198-1228), nsteps times. */
 int mxg_engine_step(mxg_engine *e, int nsteps);
 

@@ -178,6 +178,13 @@ class Potential:
 FORWARD_EULER = 0
 RUNGE_KUTTA_4 = 1
 
+# BoundaryConditionKind (MxBoundaryConditions.hpp:15-23) and the sides.
+BOUNDARY_VELOCITY = 1 << 0
+BOUNDARY_PERIODIC = 1 << 1
+BOUNDARY_FREESLIP = 1 << 2
+BOUNDARY_POTENTIAL = 1 << 3
+SIDES = ("left", "right", "front", "back", "bottom", "top")
+
 
 class Universe:
     """One engine: a periodic universe (or a batch of them) on the GPU."""
@@ -322,6 +329,38 @@ class Universe:
         self._check(_capi.lib().mxg_engine_set_bonds(self._h, M, _ip(i), _ip(j), arr,
                                                      _dp(d) if d is not None else None))
         self.M = M
+
+    def set_boundaries(self, kinds, velocity=None, restore=None, radius=None):
+        """MxBoundaryConditions: kinds is a dict or sequence over the sides
+        left, right, front, back, bottom, top of BOUNDARY_* bits or the
+        names 'periodic', 'free_slip', 'no_slip', 'potential' (potential
+        walls also reflect, as the reference's 'potential' string does);
+        velocity, restore and radius are per-side dicts or sequences."""
+        names = {"periodic": BOUNDARY_PERIODIC, "free_slip": BOUNDARY_FREESLIP,
+                 "freeslip": BOUNDARY_FREESLIP, "no_slip": BOUNDARY_VELOCITY, "noslip": BOUNDARY_VELOCITY,
+                 "velocity": BOUNDARY_VELOCITY, "potential": BOUNDARY_POTENTIAL | BOUNDARY_FREESLIP}
+
+        def per_side(v, default):
+            if v is None:
+                return [default] * 6
+            if isinstance(v, dict):
+                return [v.get(sd, default) for sd in SIDES]
+            return list(v)
+
+        k = [names[x.lower()] if isinstance(x, str) else int(x) for x in per_side(kinds, BOUNDARY_PERIODIC)]
+        vel = np.ascontiguousarray(np.asarray(per_side(velocity, (0.0, 0.0, 0.0)), np.float32).reshape(6, 3))
+        res = np.ascontiguousarray(per_side(restore, 1.0), np.float32)
+        rad = np.ascontiguousarray(per_side(radius, 0.0), np.float32)
+        karr = np.ascontiguousarray(k, np.int32)
+        self._check(_capi.lib().mxg_engine_set_boundaries(self._h, _ip(karr), _fp(vel), _fp(res), _fp(rad)))
+
+    def set_boundary_potential(self, side, type_id, pot):
+        """MxBoundaryCondition.set_potential for one side ('left'... or 0-5)."""
+        sd = SIDES.index(side) if isinstance(side, str) else int(side)
+        if pot is not None:
+            self._pots.append(pot)
+        self._check(_capi.lib().mxg_engine_set_boundary_potential(
+            self._h, sd, int(type_id), pot._h.value if pot is not None else None))
 
     def _term_pots(self, pots, M):
         if isinstance(pots, Potential):

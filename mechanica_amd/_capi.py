@@ -71,6 +71,8 @@ SIGNATURES = {
     "mxg_engine_load": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_int_p, c_int_p, c_u16_p]),
     "mxg_engine_set_bonds": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, C.POINTER(C.c_void_p),
                                        c_double_p]),
+    "mxg_engine_set_boundaries": (C.c_int, [C.c_void_p, c_int_p, c_float_p, c_float_p, c_float_p]),
+    "mxg_engine_set_boundary_potential": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "mxg_engine_set_angles": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, C.POINTER(C.c_void_p)]),
     "mxg_engine_set_dihedrals": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, c_int_p,
                                            C.POINTER(C.c_void_p)]),

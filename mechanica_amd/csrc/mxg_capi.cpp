@@ -272,6 +272,18 @@ int mxg_engine_set_bonds(mxg_engine *e, int M, const int32_t *i, const int32_t *
   return e->e.set_bonds(M, i, j, pp.data(), dissociation_energy);
 }
 
+int mxg_engine_set_boundaries(mxg_engine *e, const int kind[6], const float *velocity, const float *restore,
+                              const float *radius) {
+  MXG_CHECK(e);
+  if (!kind) return MXG_ERR_NULL;
+  return e->e.set_boundaries(kind, velocity, restore, radius);
+}
+
+int mxg_engine_set_boundary_potential(mxg_engine *e, int side, int type, const mxg_potential *p) {
+  MXG_CHECK(e);
+  return e->e.set_boundary_potential(side, type, p ? &p->p : nullptr);
+}
+
 int mxg_engine_set_angles(mxg_engine *e, int M, const int32_t *i, const int32_t *j, const int32_t *k,
                           const mxg_potential *const *pots) {
   MXG_CHECK(e);

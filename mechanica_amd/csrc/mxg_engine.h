@@ -34,6 +34,15 @@ enum : int {
 
 enum : int { kDynNewtonian = 0, kDynOverdamped = 1 };
 
+// BoundaryConditionKind (MxBoundaryConditions.hpp:15-23).
+enum : int {
+  kBoundaryVelocity = 1 << 0,
+  kBoundaryPeriodic = 1 << 1,
+  kBoundaryFreeslip = 1 << 2,
+  kBoundaryPotential = 1 << 3,
+  kBoundaryActive = kBoundaryFreeslip | kBoundaryVelocity | kBoundaryPotential,
+};
+
 // Particle flags understood by the advance (MxParticle.h PARTICLE_FROZEN_*).
 enum : uint16_t { kFrozenX = 1 << 2, kFrozenY = 1 << 3, kFrozenZ = 1 << 4 };
 
@@ -84,6 +93,15 @@ class Engine {
   int set_dihedrals(int M, const int *i, const int *j, const int *k, const int *l,
                     const Potential *const *pots);
   int set_exclusions(int M, const int *i, const int *j);
+  // The six sides of the box (MxBoundaryConditions; sides 0 left, 1 right,
+  // 2 front, 3 back, 4 bottom, 5 top): kind bits kBoundary*, the wall
+  // velocities (3 per side) and restore factors of velocity walls, and the
+  // radius each wall presents to potentials; null arrays take 0, 1 and 0.
+  // An axis is periodic when both its sides are.
+  int set_boundaries(const int kind[6], const float *vel, const float *restore, const float *radius);
+  // Potential between a side and particles of a type (MxBoundaryCondition::
+  // set_potential); null removes it.
+  int set_boundary_potential(int side, int type, const Potential *p);
   int step(int nsteps);
   int sync();
 
@@ -186,7 +204,18 @@ class Engine {
   std::vector<HostTerm> angles_, dihedrals_, exclusions_;
   std::vector<Potential> apots_, dpots_;
   std::vector<int> prep_;  // replica of each particle id
+  struct WallInfo {
+    int kind = kBoundaryPeriodic;
+    float vel[3] = {0, 0, 0};
+    float restore = 1.0f;
+    float radius = 0.0f;
+  };
+  WallInfo walls_[6];
+  std::vector<Potential> wpots_;                 // wall potentials
+  std::vector<std::pair<int, int>> wpot_key_;   // (side, type) of each
+  int upload_walls();
   int NT_ = 0;             // terms on the device
+  int NTB_ = 0;            // blocks of their energy reduction
 
   int N_ = 0, M_ = 0, ncells_ = 0, ntiles_ = 0;
   int bdim_[3] = {4, 4, 4};

@@ -66,7 +66,23 @@ struct DevReplica {
   double hd[3];
   float bias, dscale;
   int dmaxdist;
-  int pad;
+  // Bits 0-2: axis x, y, z periodic.  Bits 3-8: the left, right, front,
+  // back, bottom, top walls are active (free-slip, velocity or potential).
+  int wall;
+};
+
+enum : int { kWallPeriodicShift = 0, kWallActiveShift = 3 };
+
+// One side of the box (MxBoundaryCondition, MxBoundaryConditions.hpp:25-55):
+// sides 0 left (x low), 1 right, 2 front (y low), 3 back, 4 bottom (z low),
+// 5 top.
+struct DevWall {
+  int kind;
+  float vel[3];
+  float restore;
+  float radius;
+  float normal[3];
+  int pad[3];
 };
 
 // A neighbour-cell offset.  side: 0 same cell, +1 this cell is the
@@ -163,12 +179,18 @@ struct DeviceState {
   double *bdis = nullptr;
   unsigned char *bact[2] = {nullptr, nullptr};
   DevTerm *terms = nullptr;   // grouped by replica, then by kind
-  int *trep = nullptr;        // first term of each replica, nrep + 1
+  int4 *tblk = nullptr;       // energy blocks: replica, kind, first and end term
+  int *trep_blk = nullptr;    // first energy block of each replica, nrep + 1
+  double *tpart = nullptr;    // energy of each block
   int *toff = nullptr;        // per-particle term-list offsets, N + 1
   float4 *tf = nullptr;       // term-list entries: force on the particle
   float *te = nullptr;        // energy of each term
   DevPot *tpots = nullptr;    // angle then dihedral potentials
   double *obs_terms = nullptr;  // per replica: angle, dihedral, exclusion energy
+  DevWall *walls = nullptr;     // the six sides
+  int *wallpot = nullptr;       // 6 x ntypes: index into wpots, -1 none
+  DevPot *wpots = nullptr;      // wall potentials
+  std::vector<DevReplica> drep; // host copy of rep
   double *partials = nullptr;
   double *type_ke = nullptr;       // kinetic energy per type (Berendsen)
   double *type_ke_part = nullptr;  // per-block partial sums of it
@@ -226,6 +248,9 @@ struct StepArgs {
   const unsigned char G *__restrict__ bact_in;
   unsigned char G *__restrict__ bact_out;
   const int G *__restrict__ slot_of;
+  const DevWall G *__restrict__ walls;
+  const int G *__restrict__ wallpot;
+  const DevPot G *__restrict__ wpots;
   const int G *__restrict__ toff;   // term-list offsets (null without terms)
   const float4 G *__restrict__ tf;  // term-list entries
   float dt;
@@ -398,6 +423,34 @@ __device__ __forceinline__ void cell_coords(int lc, const DevReplica &R, int &lx
 
 __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// Neighbour cell coordinate along an axis: wrapped on a periodic axis, -1
+// beyond a wall (space_init creates no cell pair across it,
+// space.cpp:827-888).
+__device__ __forceinline__ int nbr_coord(int a, int n, int periodic) {
+  return periodic ? wrap(a, n) : ((a < 0 || a >= n) ? -1 : a);
+}
+
+// Walls of the cell (lx, ly, lz) as space_init flags them
+// (cell_active_*, space.cpp:700-723): per axis the low wall for the first
+// cell, else the high wall for the last.  Bit i is side i.
+__device__ __forceinline__ int cell_walls(const DevReplica &R, int lx, int ly, int lz) {
+  const int act = R.wall >> kWallActiveShift;
+  int m = 0;
+  if (lx == 0 && (act & 1))
+    m |= 1;
+  else if (lx + 1 == R.cdim[0] && (act & 2))
+    m |= 2;
+  if (ly == 0 && (act & 4))
+    m |= 4;
+  else if (ly + 1 == R.cdim[1] && (act & 8))
+    m |= 8;
+  if (lz == 0 && (act & 16))
+    m |= 16;
+  else if (lz + 1 == R.cdim[2] && (act & 32))
+    m |= 32;
+  return m;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
@@ -563,7 +616,9 @@ __device__ __forceinline__ void for_each_neighbour(const StepArgs &A, const Loop
   const int w0 = 2 * R.span[0] + 1, w1 = 2 * R.span[1] + 1, w2 = 2 * R.span[2] + 1;
   for (int o = 0; o < R.n_off; ++o) {
     const DevOffset off = A.offs[R.off_base + o];
-    int nx = wrap(lx + off.ox, cd0), ny = wrap(ly + off.oy, cd1), nz = wrap(lz + off.oz, cd2);
+    const int nx = nbr_coord(lx + off.ox, cd0, R.wall & 1), ny = nbr_coord(ly + off.oy, cd1, R.wall & 2),
+              nz = nbr_coord(lz + off.oz, cd2, R.wall & 4);
+    if ((nx | ny | nz) < 0) continue;
     int c2 = R.cell_base + (nx * cd1 + ny) * cd2 + nz;
     int qb = A.cstart[c2], qe = A.cstart[c2 + 1];
     if (off.side == 0) {
@@ -750,6 +805,63 @@ __device__ __forceinline__ void bond_forces(const StepArgs &A, const DevReplica 
       fx += wx;
       fy += wy;
       fz += wz;
+    }
+  }
+}
+
+// boundary_eval (boundary_eval.hpp:162-221) for a particle in a cell
+// touching walls m: each active side with a potential for the particle's
+// type acts within the potential's range b, along the axis, from the wall
+// (distance x on a low side, h - x on a high side).  As the reference does,
+// the offset vector is kept from side to side (a later side overwrites only
+// its own axis) and is replaced by normal * a when r^2 < a
+// (boundary_potential_eval_ex, :124-158); DPD walls take the reference's
+// noise-free dpd_boundary_eval (dpd_eval.hpp:71-106) against the wall's
+// velocity.
+__device__ void wall_forces(const StepArgs &A, const DevReplica &R, int m, int ti, float radius, const float4 &xs,
+                            const float4 &vs, float dpd_delta, float &fx, float &fy, float &fz, double &epot) {
+  float dx[3] = {0.0f, 0.0f, 0.0f};
+  const float X[3] = {xs.x, xs.y, xs.z};
+  for (int side = 0; side < 6; ++side) {
+    if (!(m & (1 << side))) continue;
+    const int pidx = A.wallpot[side * A.ntypes + ti];
+    if (pidx < 0) continue;
+    const DevPot P = A.wpots[pidx];
+    const int axis = side >> 1;
+    const bool low = (side & 1) == 0;
+    const float r = low ? X[axis] : (float)(R.hd[axis] - (double)X[axis]);
+    if (!(r <= P.b)) continue;
+    dx[axis] = low ? r : -r;
+    float r2 = r * r;
+    const DevWall W = A.walls[side];
+    if (r2 < P.a) {
+      r2 = P.a * P.a;
+      dx[0] = W.normal[0] * P.a;
+      dx[1] = W.normal[1] * P.a;
+      dx[2] = W.normal[2] * P.a;
+    }
+    if (P.kind == kPotKindDPD) {
+      const float cutoff = P.b;
+      if (r2 > cutoff * cutoff) continue;
+      const float rr = sqrtf(r2);
+      const float ex = dx[0] / rr, ey = dx[1] / rr, ez = dx[2] / rr;
+      const float vx = vs.x - W.vel[0], vy = vs.y - W.vel[1], vz = vs.z - W.vel[2];
+      const float omega_c = 1 - rr / cutoff;
+      const float fc = P.dpd_alpha * omega_c;
+      const float omega_d = omega_c * omega_c;
+      const float fd = -P.dpd_gamma * omega_d * (ex * vx + ey * vy + ez * vz);
+      const float fr = P.dpd_sigma * omega_c * dpd_delta;
+      const float f = fc + fd + fr;
+      fx = fx + f * ex;
+      fy = fy + f * ey;
+      fz = fz + f * ez;
+    } else {
+      float e, f;
+      if (!table_eval_ex(P, A.coef, radius, W.radius, sqrtf(r2), e, f)) continue;
+      fx -= f * dx[0];
+      fy -= f * dx[1];
+      fz -= f * dx[2];
+      epot += (double)e;
     }
   }
 }
@@ -982,21 +1094,41 @@ __global__ void __launch_bounds__(256) k_terms(TermArgs T) {
   T.te[t] = ee;
 }
 
-// Energies of the terms per replica and kind in FP64, in a fixed order: one
-// block per replica (the terms are grouped by replica).
-__global__ void __launch_bounds__(256) k_term_energy(const DevTerm *__restrict__ terms, const float *__restrict__ te,
-                                                     const int *__restrict__ trep, double *__restrict__ out) {
+// Energies of the terms in FP64 and a fixed order.  The terms are grouped
+// by replica and, within it, by kind, and cut into blocks of at most
+// kTermChunk terms of one replica and kind: each block sums its terms
+// (k_term_partial), then one block per replica adds its blocks
+// (k_term_energy), both in a fixed order.
+constexpr int kTermChunk = 4096;
+
+__global__ void __launch_bounds__(256) k_term_partial(const float *__restrict__ te, const int4 *__restrict__ tblk,
+                                                      double *__restrict__ part) {
+  __shared__ double red[256];
+  const int4 B = tblk[blockIdx.x];
+  double acc = 0.0;
+  for (int t = B.z + threadIdx.x; t < B.w; t += 256) acc += (double)te[t];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_term_energy(const int4 *__restrict__ tblk, const double *__restrict__ part,
+                                                     const int *__restrict__ trep_blk, double *__restrict__ out) {
   __shared__ double red[3][256];
   const int r = blockIdx.x;
-  double acc[3] = {0.0, 0.0, 0.0};
-  for (int t = trep[r] + threadIdx.x; t < trep[r + 1]; t += blockDim.x) {
-    const int kind = terms[t].kind;
-    const double e = (double)te[t];
-    acc[0] += kind == kTermAngle ? e : 0.0;
-    acc[1] += kind == kTermDihedral ? e : 0.0;
-    acc[2] += kind == kTermExclusion ? e : 0.0;
+  double e[3] = {0.0, 0.0, 0.0};
+  for (int b = trep_blk[r] + threadIdx.x; b < trep_blk[r + 1]; b += 256) {
+    const int kind = tblk[b].y;
+    const double v = part[b];
+    e[0] += kind == kTermAngle ? v : 0.0;
+    e[1] += kind == kTermDihedral ? v : 0.0;
+    e[2] += kind == kTermExclusion ? v : 0.0;
   }
-  for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = acc[k];
+  for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = e[k];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
@@ -1004,7 +1136,34 @@ __global__ void __launch_bounds__(256) k_term_energy(const DevTerm *__restrict__
     __syncthreads();
   }
   // exclusion_eval subtracts its energies (exclusion.cpp:255).
-  if (threadIdx.x < 3) out[r * 3 + threadIdx.x] = threadIdx.x == 2 ? -red[2][0] : red[threadIdx.x][0];
+  if (threadIdx.x < 3) out[3 * r + threadIdx.x] = threadIdx.x == 2 ? -red[2][0] : red[threadIdx.x][0];
+}
+
+// apply_update_pos_vel (boundary_eval.hpp:30-119): a particle that left a
+// wall cell through an active free-slip or velocity wall is mirrored back
+// (x -> -x on a low side, 2h - x on a high side, scaled by the restitution
+// or the wall's restore factor) and its velocity reflected (free slip:
+// that component negated; velocity wall: v -> 2 U - restore v).  Sides in
+// the reference's order; returns whether any side acted.
+__device__ __forceinline__ bool reflect(const StepArgs &A, const DevReplica &R, int m, float X[3], float V[3]) {
+  bool enforced = false;
+  for (int side = 0; side < 6; ++side) {
+    if (!(m & (1 << side))) continue;
+    const int axis = side >> 1;
+    const bool low = (side & 1) == 0;
+    if (low ? !(X[axis] <= 0.0f) : !((double)X[axis] >= R.hd[axis])) continue;
+    const DevWall W = A.walls[side];
+    if (W.kind & kBoundaryFreeslip) {
+      X[axis] = low ? -X[axis] * 1.0f : (float)(R.hd[axis] - ((double)X[axis] - R.hd[axis]) * 1.0);
+      V[axis] *= -1.0f;
+      enforced = true;
+    } else if (W.kind & kBoundaryVelocity) {
+      X[axis] = low ? -X[axis] * W.restore : (float)(2.0 * R.hd[axis] - (double)(X[axis] * W.restore));
+      for (int k = 0; k < 3; ++k) V[k] = 2.0f * W.vel[k] - V[k] * W.restore;
+      enforced = true;
+    }
+  }
+  return enforced;
 }
 
 // Advance of one particle (cell_advance_forward_euler,
@@ -1057,6 +1216,13 @@ __device__ __forceinline__ bool advance_store(const StepArgs &A, const DevReplic
     }
     if (toofast) {
       if (atomicCAS(&A.err[0], 0, (int)kErrToofast) == 0) A.err[1] = mypid;
+    }
+    if ((delta[0] | delta[1] | delta[2]) && (R.wall >> kWallActiveShift)) {
+      const int wm = cell_walls(R, lx, ly, lz);
+      if (wm && reflect(A, R, wm, X, V)) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) delta[k] = (X[k] >= R.h[k] ? 1 : 0) - (X[k] < 0.0f ? 1 : 0);
+      }
     }
     if (delta[0] | delta[1] | delta[2]) {
 #pragma unroll
@@ -1316,13 +1482,17 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
         cnt[i] = 0;
         if (h < nhalo) {
           const int hi = h / (hd1 * hd2), hj = (h / hd2) % hd1, hk = h % hd2;
-          const int gx = wrap(br.y - sp0 + hi, R.cdim[0]);
-          const int gy = wrap(br.z - sp1 + hj, R.cdim[1]);
-          const int gz = wrap(br.w - sp2 + hk, R.cdim[2]);
-          const int c = R.cell_base + (gx * R.cdim[1] + gy) * R.cdim[2] + gz;
-          const int gb = A.cstart[c];
-          cnt[i] = A.cstart[c + 1] - gb;
-          hgb[h] = gb;
+          const int gx = nbr_coord(br.y - sp0 + hi, R.cdim[0], R.wall & 1);
+          const int gy = nbr_coord(br.z - sp1 + hj, R.cdim[1], R.wall & 2);
+          const int gz = nbr_coord(br.w - sp2 + hk, R.cdim[2], R.wall & 4);
+          if ((gx | gy | gz) >= 0) {
+            const int c = R.cell_base + (gx * R.cdim[1] + gy) * R.cdim[2] + gz;
+            const int gb = A.cstart[c];
+            cnt[i] = A.cstart[c + 1] - gb;
+            hgb[h] = gb;
+          } else {
+            hgb[h] = 0;  // beyond a wall: an empty cell
+          }
         }
         local += cnt[i];
       }
@@ -1428,6 +1598,10 @@ __global__ __launch_bounds__(kBT) void k_brick(const StepArgs *__restrict__ Ap, 
       float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (!passthrough) {
         if (A.random) single_body(A, s, mypid, ti, T, t1, B.rk_stage <= 0, vs, fx, fy, fz, pfs);
+        if (R.wall >> kWallActiveShift) {
+          const int wm = cell_walls(R, lx, ly, lz);
+          if (wm) wall_forces(A, R, wm, ti, T.radius, xs, vs, C.dpd_delta, fx, fy, fz, acc[0]);
+        }
         {
           const LdsNbr nb{hv, kDPD, hstart, hgb, nhalo, A.pid};
           const unsigned dmax = (unsigned)R.dmaxdist;
@@ -1905,6 +2079,10 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
       float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (!passthrough) {
         if (A.random) single_body(A, s, mypid, ti, T, t1, B.rk_stage <= 0, vs, fx, fy, fz, pfs);
+        if (R.wall >> kWallActiveShift) {
+          const int wm = cell_walls(R, lx, ly, lz);
+          if (wm) wall_forces(A, R, wm, ti, T.radius, xs, vs, C.dpd_delta, fx, fy, fz, acc[0]);
+        }
         const GlobalNbr nb{A.v, A.pid};
         for_each_neighbour(A, C, R, s, lx, ly, lz, xs,
                            [&](int q, bool p_is_i, bool self, float dx, float dy, float dz, float r2,
@@ -2061,7 +2239,8 @@ int Engine::init(const ReplicaSpec *specs, int nrep, double cutoff, double dt) {
     return rc;
   for (auto &e : d_->ev)
     if (int rc = check_device(hipEventCreate(&e), "hipEventCreate")) return rc;
-  return build_geometry();
+  if (int rc = build_geometry()) return rc;
+  return upload_walls();
 }
 
 // Offsets, shift tables and per-replica constants.
@@ -2080,6 +2259,7 @@ int Engine::build_geometry() {
       D.span[k] = R.span[k];
     }
     D.cell_base = R.cell_base;
+    D.wall = 7 << kWallPeriodicShift;  // periodic until set_boundaries says otherwise
     for (int c = 0; c < R.ncells; ++c) cell_rep[R.cell_base + c] = (int)r;
     // engine_advance_forward_euler constants (engine_advance.cpp:262-281).
     const float dt = (float)dt_;
@@ -2165,6 +2345,7 @@ int Engine::build_geometry() {
   if (int st = dalloc(d, &d->obs, reps_.size() * kNRed)) return check_device(st, "hipMalloc");
   hipStream_t s = d->stream;
   int rc = 0;
+  d->drep = drep;
   rc |= hipMemcpyAsync(d->rep, drep.data(), drep.size() * sizeof(DevReplica), hipMemcpyHostToDevice, s);
   rc |= hipMemcpyAsync(d->offs, offs.data(), offs.size() * sizeof(DevOffset), hipMemcpyHostToDevice, s);
   rc |= hipMemcpyAsync(d->shifts, shifts.data(), shifts.size() * sizeof(float), hipMemcpyHostToDevice, s);
@@ -2321,6 +2502,12 @@ int Engine::upload_potentials() {
   std::vector<DevPot> dtp;
   for (auto &p : apots_) dtp.push_back(pack(p));
   for (auto &p : dpots_) dtp.push_back(pack(p));
+  std::vector<DevPot> dwp;
+  for (auto &p : wpots_) dwp.push_back(pack(p));
+  const int nt = (int)types_.size();
+  std::vector<int> wallpot(6 * (size_t)nt, -1);
+  for (size_t w = 0; w < wpot_key_.size(); ++w)
+    if (wpot_key_[w].second < nt) wallpot[wpot_key_[w].first * nt + wpot_key_[w].second] = (int)w;
   std::vector<DevType> dt(types_.size());
   for (size_t i = 0; i < types_.size(); ++i) {
     dt[i].mass = types_[i].mass;
@@ -2347,6 +2534,8 @@ int Engine::upload_potentials() {
   rc |= upload(&d->pots, dp);
   rc |= upload(&d->bpots, dbp);
   rc |= upload(&d->tpots, dtp);
+  rc |= upload(&d->wpots, dwp);
+  rc |= upload(&d->wallpot, wallpot);
   rc |= upload(&d->coef, coef);
   rc |= upload(&d->potmat, potmat_);
   rc |= upload(&d->types, dt);
@@ -2663,6 +2852,85 @@ int Engine::set_bonds(int M, const int *bi, const int *bj, const Potential *cons
   return kErrOk;
 }
 
+// MxBoundaryConditions_Init (MxBoundaryConditions.cpp:496-700): the six
+// sides together.  An axis is periodic when both its sides are; a lone
+// periodic side is refused (the reference warns and makes both periodic,
+// check_periodicy).
+int Engine::set_boundaries(const int kind[6], const float *vel, const float *restore, const float *radius) {
+  if (!d_) return fail(kErrNull, "engine not initialised");
+  if (!kind) return reject(kErrNull, "boundary kinds missing");
+  for (int side = 0; side < 6; ++side) {
+    const int k = kind[side];
+    if (k & ~(kBoundaryVelocity | kBoundaryPeriodic | kBoundaryFreeslip | kBoundaryPotential))
+      return reject(kErrRange, "unknown boundary kind bits");
+    if ((k & kBoundaryPeriodic) && (k & kBoundaryActive))
+      return reject(kErrRange, "a periodic side cannot also be a wall");
+  }
+  for (int a = 0; a < 3; ++a)
+    if (((kind[2 * a] & kBoundaryPeriodic) != 0) != ((kind[2 * a + 1] & kBoundaryPeriodic) != 0))
+      return reject(kErrRange, "both sides of an axis must be periodic, or neither");
+  for (int side = 0; side < 6; ++side) {
+    WallInfo &W = walls_[side];
+    W.kind = kind[side];
+    for (int k = 0; k < 3; ++k) W.vel[k] = vel ? vel[3 * side + k] : 0.0f;
+    W.restore = restore ? restore[side] : 1.0f;
+    W.radius = radius ? radius[side] : 0.0f;
+  }
+  return upload_walls();
+}
+
+int Engine::set_boundary_potential(int side, int type, const Potential *p) {
+  if (side < 0 || side > 5) return reject(kErrRange, "boundary side must be 0..5");
+  if (type < 0 || type >= (int)types_.size()) return reject(kErrRange, "type index out of range");
+  if (p && p->kind == kPotKindTable && p->n < 1) return reject(kErrPotential, "empty potential table");
+  for (size_t w = 0; w < wpot_key_.size(); ++w)
+    if (wpot_key_[w] == std::make_pair(side, type)) {
+      wpot_key_.erase(wpot_key_.begin() + w);
+      wpots_.erase(wpots_.begin() + w);
+      break;
+    }
+  if (p) {
+    wpot_key_.push_back({side, type});
+    wpots_.push_back(*p);
+  }
+  potentials_dirty_ = true;
+  return kErrOk;
+}
+
+// Periodic and active-wall bits of every replica, and the side table.
+int Engine::upload_walls() {
+  DeviceState *d = d_;
+  int bits = 0;
+  for (int a = 0; a < 3; ++a)
+    if ((walls_[2 * a].kind & kBoundaryPeriodic) && (walls_[2 * a + 1].kind & kBoundaryPeriodic))
+      bits |= 1 << (kWallPeriodicShift + a);
+  for (int side = 0; side < 6; ++side)
+    if (walls_[side].kind & kBoundaryActive) bits |= 1 << (kWallActiveShift + side);
+  static const float normals[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  std::vector<DevWall> dw(6);
+  for (int side = 0; side < 6; ++side) {
+    std::memset(&dw[side], 0, sizeof(DevWall));
+    dw[side].kind = walls_[side].kind;
+    for (int k = 0; k < 3; ++k) {
+      dw[side].vel[k] = walls_[side].vel[k];
+      dw[side].normal[k] = normals[side][k];
+    }
+    dw[side].restore = walls_[side].restore;
+    dw[side].radius = walls_[side].radius;
+  }
+  for (auto &D : d->drep) D.wall = bits;
+  if (!d->walls)
+    if (int st = dalloc(d, &d->walls, 6)) return check_device(st, "hipMalloc");
+  hipStream_t s = d->stream;
+  int rc = 0;
+  rc |= hipMemcpyAsync(d->walls, dw.data(), 6 * sizeof(DevWall), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->rep, d->drep.data(), d->drep.size() * sizeof(DevReplica), hipMemcpyHostToDevice, s);
+  rc |= hipStreamSynchronize(s);
+  if (rc) return check_device(rc, "boundary upload");
+  args_dirty_ = true;
+  return kErrOk;
+}
+
 // Shared checks of a term's particles: valid, distinct ids in one replica.
 static bool term_ids_ok(const int *ids, int n, int N, const std::vector<int> &prep) {
   for (int a = 0; a < n; ++a) {
@@ -2762,14 +3030,23 @@ int Engine::upload_terms() {
   std::stable_sort(order.begin(), order.end(), [](const Ref &x, const Ref &y) { return x.rep < y.rep; });
   const int NT = (int)order.size();
   const int R = (int)reps_.size();
-  std::vector<int> toff(N_ + 1, 0), trep(R + 1, 0);
+  std::vector<int> toff(N_ + 1, 0);
   for (const Ref &r : order) {
     const HostTerm &h = (*lists[r.kind])[r.idx];
     for (int q = 0; q < roles[r.kind]; ++q) toff[h.id[q] + 1]++;
-    trep[r.rep + 1]++;
   }
   for (int p = 0; p < N_; ++p) toff[p + 1] += toff[p];
-  for (int r = 0; r < R; ++r) trep[r + 1] += trep[r];
+  // Energy blocks: runs of one replica and kind, at most kTermChunk long.
+  std::vector<int4> tblk;
+  std::vector<int> trep_blk(R + 1, 0);
+  for (int t = 0; t < NT;) {
+    int e = t;
+    while (e < NT && e - t < kTermChunk && order[e].rep == order[t].rep && order[e].kind == order[t].kind) ++e;
+    tblk.push_back(make_int4(order[t].rep, kinds[order[t].kind], t, e));
+    trep_blk[order[t].rep + 1]++;
+    t = e;
+  }
+  for (int r = 0; r < R; ++r) trep_blk[r + 1] += trep_blk[r];
   std::vector<int> fill(toff.begin(), toff.end() - 1);
   std::vector<DevTerm> dev(NT);
   for (int t = 0; t < NT; ++t) {
@@ -2788,7 +3065,9 @@ int Engine::upload_terms() {
   DeviceState *d = d_;
   const size_t entries = toff[N_];
   if (int st = dalloc(d, &d->terms, dev.size())) return check_device(st, "hipMalloc");
-  if (int st = dalloc(d, &d->trep, trep.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->tblk, tblk.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->trep_blk, trep_blk.size())) return check_device(st, "hipMalloc");
+  if (int st = dalloc(d, &d->tpart, tblk.size())) return check_device(st, "hipMalloc");
   if (int st = dalloc(d, &d->toff, toff.size())) return check_device(st, "hipMalloc");
   if (int st = dalloc(d, &d->tf, entries)) return check_device(st, "hipMalloc");
   if (int st = dalloc(d, &d->te, dev.size())) return check_device(st, "hipMalloc");
@@ -2800,12 +3079,15 @@ int Engine::upload_terms() {
     rc |= hipMemsetAsync(d->tf, 0, entries * sizeof(float4), s);
     rc |= hipMemsetAsync(d->te, 0, dev.size() * sizeof(float), s);
   }
-  rc |= hipMemcpyAsync(d->trep, trep.data(), trep.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  if (!tblk.empty())
+    rc |= hipMemcpyAsync(d->tblk, tblk.data(), tblk.size() * sizeof(int4), hipMemcpyHostToDevice, s);
+  rc |= hipMemcpyAsync(d->trep_blk, trep_blk.data(), trep_blk.size() * sizeof(int), hipMemcpyHostToDevice, s);
   rc |= hipMemcpyAsync(d->toff, toff.data(), toff.size() * sizeof(int), hipMemcpyHostToDevice, s);
   rc |= hipMemsetAsync(d->obs_terms, 0, 3 * (size_t)R * sizeof(double), s);
   rc |= hipStreamSynchronize(s);
   if (rc) return check_device(rc, "term upload");
   NT_ = NT;
+  NTB_ = (int)tblk.size();
   args_dirty_ = true;
   return kErrOk;
 }
@@ -2872,6 +3154,9 @@ void Engine::fill_step_args(void *out, int par) const {
   A.bact_in = (decltype(A.bact_in))(M_ > 0 ? d->bact[par] : nullptr);
   A.bact_out = (decltype(A.bact_out))(M_ > 0 ? d->bact[par ^ 1] : nullptr);
   A.slot_of = (decltype(A.slot_of))(d->slot_of);
+  A.walls = (decltype(A.walls))(d->walls);
+  A.wallpot = (decltype(A.wallpot))(d->wallpot);
+  A.wpots = (decltype(A.wpots))(d->wpots);
   A.toff = (decltype(A.toff))(NT_ > 0 ? d->toff : nullptr);
   A.tf = (decltype(A.tf))(NT_ > 0 ? d->tf : nullptr);
   A.dt = (decltype(A.dt))((float)dt_);
@@ -2960,7 +3245,8 @@ int Engine::launch_substep(int rk_stage, int tick) {
     T.tf = d->tf;
     T.te = d->te;
     hipLaunchKernelGGL(k_terms, dim3((NT_ + 255) / 256), dim3(256), 0, s, T);
-    hipLaunchKernelGGL(k_term_energy, dim3(reps_.size()), dim3(256), 0, s, d->terms, d->te, d->trep,
+    hipLaunchKernelGGL(k_term_partial, dim3(NTB_), dim3(256), 0, s, d->te, d->tblk, d->tpart);
+    hipLaunchKernelGGL(k_term_energy, dim3(reps_.size()), dim3(256), 0, s, d->tblk, d->tpart, d->trep_blk,
                        d->obs_terms);
   }
   const bool rec = timing_ && timed_steps_ < (int)d->kev.size() / 2;

@@ -17,13 +17,14 @@ parameters so tests can build the same workload at small N.
              lattice of src/lattice.py:449-522; 3 n^2 (n-1) bonds), K = 100,
              r0 = 1, dissociation energy 5, Morse(d=1, a=6, r0=1, max=2)
              between all pairs, cutoff 2, overdamped, dt = 0.005.
-  molecules  four-bead chains (bond 0.3, angles near 110 deg, random
-             dihedrals) with harmonic bonds 1-2, 2-3, 3-4 (K 100, r0 0.3),
+  molecules  n^3 four-bead chains (bond 0.3, angles near 110 deg, random
+             dihedrals) centred on a cubic lattice of spacing 1.6 (jitter
+             U(+-0.15)), with harmonic bonds 1-2, 2-3, 3-4 (K 100, r0 0.3),
              harmonic angles 1-2-3, 2-3-4 (K 20, theta0 110 deg, defined on
              theta0 +- 60 deg), a harmonic dihedral 1-2-3-4 (K 1, n 3,
              delta 0 at tolerance 1e-4 or pi/4 at 1e-2), exclusions of the 1-2 and 1-3 pairs, and LJ
              12-6 (sigma 0.25, eps 0.5, table on [0.1, 1]) between all
-             particles; molecule centres at least 1.25 apart, cutoff 1,
+             particles; cutoff 1,
              Newtonian, mass 1, dt 0.002 (the bonded set of the reference's
              flexible-water example, examples/flexible/flexible.cpp:304-336,
              on a non-Coulomb pair potential).
@@ -151,36 +152,38 @@ def lattice(n=64, seed=3, jitter=0.05, dissociation=5.0):
 
 
 def _unit(v):
-    return v / np.linalg.norm(v)
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
 
 
-def _chain(rng, bond=0.3, theta0=110.0, spread=15.0):
-    """Four points: bond lengths `bond`, both angles theta0 +- spread degrees,
-    a uniform dihedral; centred on the origin."""
-    u1 = _unit(rng.normal(size=3))
-    pts = [np.zeros(3), bond * u1]
+def _chains(rng, M, bond=0.3, theta0=110.0, spread=15.0):
+    """M chains of four points: bond lengths `bond`, both angles theta0 +-
+    spread degrees, uniform dihedrals; each centred on the origin."""
+    u1 = _unit(rng.normal(size=(M, 3)))
+    pts = [np.zeros((M, 3)), bond * u1]
     prev = u1
-    ref = _unit(np.cross(u1, rng.normal(size=3)))
+    ref = _unit(np.cross(u1, rng.normal(size=(M, 3))))
     for _ in range(2):
-        th = math.radians(theta0 + rng.uniform(-spread, spread))
-        phi = rng.uniform(0, 2 * math.pi)
-        a = ref
-        b = np.cross(prev, a)
-        # direction making the angle th with -prev at the last point
-        d = -math.cos(th) * prev + math.sin(th) * (math.cos(phi) * a + math.sin(phi) * b)
+        th = np.radians(theta0 + rng.uniform(-spread, spread, size=(M, 1)))
+        phi = rng.uniform(0, 2 * math.pi, size=(M, 1))
+        b = np.cross(prev, ref)
+        # the direction making the angle th with -prev at the last point
+        d = -np.cos(th) * prev + np.sin(th) * (np.cos(phi) * ref + np.sin(phi) * b)
         pts.append(pts[-1] + bond * d)
         ref = _unit(np.cross(d, prev))
         prev = d
-    P = np.array(pts)
-    return P - P.mean(0)
+    P = np.stack(pts, 1)
+    return P - P.mean(1, keepdims=True)
 
 
-def molecules(nmol=2000, seed=8, delta=0.0, dt=0.002):
+def molecules(n=8, seed=8, delta=0.0, dt=0.002):
     rng = _rng(seed)
-    L = (nmol * 4.0) ** (1.0 / 3.0)
+    nmol = n ** 3
+    spacing = 1.6
+    L = n * spacing
     cells = max(3, int(math.floor(L / 1.0)))
-    centres = _min_dist_fill(rng, nmol, L, 1.25)
-    x = np.concatenate([c + _chain(rng) for c in centres]) % L
+    g = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    centres = (g + 0.5) * spacing + rng.uniform(-0.15, 0.15, size=(nmol, 3))
+    x = ((centres[:, None, :] + _chains(rng, nmol)) % L).reshape(-1, 3)
     N = 4 * nmol
     v = rng.normal(0.0, 0.5, size=(N, 3))
     base = 4 * np.arange(nmol, dtype=np.int32)
@@ -359,6 +362,17 @@ def build_ensemble(m, e, seed=None):
     return u
 
 
+def boundary_arrays(b):
+    """(kinds[6], velocity[6x3], restore[6], radius[6]) from a workload's
+    "boundaries" entry: {"kinds": [...], "velocity": ..., "restore": ...,
+    "radius": ...} over the sides left, right, front, back, bottom, top."""
+    kinds = np.asarray(b["kinds"], np.int32)
+    vel = np.asarray(b.get("velocity", np.zeros((6, 3))), np.float32).reshape(6, 3)
+    res = np.asarray(b.get("restore", np.ones(6)), np.float32)
+    rad = np.asarray(b.get("radius", np.zeros(6)), np.float32)
+    return kinds, vel, res, rad
+
+
 def make_potential(m, spec):
     """Builds a Potential from a (kind, args) spec with the given module."""
     kind, args = spec[0], spec[1]
@@ -391,6 +405,13 @@ def build_universe(m, w, seed=None):
     for spec in w["pots"]:
         p = make_potential(m, spec)
         u.bind(p, spec[2], spec[3])
+        pots.append(p)
+    if w.get("boundaries"):
+        k, vel, res, rad = boundary_arrays(w["boundaries"])
+        u.set_boundaries([int(x) for x in k], vel, res, rad)
+    for side, t, spec in w.get("wall_pots") or ():
+        p = make_potential(m, spec)
+        u.set_boundary_potential(side, t, p)
         pots.append(p)
     if seed is not None:
         u.set_seed(seed)

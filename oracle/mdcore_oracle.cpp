@@ -29,6 +29,10 @@
 //   angle_eval (cos theta, potential_eval_r) ........ angle.cpp:92-354
 //   dihedral_eval (cos phi, potential_eval_r) ....... dihedral.cpp:90-330
 //   bonded evaluation order ......................... engine_bonded.cpp:1510-1538
+//   boundary conditions: cell flags, pairs ........... space.cpp:700-723, 827-888
+//   boundary_eval (wall potentials) ................. boundary_eval.hpp:124-221
+//   dpd_boundary_eval ............................... dpd_eval.hpp:71-106
+//   apply_update_pos_vel (wall reflection) .......... boundary_eval.hpp:30-119
 //   cell_advance_forward_euler ...................... engine_advance.cpp:126-239
 //   engine_advance_forward_euler .................... engine_advance.cpp:248-422
 //   engine_advance_runge_kutta_4 .................... engine_advance.cpp:443-698
@@ -84,6 +88,9 @@ enum : uint32_t { kKindTable = 0, kKindDPD = 1 };
 enum : uint16_t { kFrozenX = 1 << 2, kFrozenY = 1 << 3, kFrozenZ = 1 << 4 };
 enum { kNewtonian = 0, kOverdamped = 1 };
 enum { kTaskSort = 0, kTaskSelf = 1, kTaskPair = 2 };
+
+// BoundaryConditionKind (MxBoundaryConditions.hpp:15-23).
+enum { kBcVelocity = 1, kBcPeriodic = 2, kBcFreeslip = 4, kBcPotential = 8, kBcActive = 1 | 4 | 8 };
 constexpr int kErrToofast = -29;
 constexpr int kErrRange = -6;
 
@@ -155,6 +162,7 @@ struct Part {
 
 struct Cell {
   int loc[3];
+  int walls = 0;  // cell_active_* as bit i for side i (left, right, front, back, bottom, top)
   double origin[3];
   std::vector<Part> parts;
   std::vector<Part> incoming;
@@ -224,6 +232,19 @@ struct Oracle {
 
   double ekin = 0, epot_nb = 0, epot_bond = 0;
   double epot_angle = 0, epot_dihedral = 0, epot_exclusion = 0;
+
+  // Boundary conditions (MxBoundaryConditions): per side the kind bits,
+  // wall velocity, restore factor and radius; wall potentials by side and
+  // type.  The space is rebuilt when they change.
+  int bc_kind[6] = {2, 2, 2, 2, 2, 2};
+  float bc_vel[6][3] = {};
+  float bc_restore[6] = {1, 1, 1, 1, 1, 1};
+  float bc_radius[6] = {};
+  bool periodic[3] = {true, true, true};
+  std::vector<Pot> wpots;
+  std::vector<int> wpot_side, wpot_type;
+  double init_origin[3] = {}, init_dim[3] = {};
+  int init_cells[3] = {};
   float dpd_delta = 0;
   bool dpd_delta_set = false;
 
@@ -256,6 +277,11 @@ int getsid(const Oracle &o, int &ci, int &cj, float shift[3]) {
 
 int init_space(Oracle &o, const double origin[3], const double dim[3], const int cells_in[3],
                double cutoff) {
+  for (int k = 0; k < 3; ++k) {
+    o.init_origin[k] = origin[k];
+    o.init_dim[k] = dim[k];
+    o.init_cells[k] = cells_in[k];
+  }
   int cells[3] = {cells_in[0], cells_in[1], cells_in[2]};
   for (int k = 0; k < 3; ++k)
     if (cells[k] < 3) cells[k] = 3;  // periodic needs >= 3 cells
@@ -288,6 +314,15 @@ int init_space(Oracle &o, const double origin[3], const double dim[3], const int
           c.loc[k] = l[k];
           c.origin[k] = org[k];
         }
+        // cell_active_* flags: the low side for the first cell of an axis,
+        // else the high side for the last.
+        c.walls = 0;
+        for (int a = 0; a < 3; ++a) {
+          if (l[a] == 0 && (o.bc_kind[2 * a] & kBcActive))
+            c.walls |= 1 << (2 * a);
+          else if (l[a] + 1 == o.cdim[a] && (o.bc_kind[2 * a + 1] & kBcActive))
+            c.walls |= 1 << (2 * a + 1);
+        }
       }
     }
   }
@@ -301,15 +336,21 @@ int init_space(Oracle &o, const double origin[3], const double dim[3], const int
     for (int j = 0; j < o.cdim[1]; j++)
       for (int k = 0; k < o.cdim[2]; k++) {
         int id1 = o.cellid(i, j, k);
+        // Neighbours across a non-periodic side are not paired.
+        auto nbr = [&](int a, int v) { return o.periodic[a] ? (v + o.cdim[a]) % o.cdim[a] : v; };
+        auto off = [&](int a, int v) { return !o.periodic[a] && (v < 0 || v >= o.cdim[a]); };
         for (l[0] = -o.span[0]; l[0] <= o.span[0]; l[0]++) {
-          int ii = (i + l[0] + o.cdim[0]) % o.cdim[0];
+          if (off(0, i + l[0])) continue;
+          int ii = nbr(0, i + l[0]);
           for (l[1] = -o.span[1]; l[1] <= o.span[1]; l[1]++) {
-            int jj = (j + l[1] + o.cdim[1]) % o.cdim[1];
+            if (off(1, j + l[1])) continue;
+            int jj = nbr(1, j + l[1]);
             for (l[2] = -o.span[2]; l[2] <= o.span[2]; l[2]++) {
               double lh[3];
               for (int q = 0; q < 3; ++q) lh[q] = o.h[q] * std::fmax(std::abs(l[q]) - 1, 0);
               if (lh[0] * lh[0] + lh[1] * lh[1] + lh[2] * lh[2] > o.cutoff2) continue;
-              int kk = (k + l[2] + o.cdim[2]) % o.cdim[2];
+              if (off(2, k + l[2])) continue;
+              int kk = nbr(2, k + l[2]);
               int id2 = o.cellid(ii, jj, kk);
               int ci = id1, cj = id2;
               float sh[3];
@@ -701,6 +742,119 @@ void single_body(Oracle &o, Part *p) {
   }
 }
 
+// potential_eval_ex in FP64 (the FP64 statement of the table potentials).
+bool table_ex64(const Pot &p, double ri, double rj, double r2, double *e, double *f) {
+  double r = std::sqrt(r2);
+  double ro = r < FLT_EPSILON ? FLT_EPSILON : r;
+  r = r < p.a ? p.a : r;
+  if (p.flags & kFlagScaled)
+    r = r / (ri + rj);
+  else if (p.flags & kFlagShifted)
+    r = r - (ri + rj) + p.r0_plusone;
+  double indf = std::max(0.0, p.alpha[0] + r * (p.alpha[1] + r * (double)p.alpha[2]));
+  unsigned ind = (unsigned)indf;
+  if (r > p.b || ind > (unsigned)p.n) return false;
+  const float *c = &p.c[(size_t)ind * kChunk];
+  double x = (r - c[0]) * (double)c[1];
+  double ee = c[2] * x + c[3], eff = c[2];
+  for (int k = 4; k < kChunk; ++k) {
+    eff = eff * x + ee;
+    ee = ee * x + c[k];
+  }
+  *e = ee;
+  *f = eff * c[1] / ro;
+  return true;
+}
+
+// boundary_eval / boundary_potential_eval_ex / dpd_boundary_eval: the
+// offset vector dx persists from side to side, and r^2 < a (not a^2) moves
+// the particle to the normal times a, as in the reference.  add(k, w) adds
+// w to the particle's force component k; the energy goes to *epot.
+template <typename T, typename Add>
+void boundary_eval(const Oracle &o, const Cell &c, const Part &p, T *epot, Add add) {
+  static const float normals[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  T dx[3] = {0, 0, 0};
+  for (int side = 0; side < 6; ++side) {
+    if (!(c.walls & (1 << side))) continue;
+    int pidx = -1;
+    for (size_t w = 0; w < o.wpots.size(); ++w)
+      if (o.wpot_side[w] == side && o.wpot_type[w] == p.type) pidx = (int)w;
+    if (pidx < 0) continue;
+    const Pot &pot = o.wpots[pidx];
+    const int axis = side >> 1;
+    const bool low = (side & 1) == 0;
+    const T r = low ? (T)p.x[axis] : (T)(o.h[axis] - p.x[axis]);
+    if (!(r <= (T)pot.b)) continue;
+    dx[axis] = low ? r : -r;
+    T r2 = r * r;
+    if (r2 < (T)pot.a) {
+      r2 = (T)pot.a * (T)pot.a;
+      for (int k = 0; k < 3; ++k) dx[k] = (T)normals[side][k] * (T)pot.a;
+    }
+    if (pot.kind == kKindDPD) {
+      const T cutoff = pot.b;
+      if (r2 > cutoff * cutoff) continue;
+      const T rr = std::sqrt(r2);
+      const T e[3] = {dx[0] / rr, dx[1] / rr, dx[2] / rr};
+      const T v[3] = {(T)p.v[0] - (T)o.bc_vel[side][0], (T)p.v[1] - (T)o.bc_vel[side][1],
+                      (T)p.v[2] - (T)o.bc_vel[side][2]};
+      const T omega_c = 1 - rr / cutoff;
+      const T fc = (T)pot.dpd_alpha * omega_c;
+      const T omega_d = omega_c * omega_c;
+      const T fd = -(T)pot.dpd_gamma * omega_d * (e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
+      const T fr = (T)pot.dpd_sigma * omega_c * (T)o.dpd_delta;
+      const T f = fc + fd + fr;
+      for (int k = 0; k < 3; ++k) add(k, f * e[k]);
+    } else {
+      float ee, eff;
+      if (sizeof(T) == 4) {
+        if (!eval_ex(pot, p.radius, o.bc_radius[side], (float)r2, &ee, &eff)) continue;
+        for (int k = 0; k < 3; ++k) add(k, -(T)eff * dx[k]);
+        *epot += (T)ee;
+      } else {
+        double e64, f64;
+        if (!table_ex64(pot, p.radius, o.bc_radius[side], (double)r2, &e64, &f64)) continue;
+        for (int k = 0; k < 3; ++k) add(k, -(T)f64 * dx[k]);
+        *epot += (T)e64;
+      }
+    }
+  }
+}
+
+// apply_update_pos_vel for a particle that left its cell: reflection at
+// active free-slip or velocity sides of its cell; true when it then stays.
+bool apply_update_pos_vel(const Oracle &o, const Cell &c, Part *p, const float h[3], int delta[3]) {
+  bool enforced = false;
+  for (int side = 0; side < 6; ++side) {
+    if (!(c.walls & (1 << side))) continue;
+    const int i = side >> 1;
+    const bool low = (side & 1) == 0;
+    const double dim = o.h[i];
+    if (low ? !(p->x[i] <= 0) : !(p->x[i] >= dim)) continue;
+    const int kind = o.bc_kind[side];
+    if (kind & kBcFreeslip) {
+      const float restitution = 1.0f;
+      if (low)
+        p->x[i] = -p->x[i] * restitution;
+      else
+        p->x[i] = dim - (p->x[i] - dim) * restitution;
+      p->v[i] *= -restitution;
+      enforced = true;
+    } else if (kind & kBcVelocity) {
+      const float restore = o.bc_restore[side];
+      if (low)
+        p->x[i] = -p->x[i] * restore;
+      else
+        p->x[i] = 2.f * dim - (p->x[i] * restore);
+      for (int k = 0; k < 3; ++k) p->v[k] = 2.f * o.bc_vel[side][k] - (p->v[k] * restore);
+      enforced = true;
+    }
+  }
+  if (enforced)
+    for (int k = 0; k < 3; k++) delta[k] = std::isgreaterequal(p->x[k], h[k]) - std::isless(p->x[k], 0.0);
+  return delta[0] == 0 && delta[1] == 0 && delta[2] == 0;
+}
+
 void do_self(Oracle &o, int cid) {
   Cell &c = o.cells[cid];
   int count = (int)c.parts.size();
@@ -714,6 +868,7 @@ void do_self(Oracle &o, int cid) {
     pix[1] = part_i->x[1];
     pix[2] = part_i->x[2];
     single_body(o, part_i);
+    if (c.walls) boundary_eval<float>(o, c, *part_i, &epot, [&](int k, float w) { part_i->f[k] += w; });
     for (int j = i + 1; j < count; j++) {
       Part *part_j = &c.parts[j];
       float r2 = fr2(pix, part_j->x, dx);
@@ -1113,7 +1268,10 @@ void advance_cell(Oracle &o, const AdvanceConst &K, int cid) {
       o.err = kErrToofast;
       o.err_pid = p->id;
     }
-    if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
+    if ((delta[0] != 0 || delta[1] != 0 || delta[2] != 0) && c.walls &&
+        apply_update_pos_vel(o, c, p, K.h, delta)) {
+      pid += 1;
+    } else if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0) {
       for (int k = 0; k < 3; k++) p->x[k] -= delta[k] * K.h[k];
       int dest = o.cellid((c.loc[0] + delta[0] + o.cdim[0]) % o.cdim[0],
                           (c.loc[1] + delta[1] + o.cdim[1]) % o.cdim[1],
@@ -1268,26 +1426,7 @@ void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_o
   std::fill(F, F + 3 * (size_t)o.nparts, 0.0);
   double epot = 0;
   auto table64 = [](const Pot &p, double ri, double rj, double r2, double *e, double *f) {
-    double r = std::sqrt(r2);
-    double ro = r < FLT_EPSILON ? FLT_EPSILON : r;
-    r = r < p.a ? p.a : r;
-    if (p.flags & kFlagScaled)
-      r = r / (ri + rj);
-    else if (p.flags & kFlagShifted)
-      r = r - (ri + rj) + p.r0_plusone;
-    double indf = std::max(0.0, p.alpha[0] + r * (p.alpha[1] + r * (double)p.alpha[2]));
-    unsigned ind = (unsigned)indf;
-    if (r > p.b || ind > (unsigned)p.n) return false;
-    const float *c = &p.c[(size_t)ind * kChunk];
-    double x = (r - c[0]) * (double)c[1];
-    double ee = c[2] * x + c[3], eff = c[2];
-    for (int k = 4; k < kChunk; ++k) {
-      eff = eff * x + ee;
-      ee = ee * x + c[k];
-    }
-    *e = ee;
-    *f = eff * c[1] / ro;
-    return true;
+    return table_ex64(p, ri, rj, r2, e, f);
   };
   auto pair64 = [&](const Part &pi, const Part &pj, const double dx[3], double r2) {
     int pot = o.potmat[pi.type * o.types.size() + pj.type];
@@ -1378,6 +1517,13 @@ void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_o
         for (int k = 0; k < 3; ++k) Fp[k] += -(double)t.sb_coef * v2 * p.v[k] + p.pf[k];
       }
     }
+  // walls
+  for (const Cell &c : o.cells)
+    if (c.walls)
+      for (const Part &p : c.parts) {
+        double *Fp = F + 3 * (size_t)p.id;
+        boundary_eval<double>(o, c, p, &epot, [&](int k, double w) { Fp[k] += w; });
+      }
   // bonds
   std::vector<Loc> where = locate(o);
   for (const Bond &b : o.bonds) {
@@ -1658,6 +1804,38 @@ int oracle_add_pot(void *h, int ti, int tj, uint32_t kind, uint32_t flags, const
   int nt = (int)o->types.size();
   o->potmat[ti * nt + tj] = (int)o->pots.size() - 1;
   o->potmat[tj * nt + ti] = (int)o->pots.size() - 1;
+  return 0;
+}
+
+// The six sides (MxBoundaryConditions_Init); rebuilds the space, so it
+// must come before the particles are loaded.
+int oracle_set_boundaries(void *h, const int kind[6], const float *vel, const float *restore, const float *radius) {
+  Oracle *o = (Oracle *)h;
+  if (o->nparts > 0) return kErrRange;
+  for (int side = 0; side < 6; ++side) {
+    o->bc_kind[side] = kind[side];
+    for (int k = 0; k < 3; ++k) o->bc_vel[side][k] = vel ? vel[3 * side + k] : 0.0f;
+    o->bc_restore[side] = restore ? restore[side] : 1.0f;
+    o->bc_radius[side] = radius ? radius[side] : 0.0f;
+  }
+  for (int a = 0; a < 3; ++a) o->periodic[a] = (o->bc_kind[2 * a] & kBcPeriodic) != 0;
+  double origin[3], dim[3];
+  int cells[3];
+  for (int k = 0; k < 3; ++k) {
+    origin[k] = o->init_origin[k];
+    dim[k] = o->init_dim[k];
+    cells[k] = o->init_cells[k];
+  }
+  return init_space(*o, origin, dim, cells, o->cutoff);
+}
+
+int oracle_set_boundary_pot(void *h, int side, int type, uint32_t kind, uint32_t flags, const float alpha[4], float a,
+                            float b, float r0_plusone, int n, const float *c, float dpd_alpha, float dpd_gamma,
+                            float dpd_sigma) {
+  Oracle *o = (Oracle *)h;
+  o->wpots.push_back(make_pot(kind, flags, alpha, a, b, r0_plusone, n, c, dpd_alpha, dpd_gamma, dpd_sigma));
+  o->wpot_side.push_back(side);
+  o->wpot_type.push_back(type);
   return 0;
 }
 

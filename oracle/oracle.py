@@ -39,6 +39,11 @@ _SIG = {
                               C.POINTER(C.c_int), C.POINTER(C.c_uint16)]),
     "oracle_set_bonds": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "oracle_set_boundaries": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                        C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "oracle_set_boundary_pot": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint32,
+                                          C.POINTER(C.c_float), C.c_float, C.c_float, C.c_float, C.c_int,
+                                          C.POINTER(C.c_float), C.c_float, C.c_float, C.c_float]),
     "oracle_add_angle_pot": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int,
                                        C.POINTER(C.c_float)]),
     "oracle_add_dihedral_pot": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int,
@@ -151,6 +156,22 @@ class Oracle:
         cp = _fp(c) if c.size else None
         lib().oracle_add_pot(self._h, int(ti), int(tj), inf.kind, inf.flags, _fp(alpha), inf.a, inf.b,
                              inf.r0_plusone, inf.n, cp, inf.dpd_alpha, inf.dpd_gamma, inf.dpd_sigma)
+
+    def set_boundaries(self, kinds, velocity, restore, radius):
+        """Six sides: kind bits, velocities (6 x 3), restore and radius."""
+        k = np.ascontiguousarray(kinds, np.int32)
+        vel = np.ascontiguousarray(velocity, np.float32).reshape(-1)
+        res = np.ascontiguousarray(restore, np.float32)
+        rad = np.ascontiguousarray(radius, np.float32)
+        rc = lib().oracle_set_boundaries(self._h, _ip(k), _fp(vel), _fp(res), _fp(rad))
+        if rc:
+            raise ValueError("oracle boundaries failed %d" % rc)
+
+    def set_boundary_potential(self, side, ti, pot):
+        inf, alpha, c = _pot_tables(pot)
+        cp = _fp(c) if c.size else None
+        lib().oracle_set_boundary_pot(self._h, int(side), int(ti), inf.kind, inf.flags, _fp(alpha), inf.a, inf.b,
+                                      inf.r0_plusone, inf.n, cp, inf.dpd_alpha, inf.dpd_gamma, inf.dpd_sigma)
 
     def load(self, x, v, types, flags=None):
         x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
@@ -273,6 +294,13 @@ def from_workload(w, m, threads=1, seed=None):
     for spec in w["pots"]:
         p = W.make_potential(m, spec)
         o.bind(p, spec[2], spec[3])
+        keep.append(p)
+    if w.get("boundaries"):
+        b = W.boundary_arrays(w["boundaries"])
+        o.set_boundaries(*b)
+    for side, t, spec in w.get("wall_pots") or ():
+        p = W.make_potential(m, spec)
+        o.set_boundary_potential(side, t, p)
         keep.append(p)
     if seed is not None:
         o.set_seed(seed)

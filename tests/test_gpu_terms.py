@@ -27,7 +27,7 @@ def rel_err(fg, fr):
 
 @pytest.mark.parametrize("delta", [0.0, math.pi / 4])
 def test_terms_match_oracle(m, oracle_mod, delta):
-    w = W.molecules(nmol=1500, seed=11, delta=delta)
+    w = W.molecules(n=11, seed=11, delta=delta)
     print("case: molecules delta %.3f N=%d" % (delta, len(w["x"])), flush=True)
     u = W.build_universe(m, w, seed=5)
     o = oracle_mod.from_workload(w, m, seed=5)
@@ -58,7 +58,7 @@ def test_terms_match_oracle(m, oracle_mod, delta):
 
 def test_terms_trajectory(m, oracle_mod):
     """Twenty steps side by side: positions within 1e-4 of the oracle's."""
-    w = W.molecules(nmol=800, seed=12)
+    w = W.molecules(n=9, seed=12)
     print("case: molecules trajectory", flush=True)
     u = W.build_universe(m, w, seed=5)
     o = oracle_mod.from_workload(w, m, seed=5)
@@ -99,7 +99,7 @@ def test_right_angles_and_exclusions_only(m, oracle_mod):
 
 
 def test_terms_rk4(m, oracle_mod):
-    w = W.molecules(nmol=600, seed=14)
+    w = W.molecules(n=8, seed=14)
     w["integrator"] = 1
     print("case: molecules rk4", flush=True)
     u = W.build_universe(m, w, seed=5)
@@ -113,7 +113,7 @@ def test_terms_rk4(m, oracle_mod):
 
 
 def test_terms_graph_replay_bitwise(m):
-    w = W.molecules(nmol=600, seed=15)
+    w = W.molecules(n=8, seed=15)
     print("case: molecules graphs", flush=True)
     out = []
     for graphs in (False, True):
@@ -131,7 +131,7 @@ def test_terms_graph_replay_bitwise(m):
 
 def test_terms_replace_and_clear(m, oracle_mod):
     """set_* replaces the previous set; an empty set removes the terms."""
-    w = W.molecules(nmol=300, seed=16)
+    w = W.molecules(n=7, seed=16)
     u = W.build_universe(m, w, seed=5)
     u.set_angles([], [], [], [])
     u.set_dihedrals([], [], [], [], [])

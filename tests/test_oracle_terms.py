@@ -215,7 +215,7 @@ def test_molecules_fp32_against_fp64(m, oracle_mod, delta, bar):
     """The FP32 path against FP64 on 200 four-bead molecules.  At delta pi/4
     the dihedral table's slope grows like 1 / sqrt(1 - c^2), so cos(phi)'s
     FP32 rounding reaches the force at a few 1e-4."""
-    w = W.molecules(nmol=200, seed=3, delta=delta)
+    w = W.molecules(n=6, seed=3, delta=delta)
     o = oracle_mod.from_workload(w, m)
     o.force()
     f = o.state()["f"].astype(np.float64)
