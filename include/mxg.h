@@ -252,6 +252,28 @@ int mxg_engine_set_boundaries(mxg_engine *e, const int kind[6], const float *vel
  * potential); NULL removes it. */
 int mxg_engine_set_boundary_potential(mxg_engine *e, int side, int type, const mxg_potential *p);
 
+/* Species of a particle type (MxSpecies in its CStateVector, whose layout
+ * lives in a dependency absent from the reference tree): up to four per
+ * particle; bit k of constant_mask marks species k constant (the
+ * SPECIES_KONSTANT flag MxFluxes_Integrate tests, Flux.cpp:261-267). */
+int mxg_engine_set_species(mxg_engine *e, int type, int constant_mask);
+
+/* CStateVector fvec of every particle: four floats per particle by id; the
+ * flux accumulators (q) restart at zero.  After mxg_engine_load. */
+int mxg_engine_set_species_values(mxg_engine *e, const float *values);
+
+/* Species values and pending flux accumulators, four each per particle by
+ * id (either pointer may be NULL). */
+int mxg_engine_get_species(mxg_engine *e, float *values, float *q);
+
+/* MxFluxes_FluxEx (Flux.cpp:162-211; MxFluxes_Fick / _Secrete / _Uptake,
+ * :214-259): kind 0 Fick, 1 secrete, 2 uptake between species species_a of
+ * type type_a and species_b of type type_b, rate k, decay and target.
+ * Evaluated per in-cutoff pair as flux_eval_ex (flux_eval.hpp:33-81) and
+ * integrated after the advance (MxFluxes_Integrate, Flux.cpp:261-284). */
+int mxg_engine_add_flux(mxg_engine *e, int kind, int type_a, int type_b, int species_a, int species_b, float k,
+                        float decay, float target);
+
 /* engine_step (engine.h:628, engine.cpp:1198-1228), nsteps times. */
 int mxg_engine_step(mxg_engine *e, int nsteps);
 

@@ -362,6 +362,33 @@ class Universe:
         self._check(_capi.lib().mxg_engine_set_boundary_potential(
             self._h, sd, int(type_id), pot._h.value if pot is not None else None))
 
+    # species and fluxes ----------------------------------------------------
+    FICK, SECRETE, UPTAKE = 0, 1, 2
+
+    def set_species(self, type_id, constant_mask=0):
+        """Up to four species per particle of the type; bit k of
+        constant_mask keeps species k constant."""
+        self._check(_capi.lib().mxg_engine_set_species(self._h, int(type_id), int(constant_mask)))
+
+    def set_species_values(self, values):
+        """Species values by particle id (N x up to 4)."""
+        a = np.asarray(values, np.float32)
+        v = np.zeros((self.N, 4), np.float32)
+        v[:, :a.shape[1]] = a
+        self._check(_capi.lib().mxg_engine_set_species_values(self._h, _fp(v)))
+
+    def species(self):
+        """(values, pending flux accumulators), N x 4 each, by id."""
+        v = np.zeros((self.N, 4), np.float32)
+        q = np.zeros((self.N, 4), np.float32)
+        self._check(_capi.lib().mxg_engine_get_species(self._h, _fp(v), _fp(q)))
+        return v, q
+
+    def add_flux(self, kind, type_a, type_b, species_a, species_b, k, decay=0.0, target=0.0):
+        """MxFluxes_FluxEx: kind FICK, SECRETE or UPTAKE."""
+        self._check(_capi.lib().mxg_engine_add_flux(self._h, int(kind), int(type_a), int(type_b), int(species_a),
+                                                    int(species_b), float(k), float(decay), float(target)))
+
     def _term_pots(self, pots, M):
         if isinstance(pots, Potential):
             pots = [pots] * M

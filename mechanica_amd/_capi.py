@@ -73,6 +73,11 @@ SIGNATURES = {
                                        c_double_p]),
     "mxg_engine_set_boundaries": (C.c_int, [C.c_void_p, c_int_p, c_float_p, c_float_p, c_float_p]),
     "mxg_engine_set_boundary_potential": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "mxg_engine_set_species": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "mxg_engine_set_species_values": (C.c_int, [C.c_void_p, c_float_p]),
+    "mxg_engine_get_species": (C.c_int, [C.c_void_p, c_float_p, c_float_p]),
+    "mxg_engine_add_flux": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                      C.c_float, C.c_float]),
     "mxg_engine_set_angles": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, C.POINTER(C.c_void_p)]),
     "mxg_engine_set_dihedrals": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, c_int_p,
                                            C.POINTER(C.c_void_p)]),

@@ -284,6 +284,28 @@ int mxg_engine_set_boundary_potential(mxg_engine *e, int side, int type, const m
   return e->e.set_boundary_potential(side, type, p ? &p->p : nullptr);
 }
 
+int mxg_engine_set_species(mxg_engine *e, int type, int constant_mask) {
+  MXG_CHECK(e);
+  return e->e.set_species(type, constant_mask);
+}
+
+int mxg_engine_set_species_values(mxg_engine *e, const float *values) {
+  MXG_CHECK(e);
+  if (!values) return MXG_ERR_NULL;
+  return e->e.set_species_values(values);
+}
+
+int mxg_engine_get_species(mxg_engine *e, float *values, float *q) {
+  if (!e) return MXG_ERR_NULL;
+  return e->e.get_species(values, q);
+}
+
+int mxg_engine_add_flux(mxg_engine *e, int kind, int type_a, int type_b, int species_a, int species_b, float k,
+                        float decay, float target) {
+  MXG_CHECK(e);
+  return e->e.add_flux(kind, type_a, type_b, species_a, species_b, k, decay, target);
+}
+
 int mxg_engine_set_angles(mxg_engine *e, int M, const int32_t *i, const int32_t *j, const int32_t *k,
                           const mxg_potential *const *pots) {
   MXG_CHECK(e);

@@ -102,6 +102,16 @@ class Engine {
   // Potential between a side and particles of a type (MxBoundaryCondition::
   // set_potential); null removes it.
   int set_boundary_potential(int side, int type, const Potential *p);
+  // Species of a type (MxSpecies in a CStateVector): up to four per
+  // particle; bit k of constant_mask keeps species k constant.
+  int set_species(int type, int constant_mask);
+  // Species values, four per particle by id; the flux accumulators restart
+  // at zero.
+  int set_species_values(const float *values);
+  int get_species(float *values, float *q);
+  // MxFluxes_FluxEx (Flux.cpp:162-211): kind 0 Fick, 1 secrete, 2 uptake
+  // between species ia of type ta and ib of type tb.
+  int add_flux(int kind, int ta, int tb, int ia, int ib, float k, float decay, float target);
   int step(int nsteps);
   int sync();
 
@@ -190,7 +200,15 @@ class Engine {
     float sb_coef = 0;
     float cf[3] = {0, 0, 0};
     double target_energy = 0;
+    int spec_const = 0;
   };
+  struct FluxInfo {
+    int kind, ta, tb, ia, ib;
+    float k, decay, target;
+  };
+  std::vector<FluxInfo> fluxes_;
+  bool species_on_ = false;  // species buffers allocated: k_flux runs every step
+  int ensure_species();
   std::vector<TypeInfo> types_;
   std::vector<Potential> pots_;
   std::vector<int> potmat_;  // ntypes * ntypes

@@ -118,6 +118,16 @@ struct DevType {
   unsigned rnd_steps;
   float sb_coef;         // Berendsen 1/tau, friction coefficient
   float cf[3];           // constant force
+  int spec_const;        // bit k: species k is constant (not integrated)
+  int pad;
+};
+
+// One flux of a type pair (MxFlux entries, Flux.hpp; MxFluxes_AddFlux,
+// Flux.cpp:288-316): kind 0 Fick, 1 secrete, 2 uptake; species ia of type
+// ta and ib of type tb.
+struct DevFlux {
+  int kind, ta, tb, ia, ib;
+  float coef, decay, target;
 };
 
 }  // namespace
@@ -187,6 +197,10 @@ struct DeviceState {
   float *te = nullptr;        // energy of each term
   DevPot *tpots = nullptr;    // angle then dihedral potentials
   double *obs_terms = nullptr;  // per replica: angle, dihedral, exclusion energy
+  float4 *spec[2] = {nullptr, nullptr};   // species values, up to 4 per particle
+  float4 *specq[2] = {nullptr, nullptr};  // species flux accumulators
+  DevFlux *fluxes = nullptr;              // by ordered type pair (CSR)
+  int *flux_off = nullptr;                // ntypes * ntypes + 1
   DevWall *walls = nullptr;     // the six sides
   int *wallpot = nullptr;       // 6 x ntypes: index into wpots, -1 none
   DevPot *wpots = nullptr;      // wall potentials
@@ -1139,6 +1153,104 @@ __global__ void __launch_bounds__(256) k_term_energy(const int4 *__restrict__ tb
   if (threadIdx.x < 3) out[3 * r + threadIdx.x] = threadIdx.x == 2 ? -red[2][0] : red[threadIdx.x][0];
 }
 
+__device__ __forceinline__ float &comp(float4 &v, int k) {
+  return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+__device__ __forceinline__ float comp(const float4 &v, int k) {
+  return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+struct FluxArgs {
+  const DevFlux *__restrict__ fluxes;
+  const int *__restrict__ flux_off;
+  const float4 *__restrict__ spec;   // values at the start of the step (buffer 0)
+  const float4 *__restrict__ specq;  // accumulators carried in (buffer 0)
+  float4 *__restrict__ spec_out;     // buffer 1
+  float4 *__restrict__ specq_out;
+  const int *__restrict__ cell_out;  // new cell of each slot, bit 31 set for movers
+  const int *__restrict__ cell_rep;
+  int nslots;
+  int integrate;                     // forward Euler: integrate the stayers
+  double dt;
+  double cutoff;
+};
+
+// Species transport (flux_eval_ex, flux_eval.hpp:33-81, over the pair set
+// of runner_dopair/runner_doself) and its integration
+// (MxFluxes_Integrate, Flux.cpp:261-284), one thread per particle after the
+// force kernel: every in-cutoff neighbour of a type pair with fluxes adds
+// this particle's share, -q - decay/2 s_i as the flux's a particle, q -
+// decay/2 s_j as its b particle, with q = (1 - r/cutoff)^2 times the kind's
+// rate; the roles follow the types (the reference's i particle for equal
+// types, the lower slot within a cell).  The reference integrates a cell's
+// particles after that cell's advance, so a particle that changed cells
+// keeps its accumulated flux for the next step; stayers add dt q (not for
+// constant species) and clear it.  Under RK4 nothing is integrated
+// (engine_advance_runge_kutta_4 calls no MxFluxes_Integrate).
+__global__ void __launch_bounds__(256) k_flux(const StepArgs *__restrict__ Ap, FluxArgs F) {
+  const StepArgs &A = *Ap;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= F.nslots) return;
+  const float4 xs = A.x[s];
+  const int ti = __float_as_int(xs.w) & 0xffff;
+  const int c = A.cell[s];
+  const DevReplica R = A.rep[F.cell_rep[c]];
+  int lx, ly, lz;
+  cell_coords(c - R.cell_base, R, lx, ly, lz);
+  const float4 me = F.spec[s];
+  float4 q = F.specq[s];
+  const LoopConsts C = loop_consts(A);
+  const int nt = A.ntypes;
+  for_each_neighbour(A, C, R, s, lx, ly, lz, xs,
+                     [&](int qn, bool p_is_i, bool self, float, float, float, float r2, const float4 &xq) {
+                       const int tn = __float_as_int(xq.w) & 0xffff;
+                       const int f0 = F.flux_off[ti * nt + tn], f1 = F.flux_off[ti * nt + tn + 1];
+                       if (f0 == f1) return;
+                       const bool me_i = self ? (s < qn) : p_is_i;  // this particle is the pair's i
+                       const int type_i = me_i ? ti : tn, type_j = me_i ? tn : ti;
+                       const float r = sqrtf(r2);
+                       float term = (float)(1. - (double)r / F.cutoff);
+                       term = term * term;
+                       const float4 other = F.spec[qn];
+                       for (int e = f0; e < f1; ++e) {
+                         const DevFlux X = F.fluxes[e];
+                         const bool pi_is_i = type_i == X.ta;  // pi = part_i if its type is a
+                         const bool pj_is_j = type_j == X.tb;  // pj = part_j if its type is b
+                         const bool me_pi = pi_is_i ? me_i : !me_i;
+                         const bool me_pj = pj_is_j ? !me_i : me_i;
+                         const float ssi = comp(me_pi ? me : other, X.ia);
+                         const float ssj = comp(me_pj ? me : other, X.ib);
+                         float rate;
+                         if (X.kind == 0) {
+                           rate = X.coef * (ssi - ssj);
+                         } else if (X.kind == 1) {
+                           const float qq = X.coef * (ssi - X.target);
+                           rate = (float)(qq > 0.f) * qq;
+                         } else {
+                           const float qq = X.coef * (X.target - ssj) * ssi;
+                           rate = (float)(qq > 0.f) * qq;
+                         }
+                         const float qf = term * rate;
+                         const float hd = X.decay / 2.f;
+                         if (me_pi) comp(q, X.ia) = comp(q, X.ia) - qf - hd * ssi;
+                         if (me_pj) comp(q, X.ib) = comp(q, X.ib) + qf - hd * ssj;
+                       }
+                     });
+  float4 v = me;
+  if (F.integrate && F.cell_out[s] >= 0) {
+    const int cm = A.types[ti].spec_const;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float konst = (cm >> k) & 1 ? 0.f : 1.f;
+      comp(v, k) = (float)((double)comp(v, k) + F.dt * (double)comp(q, k) * (double)konst);
+      comp(q, k) = 0.0f;
+    }
+  }
+  F.spec_out[s] = v;
+  F.specq_out[s] = q;
+}
+
 // apply_update_pos_vel (boundary_eval.hpp:30-119): a particle that left a
 // wall cell through an active free-slip or velocity wall is mirrored back
 // (x -> -x on a low side, 2h - x on a high side, scaled by the restitution
@@ -1800,6 +1912,10 @@ struct RebuildArgs {
   const float4 *__restrict__ rk_in[4];
   float4 *__restrict__ rk[4];
   int with_rk;
+  // species values and flux accumulators (buffer 1 -> 0)
+  const float4 *__restrict__ spec_in[2];
+  float4 *__restrict__ spec[2];
+  int with_spec;
 };
 
 // Exclusive scan of the per-cell counts (first pass); the counts are reset
@@ -1884,6 +2000,10 @@ __global__ __launch_bounds__(256) void k_place(RebuildArgs A) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) A.rk[k][dst] = A.rk_in[k][q];
   }
+  if (A.with_spec) {
+    A.spec[0][dst] = A.spec_in[0][q];
+    A.spec[1][dst] = A.spec_in[1][q];
+  }
   const int p = A.pid_in[q];
   A.pid[dst] = p;
   A.cell[dst] = c1;
@@ -1921,6 +2041,14 @@ __global__ __launch_bounds__(256) void k_order_incoming(RebuildArgs A) {
             const float4 t = A.rk[r][a];
             A.rk[r][a] = A.rk[r][b];
             A.rk[r][b] = t;
+          }
+        }
+        if (A.with_spec) {
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            const float4 t = A.spec[r][a];
+            A.spec[r][a] = A.spec[r][b];
+            A.spec[r][b] = t;
           }
         }
       }
@@ -2521,7 +2649,19 @@ int Engine::upload_potentials() {
     dt[i].rnd_steps = types_[i].rnd_steps;
     dt[i].sb_coef = types_[i].sb_coef;
     for (int k = 0; k < 3; ++k) dt[i].cf[k] = types_[i].cf[k];
+    dt[i].spec_const = types_[i].spec_const;
   }
+  // Fluxes by ordered type pair; both orders of a pair list the same fluxes
+  // in the order they were added (one MxFluxes per pair, Flux.cpp:197-208).
+  std::vector<int> flux_off((size_t)nt * nt + 1, 0);
+  std::vector<DevFlux> dflux;
+  for (int t1 = 0; t1 < nt; ++t1)
+    for (int t2 = 0; t2 < nt; ++t2) {
+      for (const FluxInfo &F : fluxes_)
+        if ((F.ta == t1 && F.tb == t2) || (F.ta == t2 && F.tb == t1))
+          dflux.push_back(DevFlux{F.kind, F.ta, F.tb, F.ia, F.ib, F.k, F.decay, F.target});
+      flux_off[t1 * nt + t2 + 1] = (int)dflux.size();
+    }
   // Potentials change at set-up only, so reallocating here is fine.
   auto upload = [&](auto **dst, const auto &vec) -> int {
     using T = typename std::remove_reference<decltype(vec)>::type::value_type;
@@ -2536,6 +2676,8 @@ int Engine::upload_potentials() {
   rc |= upload(&d->tpots, dtp);
   rc |= upload(&d->wpots, dwp);
   rc |= upload(&d->wallpot, wallpot);
+  rc |= upload(&d->fluxes, dflux);
+  rc |= upload(&d->flux_off, flux_off);
   rc |= upload(&d->coef, coef);
   rc |= upload(&d->potmat, potmat_);
   rc |= upload(&d->types, dt);
@@ -2616,6 +2758,9 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   prep_.assign(N, 0);
   if (replica)
     for (int i = 0; i < N; ++i) prep_[i] = replica[i];
+  // Species buffers are sized by the particle count.
+  const bool had_species = species_on_;
+  species_on_ = false;
   // Angles, dihedrals and exclusions name particle ids of the previous load.
   angles_.clear();
   dihedrals_.clear();
@@ -2664,6 +2809,8 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   parity_ = 0;
   args_dirty_ = true;
   if (int rc = ensure_rk_buffers()) return rc;
+  if (had_species || !fluxes_.empty())
+    if (int rc = ensure_species()) return rc;
   return set_brick(0, 0, 0, 0, 0);
 }
 
@@ -2928,6 +3075,89 @@ int Engine::upload_walls() {
   rc |= hipStreamSynchronize(s);
   if (rc) return check_device(rc, "boundary upload");
   args_dirty_ = true;
+  return kErrOk;
+}
+
+int Engine::set_species(int type, int constant_mask) {
+  if (type < 0 || type >= (int)types_.size()) return reject(kErrRange, "type index out of range");
+  if (constant_mask & ~0xf) return reject(kErrRange, "at most four species per particle");
+  types_[type].spec_const = constant_mask;
+  potentials_dirty_ = true;
+  return kErrOk;
+}
+
+int Engine::add_flux(int kind, int ta, int tb, int ia, int ib, float k, float decay, float target) {
+  const int nt = (int)types_.size();
+  if (kind < 0 || kind > 2) return reject(kErrRange, "flux kind must be 0 (Fick), 1 (secrete) or 2 (uptake)");
+  if (ta < 0 || ta >= nt || tb < 0 || tb >= nt) return reject(kErrRange, "type index out of range");
+  if (ia < 0 || ia > 3 || ib < 0 || ib > 3) return reject(kErrRange, "species index must be 0..3");
+  fluxes_.push_back(FluxInfo{kind, ta, tb, ia, ib, k, decay, target});
+  potentials_dirty_ = true;
+  if (N_ > 0) return ensure_species();
+  return kErrOk;
+}
+
+// Species buffers (values and accumulators, ping-pong), zero at first.
+int Engine::ensure_species() {
+  if (species_on_) return kErrOk;
+  if (!d_ || N_ == 0) return fail(kErrNull, "load particles before species");
+  DeviceState *d = d_;
+  for (int b = 0; b < 2; ++b) {
+    if (int st = dalloc(d, &d->spec[b], N_)) return check_device(st, "hipMalloc");
+    if (int st = dalloc(d, &d->specq[b], N_)) return check_device(st, "hipMalloc");
+  }
+  int rc = 0;
+  for (int b = 0; b < 2; ++b) {
+    rc |= hipMemsetAsync(d->spec[b], 0, (size_t)N_ * sizeof(float4), d->stream);
+    rc |= hipMemsetAsync(d->specq[b], 0, (size_t)N_ * sizeof(float4), d->stream);
+  }
+  rc |= hipStreamSynchronize(d->stream);
+  if (rc) return check_device(rc, "species buffers");
+  species_on_ = true;
+  args_dirty_ = true;
+  ++config_gen_;
+  return kErrOk;
+}
+
+int Engine::set_species_values(const float *values) {
+  if (!values) return reject(kErrNull, "species values missing");
+  if (int rc = ensure_species()) return rc;
+  if (int rc = sync()) return rc;
+  DeviceState *d = d_;
+  std::vector<int> pid(N_);
+  std::vector<float4> v(N_), zero(N_, make_float4(0, 0, 0, 0));
+  int rc = hipMemcpy(pid.data(), d->pid[parity_], N_ * sizeof(int), hipMemcpyDeviceToHost);
+  if (rc) return check_device(rc, "species upload");
+  for (int s = 0; s < N_; ++s) {
+    const float *p = values + 4 * (size_t)pid[s];
+    v[s] = make_float4(p[0], p[1], p[2], p[3]);
+  }
+  rc |= hipMemcpy(d->spec[0], v.data(), N_ * sizeof(float4), hipMemcpyHostToDevice);
+  rc |= hipMemcpy(d->specq[0], zero.data(), N_ * sizeof(float4), hipMemcpyHostToDevice);
+  if (rc) return check_device(rc, "species upload");
+  return kErrOk;
+}
+
+int Engine::get_species(float *values, float *q) {
+  if (!d_ || N_ == 0) return reject(kErrNull, "no particles loaded");
+  if (values) std::fill(values, values + 4 * (size_t)N_, 0.0f);
+  if (q) std::fill(q, q + 4 * (size_t)N_, 0.0f);
+  if (!species_on_) return kErrOk;
+  if (int rc = sync()) return rc;
+  DeviceState *d = d_;
+  std::vector<int> pid(N_);
+  std::vector<float4> v(N_), a(N_);
+  int rc = hipMemcpy(pid.data(), d->pid[parity_], N_ * sizeof(int), hipMemcpyDeviceToHost);
+  rc |= hipMemcpy(v.data(), d->spec[0], N_ * sizeof(float4), hipMemcpyDeviceToHost);
+  rc |= hipMemcpy(a.data(), d->specq[0], N_ * sizeof(float4), hipMemcpyDeviceToHost);
+  if (rc) return check_device(rc, "species read-back");
+  for (int s = 0; s < N_; ++s) {
+    const float vv[4] = {v[s].x, v[s].y, v[s].z, v[s].w}, qq[4] = {a[s].x, a[s].y, a[s].z, a[s].w};
+    for (int k = 0; k < 4; ++k) {
+      if (values) values[4 * (size_t)pid[s] + k] = vv[k];
+      if (q) q[4 * (size_t)pid[s] + k] = qq[k];
+    }
+  }
   return kErrOk;
 }
 
@@ -3268,6 +3498,23 @@ int Engine::launch_substep(int rk_stage, int tick) {
   kStepLaunchers[mode](reinterpret_cast<const StepArgs *>(d->dargs) + par, Bk, lds, tabs, s);
   if (rec) hipEventRecord(d->kev[2 * timed_steps_ + 1], s);
   if (rec) ++timed_steps_;
+  if (species_on_) {
+    FluxArgs F;
+    F.fluxes = d->fluxes;
+    F.flux_off = d->flux_off;
+    F.spec = d->spec[0];
+    F.specq = d->specq[0];
+    F.spec_out = d->spec[1];
+    F.specq_out = d->specq[1];
+    F.cell_out = d->cell[1];
+    F.cell_rep = d->cell_rep;
+    F.nslots = N_;
+    F.integrate = rk_stage < 0 ? 1 : 0;
+    F.dt = dt_;
+    F.cutoff = cutoff_;
+    hipLaunchKernelGGL(k_flux, dim3((N_ + 255) / 256), dim3(256), 0, s,
+                       reinterpret_cast<const StepArgs *>(d->dargs) + par, F);
+  }
 
   // Cell-list rebuild: x, v, pf, cell go from buffer 1 back to buffer 0;
   // ids and cell offsets go from parity par to par ^ 1.
@@ -3291,6 +3538,11 @@ int Engine::launch_substep(int rk_stage, int tick) {
   B.slot_of = d->slot_of;
   B.with_pf = any_random_ ? 1 : 0;
   B.with_rk = rk_stage >= 0 && rk_stage < 3 ? 1 : 0;
+  B.with_spec = species_on_ ? 1 : 0;
+  for (int k = 0; k < 2; ++k) {
+    B.spec_in[k] = k == 0 ? d->spec[1] : d->specq[1];
+    B.spec[k] = k == 0 ? d->spec[0] : d->specq[0];
+  }
   for (int k = 0; k < 4; ++k) {
     float4 *const *buf = k == 0 ? d->rk_p0 : k == 1 ? d->rk_ax : k == 2 ? d->rk_av : d->rk_vk;
     B.rk_in[k] = buf[1];

@@ -440,6 +440,12 @@ def build_universe(m, w, seed=None):
         bp = make_potential(m, b["pot"])
         u.set_bonds(b["i"], b["j"], bp, b["dissociation"])
         pots.append(bp)
+    for t, mask in (w.get("species_const") or {}).items():
+        u.set_species(t, mask)
+    for fx in w.get("fluxes") or ():
+        u.add_flux(*fx)
+    if w.get("species") is not None:
+        u.set_species_values(w["species"])
     if w.get("exclusions"):
         u.set_exclusions(w["exclusions"]["i"], w["exclusions"]["j"])
     if w.get("angles"):

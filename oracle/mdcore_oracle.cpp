@@ -33,6 +33,8 @@
 //   boundary_eval (wall potentials) ................. boundary_eval.hpp:124-221
 //   dpd_boundary_eval ............................... dpd_eval.hpp:71-106
 //   apply_update_pos_vel (wall reflection) .......... boundary_eval.hpp:30-119
+//   flux_eval_ex (species transport) ................ flux_eval.hpp:17-81
+//   MxFluxes_Integrate (after each cell's advance) .. Flux.cpp:261-284
 //   cell_advance_forward_euler ...................... engine_advance.cpp:126-239
 //   engine_advance_forward_euler .................... engine_advance.cpp:248-422
 //   engine_advance_runge_kutta_4 .................... engine_advance.cpp:443-698
@@ -142,6 +144,14 @@ struct Type {
   float sb_coef = 0;  // berendsen itau, friction coef
   float cf[3] = {0, 0, 0};
   double target_energy = 0, kinetic_energy = 0;
+  int spec_const = 0;  // bit k: species k is constant (not integrated)
+};
+
+// One flux (MxFlux entry): kind 0 Fick, 1 secrete, 2 uptake, between
+// species ia of type ta and ib of type tb.
+struct FluxEntry {
+  int kind, ta, tb, ia, ib;
+  float coef, decay, target;
 };
 
 struct Part {
@@ -149,6 +159,8 @@ struct Part {
   float v[4];
   float x[4];
   float pf[4];  // persistent random force
+  float sv[4];  // species values (CStateVector fvec), up to four
+  float sq[4];  // species flux accumulators (CStateVector q)
   // Runge-Kutta stage state (MxParticle p0, v0, xk, vk: MxParticle.h:150-153)
   float p0[3], v0[3], xk[4][3], vk[4][3];
   double imass;
@@ -228,6 +240,8 @@ struct Oracle {
   std::vector<Angle> angles;
   std::vector<Dihedral> dihedrals;
   std::vector<Exclusion> exclusions;
+  std::vector<FluxEntry> fluxes;
+  std::vector<std::vector<int>> flux_pair;  // ntypes^2: fluxes of the unordered type pair
   int nparts = 0;
 
   double ekin = 0, epot_nb = 0, epot_bond = 0;
@@ -655,6 +669,60 @@ void do_sort(Oracle &o, Cell &c) {
 // ---------------------------------------------------------------------------
 // Pair and self loops (runner_dopair.cpp:91-340, 446-682).
 
+// flux_eval_ex: the fluxes of the pair's type pair, each between the
+// particle of the flux's type a (the pair's i for equal types) and that of
+// type b, scaled by (1 - r/cutoff)^2, with half the decay taken from each.
+void flux_eval(const Oracle &o, const std::vector<int> &fl, float r2, Part *part_i, Part *part_j) {
+  float r = std::sqrt(r2);
+  float term = (1. - r / o.cutoff);
+  term = term * term;
+  for (int e : fl) {
+    const FluxEntry &F = o.fluxes[e];
+    Part *pi = part_i->type == F.ta ? part_i : part_j;
+    Part *pj = part_j->type == F.tb ? part_j : part_i;
+    const float ssi = pi->sv[F.ia], ssj = pj->sv[F.ib];
+    float q = term;
+    if (F.kind == 0) {
+      q *= F.coef * (ssi - ssj);
+    } else if (F.kind == 1) {
+      const float qq = F.coef * (ssi - F.target);
+      const float scale = qq > 0.f;
+      q *= scale * qq;
+    } else {
+      const float qq = F.coef * (F.target - ssj) * ssi;
+      const float scale = qq > 0.f;
+      q *= scale * qq;
+    }
+    const float half_decay = F.decay / 2.f;
+    pi->sq[F.ia] = pi->sq[F.ia] - q - half_decay * ssi;
+    pj->sq[F.ib] = pj->sq[F.ib] + q - half_decay * ssj;
+  }
+}
+
+void rebuild_flux_pairs(Oracle &o) {
+  const int nt = (int)o.types.size();
+  o.flux_pair.assign((size_t)nt * nt, {});
+  for (size_t e = 0; e < o.fluxes.size(); ++e) {
+    const FluxEntry &F = o.fluxes[e];
+    if (F.ta >= nt || F.tb >= nt) continue;
+    o.flux_pair[F.ta * nt + F.tb].push_back((int)e);
+    if (F.ta != F.tb) o.flux_pair[F.tb * nt + F.ta].push_back((int)e);
+  }
+}
+
+// MxFluxes_Integrate for one cell's particles (those still in it after its
+// advance): s += dt q for species that are not constant, then q = 0.
+void flux_integrate(Oracle &o, int cid) {
+  for (Part &p : o.cells[cid].parts) {
+    const int cm = o.types[p.type].spec_const;
+    for (int k = 0; k < 4; ++k) {
+      const float konst = (cm >> k) & 1 ? 0.f : 1.f;
+      p.sv[k] += o.dt_d * p.sq[k] * konst;
+      p.sq[k] = 0;
+    }
+  }
+}
+
 void do_pair(Oracle &o, int ci_, int cj_) {
   int ci = ci_, cj = cj_;
   Cell *cell_i = &o.cells[ci], *cell_j = &o.cells[cj];
@@ -690,11 +758,15 @@ void do_pair(Oracle &o, int ci_, int cj_) {
       part_j->ncount++;
       float nd = w_cubic(r2, cutoff);
       int pot = o.potmat[part_i->type * o.types.size() + part_j->type];
-      if (pot < 0) {
+      const std::vector<int> *fl = o.fluxes.empty() ? nullptr : &o.flux_pair[part_i->type * o.types.size() + part_j->type];
+      if (fl && fl->empty()) fl = nullptr;
+      if (pot < 0 && !fl) {
         part_i->f[3] += nd;
         part_j->f[3] += nd;
         continue;
       }
+      if (fl) flux_eval(o, *fl, r2, part_i, part_j);
+      if (pot < 0) continue;
       eval_pair(o, *cell_i, pot, part_i, part_j, dx, r2, nd, &epot);
     }
   }
@@ -877,11 +949,15 @@ void do_self(Oracle &o, int cid) {
       part_j->ncount++;
       float nd = w_cubic(r2, (float)o.cutoff);
       int pot = o.potmat[part_i->type * o.types.size() + part_j->type];
-      if (pot < 0) {
+      const std::vector<int> *fl = o.fluxes.empty() ? nullptr : &o.flux_pair[part_i->type * o.types.size() + part_j->type];
+      if (fl && fl->empty()) fl = nullptr;
+      if (pot < 0 && !fl) {
         part_i->f[3] += nd;
         part_j->f[3] += nd;
         continue;
       }
+      if (fl) flux_eval(o, *fl, r2, part_i, part_j);
+      if (pot < 0) continue;
       eval_pair(o, c, pot, part_i, part_j, dx, r2, nd, &epot);
     }
   }
@@ -1406,11 +1482,17 @@ int step(Oracle &o) {
 void advance(Oracle &o) {
   AdvanceConst K = advance_const(o);
   if (o.nthreads <= 1) {
-    for (int cid = 0; cid < o.ncells; ++cid) advance_cell(o, K, cid);
+    for (int cid = 0; cid < o.ncells; ++cid) {
+      advance_cell(o, K, cid);
+      flux_integrate(o, cid);
+    }
   } else {
     std::atomic<int> next{0};
     auto fn = [&]() {
-      for (int c; (c = next.fetch_add(1)) < o.ncells;) advance_cell(o, K, c);
+      for (int c; (c = next.fetch_add(1)) < o.ncells;) {
+        advance_cell(o, K, c);
+        flux_integrate(o, c);
+      }
     };
     std::vector<std::thread> th;
     for (int i = 0; i < o.nthreads; ++i) th.emplace_back(fn);
@@ -1421,10 +1503,19 @@ void advance(Oracle &o) {
 
 // FP64 evaluation of the forces on the current configuration: the same pair
 // set as the FP32 loops (in-cutoff in FP32, reference orientation) and the
-// same tables, but every subsequent operation in double.
-void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_out) {
+// same tables, but every subsequent operation in double.  Returns forces by
+// id, the potential energy and, when asked, the virial tensor W = sum over
+// pairs and bonds of (x_i - x_j) (x) f_i, f_i the force on i (row-major).
+// The reference's virial_pair never sets its force (metrics.cpp:183-246),
+// so this definition is the one the engine's virial is checked against.
+void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_out, double *virial = nullptr) {
   std::fill(F, F + 3 * (size_t)o.nparts, 0.0);
   double epot = 0;
+  double W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  auto vir = [&](const double dx[3], const double fi[3]) {
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) W[3 * a + b] += dx[a] * fi[b];
+  };
   auto table64 = [](const Pot &p, double ri, double rj, double r2, double *e, double *f) {
     return table_ex64(p, ri, rj, r2, e, f);
   };
@@ -1445,17 +1536,21 @@ void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_o
       double wc = sr < 0 ? 1.0 : 1 - sr / cut;
       double f = p.dpd_alpha * wc - p.dpd_gamma * wc * wc * (e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
       if (!o.dpd_counter_noise) f += p.dpd_sigma * wc * (double)o.dpd_delta;
+      const double fi[3] = {f * e[0], f * e[1], f * e[2]};
       for (int k = 0; k < 3; ++k) {
-        F[3 * (size_t)pi.id + k] += f * e[k];
-        F[3 * (size_t)pj.id + k] -= f * e[k];
+        F[3 * (size_t)pi.id + k] += fi[k];
+        F[3 * (size_t)pj.id + k] -= fi[k];
       }
+      vir(dx, fi);
     } else {
       double e, f;
       if (!table64(p, pi.radius, pj.radius, r2, &e, &f)) return;
+      const double fi[3] = {-f * dx[0], -f * dx[1], -f * dx[2]};
       for (int k = 0; k < 3; ++k) {
-        F[3 * (size_t)pi.id + k] -= f * dx[k];
-        F[3 * (size_t)pj.id + k] += f * dx[k];
+        F[3 * (size_t)pi.id + k] += fi[k];
+        F[3 * (size_t)pj.id + k] -= fi[k];
       }
+      vir(dx, fi);
       epot += e;
     }
   };
@@ -1554,10 +1649,13 @@ void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_o
       ee = ee * x + c[k];
     }
     eff = eff * c[1] / r;
+    if (eff >= b.dissociation) continue;  // the bond breaks in this force phase (bond.cpp:230)
+    const double fi[3] = {-eff * dx[0], -eff * dx[1], -eff * dx[2]};
     for (int k = 0; k < 3; ++k) {
-      F[3 * (size_t)pi.id + k] -= eff * dx[k];
-      F[3 * (size_t)pj.id + k] += eff * dx[k];
+      F[3 * (size_t)pi.id + k] += fi[k];
+      F[3 * (size_t)pj.id + k] -= fi[k];
     }
+    vir(dx, fi);
     epot += ee;
   }
   auto add64 = [&](int id, int k, double w) { F[3 * (size_t)id + k] += w; };
@@ -1565,6 +1663,8 @@ void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_o
   epot += angles_eval<double>(o, where, add64);
   epot += dihedrals_eval<double>(o, where, add64);
   if (epot_out) *epot_out = epot;
+  if (virial)
+    for (int k = 0; k < 9; ++k) virial[k] = W[k];
 }
 
 Pot make_pot(uint32_t kind, uint32_t flags, const float alpha[4], float a, float b, float r0_plusone,
@@ -1765,6 +1865,7 @@ int oracle_add_type(void *h, double mass, float radius, int dynamics) {
   for (int i = 0; i < nt - 1; ++i)
     for (int j = 0; j < nt - 1; ++j) m[i * nt + j] = o->potmat[i * (nt - 1) + j];
   o->potmat.swap(m);
+  rebuild_flux_pairs(*o);
   return nt - 1;
 }
 
@@ -1878,6 +1979,43 @@ int oracle_set_bonds(void *h, int M, const int *bi, const int *bj, const int *po
   Oracle *o = (Oracle *)h;
   o->bonds.clear();
   for (int b = 0; b < M; ++b) o->bonds.push_back(Bond{bi[b], bj[b], pot[b], dissoc[b], true});
+  return 0;
+}
+
+int oracle_set_species(void *h, int type, int spec_const) {
+  Oracle *o = (Oracle *)h;
+  if (type < 0 || type >= (int)o->types.size()) return kErrRange;
+  o->types[type].spec_const = spec_const;
+  return 0;
+}
+
+int oracle_add_flux(void *h, int kind, int ta, int tb, int ia, int ib, float coef, float decay, float target) {
+  Oracle *o = (Oracle *)h;
+  o->fluxes.push_back(FluxEntry{kind, ta, tb, ia, ib, coef, decay, target});
+  rebuild_flux_pairs(*o);
+  return 0;
+}
+
+// Species values by id (4 per particle); the accumulators start at zero.
+int oracle_set_species_values(void *h, const float *v) {
+  Oracle *o = (Oracle *)h;
+  for (auto &c : o->cells)
+    for (auto &p : c.parts)
+      for (int k = 0; k < 4; ++k) {
+        p.sv[k] = v[4 * (size_t)p.id + k];
+        p.sq[k] = 0;
+      }
+  return 0;
+}
+
+int oracle_get_species(void *h, float *v, float *q) {
+  Oracle *o = (Oracle *)h;
+  for (auto &c : o->cells)
+    for (auto &p : c.parts)
+      for (int k = 0; k < 4; ++k) {
+        if (v) v[4 * (size_t)p.id + k] = p.sv[k];
+        if (q) q[4 * (size_t)p.id + k] = p.sq[k];
+      }
   return 0;
 }
 
@@ -2017,6 +2155,14 @@ int oracle_observables(void *h, double *epot_nb, double *epot_bond, double *ekin
 int oracle_bond_active(void *h, uint8_t *active) {
   Oracle *o = (Oracle *)h;
   for (size_t b = 0; b < o->bonds.size(); ++b) active[b] = o->bonds[b].active ? 1 : 0;
+  return 0;
+}
+
+// Virial tensor of the FP64 force phase (pairs and bonds).
+int oracle_virial64(void *h, double *W9) {
+  Oracle *o = (Oracle *)h;
+  std::vector<double> F(3 * (size_t)std::max(o->nparts, 1));
+  forces64(*o, F.data(), nullptr, W9);
   return 0;
 }
 

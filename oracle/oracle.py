@@ -65,6 +65,12 @@ _SIG = {
     "oracle_observables": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
     "oracle_bond_active": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8)]),
+    "oracle_set_species": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "oracle_add_flux": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                  C.c_float]),
+    "oracle_set_species_values": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "oracle_get_species": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "oracle_virial64": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "oracle_forces64": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "oracle_error": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "oracle_number_density": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
@@ -270,6 +276,31 @@ class Oracle:
         lib().oracle_forces64(self._h, _dp(F), C.byref(e))
         return F, e.value
 
+    def set_species(self, t, constant_mask=0):
+        lib().oracle_set_species(self._h, int(t), int(constant_mask))
+
+    def add_flux(self, kind, ta, tb, ia, ib, k, decay=0.0, target=0.0):
+        lib().oracle_add_flux(self._h, int(kind), int(ta), int(tb), int(ia), int(ib), float(k), float(decay),
+                              float(target))
+
+    def set_species_values(self, values):
+        v = np.zeros((self.N, 4), np.float32)
+        a = np.asarray(values, np.float32)
+        v[:, :a.shape[1]] = a
+        lib().oracle_set_species_values(self._h, _fp(v))
+
+    def species(self):
+        v = np.zeros((self.N, 4), np.float32)
+        q = np.zeros((self.N, 4), np.float32)
+        lib().oracle_get_species(self._h, _fp(v), _fp(q))
+        return v, q
+
+    def virial64(self):
+        """FP64 virial tensor sum (x_i - x_j) (x) f_i over pairs and bonds."""
+        W = np.zeros(9, np.float64)
+        lib().oracle_virial64(self._h, _dp(W))
+        return W.reshape(3, 3)
+
     def error(self):
         pid = C.c_int()
         rc = lib().oracle_error(self._h, C.byref(pid))
@@ -322,6 +353,12 @@ def from_workload(w, m, threads=1, seed=None):
         bp = W.make_potential(m, b["pot"])
         o.set_bonds(b["i"], b["j"], bp, b["dissociation"])
         keep.append(bp)
+    for t, mask in (w.get("species_const") or {}).items():
+        o.set_species(t, mask)
+    for fx in w.get("fluxes") or ():
+        o.add_flux(*fx)
+    if w.get("species") is not None:
+        o.set_species_values(w["species"])
     if w.get("exclusions"):
         o.set_exclusions(w["exclusions"]["i"], w["exclusions"]["j"])
     if w.get("angles"):
