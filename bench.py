@@ -11,7 +11,9 @@ in HBM.
 Other workloads (--workload): lj (argon LJ at density 10, 1M particles),
 lattice (64^3 bonded Morse lattice, overdamped), molecules (64^3 four-bead
 chains with bonds, angles, dihedrals and exclusions in an LJ fluid, 1M
-particles), ensemble (256 cell-sorting
+particles), channel (examples/dpd1.py at 1M particles: DPD between no-slip
+walls under a body force), species (the 1M DPD fluid carrying three species
+with five fluxes), ensemble (256 cell-sorting
 replicas in one engine, compared with one universe of the same total size)
 and adjoint (65,536-particle Morse trajectory of 50 steps, forward and
 backward, compared with the plain step on the same system).
@@ -142,6 +144,18 @@ def bench_single(args, threads):
         per, bonds = BYTES_NEWTONIAN + BYTES_TERM_OFFSET, len(w["bonds"]["i"])
         per_terms = BYTES_PER_TERM_ENTRY * (3 * len(w["angles"]["i"]) + 4 * len(w["dihedrals"]["i"])
                                             + 2 * len(w["exclusions"]["i"]))
+    elif args.workload == "channel":
+        w = W.channel(N=args.n, seed=9)
+        w["dpd_noise"] = True
+        cfg = ("channel N=%d (examples/dpd1.py at scale): DPD alpha=1 gamma=1 sigma=1.5 (counter RNG) cutoff=1, "
+               "mass 10, no-slip walls in z, constant force (0.01,0,0), dt=0.1 cells=%dx%dx%d")
+        per, bonds = BYTES_NEWTONIAN, 0
+    elif args.workload == "species":
+        w = W.species_fluid(N=args.n, seed=10)
+        w["dpd_noise"] = True
+        cfg = ("species N=%d: DPD fluid at density 3 (sigma=3, counter RNG) in two types with three species, "
+               "5 fluxes (Fick within and between types, secrete, uptake, decay), dt=0.01 cells=%dx%dx%d")
+        per, bonds = BYTES_NEWTONIAN, 0
     else:  # lattice
         w = W.lattice(n=64, seed=3)
         cfg = ("lattice N=%d (64^3) Morse(d=1,a=6,r0=1) cutoff=2 + 774144 harmonic bonds k=100 r0=1 "
@@ -325,7 +339,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="dpd",
-                    choices=["dpd", "lj", "lattice", "molecules", "ensemble", "adjoint"])
+                    choices=["dpd", "lj", "lattice", "molecules", "channel", "species", "ensemble", "adjoint"])
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--integrator", default="euler", choices=["euler", "rk4"],
                     help="forward Euler (the reference default) or Runge-Kutta 4")
@@ -346,7 +360,7 @@ def main():
         copy_calibration(args)
         return
     threads = min(16, os.cpu_count() or 1)
-    if args.workload in ("dpd", "lj", "lattice", "molecules"):
+    if args.workload in ("dpd", "lj", "lattice", "molecules", "channel", "species"):
         out = bench_single(args, threads)
     elif args.workload == "ensemble":
         out = bench_ensemble(args, threads)

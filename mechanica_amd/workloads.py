@@ -28,6 +28,13 @@ parameters so tests can build the same workload at small N.
              Newtonian, mass 1, dt 0.002 (the bonded set of the reference's
              flexible-water example, examples/flexible/flexible.cpp:304-336,
              on a non-Coulomb pair potential).
+  channel    examples/dpd1.py at scale: DPD (alpha 1, gamma 1, sigma 1.5,
+             cutoff 1) particles of mass 10 at number density 1.875 in a
+             15:12:10 box, no-slip walls in z, periodic in x and y, driven
+             by a constant force (0.01, 0, 0), dt 0.1.
+  species    the DPD fluid at density 3 in two types carrying three
+             species, with Fick fluxes within and between the types,
+             secretion and uptake (Flux.cpp, flux_eval.hpp).
   cell_sort  two-type Morse cell-sorting universe (examples/cell_sorting.py)
              at density 0.8: AA and BB d = 3, AB d = 1, a = 6, r0 = 1,
              cutoff 2.5, overdamped with a persistent random force whose
@@ -209,6 +216,40 @@ def molecules(n=8, seed=8, delta=0.0, dt=0.002):
         "exclusions": {"i": np.concatenate([base, base + 1, base + 2, base, base + 1]),
                        "j": np.concatenate([base + 1, base + 2, base + 3, base + 2, base + 3])},
     }
+
+
+def channel(N=1048576, seed=9):
+    rng = _rng(seed)
+    density = 15 ** 3 / (15.0 * 12.0 * 10.0)
+    s = (N / density / 1800.0) ** (1.0 / 3.0)
+    dim = [15.0 * s, 12.0 * s, 10.0 * s]
+    cells = [max(3, int(math.floor(d))) for d in dim]
+    x = rng.random((N, 3)) * dim
+    return {
+        "name": "channel",
+        "dim": dim, "cells": cells, "cutoff": 1.0, "dt": 0.1,
+        "types": [{"mass": 10.0, "radius": 0.3, "dynamics": 0}],
+        "x": x, "v": np.zeros((N, 3)), "type": np.zeros(N, np.int32),
+        "pots": [("dpd", (1.0, 1.0, 1.5, 1.0, 0), 0, 0)],
+        "bonds": None,
+        "boundaries": {"kinds": [2, 2, 2, 2, 1, 1]},
+        "single_body": {0: (3, np.array([0.01, 0.0, 0.0], np.float32))},
+    }
+
+
+def species_fluid(N=1048576, seed=10):
+    w = dpd_fluid(N=N, sigma=3.0, seed=seed)
+    rng = _rng(seed + 1)
+    w["name"] = "species"
+    w["types"] = [dict(w["types"][0]), dict(w["types"][0])]
+    w["type"] = (rng.random(N) < 0.5).astype(np.int32)
+    dpd = w["pots"][0]
+    w["pots"] = [dpd[:2] + (0, 0), dpd[:2] + (0, 1), dpd[:2] + (1, 1)]
+    w["species"] = (rng.random((N, 3)) * [2.0, 1.0, 3.0]).astype(np.float32)
+    w["fluxes"] = [(0, 0, 0, 0, 0, 1.0, 0.0, 0.0), (0, 1, 1, 0, 0, 1.0, 0.0, 0.0),
+                   (0, 0, 1, 0, 0, 0.5, 0.01, 0.0), (1, 0, 1, 1, 1, 0.3, 0.0, 0.5),
+                   (2, 1, 0, 2, 2, 0.2, 0.0, 2.0)]
+    return w
 
 
 def cell_sort_replica(N, seed, kT=0.1, dt=0.01, density=0.8):
