@@ -241,6 +241,7 @@ class Engine {
   int integrator_ = 0;
   int cap_ = 12, pmax_ = 0, pmax_req_ = 0, hmax_ = 0, hdmax_ = 0, noff_ = 0, nbricks_ = 0;
   bool brick_dpd_ = false;
+  bool brick_flux_ = false;
   std::vector<double> mean_cell_count_;
   int block_ = 256;
   bool any_dpd_ = false, any_table_ = false, any_random_ = false, any_sb_ = false, any_berendsen_ = false;

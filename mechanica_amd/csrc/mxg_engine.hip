@@ -265,6 +265,15 @@ struct StepArgs {
   const DevWall G *__restrict__ walls;
   const int G *__restrict__ wallpot;
   const DevPot G *__restrict__ wpots;
+  // species transport (flux mode): values and accumulators in (buffer 0)
+  // and out (buffer 1), the flux CSR by ordered type pair
+  const float4 G *__restrict__ spec;
+  const float4 G *__restrict__ specq;
+  float4 G *__restrict__ spec_out;
+  float4 G *__restrict__ specq_out;
+  const DevFlux G *__restrict__ fluxes;
+  const int G *__restrict__ flux_off;
+  double dt_d, cutoff_d;
   const int G *__restrict__ toff;   // term-list offsets (null without terms)
   const float4 G *__restrict__ tf;  // term-list entries
   float dt;
@@ -319,6 +328,7 @@ enum : int {
   kModeDPD = 2,
   kModeVirial = 4,
   kModeBonds = 8,
+  kModeFlux = 16,
 };
 
 // Uniform noise of zero mean and unit variance on [-sqrt(3), sqrt(3)).
@@ -1153,102 +1163,78 @@ __global__ void __launch_bounds__(256) k_term_energy(const int4 *__restrict__ tb
   if (threadIdx.x < 3) out[3 * r + threadIdx.x] = threadIdx.x == 2 ? -red[2][0] : red[threadIdx.x][0];
 }
 
-__device__ __forceinline__ float &comp(float4 &v, int k) {
-  return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
-}
-
+// Component k of a float4 by selects (a runtime index into a register
+// vector would otherwise send it to scratch memory).
 __device__ __forceinline__ float comp(const float4 &v, int k) {
   return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
 }
 
-struct FluxArgs {
-  const DevFlux *__restrict__ fluxes;
-  const int *__restrict__ flux_off;
-  const float4 *__restrict__ spec;   // values at the start of the step (buffer 0)
-  const float4 *__restrict__ specq;  // accumulators carried in (buffer 0)
-  float4 *__restrict__ spec_out;     // buffer 1
-  float4 *__restrict__ specq_out;
-  const int *__restrict__ cell_out;  // new cell of each slot, bit 31 set for movers
-  const int *__restrict__ cell_rep;
-  int nslots;
-  int integrate;                     // forward Euler: integrate the stayers
-  double dt;
-  double cutoff;
-};
+__device__ __forceinline__ void add_comp(float4 &v, int k, float d, float e) {
+  // v[k] = v[k] + d - e, as the reference orders it (flux_eval.hpp:79-80)
+  const float n = comp(v, k) + d - e;
+  v.x = k == 0 ? n : v.x;
+  v.y = k == 1 ? n : v.y;
+  v.z = k == 2 ? n : v.z;
+  v.w = k == 3 ? n : v.w;
+}
 
-// Species transport (flux_eval_ex, flux_eval.hpp:33-81, over the pair set
-// of runner_dopair/runner_doself) and its integration
-// (MxFluxes_Integrate, Flux.cpp:261-284), one thread per particle after the
-// force kernel: every in-cutoff neighbour of a type pair with fluxes adds
-// this particle's share, -q - decay/2 s_i as the flux's a particle, q -
-// decay/2 s_j as its b particle, with q = (1 - r/cutoff)^2 times the kind's
-// rate; the roles follow the types (the reference's i particle for equal
-// types, the lower slot within a cell).  The reference integrates a cell's
-// particles after that cell's advance, so a particle that changed cells
-// keeps its accumulated flux for the next step; stayers add dt q (not for
-// constant species) and clear it.  Under RK4 nothing is integrated
-// (engine_advance_runge_kutta_4 calls no MxFluxes_Integrate).
-__global__ void __launch_bounds__(256) k_flux(const StepArgs *__restrict__ Ap, FluxArgs F) {
-  const StepArgs &A = *Ap;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= F.nslots) return;
-  const float4 xs = A.x[s];
-  const int ti = __float_as_int(xs.w) & 0xffff;
-  const int c = A.cell[s];
-  const DevReplica R = A.rep[F.cell_rep[c]];
-  int lx, ly, lz;
-  cell_coords(c - R.cell_base, R, lx, ly, lz);
-  const float4 me = F.spec[s];
-  float4 q = F.specq[s];
-  const LoopConsts C = loop_consts(A);
-  const int nt = A.ntypes;
-  for_each_neighbour(A, C, R, s, lx, ly, lz, xs,
-                     [&](int qn, bool p_is_i, bool self, float, float, float, float r2, const float4 &xq) {
-                       const int tn = __float_as_int(xq.w) & 0xffff;
-                       const int f0 = F.flux_off[ti * nt + tn], f1 = F.flux_off[ti * nt + tn + 1];
-                       if (f0 == f1) return;
-                       const bool me_i = self ? (s < qn) : p_is_i;  // this particle is the pair's i
-                       const int type_i = me_i ? ti : tn, type_j = me_i ? tn : ti;
-                       const float r = sqrtf(r2);
-                       float term = (float)(1. - (double)r / F.cutoff);
-                       term = term * term;
-                       const float4 other = F.spec[qn];
-                       for (int e = f0; e < f1; ++e) {
-                         const DevFlux X = F.fluxes[e];
-                         const bool pi_is_i = type_i == X.ta;  // pi = part_i if its type is a
-                         const bool pj_is_j = type_j == X.tb;  // pj = part_j if its type is b
-                         const bool me_pi = pi_is_i ? me_i : !me_i;
-                         const bool me_pj = pj_is_j ? !me_i : me_i;
-                         const float ssi = comp(me_pi ? me : other, X.ia);
-                         const float ssj = comp(me_pj ? me : other, X.ib);
-                         float rate;
-                         if (X.kind == 0) {
-                           rate = X.coef * (ssi - ssj);
-                         } else if (X.kind == 1) {
-                           const float qq = X.coef * (ssi - X.target);
-                           rate = (float)(qq > 0.f) * qq;
-                         } else {
-                           const float qq = X.coef * (X.target - ssj) * ssi;
-                           rate = (float)(qq > 0.f) * qq;
-                         }
-                         const float qf = term * rate;
-                         const float hd = X.decay / 2.f;
-                         if (me_pi) comp(q, X.ia) = comp(q, X.ia) - qf - hd * ssi;
-                         if (me_pj) comp(q, X.ib) = comp(q, X.ib) + qf - hd * ssj;
-                       }
-                     });
-  float4 v = me;
-  if (F.integrate && F.cell_out[s] >= 0) {
-    const int cm = A.types[ti].spec_const;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const float konst = (cm >> k) & 1 ? 0.f : 1.f;
-      comp(v, k) = (float)((double)comp(v, k) + F.dt * (double)comp(q, k) * (double)konst);
-      comp(q, k) = 0.0f;
+// flux_eval_ex (flux_eval.hpp:33-81) for one in-cutoff pair of the
+// reference's pair set, this particle's share: every flux of the pair's
+// types adds -q - decay/2 s_i to the flux's a particle and q - decay/2 s_j
+// to its b particle, q = (1 - r/cutoff)^2 times the kind's rate.  The roles
+// follow the types; for equal types the pair's i particle is a (me_i: the
+// reference orientation, or the lower slot within a cell).
+__device__ __forceinline__ void flux_pair(const DevFlux G *fluxes, const int G *flux_off, int nt, double cutoff,
+                                          int ti, int tj, bool me_i, float r, const float4 &me,
+                                          const float4 &other, float4 &q) {
+  const int f0 = flux_off[ti * nt + tj], f1 = flux_off[ti * nt + tj + 1];
+  if (f0 == f1) return;
+  float term = (float)(1. - (double)r / cutoff);
+  term = term * term;
+  const int type_i = me_i ? ti : tj, type_j = me_i ? tj : ti;
+  for (int e = f0; e < f1; ++e) {
+    const DevFlux X = fluxes[e];
+    const bool pi_is_i = type_i == X.ta;  // pi = part_i if its type is a
+    const bool pj_is_j = type_j == X.tb;  // pj = part_j if its type is b
+    const bool me_pi = pi_is_i ? me_i : !me_i;
+    const bool me_pj = pj_is_j ? !me_i : me_i;
+    const float ssi = me_pi ? comp(me, X.ia) : comp(other, X.ia);
+    const float ssj = me_pj ? comp(me, X.ib) : comp(other, X.ib);
+    float rate;
+    if (X.kind == 0) {
+      rate = X.coef * (ssi - ssj);
+    } else if (X.kind == 1) {
+      const float qq = X.coef * (ssi - X.target);
+      rate = (float)(qq > 0.f) * qq;
+    } else {
+      const float qq = X.coef * (X.target - ssj) * ssi;
+      rate = (float)(qq > 0.f) * qq;
     }
+    const float qf = term * rate;
+    const float hd = X.decay / 2.f;
+    if (me_pi) add_comp(q, X.ia, -qf, hd * ssi);
+    if (me_pj) add_comp(q, X.ib, qf, hd * ssj);
   }
-  F.spec_out[s] = v;
-  F.specq_out[s] = q;
+}
+
+// MxFluxes_Integrate (Flux.cpp:261-284) for one particle after its advance:
+// the reference integrates a cell's particles after that cell's advance, so
+// a particle that changed cells keeps its accumulated flux for the next
+// step; stayers add dt q (not for constant species) and clear it.  Under
+// RK4 nothing is integrated (engine_advance_runge_kutta_4 calls no
+// MxFluxes_Integrate).
+__device__ __forceinline__ void flux_store(const StepArgs &A, int s, bool integrate, int spec_const, float4 v,
+                                           float4 q) {
+  if (integrate) {
+    auto integ = [&](float s0, float q0, int k) {
+      const float konst = (spec_const >> k) & 1 ? 0.f : 1.f;
+      return (float)((double)s0 + A.dt_d * (double)q0 * (double)konst);
+    };
+    v = make_float4(integ(v.x, q.x, 0), integ(v.y, q.y, 1), integ(v.z, q.z, 2), integ(v.w, q.w, 3));
+    q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  A.spec_out[s] = v;
+  A.specq_out[s] = q;
 }
 
 // apply_update_pos_vel (boundary_eval.hpp:30-119): a particle that left a
@@ -1503,10 +1489,11 @@ __device__ __forceinline__ PairTabs stage_pair_tabs(const StepArgs &A, unsigned 
 }
 
 __host__ __device__ inline size_t brick_lds_bytes(int pmax, int hmax, int hdmax, int noff, int cap, bool dpd,
-                                                  size_t tabs) {
+                                                  bool flux, size_t tabs) {
   size_t b = 0;
   b += (size_t)pmax * 16;              // positions (w: type and flags)
   if (dpd) b += (size_t)pmax * 16;     // velocities (w: particle id)
+  if (flux) b += (size_t)pmax * 16;    // species values
   b += (size_t)noff * sizeof(DevOffset);
   b += (size_t)(hmax + 1) * 4 * 4;     // halo starts, global starts, home starts, home stayers
   b += (size_t)3 * hdmax * (2 * kSpanMax + 1) * 4;  // shifts
@@ -1554,9 +1541,11 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
   const StepArgs &A = *Ap;
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool kDPD = (MODE & kModeDPD) != 0;
+  constexpr bool kFlux = (MODE & kModeFlux) != 0;
   float4 *hx = reinterpret_cast<float4 *>(smem);
   float4 *hv = hx + B.pmax;
-  DevOffset *hoff = reinterpret_cast<DevOffset *>(hv + (kDPD ? B.pmax : 0));
+  float4 *hsp = hv + (kDPD ? B.pmax : 0);  // species values (flux mode)
+  DevOffset *hoff = reinterpret_cast<DevOffset *>(hsp + (kFlux ? B.pmax : 0));
   int *hstart = reinterpret_cast<int *>(hoff + B.noff);
   int *hgb = hstart + (B.hmax + 1);
   int *homestart = hgb + (B.hmax + 1);
@@ -1574,6 +1563,9 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
   const PairTabs tb = stage_pair_tabs(A, reinterpret_cast<unsigned char *>(nlist + B.cap * kBT), A.npots);
   const LoopConsts C = loop_consts(A);
   (void)kOffMax;
+  const DevFlux G *const fluxes = kFlux ? A.fluxes : nullptr;
+  const int G *const flux_off = kFlux ? A.flux_off : nullptr;
+  const double flux_cut = kFlux ? A.cutoff_d : 0.0;
 
   for (int g = blockIdx.x; g < B.nbricks; g += gridDim.x) {
     const int bi = xcd_brick(g, B.nbricks);
@@ -1680,6 +1672,7 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
       const int gs = hgb[h] + (f - hstart[h]);
       hx[f] = A.x[gs];
       if (kDPD) hv[f] = A.v[gs];
+      if (kFlux) hsp[f] = A.spec[gs];
     }
     __syncthreads();
     const int nhp = homestart[nhome];
@@ -1700,6 +1693,8 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
       const int c = R.cell_base + (lx * R.cdim[1] + ly) * R.cdim[2] + lz;
       const float4 xs = hx[p];
       const float4 vs = kDPD ? float4(hv[p]) : float4(A.v[s]);
+      const float4 sme = kFlux ? float4(hsp[p]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      float4 sq = kFlux ? float4(A.specq[s]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       const int mypid = __float_as_int(vs.w);
       const int ti = __float_as_int(xs.w) & 0xffff;
       const DevType T = A.types[ti];
@@ -1757,6 +1752,9 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
               if (pair_force<MODE>(C, tb, nb, ti, tj, side >= 0, mypid, q, side == 0, dx, dy, dz, r2, r, vs, fx, fy,
                                    fz, acc, t1, tk))
                 nd += w_cubic(r, C.cutoff, C.nd_norm);
+              if (kFlux)
+                flux_pair(fluxes, flux_off, C.ntypes, flux_cut, ti, tj, side == 0 ? p < q : side > 0, r, sme,
+                          hsp[q], sq);
             }
           };
           // Gather in-cutoff neighbours into the per-thread list, process
@@ -1848,6 +1846,7 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
                          : rk_store(A, R, B.rk_stage, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs,
                                     mypid, nd);
       if (stayed) atomicAdd(&home_stay[hc], 1);
+      if (kFlux) flux_store(A, s, stayed && B.rk_stage < 0, T.spec_const, sme, sq);
     }
 
     // Deterministic block reduction of the FP64 partials for this brick.
@@ -2148,6 +2147,10 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
   const uint32_t tk = step_key(A.seed_lo, t1);
   const PairTabs tb = stage_pair_tabs(A, gsmem, A.npots);
   const LoopConsts C = loop_consts(A);
+  constexpr bool kFlux = (MODE & kModeFlux) != 0;
+  const DevFlux G *const fluxes = kFlux ? A.fluxes : nullptr;
+  const int G *const flux_off = kFlux ? A.flux_off : nullptr;
+  const double flux_cut = kFlux ? A.cutoff_d : 0.0;
   __syncthreads();
   const int nov = *B.ovf_count;
   for (int i = blockIdx.x; i < nov; i += gridDim.x) {
@@ -2198,6 +2201,8 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
       const float4 xs = A.x[s];
       const float4 vs = A.v[s];
       const int mypid = A.pid[s];
+      const float4 sme = kFlux ? float4(A.spec[s]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      float4 sq = kFlux ? float4(A.specq[s]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       const int ti = __float_as_int(xs.w) & 0xffff;
       const DevType T = A.types[ti];
       {
@@ -2223,6 +2228,9 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
                              if (pair_force<MODE>(C, tb, nb, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, r, vs,
                                                   fx, fy, fz, acc, t1, tk))
                                nd += w_cubic(r, C.cutoff, C.nd_norm);
+                             if (kFlux)
+                               flux_pair(fluxes, flux_off, C.ntypes, flux_cut, ti, tj, self ? s < q : p_is_i, r, sme,
+                                         float4(A.spec[q]), sq);
                            });
         if (MODE & kModeBonds) {
           bond_forces<MODE>(A, R, s, mypid, fx, fy, fz, acc);
@@ -2234,6 +2242,7 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
                          : rk_store(A, R, B.rk_stage, s, c, lx, ly, lz, xs, vs, T, passthrough, fx, fy, fz, pfs,
                                     mypid, nd);
       if (stayed) atomicAdd(&home_stay[hc], 1);
+      if (kFlux) flux_store(A, s, stayed && B.rk_stage < 0, T.spec_const, sme, sq);
     }
 #pragma unroll
     for (int k = 0; k < kNRed; ++k) {
@@ -2264,12 +2273,16 @@ void launch_step_mode(const StepArgs *A, const BrickArgs &B, size_t lds, size_t 
 
 using StepLauncher = void (*)(const StepArgs *, const BrickArgs &, size_t, size_t, hipStream_t);
 
-// One instantiation per combination of table / DPD / virial / bonds.
-const StepLauncher kStepLaunchers[16] = {
+// One instantiation per combination of table / DPD / virial / bonds / flux.
+const StepLauncher kStepLaunchers[32] = {
     launch_step_mode<0>,  launch_step_mode<1>,  launch_step_mode<2>,  launch_step_mode<3>,
     launch_step_mode<4>,  launch_step_mode<5>,  launch_step_mode<6>,  launch_step_mode<7>,
     launch_step_mode<8>,  launch_step_mode<9>,  launch_step_mode<10>, launch_step_mode<11>,
     launch_step_mode<12>, launch_step_mode<13>, launch_step_mode<14>, launch_step_mode<15>,
+    launch_step_mode<16>, launch_step_mode<17>, launch_step_mode<18>, launch_step_mode<19>,
+    launch_step_mode<20>, launch_step_mode<21>, launch_step_mode<22>, launch_step_mode<23>,
+    launch_step_mode<24>, launch_step_mode<25>, launch_step_mode<26>, launch_step_mode<27>,
+    launch_step_mode<28>, launch_step_mode<29>, launch_step_mode<30>, launch_step_mode<31>,
 };
 
 }  // namespace
@@ -2826,7 +2839,7 @@ struct BrickFit {
 };
 
 static BrickFit brick_fit(const std::vector<HostReplica> &reps, const std::vector<double> &mean, const int bd[3],
-                          int cap, bool dpd, size_t tabs, size_t budget) {
+                          int cap, bool dpd, bool flux, size_t tabs, size_t budget) {
   BrickFit F;
   for (size_t r = 0; r < reps.size(); ++r) {
     int hc = 1;
@@ -2844,7 +2857,7 @@ static BrickFit brick_fit(const std::vector<HostReplica> &reps, const std::vecto
   want = (want + 31) / 32 * 32;
   int fit = 0;
   for (int pm = 32; pm <= 16384; pm += 32)
-    if (brick_lds_bytes(pm, F.hmax, F.hdmax, F.noff, cap, dpd, tabs) <= budget) fit = pm;
+    if (brick_lds_bytes(pm, F.hmax, F.hdmax, F.noff, cap, dpd, flux, tabs) <= budget) fit = pm;
   F.pmax = std::min(want, fit);
   return F;
 }
@@ -2871,7 +2884,7 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
     bool ok = false;
     for (const auto &sh : shapes) {
       for (int k = 0; k < 3; ++k) bd[k] = sh[k];
-      fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, tabs, budget);
+      fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, species_on_, tabs, budget);
       if (fitv.pmax >= 1.1 * fitv.worst && fitv.hmax <= kScanItems * kBT) {
         ok = true;
         break;
@@ -2879,11 +2892,11 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
     }
     if (!ok) {
       for (int k = 0; k < 3; ++k) bd[k] = 1;
-      fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, tabs, budget);
+      fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, species_on_, tabs, budget);
     }
   } else {
     if (bx < 1 || by < 1 || bz < 1) return reject(kErrRange, "brick dimensions must be positive");
-    fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, tabs, budget);
+    fitv = brick_fit(reps_, mean_cell_count_, bd, c, dpd, species_on_, tabs, budget);
   }
   hmax = fitv.hmax;
   pm = fitv.pmax;
@@ -2922,6 +2935,7 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
   noff_ = fitv.noff;
   nbricks_ = (int)bricks.size();
   brick_dpd_ = dpd;
+  brick_flux_ = species_on_;
   brick_tabs_ = tabs;
   if (int st = dalloc(d, &d->ovf_list, bricks.size())) return check_device(st, "hipMalloc");
   if (!d->ovf_count) {
@@ -3387,6 +3401,14 @@ void Engine::fill_step_args(void *out, int par) const {
   A.bact_in = (decltype(A.bact_in))(M_ > 0 ? d->bact[par] : nullptr);
   A.bact_out = (decltype(A.bact_out))(M_ > 0 ? d->bact[par ^ 1] : nullptr);
   A.slot_of = (decltype(A.slot_of))(d->slot_of);
+  A.spec = (decltype(A.spec))(d->spec[0]);
+  A.specq = (decltype(A.specq))(d->specq[0]);
+  A.spec_out = (decltype(A.spec_out))(d->spec[1]);
+  A.specq_out = (decltype(A.specq_out))(d->specq[1]);
+  A.fluxes = (decltype(A.fluxes))(d->fluxes);
+  A.flux_off = (decltype(A.flux_off))(d->flux_off);
+  A.dt_d = dt_;
+  A.cutoff_d = cutoff_;
   A.walls = (decltype(A.walls))(d->walls);
   A.wallpot = (decltype(A.wallpot))(d->wallpot);
   A.wpots = (decltype(A.wpots))(d->wpots);
@@ -3451,6 +3473,7 @@ int Engine::launch_substep(int rk_stage, int tick) {
   if (any_dpd_) mode |= kModeDPD;
   if (virial_) mode |= kModeVirial;
   if (M_ > 0 || NT_ > 0) mode |= kModeBonds;
+  if (species_on_) mode |= kModeFlux;
   hipStream_t s = d->stream;
   if (any_berendsen_) {
     hipLaunchKernelGGL(k_type_ke, dim3(kKeBlocks), dim3(256), 0, s, d->x[0], d->v[0], d->types, N_,
@@ -3497,27 +3520,11 @@ int Engine::launch_substep(int rk_stage, int tick) {
   Bk.ovf_count = d->ovf_count;
   Bk.rk_stage = rk_stage;
   const size_t tabs = pair_tab_bytes((int)pots_.size(), (int)types_.size());
-  const size_t lds = brick_lds_bytes(pmax_, hmax_, hdmax_, noff_, cap_, (mode & kModeDPD) != 0, tabs);
+  const size_t lds =
+      brick_lds_bytes(pmax_, hmax_, hdmax_, noff_, cap_, (mode & kModeDPD) != 0, (mode & kModeFlux) != 0, tabs);
   kStepLaunchers[mode](reinterpret_cast<const StepArgs *>(d->dargs) + par, Bk, lds, tabs, s);
   if (rec) hipEventRecord(d->kev[2 * timed_steps_ + 1], s);
   if (rec) ++timed_steps_;
-  if (species_on_) {
-    FluxArgs F;
-    F.fluxes = d->fluxes;
-    F.flux_off = d->flux_off;
-    F.spec = d->spec[0];
-    F.specq = d->specq[0];
-    F.spec_out = d->spec[1];
-    F.specq_out = d->specq[1];
-    F.cell_out = d->cell[1];
-    F.cell_rep = d->cell_rep;
-    F.nslots = N_;
-    F.integrate = rk_stage < 0 ? 1 : 0;
-    F.dt = dt_;
-    F.cutoff = cutoff_;
-    hipLaunchKernelGGL(k_flux, dim3((N_ + 255) / 256), dim3(256), 0, s,
-                       reinterpret_cast<const StepArgs *>(d->dargs) + par, F);
-  }
 
   // Cell-list rebuild: x, v, pf, cell go from buffer 1 back to buffer 0;
   // ids and cell offsets go from parity par to par ^ 1.
@@ -3683,7 +3690,8 @@ int Engine::step(int nsteps) {
     if (int rc = upload_potentials()) return rc;
   // Potentials bound after the load can change the LDS layout (velocities
   // are staged for DPD only).
-  if (brick_dpd_ != any_dpd_ || brick_tabs_ != pair_tab_bytes((int)pots_.size(), (int)types_.size()))
+  if (brick_dpd_ != any_dpd_ || brick_flux_ != species_on_ ||
+      brick_tabs_ != pair_tab_bytes((int)pots_.size(), (int)types_.size()))
     if (int rc = set_brick(bdim_[0], bdim_[1], bdim_[2], cap_, pmax_req_)) return rc;
   if (args_dirty_)
     if (int rc = refresh_args()) return rc;
