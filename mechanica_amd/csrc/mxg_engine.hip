@@ -1463,6 +1463,10 @@ struct BrickArgs {
 
 constexpr int kBT = 256;        // threads per brick block
 constexpr int kChunk = 2;       // candidates loaded together in the scan
+// 1/sqrt(2) and 1/sqrt(3) rounded to float, as (float)(1.0 / sqrt(k)) on the
+// host, the unit forward directions of the offset table.
+constexpr float kInvSqrt2 = 0.70710677f;
+constexpr float kInvSqrt3 = 0.57735026f;
 constexpr int kHaloAxis = 64;   // max halo cells per axis
 constexpr int kSpanMax = 2;     // max cell span
 constexpr int kOffMax = 125;    // (2 * kSpanMax + 1)^3
@@ -1739,9 +1743,21 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
               // Self pairs: x_i - x_j; others (a - shift) - b in the
               // reference orientation (x - 0 is exact, so one form serves).
               float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-              if (side != 0)
-                shift_of((int)((ent >> 6) & 7u) - 2, (int)((ent >> 3) & 7u) - 2, (int)(ent & 7u) - 2, side, sx, sy,
-                         sz);
+              if (side != 0) {
+                const int ox = (int)((ent >> 6) & 7u) - 2, oy = (int)((ent >> 3) & 7u) - 2, oz = (int)(ent & 7u) - 2;
+                shift_of(ox, oy, oz, side, sx, sy, sz);
+                // runner_dopair's sort-key window (runner_dopair.cpp:170,182) along the
+                // pair's forward direction f = side * sign(ox, oy, oz), n = f / |f| as the
+                // offset table holds it (components 0 or +-1/sqrt(|f|^2)).
+                const int gx = (ox > 0) - (ox < 0), gy = (oy > 0) - (oy < 0), gz = (oz > 0) - (oz < 0);
+                const int l2 = gx * gx + gy * gy + gz * gz;
+                const float il = l2 == 1 ? 1.0f : (l2 == 2 ? kInvSqrt2 : kInvSqrt3);
+                const float nx = (float)(side * gx) * il, ny = (float)(side * gy) * il, nz = (float)(side * gz) * il;
+                const unsigned dnshift = (unsigned)(int)(R.dscale * sqrtf(sx * sx + sy * sy + sz * sz));
+                const unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, nx, ny, nz);
+                const unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, nx, ny, nz);
+                if (!(side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax))) continue;
+              }
               const bool fwd = side >= 0;
               const float ax = fwd ? xs.x : xq.x, ay = fwd ? xs.y : xq.y, az = fwd ? xs.z : xq.z;
               const float bx = fwd ? xq.x : xs.x, by = fwd ? xq.y : xs.y, bz = fwd ? xq.z : xs.z;
@@ -1796,9 +1812,6 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
                 shift_of(off.ox, off.oy, off.oz, off.side, sx, sy, sz);
                 const unsigned geo = ((unsigned)(off.side + 1) << 9) | ((unsigned)(off.ox + 2) << 6) |
                                      ((unsigned)(off.oy + 2) << 3) | (unsigned)(off.oz + 2);
-                const float nshift = sqrtf(sx * sx + sy * sy + sz * sz);
-                const unsigned dnshift = (unsigned)(int)(R.dscale * nshift);
-                const unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, off.nx, off.ny, off.nz);
                 for (; q < qe; q += kChunk) {
                   if (nn > B.cap - kChunk) {
                     full = true;
@@ -1816,14 +1829,11 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
                     const float bx = fwd ? xq[u].x : xs.x, by = fwd ? xq[u].y : xs.y, bz = fwd ? xq[u].z : xs.z;
                     const float dx = (ax - sx) - bx, dy = (ay - sy) - by, dz = (az - sz) - bz;
                     const float r2 = dx * dx + dy * dy + dz * dz;
+                    // In-cutoff candidates go to the list; the sort-key window
+                    // of runner_dopair is applied when the list is processed.
                     if (q + u < qe && r2 <= C.cut2_pair) {
-                      const unsigned kq =
-                          sort_key(R.bias, R.dscale, xq[u].x, xq[u].y, xq[u].z, off.nx, off.ny, off.nz);
-                      const bool pass = off.side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax);
-                      if (pass) {
-                        nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
-                        ++nn;
-                      }
+                      nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
+                      ++nn;
                     }
                   }
                 }
