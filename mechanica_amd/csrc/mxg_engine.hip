@@ -69,6 +69,11 @@ struct DevReplica {
   // Bits 0-2: axis x, y, z periodic.  Bits 3-8: the left, right, front,
   // back, bottom, top walls are active (free-slip, velocity or potential).
   int wall;
+  // A particle can take part in dropping a cell pair (see DevOffset::flags)
+  // only if each coordinate along the pair's axes lies within near_eps of
+  // the cell's bound; drop_check is set when any offset can drop.
+  float near_eps;
+  int drop_check;
 };
 
 enum : int { kWallPeriodicShift = 0, kWallActiveShift = 3 };
@@ -88,11 +93,33 @@ struct DevWall {
 // A neighbour-cell offset.  side: 0 same cell, +1 this cell is the
 // reference's i cell, -1 the neighbour is.  n: unit direction of the
 // forward shift (the sort axis runner_dosort uses for this pair).
+// flags, bit 31: the sort keys of this offset can drop the whole cell pair
+// (runner_dopair's outer-loop break on a negative key distance); bits 0-5
+// are then the near-bound bits a witness particle of this cell must show and
+// bits 8-13 those of the neighbour cell (bit 2k: coordinate k near 0, bit
+// 2k+1: near the cell width).  Bit 30: the sort-key window can reject a pair
+// inside the cutoff; without it, every in-cutoff pair of a cell pair that is
+// not dropped passes the window.
 struct DevOffset {
   int ox, oy, oz, side;
   float nx, ny, nz;
-  int pad;
+  int flags;
 };
+
+constexpr int kOffDrop = (int)0x80000000u, kOffWindow = 0x40000000;
+
+// Unit direction n = f / |f| of the forward shift f = side * sign(ox, oy, oz)
+// as the offset table holds it: components 0 or +-1/sqrt(|f|^2), with
+// 1/sqrt(2) and 1/sqrt(3) rounded to float as (float)(1.0 / sqrt(k)) on the
+// host.
+__host__ __device__ inline void offset_dir(int ox, int oy, int oz, int side, float &nx, float &ny, float &nz) {
+  const int gx = (ox > 0) - (ox < 0), gy = (oy > 0) - (oy < 0), gz = (oz > 0) - (oz < 0);
+  const int l2 = gx * gx + gy * gy + gz * gz;
+  const float il = l2 == 1 ? 1.0f : (l2 == 2 ? 0.70710677f : 0.57735026f);
+  nx = (float)(side * gx) * il;
+  ny = (float)(side * gy) * il;
+  nz = (float)(side * gz) * il;
+}
 
 struct DevPot {
   uint32_t kind, flags;
@@ -421,6 +448,35 @@ __device__ __forceinline__ unsigned sort_key(float bias, float dscale, float x, 
   return ((unsigned)(dscale * (bias + x * nx + y * ny + z * nz))) & 0xffffu;
 }
 
+// True when runner_dopair yields no pair for the cell pair with i-cell
+// particles x[ib, ie) and j-cell particles x[jb, je) (its loop over the i
+// cell stops at once, runner_dopair.cpp:164-166): the smallest j key plus
+// dnshift minus the largest i key is negative.  (sx, sy, sz): the pair's
+// shift in either orientation (only its norm enters).
+__device__ __forceinline__ bool pair_dropped(const DevReplica &R, float nx, float ny, float nz, float sx, float sy,
+                                             float sz, const float4 *x, int ib, int ie, int jb, int je) {
+  if (ib >= ie || jb >= je) return false;
+  unsigned kimax = 0u, kjmin = 0xffffu;
+  for (int q = ib; q < ie; ++q) {
+    const float4 p = x[q];
+    kimax = max(kimax, sort_key(R.bias, R.dscale, p.x, p.y, p.z, nx, ny, nz));
+  }
+  for (int q = jb; q < je; ++q) {
+    const float4 p = x[q];
+    kjmin = min(kjmin, sort_key(R.bias, R.dscale, p.x, p.y, p.z, nx, ny, nz));
+  }
+  const int dn = (int)(R.dscale * sqrtf(sx * sx + sy * sy + sz * sz));
+  return (int)kjmin + dn - (int)kimax < 0;
+}
+
+// Near-bound bits of a cell-relative position (bit 2k: coordinate k below
+// eps, bit 2k+1: above the cell width minus eps).
+__device__ __forceinline__ int near_bits(const DevReplica &R, const float4 &p) {
+  const float e = R.near_eps;
+  return (int)(p.x < e) | ((int)(p.x > R.h[0] - e) << 1) | ((int)(p.y < e) << 2) | ((int)(p.y > R.h[1] - e) << 3) |
+         ((int)(p.z < e) << 4) | ((int)(p.z > R.h[2] - e) << 5);
+}
+
 // Cubic-spline number-density kernel W(r^2, h) (smoothing_kernel.hpp:28-50)
 // with the exact square root (the reference uses an approximate x86 rsqrt).
 __device__ __forceinline__ float w_cubic(float r, float h, float inv_norm) {
@@ -666,6 +722,13 @@ __device__ __forceinline__ void for_each_neighbour(const StepArgs &A, const Loop
       sx = A.shifts[R.shift_base[0] + nx * w0 + (-off.ox + R.span[0])];
       sy = A.shifts[R.shift_base[1] + ny * w1 + (-off.oy + R.span[1])];
       sz = A.shifts[R.shift_base[2] + nz * w2 + (-off.oz + R.span[2])];
+    }
+    if (off.flags & kOffDrop) {
+      const int c = R.cell_base + (lx * cd1 + ly) * cd2 + lz;
+      const int hb = A.cstart[c], he = A.cstart[c + 1];
+      if (off.side > 0 ? pair_dropped(R, off.nx, off.ny, off.nz, sx, sy, sz, A.x, hb, he, qb, qe)
+                       : pair_dropped(R, off.nx, off.ny, off.nz, sx, sy, sz, A.x, qb, qe, hb, he))
+        continue;
     }
     float nshift = sqrtf(sx * sx + sy * sy + sz * sz);
     unsigned dnshift = (unsigned)(int)(R.dscale * nshift);
@@ -1463,10 +1526,6 @@ struct BrickArgs {
 
 constexpr int kBT = 256;        // threads per brick block
 constexpr int kChunk = 2;       // candidates loaded together in the scan
-// 1/sqrt(2) and 1/sqrt(3) rounded to float, as (float)(1.0 / sqrt(k)) on the
-// host, the unit forward directions of the offset table.
-constexpr float kInvSqrt2 = 0.70710677f;
-constexpr float kInvSqrt3 = 0.57735026f;
 constexpr int kHaloAxis = 64;   // max halo cells per axis
 constexpr int kSpanMax = 2;     // max cell span
 constexpr int kOffMax = 125;    // (2 * kSpanMax + 1)^3
@@ -1498,8 +1557,8 @@ __host__ __device__ inline size_t brick_lds_bytes(int pmax, int hmax, int hdmax,
   b += (size_t)pmax * 16;              // positions (w: type and flags)
   if (dpd) b += (size_t)pmax * 16;     // velocities (w: particle id)
   if (flux) b += (size_t)pmax * 16;    // species values
-  b += (size_t)noff * sizeof(DevOffset);
-  b += (size_t)(hmax + 1) * 4 * 4;     // halo starts, global starts, home starts, home stayers
+  b += (size_t)noff * 16;              // offsets: halo-index delta, list geometry, flags
+  b += (size_t)(hmax + 1) * 4 * 5;     // halo starts, global starts, home starts, home stayers, near bits
   b += (size_t)3 * hdmax * (2 * kSpanMax + 1) * 4;  // shifts
   b += (size_t)cap * kBT * 4;          // neighbour lists
   b += tabs;                           // potentials, type pairs, radii
@@ -1549,12 +1608,16 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
   float4 *hx = reinterpret_cast<float4 *>(smem);
   float4 *hv = hx + B.pmax;
   float4 *hsp = hv + (kDPD ? B.pmax : 0);  // species values (flux mode)
-  DevOffset *hoff = reinterpret_cast<DevOffset *>(hsp + (kFlux ? B.pmax : 0));
+  // Per offset: halo-index delta, the low 12 bits of its list entries
+  // (window << 11 | (side + 1) << 9 | (ox + 2) << 6 | (oy + 2) << 3 |
+  // (oz + 2)) and DevOffset::flags.
+  int4 *hoff = reinterpret_cast<int4 *>(hsp + (kFlux ? B.pmax : 0));
   int *hstart = reinterpret_cast<int *>(hoff + B.noff);
   int *hgb = hstart + (B.hmax + 1);
   int *homestart = hgb + (B.hmax + 1);
   int *home_stay = homestart + (B.hmax + 1);  // stayers per home cell
-  float *hs = reinterpret_cast<float *>(home_stay + (B.hmax + 1));
+  int *hmask = home_stay + (B.hmax + 1);      // near-bound bits per halo cell
+  float *hs = reinterpret_cast<float *>(hmask + (B.hmax + 1));
   unsigned *nlist = reinterpret_cast<unsigned *>(hs + 3 * B.hdmax * (2 * kSpanMax + 1));
   __shared__ int scan_lds[kBT / 64];
   __shared__ double red[kNRed][kBT / 64];
@@ -1604,6 +1667,7 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
           } else {
             hgb[h] = 0;  // beyond a wall: an empty cell
           }
+          hmask[h] = 0;
         }
         local += cnt[i];
       }
@@ -1633,7 +1697,12 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
       hs[(2 * B.hdmax + hl) * (2 * kSpanMax + 1) + d] =
           A.shifts[R.shift_base[2] + wrap(br.w - sp2 + hl, R.cdim[2]) * (2 * sp2 + 1) + d];
     }
-    for (int i = tid; i < R.n_off; i += kBT) hoff[i] = A.offs[R.off_base + i];
+    for (int i = tid; i < R.n_off; i += kBT) {
+      const DevOffset d = A.offs[R.off_base + i];
+      const int geo = ((d.flags & kOffWindow) ? 0x800 : 0) | ((d.side + 1) << 9) | ((d.ox + 2) << 6) |
+                      ((d.oy + 2) << 3) | (d.oz + 2);
+      hoff[i] = make_int4((d.ox * hd1 + d.oy) * hd2 + d.oz, geo, d.flags, 0);
+    }
     __syncthreads();
 
     // 2. Home cells: exclusive scan of their particle counts.
@@ -1674,9 +1743,14 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
     for (int f = tid; f < P; f += kBT) {
       const int h = upper_index(hstart, nhalo + 1, f);
       const int gs = hgb[h] + (f - hstart[h]);
-      hx[f] = A.x[gs];
+      const float4 xf = A.x[gs];
+      hx[f] = xf;
       if (kDPD) hv[f] = A.v[gs];
       if (kFlux) hsp[f] = A.spec[gs];
+      if (R.drop_check) {
+        const int m = near_bits(R, xf);
+        if (m) atomicOr(&hmask[h], m);
+      }
     }
     __syncthreads();
     const int nhp = homestart[nhome];
@@ -1734,8 +1808,8 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
           };
           auto process = [&](int n) {
             for (int e = 0; e < n; ++e) {
-              // Entry: LDS index << 12 | (side + 1) << 9 | (ox + 2) << 6 |
-              // (oy + 2) << 3 | (oz + 2).
+              // Entry: LDS index << 12 | window << 11 | (side + 1) << 9 |
+              // (ox + 2) << 6 | (oy + 2) << 3 | (oz + 2).
               const unsigned ent = nlist[e * kBT + tid];
               const int q = (int)(ent >> 12);
               const int side = (int)((ent >> 9) & 3u) - 1;
@@ -1748,22 +1822,25 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
                 shift_of(ox, oy, oz, side, sx, sy, sz);
                 // runner_dopair's sort-key window (runner_dopair.cpp:170,182) along the
                 // pair's forward direction f = side * sign(ox, oy, oz), n = f / |f| as the
-                // offset table holds it (components 0 or +-1/sqrt(|f|^2)).
-                const int gx = (ox > 0) - (ox < 0), gy = (oy > 0) - (oy < 0), gz = (oz > 0) - (oz < 0);
-                const int l2 = gx * gx + gy * gy + gz * gz;
-                const float il = l2 == 1 ? 1.0f : (l2 == 2 ? kInvSqrt2 : kInvSqrt3);
-                const float nx = (float)(side * gx) * il, ny = (float)(side * gy) * il, nz = (float)(side * gz) * il;
-                const unsigned dnshift = (unsigned)(int)(R.dscale * sqrtf(sx * sx + sy * sy + sz * sz));
-                const unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, nx, ny, nz);
-                const unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, nx, ny, nz);
-                if (!(side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax))) continue;
+                // offset table holds it (components 0 or +-1/sqrt(|f|^2)); only on
+                // offsets where it can reject an in-cutoff pair (kOffWindow).
+                if (ent & 0x800u) {
+                  float nx, ny, nz;
+                  offset_dir(ox, oy, oz, side, nx, ny, nz);
+                  const unsigned dnshift = (unsigned)(int)(R.dscale * sqrtf(sx * sx + sy * sy + sz * sz));
+                  const unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, nx, ny, nz);
+                  const unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, nx, ny, nz);
+                  if (!(side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax))) continue;
+                }
               }
               const bool fwd = side >= 0;
               const float ax = fwd ? xs.x : xq.x, ay = fwd ? xs.y : xq.y, az = fwd ? xs.z : xq.z;
               const float bx = fwd ? xq.x : xs.x, by = fwd ? xq.y : xs.y, bz = fwd ? xq.z : xs.z;
               const float dx = (ax - sx) - bx, dy = (ay - sy) - by, dz = (az - sz) - bz;
               const float r2 = dx * dx + dy * dy + dz * dz;
-              const float r = sqrtf(r2);
+              // Tables index by r and keep the exact root; DPD alone takes the
+              // one-ulp hardware square root.
+              const float r = (MODE & kModeTable) ? sqrtf(r2) : __builtin_amdgcn_sqrtf(r2);
               const int tj = __float_as_int(xq.w) & 0xffff;
               if (pair_force<MODE>(C, tb, nb, ti, tj, side >= 0, mypid, q, side == 0, dx, dy, dz, r2, r, vs, fx, fy,
                                    fz, acc, t1, tk))
@@ -1777,18 +1854,43 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
           // the list when it is full or the scan is done (one call site, so
           // the pair code is instantiated once).
           int o = 0, q = -1;
+          const int hm = R.drop_check ? hmask[h] : 0;  // this cell's near-bound bits
           for (;;) {
             int nn = 0;
             while (o < R.n_off) {
-              const DevOffset off = hoff[o];
-              const int h2 = h + (off.ox * hd1 + off.oy) * hd2 + off.oz;
+              const int4 off = hoff[o];
+              const int h2 = h + off.x;
+              const unsigned geo = (unsigned)off.y;
+              const int oside = (int)((geo >> 9) & 3u) - 1;
               const int qb = hstart[h2], qe = hstart[h2 + 1];
-              if (q < 0) q = qb;
+              if (q < 0) {
+                q = qb;
+                // A cell pair that runner_dopair drops whole: only when both
+                // cells hold particles near the bounds the offset's keys
+                // favour (rare), the exact key test.
+                if ((off.z & kOffDrop) && (hm & off.z & 0x3f) == (off.z & 0x3f)) {
+                  const int need = (off.z >> 8) & 0x3f;
+                  if ((hmask[h2] & need) == need) {
+                    const int ox = (int)((geo >> 6) & 7u) - 2, oy = (int)((geo >> 3) & 7u) - 2,
+                              oz = (int)(geo & 7u) - 2;
+                    float sx, sy, sz, nx, ny, nz;
+                    shift_of(ox, oy, oz, oside, sx, sy, sz);
+                    offset_dir(ox, oy, oz, oside, nx, ny, nz);
+                    const int hb = hstart[h], he = hstart[h + 1];
+                    if (oside > 0 ? pair_dropped(R, nx, ny, nz, sx, sy, sz, hx, hb, he, qb, qe)
+                                  : pair_dropped(R, nx, ny, nz, sx, sy, sz, hx, qb, qe, hb, he)) {
+                      ++o;
+                      q = -1;
+                      continue;
+                    }
+                  }
+                }
+              }
               bool full = false;
               // Candidates go in chunks of kChunk: their LDS loads are issued
               // together, and the list is flushed before a chunk that might
               // not fit, so the visiting order is unchanged.
-              if (off.side == 0) {
+              if (oside == 0) {
                 for (; q < qe; q += kChunk) {
                   if (nn > B.cap - kChunk) {
                     full = true;
@@ -1802,38 +1904,58 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
                     const float dx = xs.x - xq[u].x, dy = xs.y - xq[u].y, dz = xs.z - xq[u].z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     if (q + u < qe && q + u != p && r2 <= C.cut2_self) {
-                      nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | (1u << 9) | (2u << 6) | (2u << 3) | 2u;
+                      nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
                       ++nn;
                     }
                   }
                 }
               } else {
                 float sx, sy, sz;
-                shift_of(off.ox, off.oy, off.oz, off.side, sx, sy, sz);
-                const unsigned geo = ((unsigned)(off.side + 1) << 9) | ((unsigned)(off.ox + 2) << 6) |
-                                     ((unsigned)(off.oy + 2) << 3) | (unsigned)(off.oz + 2);
-                for (; q < qe; q += kChunk) {
-                  if (nn > B.cap - kChunk) {
-                    full = true;
-                    break;
+                shift_of((int)((geo >> 6) & 7u) - 2, (int)((geo >> 3) & 7u) - 2, (int)(geo & 7u) - 2, oside, sx, sy,
+                         sz);
+                // (a - shift) - b with (a, b) = (i, j) in the reference
+                // orientation.  This cell is i (side +1): a - shift is the
+                // reference's pix, the same for every candidate.  The
+                // neighbour is i (side -1): each candidate is shifted.
+                // In-cutoff candidates go to the list; the sort-key window
+                // of runner_dopair is applied when the list is processed.
+                if (oside > 0) {
+                  const float pix = xs.x - sx, piy = xs.y - sy, piz = xs.z - sz;
+                  for (; q < qe; q += kChunk) {
+                    if (nn > B.cap - kChunk) {
+                      full = true;
+                      break;
+                    }
+                    float4 xq[kChunk];
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) xq[u] = hx[min(q + u, qe - 1)];
+#pragma unroll
+                    for (int u = 0; u < kChunk; ++u) {
+                      const float dx = pix - xq[u].x, dy = piy - xq[u].y, dz = piz - xq[u].z;
+                      const float r2 = dx * dx + dy * dy + dz * dz;
+                      if (q + u < qe && r2 <= C.cut2_pair) {
+                        nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
+                        ++nn;
+                      }
+                    }
                   }
-                  float4 xq[kChunk];
+                } else {
+                  for (; q < qe; q += kChunk) {
+                    if (nn > B.cap - kChunk) {
+                      full = true;
+                      break;
+                    }
+                    float4 xq[kChunk];
 #pragma unroll
-                  for (int u = 0; u < kChunk; ++u) xq[u] = hx[min(q + u, qe - 1)];
+                    for (int u = 0; u < kChunk; ++u) xq[u] = hx[min(q + u, qe - 1)];
 #pragma unroll
-                  for (int u = 0; u < kChunk; ++u) {
-                    // (a - shift) - b with (a, b) = (i, j) in the reference
-                    // orientation: selects instead of a per-lane branch.
-                    const bool fwd = off.side > 0;
-                    const float ax = fwd ? xs.x : xq[u].x, ay = fwd ? xs.y : xq[u].y, az = fwd ? xs.z : xq[u].z;
-                    const float bx = fwd ? xq[u].x : xs.x, by = fwd ? xq[u].y : xs.y, bz = fwd ? xq[u].z : xs.z;
-                    const float dx = (ax - sx) - bx, dy = (ay - sy) - by, dz = (az - sz) - bz;
-                    const float r2 = dx * dx + dy * dy + dz * dz;
-                    // In-cutoff candidates go to the list; the sort-key window
-                    // of runner_dopair is applied when the list is processed.
-                    if (q + u < qe && r2 <= C.cut2_pair) {
-                      nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
-                      ++nn;
+                    for (int u = 0; u < kChunk; ++u) {
+                      const float dx = (xq[u].x - sx) - xs.x, dy = (xq[u].y - sy) - xs.y, dz = (xq[u].z - sz) - xs.z;
+                      const float r2 = dx * dx + dy * dy + dz * dz;
+                      if (q + u < qe && r2 <= C.cut2_pair) {
+                        nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
+                        ++nn;
+                      }
                     }
                   }
                 }
@@ -2397,6 +2519,96 @@ int Engine::init(const ReplicaSpec *specs, int nrep, double cutoff, double dt) {
   return upload_walls();
 }
 
+// sort_key in host float arithmetic (same operations and order).
+static unsigned host_sort_key(const DevReplica &D, const float x[3], const float n[3]) {
+  const float v = D.dscale * (D.bias + x[0] * n[0] + x[1] * n[1] + x[2] * n[2]);
+  return v <= 0.0f ? 0u : ((unsigned)v & 0xffffu);
+}
+
+// runner_dopair (runner_dopair.cpp:164-166) leaves its loop over the i cell
+// when (smallest j key) + dnshift - (i key) exceeds dmaxdist as an unsigned
+// value, and a negative distance wraps to a huge one: if the i particle with
+// the largest key already sees a negative distance, the cell pair yields no
+// pair at all, in the cutoff or not.  Keys are monotone in the position, so
+// bounding them at the cell corners (positions in [-delta, h + delta]) and
+// the shift norm from below over the shift table finds every offset where
+// this can happen (kOffDrop); the others never drop.
+//
+// A dropping pair (i, j) has kj + dn < ki, hence
+//   sum_k |n_k| (g_ik + g_jk) < sum_k |n_k| h_k - |shift| + 2.05 / dscale,
+// with g the distance of each coordinate from the bound the key favours
+// (g_ik = h_k - x_ik for n_k > 0, x_ik for n_k < 0, and the mirror for j).
+// Every such gap is then below near_eps, which makes "both cells hold a
+// particle that near their bounds" a filter that only rare cell pairs pass.
+//
+// Bit 30 (kOffWindow): in a cell pair that is not dropped every key distance
+// kj + dn - ki is at least kjmin + dn - kimax >= 0, so the window can only
+// reject a pair from above.  An in-cutoff pair has
+//   kj + dn - ki < dscale (r + |shift| - shift.n) + 1 + (key rounding),
+// and r <= cutoff: the window matters only where that bound reaches
+// dmaxdist (non-cubic cells, offsets beyond the nearest cells, or a
+// dscale * cutoff just below an integer).
+static void flag_offsets(DevReplica &D, DevOffset *off, const std::vector<float> &shifts, float cut2) {
+  float delta[3];
+  for (int k = 0; k < 3; ++k) delta[k] = D.h[k] * 1e-6f;
+  double slack = 0.0, dmax_delta = std::max(delta[0], std::max(delta[1], delta[2]));
+  const double hmax = std::max(D.h[0], std::max(D.h[1], D.h[2]));
+  // largest separation of a pair that passes r2 <= cut2 (float r2 and
+  // float separations)
+  const double rmax = std::sqrt((double)cut2) * (1.0 + 1e-6) + 2e-6 * hmax;
+  D.drop_check = 0;
+  for (int o = 0; o < D.n_off; ++o) {
+    DevOffset &f = off[o];
+    f.flags = 0;
+    if (f.side == 0) continue;
+    const int od[3] = {f.ox, f.oy, f.oz};
+    const float n[3] = {f.nx, f.ny, f.nz};
+    float xi[3], xj[3], smin[3], smax[3];
+    int need_i = 0, need_j = 0;
+    for (int k = 0; k < 3; ++k) {
+      xi[k] = n[k] > 0 ? D.h[k] + delta[k] : (n[k] < 0 ? -delta[k] : 0.0f);
+      xj[k] = n[k] > 0 ? -delta[k] : (n[k] < 0 ? D.h[k] + delta[k] : 0.0f);
+      if (n[k] > 0) {
+        need_i |= 2 << (2 * k);
+        need_j |= 1 << (2 * k);
+      } else if (n[k] < 0) {
+        need_i |= 1 << (2 * k);
+        need_j |= 2 << (2 * k);
+      }
+      // smallest |shift| of this offset's component over the cell rows
+      const int w = 2 * D.span[k] + 1;
+      float lo = INFINITY, hi = 0.0f;
+      for (int l = 0; l < D.cdim[k]; ++l) {
+        const float a = std::fabs(shifts[D.shift_base[k] + l * w + od[k] + D.span[k]]);
+        lo = std::min(lo, a);
+        hi = std::max(hi, a);
+      }
+      smin[k] = lo;
+      smax[k] = hi;
+    }
+    {
+      double ns = 0.0, dot = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        ns += (double)smax[k] * smax[k];
+        dot += std::fabs((double)n[k]) * smin[k];
+      }
+      const double bound = (double)D.dscale * (rmax + std::sqrt(ns) - dot) + 1.05;
+      if (bound > (double)D.dmaxdist) f.flags |= kOffWindow;
+    }
+    const float nshift = std::sqrt(smin[0] * smin[0] + smin[1] * smin[1] + smin[2] * smin[2]);
+    const int dn = (int)(D.dscale * nshift);
+    const int ki = (int)host_sort_key(D, xi, n), kj = (int)host_sort_key(D, xj, n);
+    if (kj + dn - ki >= 0) continue;
+    const int need_home = f.side > 0 ? need_i : need_j, need_nbr = f.side > 0 ? need_j : need_i;
+    f.flags |= (int)(0x80000000u | (unsigned)need_home | ((unsigned)need_nbr << 8));
+    D.drop_check = 1;
+    double proj = 0.0;
+    for (int k = 0; k < 3; ++k) proj += std::fabs((double)n[k]) * ((double)D.h[k] + 2.0 * delta[k]);
+    slack = std::max(slack, proj - (double)nshift);
+  }
+  D.near_eps = (float)(1.5 * std::sqrt(3.0) * (slack + 2.05 / D.dscale + 6.0 * dmax_delta));
+}
+
 // Offsets, shift tables and per-replica constants.
 int Engine::build_geometry() {
   std::vector<DevReplica> drep(reps_.size());
@@ -2478,10 +2690,11 @@ int Engine::build_geometry() {
           o.nx = (float)(f[0] / len);
           o.ny = (float)(f[1] / len);
           o.nz = (float)(f[2] / len);
-          o.pad = 0;
+          o.flags = 0;
           offs.push_back(o);
         }
     D.n_off = (int)offs.size() - D.off_base;
+    flag_offsets(D, &offs[D.off_base], shifts, cutoff_f * cutoff_f);
   }
   DeviceState *d = d_;
   if (int st = dalloc(d, &d->rep, drep.size())) return check_device(st, "hipMalloc");
