@@ -257,6 +257,7 @@ class Engine {
   bool pid_prev_valid_ = false;  // pid[1] holds the order forces were computed in
   int parity_ = 0;               // which ping-pong ids/offsets/bond flags are current
   bool args_dirty_ = true;        // device copy of the kernel arguments is stale
+  bool slots_valid_ = true;       // device slot_of matches the current slot order
   bool use_graphs_ = true;
   uint64_t config_gen_ = 1;       // bumped when the captured launches change
   uint64_t graph_gen_[2] = {0, 0};

@@ -13,7 +13,7 @@
 //   k_scan_*        exclusive scan of the counts into the new cell offsets
 //   k_place         stayers keep their rank in the cell, movers take a cursor
 //   k_order_incoming  movers of each cell ordered by particle id
-//   k_finalize      per replica: sum the brick partials in brick order
+//   k_finalize      per replica: sum the brick partials in a fixed order
 //
 // Every pair is evaluated from both sides (no Newton-3 write-back), so each
 // particle's force is written once by the thread that owns it and the result
@@ -1989,11 +1989,13 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
       if (lane == 0) red[k][wave] = w;
     }
     __syncthreads();
-    if (tid < kNRed) {
+    // kNRed partials per brick with the virial, 3 without (k_finalize reads
+    // the same stride).
+    constexpr int kPart = (MODE & kModeVirial) ? kNRed : 3;
+    if (tid < kPart) {
       double sum = 0.0;
-      if ((MODE & kModeVirial) || tid < 3)
-        for (int w = 0; w < kBT / 64; ++w) sum += red[tid][w];
-      A.partials[(size_t)bi * kNRed + tid] = sum;
+      for (int w = 0; w < kBT / 64; ++w) sum += red[tid][w];
+      A.partials[(size_t)bi * kPart + tid] = sum;
     }
     // Stayer counts of the home cells (this brick is their only writer).
     for (int hc = tid; hc < nhome; hc += kBT) {
@@ -2041,6 +2043,7 @@ struct RebuildArgs {
   int *__restrict__ pid;
   int *__restrict__ cell;
   int *__restrict__ slot_of;
+  int with_slots;  // bonds or terms look partners up by id: keep slot_of current
   int with_pf;
   // Runge-Kutta 4 state, carried along between stages (buffer 1 -> 0)
   const float4 *__restrict__ rk_in[4];
@@ -2141,7 +2144,13 @@ __global__ __launch_bounds__(256) void k_place(RebuildArgs A) {
   const int p = A.pid_in[q];
   A.pid[dst] = p;
   A.cell[dst] = c1;
-  A.slot_of[p] = dst;
+  if (A.with_slots) A.slot_of[p] = dst;
+}
+
+// slot_of from the slot order (after steps that did not keep it current).
+__global__ __launch_bounds__(256) void k_slots(const int *__restrict__ pid, int n, int *__restrict__ slot_of) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) slot_of[pid[k]] = k;
 }
 
 // Sorts the particles that entered each cell by id (their arrival order in
@@ -2187,38 +2196,51 @@ __global__ __launch_bounds__(256) void k_order_incoming(RebuildArgs A) {
         }
       }
     }
-    for (int k = base; k < base + n_in; ++k) A.slot_of[A.pid[k]] = k;
+    if (A.with_slots)
+      for (int k = base; k < base + n_in; ++k) A.slot_of[A.pid[k]] = k;
   }
   if (n_in) A.inc[c] = 0;
   A.cursor[c] = 0;
 }
 
-// Sums the tile partials of each replica (one block per replica, each
-// thread a fixed strided subset, then a fixed-order block reduction).
-__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ partials,
-                                                  const int2 *__restrict__ rep_tiles, int nrep,
-                                                  double *__restrict__ obs, const int *err) {
-  __shared__ double red[kNRed][4];
+// Sums a replica's per-brick partials in a fixed order (thread t takes bricks
+// t, t + 1024, ..., then waves in order), so the result is the same every
+// run.  The force kernels store nred partials per brick, brick-major: 3
+// without the virial, kNRed with it; the slots past nred are stored as zero.
+constexpr int kFinalThreads = 1024;
+
+__global__ __launch_bounds__(kFinalThreads) void k_finalize(const double *__restrict__ partials,
+                                                            const int2 *__restrict__ rep_tiles, int nrep,
+                                                            double *__restrict__ obs, const int *err, int nred) {
+  __shared__ double red[kNRed][kFinalThreads / 64];
   const int r = blockIdx.x;
   if (r >= nrep || err[3]) return;
   const int2 tr = rep_tiles[r];
   double acc[kNRed];
 #pragma unroll
   for (int k = 0; k < kNRed; ++k) acc[k] = 0.0;
-  for (int t = tr.x + (int)threadIdx.x; t < tr.y; t += blockDim.x)
+  if (nred > 3) {
+    for (int t = tr.x + (int)threadIdx.x; t < tr.y; t += kFinalThreads)
 #pragma unroll
-    for (int k = 0; k < kNRed; ++k) acc[k] += partials[(size_t)t * kNRed + k];
+      for (int k = 0; k < kNRed; ++k) acc[k] += partials[(size_t)t * kNRed + k];
+  } else {
+    for (int t = tr.x + (int)threadIdx.x; t < tr.y; t += kFinalThreads)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc[k] += partials[(size_t)t * 3 + k];
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < kNRed; ++k) {
+    if (k >= nred) break;
     double w = wave_sum(acc[k]);
     if (lane == 0) red[k][wave] = w;
   }
   __syncthreads();
   if (threadIdx.x < kNRed) {
-    double s = 0.0;
-    for (int w = 0; w < 4; ++w) s += red[threadIdx.x][w];
-    obs[(size_t)r * kNRed + threadIdx.x] = s;
+    double sum = 0.0;
+    if ((int)threadIdx.x < nred)
+      for (int w = 0; w < kFinalThreads / 64; ++w) sum += red[threadIdx.x][w];
+    obs[(size_t)r * kNRed + threadIdx.x] = sum;
   }
 }
 
@@ -2274,6 +2296,8 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
   extern __shared__ __align__(16) unsigned char gsmem[];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  const int nov = *B.ovf_count;
+  if ((int)blockIdx.x >= nov) return;  // usually no brick overflowed
   const int passthrough = A.err[2];
   const unsigned t1 = *A.time + 1u;
   const uint32_t tk = step_key(A.seed_lo, t1);
@@ -2284,7 +2308,6 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
   const int G *const flux_off = kFlux ? A.flux_off : nullptr;
   const double flux_cut = kFlux ? A.cutoff_d : 0.0;
   __syncthreads();
-  const int nov = *B.ovf_count;
   for (int i = blockIdx.x; i < nov; i += gridDim.x) {
     const int bi = B.ovf_list[i];
     const int4 br = B.bricks[bi];
@@ -2383,11 +2406,13 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
       if (lane == 0) red[k][wave] = w;
     }
     __syncthreads();
-    if (tid < kNRed) {
+    // kNRed partials per brick with the virial, 3 without (k_finalize reads
+    // the same stride).
+    constexpr int kPart = (MODE & kModeVirial) ? kNRed : 3;
+    if (tid < kPart) {
       double sum = 0.0;
-      if ((MODE & kModeVirial) || tid < 3)
-        for (int w = 0; w < kBT / 64; ++w) sum += red[tid][w];
-      A.partials[(size_t)bi * kNRed + tid] = sum;
+      for (int w = 0; w < kBT / 64; ++w) sum += red[tid][w];
+      A.partials[(size_t)bi * kPart + tid] = sum;
     }
     for (int hc = tid; hc < nhome; hc += kBT) {
       const int a = hc / (nb1 * nb2), b = (hc / nb2) % nb1, cc = hc % nb2;
@@ -3040,6 +3065,7 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   rc |= hipMemcpyAsync(d->cell[0], cell.data(), N * sizeof(int), hipMemcpyHostToDevice, s);
   rc |= hipMemcpyAsync(d->cstart[0], cstart.data(), (ncells_ + 1) * sizeof(int), hipMemcpyHostToDevice, s);
   rc |= hipMemcpyAsync(d->slot_of, slot_of.data(), N * sizeof(int), hipMemcpyHostToDevice, s);
+  slots_valid_ = true;
   rc |= hipMemcpyAsync(d->tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s);
   rc |= hipMemcpyAsync(d->rep_tiles, rep_tiles.data(), rep_tiles.size() * sizeof(int2), hipMemcpyHostToDevice, s);
   rc |= hipStreamSynchronize(s);
@@ -3769,6 +3795,7 @@ int Engine::launch_substep(int rk_stage, int tick) {
   B.pid = d->pid[par ^ 1];
   B.cell = d->cell[0];
   B.slot_of = d->slot_of;
+  B.with_slots = (M_ > 0 || NT_ > 0) ? 1 : 0;
   B.with_pf = any_random_ ? 1 : 0;
   B.with_rk = rk_stage >= 0 && rk_stage < 3 ? 1 : 0;
   B.with_spec = species_on_ ? 1 : 0;
@@ -3791,8 +3818,8 @@ int Engine::launch_substep(int rk_stage, int tick) {
   hipLaunchKernelGGL(k_scan_add, dim3(cg), dim3(256), 0, s, d->cstart[par ^ 1], d->bsum, ncells_);
   hipLaunchKernelGGL(k_place, dim3(pg), dim3(256), 0, s, B);
   hipLaunchKernelGGL(k_order_incoming, dim3(cg), dim3(256), 0, s, B);
-  hipLaunchKernelGGL(k_finalize, dim3(reps_.size()), dim3(256), 0, s, d->partials, d->rep_bricks,
-                     (int)reps_.size(), d->obs, d->err);
+  hipLaunchKernelGGL(k_finalize, dim3(reps_.size()), dim3(kFinalThreads), 0, s, d->partials, d->rep_bricks,
+                     (int)reps_.size(), d->obs, d->err, virial_ ? kNRed : 3);
   parity_ ^= 1;
   pid_prev_valid_ = true;
   return (int)hipGetLastError();
@@ -3918,6 +3945,18 @@ int Engine::step(int nsteps) {
     if (int rc = set_brick(bdim_[0], bdim_[1], bdim_[2], cap_, pmax_req_)) return rc;
   if (args_dirty_)
     if (int rc = refresh_args()) return rc;
+  // Without bonds or terms the rebuild skips slot_of (a scattered write per
+  // particle); rebuild it before bonds or terms use it again.
+  if (M_ > 0 || NT_ > 0) {
+    if (!slots_valid_) {
+      hipLaunchKernelGGL(k_slots, dim3((N_ + 255) / 256), dim3(256), 0, d_->stream, d_->pid[parity_], N_,
+                         d_->slot_of);
+      if (int st = hipGetLastError()) return check_device(st, "kernel launch");
+      slots_valid_ = true;
+    }
+  } else if (nsteps > 0) {
+    slots_valid_ = false;
+  }
   if (timing_) {
     // Two events per force launch, for up to 4096 launches per call.
     const int per = integrator_ == kIntegratorRK4 ? 4 : 1;
