@@ -1,6 +1,6 @@
 // MI355X (gfx950) engine for mdcore's force-and-advance step.
 //
-// One step is eight launches on the engine's stream (capturable as a HIP
+// One step is six launches on the engine's stream (capturable as a HIP
 // graph):
 //   k_brick         per brick of cells: stage the brick and its halo in LDS,
 //                   then per particle the non-bonded gather over the
@@ -10,10 +10,12 @@
 //                   integer per-cell counts and FP64 per-brick partial sums
 //   k_brick_global  the same for bricks whose halo overflowed LDS, with
 //                   neighbours read from global memory
-//   k_scan_*        exclusive scan of the counts into the new cell offsets
-//   k_place         stayers keep their rank in the cell, movers take a cursor
+//   k_scan_block, k_scan_add  exclusive scan of the counts into the new cell
+//                   offsets
+//   k_place         stayers keep their rank in the cell, movers take a cursor;
+//                   its first blocks sum each replica's brick partials in a
+//                   fixed order
 //   k_order_incoming  movers of each cell ordered by particle id
-//   k_finalize      per replica: sum the brick partials in a fixed order
 //
 // Every pair is evaluated from both sides (no Newton-3 write-back), so each
 // particle's force is written once by the thread that owns it and the result
@@ -1989,8 +1991,8 @@ __global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ A
       if (lane == 0) red[k][wave] = w;
     }
     __syncthreads();
-    // kNRed partials per brick with the virial, 3 without (k_finalize reads
-    // the same stride).
+    // kNRed partials per brick with the virial, 3 without (finalize_replica
+    // reads the same stride).
     constexpr int kPart = (MODE & kModeVirial) ? kNRed : 3;
     if (tid < kPart) {
       double sum = 0.0;
@@ -2081,42 +2083,106 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_block(int *__restrict__ cnt
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// Scan of the block totals (one block).  Also advances the time unless the
-// step was skipped because of an earlier error, and records the error state:
+// Second pass: each block adds the total of the chunks before its own (it
+// sums those chunk totals itself; a block's 256 cells lie in one chunk).
+// Block 0 also writes the grand total, advances the time unless the step was
+// skipped because of an earlier error, and records the error state:
 // err[3] = this step was skipped, err[2] = error state for the next step.
-__global__ __launch_bounds__(kScanBlock) void k_scan_top(int *__restrict__ bsum, int nb, int *__restrict__ out,
-                                                        int n, unsigned *time, int *err, int *ovf_count, int tick) {
-  __shared__ int lds[kScanBlock / 64];
-  int carry = 0;
-  for (int base = 0; base < nb; base += kScanBlock) {
-    const int i = base + threadIdx.x;
-    int v = i < nb ? bsum[i] : 0;
-    int total;
-    int ex = block_exclusive_scan(v, lds, total);
-    if (i < nb) bsum[i] = ex + carry;
-    carry += total;
-    __syncthreads();
+static_assert(kScanChunk % 256 == 0, "a k_scan_add block must lie in one chunk");
+
+__global__ __launch_bounds__(256) void k_scan_add(int *__restrict__ out, const int *__restrict__ bsum, int nb,
+                                                 int n, unsigned *time, int *err, int *ovf_count, int tick) {
+  __shared__ int lds[256 / 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int chunk = (blockIdx.x * 256) / kScanChunk;
+  const int upto = blockIdx.x == 0 ? nb : chunk;  // block 0 also needs the grand total
+  int v = 0;
+  for (int j = threadIdx.x; j < upto; j += 256) v += bsum[j];
+  int total;
+  block_exclusive_scan(v, lds, total);
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      out[n] = total;
+      *ovf_count = 0;
+      const int skipped = err[2];
+      err[3] = skipped;
+      if (!skipped && tick) *time += 1u;  // once per step (the last Runge-Kutta stage)
+      err[2] = err[0];
+    }
+    return;  // chunk 0 has nothing before it
   }
-  if (threadIdx.x == 0) {
-    out[n] = carry;
-    *ovf_count = 0;
-    const int skipped = err[2];
-    err[3] = skipped;
-    if (!skipped && tick) *time += 1u;  // once per step (the last Runge-Kutta stage)
-    err[2] = err[0];
-  }
+  if (i < n) out[i] += total;
 }
 
-__global__ __launch_bounds__(256) void k_scan_add(int *__restrict__ out, const int *__restrict__ bsum, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] += bsum[i / kScanChunk];
+// Per-replica sums of the force kernels' per-brick partials: nred doubles per
+// brick, brick-major (3 without the virial, kNRed with it).
+struct FinalArgs {
+  const double *__restrict__ partials;
+  const int2 *__restrict__ rep_tiles;  // brick range of each replica
+  int nrep, nred;
+  double *__restrict__ obs;
+  const int *__restrict__ err;
+};
+
+// One block (256 threads) sums replica r's partials in a fixed order: thread
+// t takes bricks t, t + 256, ... in order, then waves in order, so the
+// result is the same every run.  The slots past nred are stored as zero.
+__device__ __forceinline__ void finalize_replica(const FinalArgs &F, int r) {
+  __shared__ double red[kNRed][4];
+  if (F.err[3]) return;  // the step was skipped: keep the last observables
+  const int2 tr = F.rep_tiles[r];
+  double acc[kNRed];
+#pragma unroll
+  for (int k = 0; k < kNRed; ++k) acc[k] = 0.0;
+  int t = tr.x + (int)threadIdx.x;
+  if (F.nred > 3) {
+    for (; t < tr.y; t += 256)
+#pragma unroll
+      for (int k = 0; k < kNRed; ++k) acc[k] += F.partials[(size_t)t * kNRed + k];
+  } else {
+    // four bricks' loads in flight, added in brick order
+    for (; t + 3 * 256 < tr.y; t += 4 * 256) {
+      double p[4][3];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) p[u][k] = F.partials[(size_t)(t + u * 256) * 3 + k];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] += p[u][k];
+    }
+    for (; t < tr.y; t += 256)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc[k] += F.partials[(size_t)t * 3 + k];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kNRed; ++k) {
+    if (k >= F.nred) break;
+    double w = wave_sum(acc[k]);
+    if (lane == 0) red[k][wave] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < kNRed) {
+    double sum = 0.0;
+    if ((int)threadIdx.x < F.nred)
+      for (int w = 0; w < 4; ++w) sum += red[threadIdx.x][w];
+    F.obs[(size_t)r * kNRed + threadIdx.x] = sum;
+  }
 }
 
 // Moves each particle to its slot of the next step.  A particle that stays in
 // its cell keeps its rank among the cell's stayers; particles entering a cell
 // follow its stayers (their order is made deterministic by k_order_incoming).
-__global__ __launch_bounds__(256) void k_place(RebuildArgs A) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// The first F.nrep blocks sum the replicas' partials instead (independent of
+// the move; they run alongside it).
+__global__ __launch_bounds__(256) void k_place(RebuildArgs A, FinalArgs F) {
+  if ((int)blockIdx.x < F.nrep) {
+    finalize_replica(F, blockIdx.x);
+    return;
+  }
+  const int q = (blockIdx.x - F.nrep) * blockDim.x + threadIdx.x;
   if (q >= A.nslots) return;
   const int raw = A.cell_out[q];
   const int c1 = raw & 0x7fffffff;
@@ -2203,46 +2269,6 @@ __global__ __launch_bounds__(256) void k_order_incoming(RebuildArgs A) {
   A.cursor[c] = 0;
 }
 
-// Sums a replica's per-brick partials in a fixed order (thread t takes bricks
-// t, t + 1024, ..., then waves in order), so the result is the same every
-// run.  The force kernels store nred partials per brick, brick-major: 3
-// without the virial, kNRed with it; the slots past nred are stored as zero.
-constexpr int kFinalThreads = 1024;
-
-__global__ __launch_bounds__(kFinalThreads) void k_finalize(const double *__restrict__ partials,
-                                                            const int2 *__restrict__ rep_tiles, int nrep,
-                                                            double *__restrict__ obs, const int *err, int nred) {
-  __shared__ double red[kNRed][kFinalThreads / 64];
-  const int r = blockIdx.x;
-  if (r >= nrep || err[3]) return;
-  const int2 tr = rep_tiles[r];
-  double acc[kNRed];
-#pragma unroll
-  for (int k = 0; k < kNRed; ++k) acc[k] = 0.0;
-  if (nred > 3) {
-    for (int t = tr.x + (int)threadIdx.x; t < tr.y; t += kFinalThreads)
-#pragma unroll
-      for (int k = 0; k < kNRed; ++k) acc[k] += partials[(size_t)t * kNRed + k];
-  } else {
-    for (int t = tr.x + (int)threadIdx.x; t < tr.y; t += kFinalThreads)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) acc[k] += partials[(size_t)t * 3 + k];
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < kNRed; ++k) {
-    if (k >= nred) break;
-    double w = wave_sum(acc[k]);
-    if (lane == 0) red[k][wave] = w;
-  }
-  __syncthreads();
-  if (threadIdx.x < kNRed) {
-    double sum = 0.0;
-    if ((int)threadIdx.x < nred)
-      for (int w = 0; w < kFinalThreads / 64; ++w) sum += red[threadIdx.x][w];
-    obs[(size_t)r * kNRed + threadIdx.x] = sum;
-  }
-}
 
 // engine_kinetic_energy's per-type sums (engine.cpp:1698-1719) for the
 // Berendsen force: 1/2 m v^2 per particle in FP32, summed in FP64 per block
@@ -2406,8 +2432,8 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
       if (lane == 0) red[k][wave] = w;
     }
     __syncthreads();
-    // kNRed partials per brick with the virial, 3 without (k_finalize reads
-    // the same stride).
+    // kNRed partials per brick with the virial, 3 without (finalize_replica
+    // reads the same stride).
     constexpr int kPart = (MODE & kModeVirial) ? kNRed : 3;
     if (tid < kPart) {
       double sum = 0.0;
@@ -3813,13 +3839,17 @@ int Engine::launch_substep(int rk_stage, int tick) {
   const int nb = (ncells_ + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, d->cnt, d->inc, d->cstart[par ^ 1], d->bsum,
                      ncells_);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, d->bsum, nb, d->cstart[par ^ 1], ncells_,
-                     d->time, d->err, d->ovf_count, tick);
-  hipLaunchKernelGGL(k_scan_add, dim3(cg), dim3(256), 0, s, d->cstart[par ^ 1], d->bsum, ncells_);
-  hipLaunchKernelGGL(k_place, dim3(pg), dim3(256), 0, s, B);
+  hipLaunchKernelGGL(k_scan_add, dim3(cg), dim3(256), 0, s, d->cstart[par ^ 1], d->bsum, nb, ncells_, d->time,
+                     d->err, d->ovf_count, tick);
+  FinalArgs F;
+  F.partials = d->partials;
+  F.rep_tiles = d->rep_bricks;
+  F.nrep = (int)reps_.size();
+  F.nred = virial_ ? kNRed : 3;
+  F.obs = d->obs;
+  F.err = d->err;
+  hipLaunchKernelGGL(k_place, dim3(F.nrep + pg), dim3(256), 0, s, B, F);
   hipLaunchKernelGGL(k_order_incoming, dim3(cg), dim3(256), 0, s, B);
-  hipLaunchKernelGGL(k_finalize, dim3(reps_.size()), dim3(kFinalThreads), 0, s, d->partials, d->rep_bricks,
-                     (int)reps_.size(), d->obs, d->err, virial_ ? kNRed : 3);
   parity_ ^= 1;
   pid_prev_valid_ = true;
   return (int)hipGetLastError();
