@@ -72,3 +72,18 @@ def test_snapped_fluid(m, oracle_mod, frac, pmax):
     x = snapped_fluid(3000, 7, frac)
     n = compare(m, oracle_mod, workload(x), pmax)
     assert (n < brute_counts(x)).sum() > 0
+
+
+@pytest.mark.parametrize("pmax", [None, 64])
+def test_window_rejections(m, oracle_mod, monkeypatch, pmax):
+    """A random fluid on cells 1.0 x 0.9 x 1.1 with cutoff 1: runner_dopair's
+    sort-key window rejects some pairs inside the cutoff (edge and corner
+    neighbours of non-cubic cells); the pair phase tests the window on those
+    offsets."""
+    monkeypatch.setattr(T, "CUT", 1.0)
+    print("case: window rejections pmax", pmax, flush=True)
+    x = T.random_fluid(T.DIM)
+    w = workload(x)
+    w["pots"] = [("dpd", (25.0, 4.5, 0.0, 1.0, 0), 0, 0)]
+    n = compare(m, oracle_mod, w, pmax)
+    assert (n < brute_counts(x)).sum() > 0

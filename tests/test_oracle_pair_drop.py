@@ -12,7 +12,11 @@ cells 1.0 x 0.9 x 1.1).
 Checks against a brute-force minimum-image count: the dropped
 configuration loses every cross-cell pair of that cell pair, the same
 configuration with wider gaps or one particle moved away keeps all of them.
+A random fluid on the same grid shows the window itself rejecting a few
+pairs inside the cutoff, which a cubic grid never does.
 """
+import sys
+
 import numpy as np
 
 DIM = [6.0, 5.4, 6.6]
@@ -23,7 +27,7 @@ CUT = 0.85
 def workload(x):
     x = np.asarray(x, np.float64)
     N = len(x)
-    return {"name": "pair_drop", "dim": DIM, "cells": CELLS, "cutoff": CUT, "dt": 0.001,
+    return {"name": "pair_drop", "dim": list(DIM), "cells": CELLS, "cutoff": CUT, "dt": 0.001,
             "types": [{"mass": 1.0, "radius": 0.1, "dynamics": 0}],
             "x": x, "v": np.zeros((N, 3)), "type": np.zeros(N, np.int32),
             "pots": [("dpd", (25.0, 4.5, 0.0, CUT, 0), 0, 0)], "bonds": None}
@@ -83,3 +87,32 @@ def test_drop_only_on_axes_whose_shift_rounds_up(m, oracle_mod):
     for e in (2e-6, 1e-5, 4e-5):
         x = four(e, e, axis=2)
         assert list(counts(m, oracle_mod, x)) == list(brute_counts(x))
+
+
+def random_fluid(dim, N=3000, seed=5):
+    return np.random.default_rng(seed).random((N, 3)) * np.asarray(dim)
+
+
+def test_window_rejects_in_cutoff_pairs_only_on_non_cubic_cells(m, oracle_mod, monkeypatch):
+    """runner_dopair's window (key distance below dmaxdist) can reject a pair
+    inside the cutoff only when |shift| exceeds shift.n, i.e. for edge and
+    corner neighbours of non-cubic cells: on a cubic grid every in-cutoff
+    pair's key distance stays below dscale * cutoff + 1 < dmaxdist."""
+    here = sys.modules[__name__]
+    monkeypatch.setattr(here, "CUT", 1.0)
+    x = random_fluid(DIM)
+    w = workload(x)
+    w["pots"] = [("dpd", (25.0, 4.5, 0.0, 1.0, 0), 0, 0)]
+    n = counts_of(m, oracle_mod, w)
+    assert (n < brute_counts(x)).sum() > 0 and (n > brute_counts(x)).sum() == 0
+    monkeypatch.setattr(here, "DIM", [6.0, 6.0, 6.0])
+    x = random_fluid(here.DIM)
+    w = workload(x)
+    w["pots"] = [("dpd", (25.0, 4.5, 0.0, 1.0, 0), 0, 0)]
+    assert list(counts_of(m, oracle_mod, w)) == list(brute_counts(x))
+
+
+def counts_of(m, oracle_mod, w):
+    o = oracle_mod.from_workload(w, m)
+    o.force()
+    return np.asarray(o.state()["ncount"])
