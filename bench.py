@@ -184,7 +184,7 @@ def bench_single(args, threads):
     cpu = None if args.no_cpu_baseline else cpu_baseline_oracle(w, args.cpu_seconds, threads,
                                                                 dpd_noise=bool(w.get("dpd_noise")))
     config = {"workload": cfg % ((N,) + tuple(w["cells"])), "particles": N, "cells": w["cells"],
-              "brick": u.brick()}
+              "brick": u.brick(), "waves_per_simd": u.waves()}
     extra = {"autotune": tune} if tune else {}
     return line(args, value, wall, "f32", config, roof, cpu, **extra)
 

@@ -184,11 +184,19 @@ int mxg_engine_set_integrator(mxg_engine *e, int kind);
 /* Autotuner: times the force kernel for each brick shape and list capacity
  * that fits (steps_per real steps each: the layouts give identical
  * per-particle results, so the trajectory is unchanged) and keeps the
- * fastest; *taken = steps advanced.  mxg_engine_tune_log returns the number
- * of entries and copies up to max of them: layout (bx, by, bz, cap) and the
- * mean force-kernel time in ms.  No reference counterpart. */
+ * fastest; with bonds or terms it also times the four fastest at four waves
+ * per SIMD.  *taken = steps advanced.  mxg_engine_tune_log returns the
+ * number of entries and copies up to max of them: layout (bx, by, bz, cap,
+ * waves), 5 ints each, and the mean force-kernel time in ms.  No reference
+ * counterpart. */
 int mxg_engine_autotune(mxg_engine *e, int steps_per, int *taken);
 int mxg_engine_tune_log(mxg_engine *e, int max, int *layout, double *force_ms);
+
+/* Waves per SIMD of the force kernel: 3 (default) or 4.  Four applies to
+ * steps with bonds or terms only; other steps run at three.  get returns the
+ * setting.  No reference counterpart. */
+int mxg_engine_set_waves(mxg_engine *e, int waves);
+int mxg_engine_get_waves(mxg_engine *e);
 
 /* Replay steps from captured HIP graphs (two steps per graph); on by
  * default.  No reference counterpart (mdcore drives pthread runners). */

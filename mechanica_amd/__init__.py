@@ -286,11 +286,19 @@ class Universe:
         taken = C.c_int()
         self._check(L.mxg_engine_autotune(self._h, int(steps_per), C.byref(taken)))
         n = L.mxg_engine_tune_log(self._h, 0, None, None)
-        lay = np.zeros((max(n, 1), 4), np.int32)
+        lay = np.zeros((max(n, 1), 5), np.int32)
         ms = np.zeros(max(n, 1), np.float64)
         L.mxg_engine_tune_log(self._h, n, _ip(lay), _dp(ms))
         log = [(tuple(int(v) for v in lay[k]), float(ms[k])) for k in range(n)]
-        return {"steps": taken.value, "brick": self.brick(), "log": log}
+        return {"steps": taken.value, "brick": self.brick(), "waves": self.waves(), "log": log}
+
+    def set_waves(self, waves):
+        """Waves per SIMD of the force kernel: 3 or 4 (4 applies to steps
+        with bonds or terms)."""
+        self._check(_capi.lib().mxg_engine_set_waves(self._h, int(waves)))
+
+    def waves(self):
+        return int(_capi.lib().mxg_engine_get_waves(self._h))
 
     def set_integrator(self, kind):
         """FORWARD_EULER (0) or RUNGE_KUTTA_4 (1), as Mechanica's integrator."""

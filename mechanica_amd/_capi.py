@@ -68,6 +68,8 @@ SIGNATURES = {
     "mxg_engine_get_brick": (C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p]),
     "mxg_engine_autotune": (C.c_int, [C.c_void_p, C.c_int, c_int_p]),
     "mxg_engine_tune_log": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_double_p]),
+    "mxg_engine_set_waves": (C.c_int, [C.c_void_p, C.c_int]),
+    "mxg_engine_get_waves": (C.c_int, [C.c_void_p]),
     "mxg_engine_load": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_int_p, c_int_p, c_u16_p]),
     "mxg_engine_set_bonds": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, C.POINTER(C.c_void_p),
                                        c_double_p]),

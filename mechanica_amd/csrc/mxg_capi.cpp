@@ -232,14 +232,25 @@ int mxg_engine_tune_log(mxg_engine *e, int max, int *layout, double *force_ms) {
   const int n = (int)log.size();
   for (int k = 0; k < n && k < max; ++k) {
     if (layout) {
-      layout[4 * k + 0] = log[k].bx;
-      layout[4 * k + 1] = log[k].by;
-      layout[4 * k + 2] = log[k].bz;
-      layout[4 * k + 3] = log[k].cap;
+      layout[5 * k + 0] = log[k].bx;
+      layout[5 * k + 1] = log[k].by;
+      layout[5 * k + 2] = log[k].bz;
+      layout[5 * k + 3] = log[k].cap;
+      layout[5 * k + 4] = log[k].waves;
     }
     if (force_ms) force_ms[k] = log[k].force_ms;
   }
   return n;
+}
+
+int mxg_engine_set_waves(mxg_engine *e, int waves) {
+  MXG_CHECK(e);
+  return e->e.set_waves(waves);
+}
+
+int mxg_engine_get_waves(mxg_engine *e) {
+  if (!e) return MXG_ERR_NULL;
+  return e->e.waves();
 }
 
 int mxg_engine_set_integrator(mxg_engine *e, int kind) {

@@ -156,11 +156,15 @@ class Engine {
   // set_brick(0, 0, 0, 0, 0) picks the largest brick whose halo fits; pmax
   // (> 0) lowers the LDS particle capacity.
   int set_brick(int bx, int by, int bz, int cap, int pmax);
-  // Tries brick shapes and list capacities, steps_per steps each, keeps the
-  // fastest; *taken = steps advanced.
+  // Tries brick shapes and list capacities, steps_per steps each (with
+  // bonds or terms, the four fastest again at four waves per SIMD), keeps
+  // the fastest; *taken = steps advanced.
   int autotune(int steps_per, int *taken);
+  // Waves per SIMD of the force kernel: 3, or 4 (used by the bonded modes).
+  int set_waves(int waves);
+  int waves() const { return waves_; }
   struct TuneEntry {
-    int bx, by, bz, cap;
+    int bx, by, bz, cap, waves;
     double force_ms;
   };
   const std::vector<TuneEntry> &tune_log() const { return tune_log_; }
@@ -258,6 +262,7 @@ class Engine {
   int parity_ = 0;               // which ping-pong ids/offsets/bond flags are current
   bool args_dirty_ = true;        // device copy of the kernel arguments is stale
   bool slots_valid_ = true;       // device slot_of matches the current slot order
+  int waves_ = 3;                 // force kernel waves per SIMD (4: bonded modes only)
   bool use_graphs_ = true;
   uint64_t config_gen_ = 1;       // bumped when the captured launches change
   uint64_t graph_gen_[2] = {0, 0};

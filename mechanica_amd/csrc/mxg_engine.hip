@@ -1600,9 +1600,12 @@ __device__ __forceinline__ int xcd_brick(int g, int n) {
 
 // Three waves per SIMD: the register budget (168 VGPRs) holds without
 // spills, and bricks whose LDS footprint allows three blocks per CU keep
-// more candidate loads in flight (up to 30% faster than two waves).
-template <int MODE>
-__global__ __launch_bounds__(kBT, 3) void k_brick(const StepArgs *__restrict__ Ap, BrickArgs B) {
+// more candidate loads in flight (up to 30% faster than two waves).  The
+// bonded modes also come with four waves (128 VGPRs), which the autotuner
+// may pick: their term gathers gain more from the extra wave than the pair
+// loops lose to the tighter budget.
+template <int MODE, int WAVES>
+__global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict__ Ap, BrickArgs B) {
   const StepArgs &A = *Ap;
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool kDPD = (MODE & kModeDPD) != 0;
@@ -2448,9 +2451,9 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
   }
 }
 
-template <int MODE>
+template <int MODE, int WAVES = 3>
 void launch_step_mode(const StepArgs *A, const BrickArgs &B, size_t lds, size_t tabs, hipStream_t st) {
-  hipLaunchKernelGGL(k_brick<MODE>, dim3(B.nbricks), dim3(kBT), lds, st, A, B);
+  hipLaunchKernelGGL((k_brick<MODE, WAVES>), dim3(B.nbricks), dim3(kBT), lds, st, A, B);
   hipLaunchKernelGGL(k_brick_global<MODE>, dim3(64), dim3(kBT), tabs, st, A, B);
 }
 
@@ -2466,6 +2469,16 @@ const StepLauncher kStepLaunchers[32] = {
     launch_step_mode<20>, launch_step_mode<21>, launch_step_mode<22>, launch_step_mode<23>,
     launch_step_mode<24>, launch_step_mode<25>, launch_step_mode<26>, launch_step_mode<27>,
     launch_step_mode<28>, launch_step_mode<29>, launch_step_mode<30>, launch_step_mode<31>,
+};
+
+// Four waves per SIMD, bonded modes only (nullptr: three waves).
+const StepLauncher kStepLaunchers4[32] = {
+    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+    launch_step_mode<8, 4>,  launch_step_mode<9, 4>,  launch_step_mode<10, 4>, launch_step_mode<11, 4>,
+    launch_step_mode<12, 4>, launch_step_mode<13, 4>, launch_step_mode<14, 4>, launch_step_mode<15, 4>,
+    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+    launch_step_mode<24, 4>, launch_step_mode<25, 4>, launch_step_mode<26, 4>, launch_step_mode<27, 4>,
+    launch_step_mode<28, 4>, launch_step_mode<29, 4>, launch_step_mode<30, 4>, launch_step_mode<31, 4>,
 };
 
 }  // namespace
@@ -3797,7 +3810,8 @@ int Engine::launch_substep(int rk_stage, int tick) {
   const size_t tabs = pair_tab_bytes((int)pots_.size(), (int)types_.size());
   const size_t lds =
       brick_lds_bytes(pmax_, hmax_, hdmax_, noff_, cap_, (mode & kModeDPD) != 0, (mode & kModeFlux) != 0, tabs);
-  kStepLaunchers[mode](reinterpret_cast<const StepArgs *>(d->dargs) + par, Bk, lds, tabs, s);
+  const StepLauncher launch = (waves_ == 4 && kStepLaunchers4[mode]) ? kStepLaunchers4[mode] : kStepLaunchers[mode];
+  launch(reinterpret_cast<const StepArgs *>(d->dargs) + par, Bk, lds, tabs, s);
   if (rec) hipEventRecord(d->kev[2 * timed_steps_ + 1], s);
   if (rec) ++timed_steps_;
 
@@ -3901,6 +3915,7 @@ int Engine::autotune(int steps_per, int *taken) {
   static const int caps[] = {8, 12, 16, 24};
   const bool was_timing = timing_;
   int best[4] = {bdim_[0], bdim_[1], bdim_[2], cap_};
+  int best_waves = waves_;
   double best_ms = 1e30;
   int steps = 0;
   timing_ = true;
@@ -3919,20 +3934,72 @@ int Engine::autotune(int steps_per, int *taken) {
         if (taken) *taken = steps;
         return rc;
       }
-      tune_log_.push_back({sh[0], sh[1], sh[2], c, force_ms_});
+      tune_log_.push_back({sh[0], sh[1], sh[2], c, waves_, force_ms_});
       if (force_ms_ > 0 && force_ms_ < best_ms) {
         best_ms = force_ms_;
         best[0] = sh[0];
         best[1] = sh[1];
         best[2] = sh[2];
         best[3] = c;
+        best_waves = waves_;
       }
     }
+  }
+  // Bonded modes: the four fastest layouts again at four waves per SIMD.
+  if (M_ > 0 || NT_ > 0) {
+    std::vector<TuneEntry> top = tune_log_;
+    std::sort(top.begin(), top.end(), [](const TuneEntry &a, const TuneEntry &b) {
+      return (a.force_ms > 0 ? a.force_ms : 1e30) < (b.force_ms > 0 ? b.force_ms : 1e30);
+    });
+    top.resize(std::min<size_t>(top.size(), 4));
+    const int saved = waves_;
+    waves_ = 4;
+    ++config_gen_;
+    for (const TuneEntry &t : top) {
+      if (set_brick(t.bx, t.by, t.bz, t.cap, pmax_req_) != kErrOk) continue;
+      timing_ = false;
+      int rc = step(1);
+      timing_ = true;
+      if (rc == kErrOk) rc = step(steps_per);
+      steps += steps_per + 1;
+      if (rc != kErrOk) {
+        waves_ = saved;
+        ++config_gen_;
+        timing_ = was_timing;
+        if (taken) *taken = steps;
+        return rc;
+      }
+      tune_log_.push_back({t.bx, t.by, t.bz, t.cap, 4, force_ms_});
+      if (force_ms_ > 0 && force_ms_ < best_ms) {
+        best_ms = force_ms_;
+        best[0] = t.bx;
+        best[1] = t.by;
+        best[2] = t.bz;
+        best[3] = t.cap;
+        best_waves = 4;
+      }
+    }
+    waves_ = saved;
   }
   timing_ = was_timing;
   if (taken) *taken = steps;
   msg_.clear();
+  if (best_waves != waves_) {
+    waves_ = best_waves;
+    ++config_gen_;
+  }
   return set_brick(best[0], best[1], best[2], best[3], pmax_req_);
+}
+
+// Waves per SIMD of the force kernel (3, or 4 for bonded modes; modes
+// without bonds or terms always run at three).
+int Engine::set_waves(int waves) {
+  if (waves != 3 && waves != 4) return reject(kErrRange, "waves per SIMD must be 3 or 4");
+  if (waves != waves_) {
+    waves_ = waves;
+    ++config_gen_;  // captured graphs hold the old kernel
+  }
+  return kErrOk;
 }
 
 // engine.integrator (EngineIntegrator FORWARD_EULER / RUNGE_KUTTA_4, chosen

@@ -210,3 +210,30 @@ def test_autotune_keeps_trajectory(m):
     a = u.state()
     b, _, _ = run_steps(m, w, 6, total)
     bitwise_same(a, b)
+
+
+@pytest.mark.parametrize("name", ["lattice", "molecules"])
+def test_four_waves_bitwise(m, name):
+    """Bonded modes also run at four waves per SIMD (a tighter register
+    budget, the same operations in the same order): the states after
+    several steps are bit for bit those of the three-wave kernel."""
+    w = W.lattice(n=12) if name == "lattice" else W.molecules(n=8)
+    print("case: four waves", name, flush=True)
+    out = []
+    for waves in (3, 4):
+        u = W.build_universe(m, w, seed=3)
+        u.set_waves(waves)
+        assert u.waves() == waves
+        u.step(6)
+        g = u.state()
+        out.append((g["x"], g["v"], g["f"], g["cell"]))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                              b.view(np.uint32) if b.dtype == np.float32 else b)
+
+
+def test_waves_rejects_other_values(m):
+    w = W.lattice(n=6)
+    u = W.build_universe(m, w, seed=3)
+    with pytest.raises(Exception):
+        u.set_waves(2)
