@@ -2663,9 +2663,17 @@ static void flag_offsets(DevReplica &D, DevOffset *off, const std::vector<float>
     const int dn = (int)(D.dscale * nshift);
     const int ki = (int)host_sort_key(D, xi, n), kj = (int)host_sort_key(D, xj, n);
     if (kj + dn - ki >= 0) continue;
-    const int need_home = f.side > 0 ? need_i : need_j, need_nbr = f.side > 0 ? need_j : need_i;
+    int need_home = f.side > 0 ? need_i : need_j, need_nbr = f.side > 0 ? need_j : need_i;
+    if (std::abs(f.ox) > 1 || std::abs(f.oy) > 1 || std::abs(f.oz) > 1) {
+      // The near-bound bound above assumes adjacent cells (shift = cell
+      // width per axis); a non-adjacent offset that can drop (none on any
+      // grid seen so far) takes the exact test every time.
+      need_home = 0;
+      need_nbr = 0;
+    }
     f.flags |= (int)(0x80000000u | (unsigned)need_home | ((unsigned)need_nbr << 8));
     D.drop_check = 1;
+    if (need_home == 0) continue;  // no filter: near_eps does not apply
     double proj = 0.0;
     for (int k = 0; k < 3; ++k) proj += std::fabs((double)n[k]) * ((double)D.h[k] + 2.0 * delta[k]);
     slack = std::max(slack, proj - (double)nshift);
