@@ -1908,10 +1908,11 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                   for (int u = 0; u < kChunk; ++u) {
                     const float dx = xs.x - xq[u].x, dy = xs.y - xq[u].y, dz = xs.z - xq[u].z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    if (q + u < qe && q + u != p && r2 <= C.cut2_self) {
-                      nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
-                      ++nn;
-                    }
+                    // Written unconditionally (the flush check above leaves room
+                    // for a whole chunk), kept by advancing nn: no divergent
+                    // branch, so no exec-mask bookkeeping per candidate.
+                    nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
+                    nn += (q + u < qe && q + u != p && r2 <= C.cut2_self) ? 1 : 0;
                   }
                 }
               } else {
@@ -1938,10 +1939,8 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                     for (int u = 0; u < kChunk; ++u) {
                       const float dx = pix - xq[u].x, dy = piy - xq[u].y, dz = piz - xq[u].z;
                       const float r2 = dx * dx + dy * dy + dz * dz;
-                      if (q + u < qe && r2 <= C.cut2_pair) {
-                        nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
-                        ++nn;
-                      }
+                      nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
+                      nn += (q + u < qe && r2 <= C.cut2_pair) ? 1 : 0;
                     }
                   }
                 } else {
@@ -1957,10 +1956,8 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                     for (int u = 0; u < kChunk; ++u) {
                       const float dx = (xq[u].x - sx) - xs.x, dy = (xq[u].y - sy) - xs.y, dz = (xq[u].z - sz) - xs.z;
                       const float r2 = dx * dx + dy * dy + dz * dz;
-                      if (q + u < qe && r2 <= C.cut2_pair) {
-                        nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
-                        ++nn;
-                      }
+                      nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
+                      nn += (q + u < qe && r2 <= C.cut2_pair) ? 1 : 0;
                     }
                   }
                 }
