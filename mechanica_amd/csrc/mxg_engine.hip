@@ -1801,15 +1801,16 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
           // Shift of the pair for offset (ox, oy, oz) in the reference
           // orientation, from this particle's cell's rows of the shift table;
           // the reverse shift (side -1: the neighbour is the reference's i
-          // cell) is the exact negation (checked when the tables are built).
+          // cell) is the exact negation (checked when the tables are built),
+          // and side 0 (the same cell) reads the table's exact +0.
           constexpr int W = 2 * kSpanMax + 1;
           const float *hsx = hs + (0 * B.hdmax + ha) * W + sp0, *hsy = hs + (1 * B.hdmax + hb) * W + sp1,
                       *hsz = hs + (2 * B.hdmax + hcz) * W + sp2;
           auto shift_of = [&](int ox, int oy, int oz, int side, float &sx, float &sy, float &sz) {
             const float x = hsx[ox], y = hsy[oy], z = hsz[oz];
-            sx = side > 0 ? x : -x;
-            sy = side > 0 ? y : -y;
-            sz = side > 0 ? z : -z;
+            sx = side >= 0 ? x : -x;
+            sy = side >= 0 ? y : -y;
+            sz = side >= 0 ? z : -z;
           };
           auto process = [&](int n) {
             for (int e = 0; e < n; ++e) {
@@ -1819,24 +1820,24 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
               const int q = (int)(ent >> 12);
               const int side = (int)((ent >> 9) & 3u) - 1;
               const float4 xq = hx[q];
-              // Self pairs: x_i - x_j; others (a - shift) - b in the
-              // reference orientation (x - 0 is exact, so one form serves).
-              float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-              if (side != 0) {
-                const int ox = (int)((ent >> 6) & 7u) - 2, oy = (int)((ent >> 3) & 7u) - 2, oz = (int)(ent & 7u) - 2;
-                shift_of(ox, oy, oz, side, sx, sy, sz);
-                // runner_dopair's sort-key window (runner_dopair.cpp:170,182) along the
-                // pair's forward direction f = side * sign(ox, oy, oz), n = f / |f| as the
-                // offset table holds it (components 0 or +-1/sqrt(|f|^2)); only on
-                // offsets where it can reject an in-cutoff pair (kOffWindow).
-                if (ent & 0x800u) {
-                  float nx, ny, nz;
-                  offset_dir(ox, oy, oz, side, nx, ny, nz);
-                  const unsigned dnshift = (unsigned)(int)(R.dscale * sqrtf(sx * sx + sy * sy + sz * sz));
-                  const unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, nx, ny, nz);
-                  const unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, nx, ny, nz);
-                  if (!(side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax))) continue;
-                }
+              // (a - shift) - b in the reference orientation; a self entry's
+              // offset is (0, 0, 0), whose table shift is exactly +0, and
+              // x - 0 is exact, so self pairs (x_i - x_j) take the same form
+              // and no entry branches on its side.
+              const int ox = (int)((ent >> 6) & 7u) - 2, oy = (int)((ent >> 3) & 7u) - 2, oz = (int)(ent & 7u) - 2;
+              float sx, sy, sz;
+              shift_of(ox, oy, oz, side, sx, sy, sz);
+              // runner_dopair's sort-key window (runner_dopair.cpp:170,182) along the
+              // pair's forward direction f = side * sign(ox, oy, oz), n = f / |f| as the
+              // offset table holds it (components 0 or +-1/sqrt(|f|^2)); only on
+              // offsets where it can reject an in-cutoff pair (kOffWindow).
+              if (side != 0 && (ent & 0x800u)) {
+                float nx, ny, nz;
+                offset_dir(ox, oy, oz, side, nx, ny, nz);
+                const unsigned dnshift = (unsigned)(int)(R.dscale * sqrtf(sx * sx + sy * sy + sz * sz));
+                const unsigned kp = sort_key(R.bias, R.dscale, xs.x, xs.y, xs.z, nx, ny, nz);
+                const unsigned kq = sort_key(R.bias, R.dscale, xq.x, xq.y, xq.z, nx, ny, nz);
+                if (!(side > 0 ? (kq + dnshift - kp < dmax) : (kp + dnshift - kq < dmax))) continue;
               }
               const bool fwd = side >= 0;
               const float ax = fwd ? xs.x : xq.x, ay = fwd ? xs.y : xq.y, az = fwd ? xs.z : xq.z;
