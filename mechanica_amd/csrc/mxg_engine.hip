@@ -603,10 +603,14 @@ __device__ __forceinline__ bool pair_force(const LoopConsts &A, const PairTabs &
     r = sqrtf(r2);
   }
   const float ri = tb.radius[ti], rj = tb.radius[tj];
-  if ((MODE & kModeDPD) && P.kind == kPotKindDPD) {
+  // Without table potentials in the mode every potential is DPD.
+  if ((MODE & kModeDPD) && (!(MODE & kModeTable) || P.kind == kPotKindDPD)) {
     bool shifted = P.flags & kPotFlagShifted;
     float cutoff = shifted ? (P.b + ri + rj) : P.b;
-    if (r2 > cutoff * cutoff) return false;
+    // Beyond the DPD cutoff the pair adds an exact zero (a mask, not a
+    // branch; the flux mode keeps the branch, where it measures faster).
+    const bool in_range = r2 <= cutoff * cutoff;
+    if ((MODE & kModeFlux) && !in_range) return false;
     const float ir = __fdividef(1.0f, r);
     float ex = dxx * ir, ey = dxy * ir, ez = dxz * ir;
     float4 vq = nb.v(q);
@@ -634,7 +638,7 @@ __device__ __forceinline__ bool pair_force(const LoopConsts &A, const PairTabs &
     } else {
       fr = P.dpd_sigma * omega_c * A.dpd_delta;
     }
-    float f = fc + fd + fr;
+    float f = in_range ? fc + fd + fr : 0.0f;
     float wx = f * ex, wy = f * ey, wz = f * ez;  // force on i
     if (p_is_i) {
       fx += wx;
@@ -656,7 +660,7 @@ __device__ __forceinline__ bool pair_force(const LoopConsts &A, const PairTabs &
       acc[10] += 0.5 * dxz * wy;
       acc[11] += 0.5 * dxz * wz;
     }
-    return true;
+    return in_range;
   }
   if (MODE & kModeTable) {
     float e, f;
