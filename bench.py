@@ -51,7 +51,7 @@ BYTES_TERM_OFFSET = 4
 # scaled by the copy-kernel calibration (bench.py --copy-calibration under
 # the same counters); the summary and its commands are kept in
 # profiles/k_brick_dpd1m_counters.md (bytes per launch).
-MEASURED_TRAFFIC = {"dpd": 111.9e6}
+MEASURED_TRAFFIC = {("dpd", 1048576): 111.9e6}
 
 METRIC = "particle-steps/s (1M LJ/DPD)"
 
@@ -179,7 +179,7 @@ def bench_single(args, threads):
     force_ms, step_ms = kernel_ms(u, args.steps)
     bytes_alg = per * N + BYTES_PER_BOND * bonds + (per_terms if args.workload == "molecules" else 0)
     roof = roofline(bytes_alg, force_ms, "k_brick (LDS brick force + advance) with its global-memory fallback",
-                    MEASURED_TRAFFIC.get(args.workload))
+                    MEASURED_TRAFFIC.get((args.workload, args.n)))
     roof["step_ms_events"] = step_ms
     cpu = None if args.no_cpu_baseline else cpu_baseline_oracle(w, args.cpu_seconds, threads,
                                                                 dpd_noise=bool(w.get("dpd_noise")))
