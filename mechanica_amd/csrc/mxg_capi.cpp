@@ -277,6 +277,7 @@ int mxg_engine_load(mxg_engine *e, int N, const double *x, const double *v, cons
 int mxg_engine_set_bonds(mxg_engine *e, int M, const int32_t *i, const int32_t *j, const mxg_potential *const *pots,
                          const double *dissociation_energy) {
   MXG_CHECK(e);
+  if (M < 0) return MXG_ERR_RANGE;
   if (M > 0 && (!i || !j || !pots)) return MXG_ERR_NULL;
   std::vector<const mxg::Potential *> pp(M);
   for (int b = 0; b < M; ++b) pp[b] = pots[b] ? &pots[b]->p : nullptr;

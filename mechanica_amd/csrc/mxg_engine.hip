@@ -561,6 +561,15 @@ struct LdsNbr {
   __device__ int pid(int q) const;
 };
 
+// Pair distance from r^2, the same in the LDS kernel and its global-memory
+// fallback so a particle's force does not depend on which one ran its brick:
+// table potentials index by r and keep the correctly rounded root; a mode
+// with DPD potentials only takes the one-ulp hardware square root.
+template <int MODE>
+__device__ __forceinline__ float pair_root(float r2) {
+  return (MODE & kModeTable) ? sqrtf(r2) : __builtin_amdgcn_sqrtf(r2);
+}
+
 // Accumulates one pair into the owner's force.  dx is in the reference
 // orientation (x_i - x_j); p_is_i says which end the owner is.
 // Returns whether the pair adds to the number density (no potential, or the
@@ -1848,9 +1857,7 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
               const float bx = fwd ? xq.x : xs.x, by = fwd ? xq.y : xs.y, bz = fwd ? xq.z : xs.z;
               const float dx = (ax - sx) - bx, dy = (ay - sy) - by, dz = (az - sz) - bz;
               const float r2 = dx * dx + dy * dy + dz * dz;
-              // Tables index by r and keep the exact root; DPD alone takes the
-              // one-ulp hardware square root.
-              const float r = (MODE & kModeTable) ? sqrtf(r2) : __builtin_amdgcn_sqrtf(r2);
+              const float r = pair_root<MODE>(r2);
               const int tj = __float_as_int(xq.w) & 0xffff;
               if (pair_force<MODE>(C, tb, nb, ti, tj, side >= 0, mypid, q, side == 0, dx, dy, dz, r2, r, vs, fx, fy,
                                    fz, acc, t1, tk))
@@ -2410,7 +2417,7 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
                            [&](int q, bool p_is_i, bool self, float dx, float dy, float dz, float r2,
                                const float4 &xq) {
                              const int tj = __float_as_int(xq.w) & 0xffff;
-                             const float r = sqrtf(r2);
+                             const float r = pair_root<MODE>(r2);
                              if (pair_force<MODE>(C, tb, nb, ti, tj, p_is_i, mypid, q, self, dx, dy, dz, r2, r, vs,
                                                   fx, fy, fz, acc, t1, tk))
                                nd += w_cubic(r, C.cutoff, C.nd_norm);
@@ -3169,7 +3176,8 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
                      return p.kind == kPotKindDPD;
                    });
   int c = cap > 0 ? cap : cap_;
-  if (c < kChunk || c > 64) return reject(kErrRange, "neighbour-list capacity must lie in [4, 64]");
+  if (c < kChunk || c > 64)
+    return reject(kErrRange, "neighbour-list capacity must lie in [" + std::to_string(kChunk) + ", 64]");
   const size_t tabs = pair_tab_bytes((int)pots_.size(), (int)types_.size());
   int bd[3] = {bx, by, bz};
   int hmax = 0, pm = 0;

@@ -134,12 +134,21 @@ def test_graph_replay_bitwise(m):
     assert oa["ekin"] == ob["ekin"]
 
 
+def layout_workload(name, seed):
+    if name == "argon":
+        return W.argon_lj(N=3000, seed=seed, min_dist=0.275)
+    # DPD with counter noise: a DPD-only mode takes the hardware square root
+    # for the pair distance in both the LDS kernel and the global fallback.
+    return W.dpd_fluid(N=4000, sigma=3.0, seed=seed)
+
+
+@pytest.mark.parametrize("name", ["argon", "dpd"])
 @pytest.mark.parametrize("brick", [(1, 1, 1, 0, 0), (2, 2, 2, 0, 0), (4, 2, 1, 8, 0), (2, 2, 2, 0, 64)])
-def test_brick_layout_bitwise(m, brick):
+def test_brick_layout_bitwise(m, brick, name):
     """Brick shapes, list capacities and the global fallback (pmax = 64
     sends nearly every brick there) give the same per-particle results."""
-    w = W.argon_lj(N=3000, seed=45, min_dist=0.275)
-    print("case: brick", brick, flush=True)
+    w = layout_workload(name, 45)
+    print("case: brick", name, brick, flush=True)
     a, oa, _ = run_steps(m, w, 6, 5)
     b, ob, ub = run_steps(m, w, 6, 5, brick=brick)
     dims, cap, pmax = ub.brick()
@@ -196,12 +205,15 @@ def test_ensemble_matches_independent(m):
     assert abs(tot["ekin"] - sum(o["ekin"] for _, o in singles)) <= 1e-12 * tot["ekin"]
 
 
-def test_autotune_keeps_trajectory(m):
+@pytest.mark.parametrize("name", ["argon", "dpd"])
+def test_autotune_keeps_trajectory(m, name):
     """The tuner's timing steps are real steps; the trajectory equals an
     untuned run of the same length."""
-    w = W.argon_lj(N=3000, seed=46, min_dist=0.275)
-    print("case: autotune argon N=3000", flush=True)
+    w = layout_workload(name, 46)
+    print("case: autotune", name, flush=True)
     u = W.build_universe(m, w, seed=6)
+    if name == "dpd":
+        u.set_dpd_noise(1)
     res = u.autotune(steps_per=1)
     print("autotune", res["brick"], res["steps"], res["log"][:3])
     assert len(res["log"]) > 0 and res["steps"] == 2 * len(res["log"])  # one untimed step each
