@@ -11,6 +11,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 namespace mxg {
@@ -306,7 +307,7 @@ void fix_first_interval(std::vector<float> &c, float a) {
 }  // namespace
 
 int build_table(Potential &p, const Fn &f, const Fn &fp, const Fn &f6p,
-                float a, float b, float tol) {
+                float a, float b, float tol, BuildTrace *trace) {
   if (!f || !f6p) return kPotErrNull;
   p.a = a;
   p.b = b;
@@ -327,9 +328,23 @@ int build_table(Potential &p, const Fn &f, const Fn &fp, const Fn &f6p,
   w = 2 * FLT_EPSILON * (std::fabs(al[0]) + std::fabs(al[1]) + std::fabs(al[2]));
   al[0] -= w * a / (a - b);
   al[1] += w / (a - b);
+  if (trace) {
+    trace->a = a;
+    trace->b = b;
+    trace->exponent = ex;
+    for (int k = 0; k < 3; ++k) trace->alpha[k] = al[k];
+    trace->tried_n.clear();
+    trace->tried_err.clear();
+  }
+  auto tried = [trace](const Candidate &c) {
+    if (!trace) return;
+    trace->tried_n.push_back(c.n);
+    trace->tried_err.push_back(c.err);
+  };
 
   Candidate lo, hi, mid;
   if (!make_candidate(lo, f, fp, al, a, b, kPotIvalsA)) return kPotErrBounds;
+  tried(lo);
 
   if (lo.err < tol) {
     p.n = lo.n;
@@ -347,6 +362,7 @@ int build_table(Potential &p, const Fn &f, const Fn &fp, const Fn &f6p,
   int r = kPotIvalsB;
   for (;;) {
     if (!make_candidate(hi, f, fp, al, a, b, r)) return kPotErrBounds;
+    tried(hi);
     if (hi.err < tol) break;
     if (2 * r > kPotIvalsMax) return kPotErrIvalsMax;
     lo = std::move(hi);
@@ -357,6 +373,7 @@ int build_table(Potential &p, const Fn &f, const Fn &fp, const Fn &f6p,
   while (r - l > 1) {
     int m = 0.5 * (r + l);
     if (!make_candidate(mid, f, fp, al, a, b, m)) return kPotErrBounds;
+    tried(mid);
     if (mid.err > tol) {
       l = m;
       lo = std::move(mid);
@@ -378,110 +395,68 @@ int build_table(Potential &p, const Fn &f, const Fn &fp, const Fn &f6p,
   return kPotErrOk;
 }
 
-int make_lj126(Potential &p, double a, double b, double A, double B,
-               double tol) {
-  Potential q;
-  q.flags = kPotFlagR2 | kPotFlagLJ126;
-  q.name = "Lennard-Jones";
-  Fn f = [A, B](double r) {
+PotentialFns lj126_fns(double A, double B) {
+  PotentialFns F;
+  F.f = [A, B](double r) {
     double ir = 1.0 / r, ir2 = ir * ir, ir6 = ir2 * ir2 * ir2, ir12 = ir6 * ir6;
     return A * ir12 - B * ir6;
   };
-  Fn fp = [A, B](double r) {
+  F.fp = [A, B](double r) {
     double ir = 1.0 / r, ir2 = ir * ir, ir4 = ir2 * ir2, ir12 = ir4 * ir4 * ir4;
     return 6.0 * ir * (-2.0 * A * ir12 + B * ir4 * ir2);
   };
-  Fn f6p = [A, B](double r) {
+  F.f6p = [A, B](double r) {
     double r2 = r * r, ir2 = 1.0 / r2, ir6 = ir2 * ir2 * ir2, ir12 = ir6 * ir6;
     return 10080.0 * ir12 * (884.0 * A * ir6 - 33.0 * B);
   };
-  int rc = build_table(q, f, fp, f6p, (float)a, (float)b, (float)tol);
-  if (rc < 0) return rc;
-  p = std::move(q);
-  return kPotErrOk;
+  return F;
 }
 
-int make_morse(Potential &p, double d, double alpha, double r0, double min,
-               double max, double tol) {
-  Potential q;
-  q.flags = kPotFlagR2 | kPotFlagShifted;
-  q.name = "Morse";
-  q.r0_plusone = r0 + 1;
-  // d (exp(-2 alpha (r-1)) - 2 exp(-alpha (r-1))); powers of e as the
-  // reference evaluates them (std::pow(M_E, .)).
-  Fn f = [d, alpha](double r) {
+// d (exp(-2 alpha (r-1)) - 2 exp(-alpha (r-1))); powers of e as the
+// reference evaluates them (std::pow(M_E, .)).
+PotentialFns morse_fns(double d, double alpha) {
+  PotentialFns F;
+  F.f = [d, alpha](double r) {
     return d * (std::pow(M_E, -2 * alpha * (-1 + r)) - 2 / std::pow(M_E, alpha * (-1 + r)));
   };
-  Fn fp = [d, alpha](double r) {
+  F.fp = [d, alpha](double r) {
     return d * ((-2 * alpha) / std::pow(M_E, 2 * alpha * (-1 + r)) +
                 (2 * alpha) / std::pow(M_E, alpha * (-1 + r)));
   };
-  Fn f6p = [d, alpha](double r) {
+  F.f6p = [d, alpha](double r) {
     return d * ((64 * std::pow(alpha, 6)) / std::pow(M_E, 2 * alpha * (-1 + r)) -
                 (2 * std::pow(alpha, 6)) / std::pow(M_E, alpha * (-1 + r)));
   };
-  int rc = build_table(q, f, fp, f6p, (float)min, (float)max, (float)tol);
-  if (rc < 0) return rc;
-  p = std::move(q);
-  return kPotErrOk;
+  return F;
 }
 
-int make_harmonic(Potential &p, double a, double b, double K, double r0,
-                  double tol) {
-  Potential q;
-  // The reference sets POTENTIAL_HARMONIC & POTENTIAL_R2, which is 0.
-  q.flags = kPotFlagHarmonic & kPotFlagR2;
-  q.name = "Harmonic";
-  Fn f = [K, r0](double r) { return K * (r - r0) * (r - r0); };
-  Fn fp = [K, r0](double r) { return 2.0 * K * (r - r0); };
-  Fn f6p = [](double) { return 0.0; };
-  int rc = build_table(q, f, fp, f6p, (float)a, (float)b, (float)tol);
-  if (rc < 0) return rc;
-  p = std::move(q);
-  return kPotErrOk;
+PotentialFns harmonic_fns(double K, double r0) {
+  PotentialFns F;
+  F.f = [K, r0](double r) { return K * (r - r0) * (r - r0); };
+  F.fp = [K, r0](double r) { return 2.0 * K * (r - r0); };
+  F.f6p = [](double) { return 0.0; };
+  return F;
 }
 
-// potential_create_harmonic_angle (MxPotential.cpp:555-597): K (acos(c) -
-// theta0)^2 tabulated in c = cos(theta) on [cos(b), cos(a)], the angles
-// clamped to [0, pi]; no derivative is supplied, so the slopes are the
-// free-end least-squares ones.
-int make_harmonic_angle(Potential &p, double a, double b, double K, double theta0, double tol) {
-  Potential q;
-  q.flags = kPotFlagAngle | kPotFlagHarmonic;
-  q.name = "Harmonic Angle";
-  if (a < 0.0) a = 0.0;
-  if (b > M_PI) b = M_PI;
-  const double left = std::cos(b), right = std::cos(a);
-  Fn f = [K, theta0](double r) {
+// K (acos(c) - theta0)^2 in c = cos(theta); no derivative is supplied, so
+// the slopes are the free-end least-squares ones.
+PotentialFns harmonic_angle_fns(double K, double theta0) {
+  PotentialFns F;
+  F.f = [K, theta0](double r) {
     r = std::fmin(1.0, std::fmax(-1.0, r));
     const double theta = std::acos(r);
     return K * (theta - theta0) * (theta - theta0);
   };
-  Fn f6p = [](double) { return 0.0; };
-  int rc = build_table(q, f, Fn(), f6p, (float)left, (float)right, (float)tol);
-  if (rc < 0) return rc;
-  p = std::move(q);
-  return kPotErrOk;
+  F.f6p = [](double) { return 0.0; };
+  return F;
 }
 
-// potential_create_harmonic_dihedral (MxPotential.cpp:428-517):
-// K (1 + cos(n acos(c) - delta)) tabulated in c = cos(phi).  With
-// theta = acos(c), cos(n theta) = T_n(c) and sin(n theta) = U_{n-1}(c)
-// sin(theta), the Chebyshev polynomials of the first and second kind; the
-// domain is [-1, 1], pulled in by a relative sqrt(eps) when delta is not a
-// multiple of pi (the sin(theta) factor has unbounded slope at the ends).
-int make_harmonic_dihedral(Potential &p, double K, int n, double delta, double tol) {
-  if (n < 1) return kPotErrBounds;
-  Potential q;
-  q.flags = kPotFlagR | kPotFlagHarmonic | kPotFlagDihedral;
-  q.name = "Harmonic Dihedral";
-  double a = -1.0, b = 1.0;
-  if (std::fmod(delta, M_PI) != 0) {
-    const double e = std::sqrt((double)std::numeric_limits<float>::epsilon());
-    a = -1.0 / (1.0 + e);
-    b = 1.0 / (1.0 + e);
-  }
-  Fn f = [K, n, delta](double r) {
+// K (1 + cos(n acos(c) - delta)) in c = cos(phi).  With theta = acos(c),
+// cos(n theta) = T_n(c) and sin(n theta) = U_{n-1}(c) sin(theta), the
+// Chebyshev polynomials of the first and second kind.
+PotentialFns harmonic_dihedral_fns(double K, int n, double delta) {
+  PotentialFns F;
+  F.f = [K, n, delta](double r) {
     // T_k and U_k by their common three-term recurrence.
     double t0 = 1.0, t1 = r, u0 = 1.0, u1 = 2 * r;
     for (int k = 2; k <= n; ++k) {
@@ -497,8 +472,81 @@ int make_harmonic_dihedral(Potential &p, double K, int n, double delta, double t
     if (std::fabs(r) < 1.0) return K * (1.0 + Tn * std::cos(delta) + Un1 * std::sqrt(1.0 - r * r) * std::sin(delta));
     return K * (1.0 + Tn * std::cos(delta));
   };
-  Fn f6p = [](double) { return 0.0; };
-  int rc = build_table(q, f, Fn(), f6p, (float)a, (float)b, (float)tol);
+  F.f6p = [](double) { return 0.0; };
+  return F;
+}
+
+int make_lj126(Potential &p, double a, double b, double A, double B,
+               double tol, BuildTrace *trace) {
+  Potential q;
+  q.flags = kPotFlagR2 | kPotFlagLJ126;
+  q.name = "Lennard-Jones";
+  const PotentialFns F = lj126_fns(A, B);
+  int rc = build_table(q, F.f, F.fp, F.f6p, (float)a, (float)b, (float)tol, trace);
+  if (rc < 0) return rc;
+  p = std::move(q);
+  return kPotErrOk;
+}
+
+int make_morse(Potential &p, double d, double alpha, double r0, double min,
+               double max, double tol, BuildTrace *trace) {
+  Potential q;
+  q.flags = kPotFlagR2 | kPotFlagShifted;
+  q.name = "Morse";
+  q.r0_plusone = r0 + 1;
+  const PotentialFns F = morse_fns(d, alpha);
+  int rc = build_table(q, F.f, F.fp, F.f6p, (float)min, (float)max, (float)tol, trace);
+  if (rc < 0) return rc;
+  p = std::move(q);
+  return kPotErrOk;
+}
+
+int make_harmonic(Potential &p, double a, double b, double K, double r0,
+                  double tol, BuildTrace *trace) {
+  Potential q;
+  // The reference sets POTENTIAL_HARMONIC & POTENTIAL_R2, which is 0.
+  q.flags = kPotFlagHarmonic & kPotFlagR2;
+  q.name = "Harmonic";
+  const PotentialFns F = harmonic_fns(K, r0);
+  int rc = build_table(q, F.f, F.fp, F.f6p, (float)a, (float)b, (float)tol, trace);
+  if (rc < 0) return rc;
+  p = std::move(q);
+  return kPotErrOk;
+}
+
+// potential_create_harmonic_angle (MxPotential.cpp:555-597): tabulated on
+// [cos(b), cos(a)], the angles clamped to [0, pi].
+int make_harmonic_angle(Potential &p, double a, double b, double K, double theta0, double tol,
+                        BuildTrace *trace) {
+  Potential q;
+  q.flags = kPotFlagAngle | kPotFlagHarmonic;
+  q.name = "Harmonic Angle";
+  if (a < 0.0) a = 0.0;
+  if (b > M_PI) b = M_PI;
+  const double left = std::cos(b), right = std::cos(a);
+  const PotentialFns F = harmonic_angle_fns(K, theta0);
+  int rc = build_table(q, F.f, F.fp, F.f6p, (float)left, (float)right, (float)tol, trace);
+  if (rc < 0) return rc;
+  p = std::move(q);
+  return kPotErrOk;
+}
+
+// potential_create_harmonic_dihedral (MxPotential.cpp:428-517): the domain
+// is [-1, 1], pulled in by a relative sqrt(eps) when delta is not a multiple
+// of pi (the sin(theta) factor has unbounded slope at the ends).
+int make_harmonic_dihedral(Potential &p, double K, int n, double delta, double tol, BuildTrace *trace) {
+  if (n < 1) return kPotErrBounds;
+  Potential q;
+  q.flags = kPotFlagR | kPotFlagHarmonic | kPotFlagDihedral;
+  q.name = "Harmonic Dihedral";
+  double a = -1.0, b = 1.0;
+  if (std::fmod(delta, M_PI) != 0) {
+    const double e = std::sqrt((double)std::numeric_limits<float>::epsilon());
+    a = -1.0 / (1.0 + e);
+    b = 1.0 / (1.0 + e);
+  }
+  const PotentialFns F = harmonic_dihedral_fns(K, n, delta);
+  int rc = build_table(q, F.f, F.fp, F.f6p, (float)a, (float)b, (float)tol, trace);
   if (rc < 0) return rc;
   p = std::move(q);
   return kPotErrOk;

@@ -65,28 +65,56 @@ struct Potential {
 
 using Fn = std::function<double(double)>;
 
+// The function a table interpolates, its first derivative (empty: the edge
+// slopes are the free-end least-squares ones) and its sixth derivative
+// (which places the interval edges).
+struct PotentialFns {
+  Fn f, fp, f6p;
+};
+
+// Closed forms of the reference's potentials (MxPotential.cpp:149-195
+// Lennard-Jones 12-6, :305-346 harmonic, :3353-3394 Morse, :428-597 harmonic
+// angle and dihedral in the cosine).
+PotentialFns lj126_fns(double A, double B);
+PotentialFns morse_fns(double d, double alpha);
+PotentialFns harmonic_fns(double K, double r0);
+PotentialFns harmonic_angle_fns(double K, double theta0);
+PotentialFns harmonic_dihedral_fns(double K, int n, double delta);
+
+// Intermediate results of build_table (potential_init's locals), recorded
+// when a trace is passed: the stretched domain, the node exponent
+// (potential_getalpha), the interval map before it is scaled by n, and the
+// interval count and error of every candidate table in the order tried.
+struct BuildTrace {
+  float a = 0.0f, b = 0.0f;
+  double exponent = 0.0;
+  float alpha[3] = {0.0f, 0.0f, 0.0f};
+  std::vector<int> tried_n;
+  std::vector<float> tried_err;
+};
+
 // Builds the interpolation table of f on [a, b] to tolerance tol.
 // Returns kPotErrOk or a negative kPotErr* code.
 int build_table(Potential &p, const Fn &f, const Fn &fp, const Fn &f6p,
-                float a, float b, float tol);
+                float a, float b, float tol, BuildTrace *trace = nullptr);
 
 // Factories (MxPotential.cpp:305-366, 932-991, 3353-3424 and
 // DissapativeParticleDynamics.cpp:134-148).  They return a negative code on
 // failure and leave p untouched.
 int make_lj126(Potential &p, double a, double b, double A, double B,
-               double tol);
+               double tol, BuildTrace *trace = nullptr);
 int make_morse(Potential &p, double d, double alpha, double r0, double min,
-               double max, double tol);
+               double max, double tol, BuildTrace *trace = nullptr);
 int make_harmonic(Potential &p, double a, double b, double K, double r0,
-                  double tol);
+                  double tol, BuildTrace *trace = nullptr);
 // potential_create_harmonic_angle (MxPotential.cpp:555-597): a table in
 // cos(theta) on [cos(b), cos(a)] of K (theta - theta0)^2.
 int make_harmonic_angle(Potential &p, double a, double b, double K,
-                        double theta0, double tol);
+                        double theta0, double tol, BuildTrace *trace = nullptr);
 // potential_create_harmonic_dihedral (MxPotential.cpp:428-517): a table in
 // cos(phi) of K (1 + cos(n phi - delta)).
 int make_harmonic_dihedral(Potential &p, double K, int n, double delta,
-                           double tol);
+                           double tol, BuildTrace *trace = nullptr);
 int make_dpd(Potential &p, float alpha, float gamma, float sigma, float cutoff,
              bool shifted);
 

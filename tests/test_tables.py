@@ -6,7 +6,8 @@ by what the mathematics fixes:
   * the interpolant matches the closed-form potential to the requested
     tolerance, measured like the builder measures it (relative to
     max(max |f| on the interval, 1));
-  * the force it returns is the derivative of its own energy;
+  * the force it returns is the derivative of its own energy (to float
+    rounding);
   * the interval map is monotone and sends a to interval 1, b to <= n;
   * harmonic (a quadratic) is reproduced exactly by one interval;
   * the continuation below a matches the first interval at a;
@@ -57,14 +58,14 @@ def check_tolerance(p, f, tol, samples=4000):
 def test_lj_table_within_tolerance(m):
     p = m.Potential.lennard_jones_12_6(0.275, 1.0, W.LJ_A, W.LJ_B, 1e-3)
     assert 1 < p.n < 3048
-    assert check_tolerance(p, LJ, 1e-3) <= 1.5e-3
+    assert check_tolerance(p, LJ, 1e-3) <= 1e-3
 
 
 def test_morse_table_within_tolerance(m):
     p = m.Potential.morse(d=1.0, a=6.0, r0=1.0, min=1e-4, max=2.0, tol=1e-3)
     assert p.flags & (1 << 12)  # SHIFTED
     assert p.r0_plusone == pytest.approx(2.0)
-    assert check_tolerance(p, morse(1.0, 6.0), 1e-3) <= 1.5e-3
+    assert check_tolerance(p, morse(1.0, 6.0), 1e-3) <= 1e-3
 
 
 def test_tighter_tolerance_needs_more_intervals(m):
@@ -73,26 +74,33 @@ def test_tighter_tolerance_needs_more_intervals(m):
 
 
 def test_force_is_derivative_of_energy(m):
-    """f * r equals dE/dr of the interpolant (central differences in r)."""
+    """f * r equals dE/dr of the interpolant: the derivative of the stored
+    polynomial of r's interval, in double from the float coefficients, to
+    float rounding (1e-5 of the interval's derivative scale)."""
     for p in (m.Potential.lennard_jones_12_6(0.275, 1.0, W.LJ_A, W.LJ_B, 1e-3),
               m.Potential.morse(d=1.0, a=6.0, r0=1.0, min=1e-4, max=2.0, tol=1e-3)):
         a, b = p.domain
-        r = np.linspace(a + 0.02, b - 0.02, 60)
-        h = 1e-3
-        de = (table_e(p, r + h) - table_e(p, r - h)) / (2 * h)
-        fr = table_f(p, r) * r
-        scale = np.maximum(np.abs(de), 1.0)
-        assert (np.abs(fr - de) / scale).max() < 2e-2
+        c = p.coefficients.reshape(-1, 8).astype(np.float64)
+        for r in np.linspace(a + 1e-3, b - 1e-3, 300).astype(np.float32):
+            i = interval_of(p, r)
+            ch = c[i]
+            poly = np.poly1d(ch[2:])
+            s = (float(r) - ch[0]) * ch[1]
+            de = poly.deriv()(s) * ch[1]
+            scale = np.abs(np.polyder(ch[2:])).sum() * ch[1]
+            fr = p.eval(np.float32(r) ** 2)[1] * float(r)
+            assert abs(fr - de) <= 1e-5 * scale, (r, fr, de)
 
 
 def test_force_matches_closed_form_derivative(m):
-    """The table bounds the energy error (tol relative to the interval's
-    energy scale); its derivative is looser: within 10% of max(|E'|, 1)."""
+    """The builder bounds the energy error only (tol relative to the
+    interval's energy scale); the interpolant's slope follows E' to within
+    8% of max(|E'|, 1) on the argon table (measured 6.0%, at the steep end)."""
     p = m.Potential.lennard_jones_12_6(0.275, 1.0, W.LJ_A, W.LJ_B, 1e-3)
     r = np.linspace(0.3, 0.99, 50)
     dlj = -12 * W.LJ_A / r ** 13 + 6 * W.LJ_B / r ** 7
     fr = table_f(p, r) * r
-    assert (np.abs(fr - dlj) / np.maximum(np.abs(dlj), 1.0)).max() < 0.1
+    assert (np.abs(fr - dlj) / np.maximum(np.abs(dlj), 1.0)).max() < 0.08
 
 
 def test_interval_map_monotone(m):
