@@ -150,9 +150,17 @@ int mxg_engine_add_force_constant(mxg_engine *e, int type, const float force[3])
 int mxg_engine_add_force_friction(mxg_engine *e, int type, float coef, float mean, float std, float duration);
 int mxg_engine_set_target_energy(mxg_engine *e, int type, double energy);
 
-/* Seed of the counter-based generator (no reference counterpart: the
- * reference uses a global Mersenne twister). */
+/* Seed of the counter-based generators (no reference counterpart: the
+ * reference uses a global Mersenne twister).  Replica r draws from its own
+ * seed, seed + r * 0x9E3779B97F4A7C15 (mod 2^64) unless set with
+ * mxg_engine_set_replica_seed; a particle's random streams (persistent
+ * random force, DPD counter noise, overlap direction) are keyed by its
+ * replica's seed and its rank among the replica's particles in load order,
+ * so a replica evolves as a standalone engine loaded with the same particles
+ * and that seed.  Setting the engine seed resets every replica seed. */
 int mxg_engine_set_seed(mxg_engine *e, uint64_t seed);
+int mxg_engine_set_replica_seed(mxg_engine *e, int replica, uint64_t seed);
+int mxg_engine_replica_seed(mxg_engine *e, int replica, uint64_t *seed);
 
 /* 0: DPD random term as the reference computes it (sigma*omega/sqrt(dt),
  * dpd_eval.hpp:57); 1: a zero-mean, unit-variance uniform number per pair
@@ -202,9 +210,14 @@ int mxg_engine_get_waves(mxg_engine *e);
  * default.  No reference counterpart (mdcore drives pthread runners). */
 int mxg_engine_set_graphs(mxg_engine *e, int on);
 
-/* engine_load (engine.h:612-613): x, v hold 3N doubles; particle ids are
- * 0..N-1 in argument order; replica and flags may be NULL. */
-int mxg_engine_load(mxg_engine *e, int N, const double *x, const double *v, const int *type,
+/* engine_load (engine.h:612-613): x, v hold 3N doubles; pid[k] is the id of
+ * particle k (MxParticle::id; NULL: ids 0..N-1 in argument order), unique
+ * and below 64 N + 65536 (partlist is sized by the largest id); bonds,
+ * angles, dihedrals and exclusions name particles by these ids.  replica
+ * and flags may be NULL.  Per-particle read-backs (state, unload, number
+ * density, neighbour counts, species) are in argument order.  A load
+ * replaces the particles and drops the previous bonds and terms. */
+int mxg_engine_load(mxg_engine *e, int N, const double *x, const double *v, const int *type, const int *pid,
                     const int *replica, const uint16_t *flags);
 
 /* engine_bond_alloc + _bond_init (engine_bonded.cpp:1317-1360,

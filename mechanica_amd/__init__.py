@@ -260,7 +260,20 @@ class Universe:
         self._check(_capi.lib().mxg_engine_set_target_energy(self._h, int(type_id), float(energy)))
 
     def set_seed(self, seed):
+        """Seed of the counter-based generators; resets every replica's seed
+        to seed + r * 0x9E3779B97F4A7C15."""
         self._check(_capi.lib().mxg_engine_set_seed(self._h, int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def set_replica_seed(self, replica, seed):
+        """Seed of one replica's random streams (a replica evolves as a
+        standalone engine with the same particles and this seed)."""
+        self._check(_capi.lib().mxg_engine_set_replica_seed(self._h, int(replica),
+                                                            int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def replica_seed(self, replica):
+        s = C.c_uint64()
+        self._check(_capi.lib().mxg_engine_replica_seed(self._h, int(replica), C.byref(s)))
+        return s.value
 
     def set_dpd_noise(self, mode):
         self._check(_capi.lib().mxg_engine_set_dpd_noise(self._h, int(mode)))
@@ -311,17 +324,22 @@ class Universe:
     def set_virial(self, on=True):
         self._check(_capi.lib().mxg_engine_set_virial(self._h, int(bool(on))))
 
-    def load(self, x, v=None, types=None, replica=None, flags=None):
+    def load(self, x, v=None, types=None, replica=None, flags=None, pid=None):
+        """engine_load: particle k gets id pid[k] (k without pid); bonds and
+        terms name particles by id, read-backs are in load order."""
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
         N = x.shape[0]
         v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
         t = np.zeros(N, np.int32) if types is None else np.ascontiguousarray(types, dtype=np.int32)
         r = None if replica is None else np.ascontiguousarray(replica, dtype=np.int32)
+        p = None if pid is None else np.ascontiguousarray(pid, dtype=np.int32)
         fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint16)
         self._check(_capi.lib().mxg_engine_load(
-            self._h, N, _dp(x), _dp(v), _ip(t), _ip(r) if r is not None else None,
+            self._h, N, _dp(x), _dp(v), _ip(t), _ip(p) if p is not None else None,
+            _ip(r) if r is not None else None,
             fl.ctypes.data_as(_capi.c_u16_p) if fl is not None else None))
         self.N = N
+        self.M = 0
 
     def set_bonds(self, i, j, pots, dissociation=None):
         i = np.ascontiguousarray(i, dtype=np.int32)

@@ -70,7 +70,10 @@ SIGNATURES = {
     "mxg_engine_tune_log": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_double_p]),
     "mxg_engine_set_waves": (C.c_int, [C.c_void_p, C.c_int]),
     "mxg_engine_get_waves": (C.c_int, [C.c_void_p]),
-    "mxg_engine_load": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_int_p, c_int_p, c_u16_p]),
+    "mxg_engine_load": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_int_p, c_int_p, c_int_p,
+                                  c_u16_p]),
+    "mxg_engine_set_replica_seed": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64]),
+    "mxg_engine_replica_seed": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
     "mxg_engine_set_bonds": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, C.POINTER(C.c_void_p),
                                        c_double_p]),
     "mxg_engine_set_boundaries": (C.c_int, [C.c_void_p, c_int_p, c_float_p, c_float_p, c_float_p]),

@@ -49,5 +49,21 @@ def build(verbose=False):
     return LIB
 
 
+def build_driver(src, exe):
+    """Compiles a C++ program that drives the engine through include/mxg.hpp
+    (host code only, g++), linked against libmxg.so with a run path
+    relative to the executable."""
+    root = os.path.dirname(HERE)
+    rel = os.path.relpath(HERE, os.path.dirname(os.path.abspath(exe)))
+    tmp = exe + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src, "-L", HERE, "-lmxg",
+           "-Wl,-rpath,$ORIGIN/" + rel, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, exe)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

@@ -200,6 +200,16 @@ int mxg_engine_set_seed(mxg_engine *e, uint64_t seed) {
   return e->e.set_seed(seed);
 }
 
+int mxg_engine_set_replica_seed(mxg_engine *e, int replica, uint64_t seed) {
+  MXG_CHECK(e);
+  return e->e.set_replica_seed(replica, seed);
+}
+
+int mxg_engine_replica_seed(mxg_engine *e, int replica, uint64_t *seed) {
+  MXG_CHECK(e);
+  return e->e.replica_seed(replica, seed);
+}
+
 int mxg_engine_set_dpd_noise(mxg_engine *e, int mode) {
   MXG_CHECK(e);
   return e->e.set_dpd_counter_noise(mode);
@@ -268,10 +278,10 @@ int mxg_engine_set_virial(mxg_engine *e, int on) {
   return e->e.set_virial(on);
 }
 
-int mxg_engine_load(mxg_engine *e, int N, const double *x, const double *v, const int *type, const int *replica,
-                    const uint16_t *flags) {
+int mxg_engine_load(mxg_engine *e, int N, const double *x, const double *v, const int *type, const int *pid,
+                    const int *replica, const uint16_t *flags) {
   MXG_CHECK(e);
-  return e->e.load(N, x, v, type, replica, flags);
+  return e->e.load(N, x, v, type, pid, replica, flags);
 }
 
 int mxg_engine_set_bonds(mxg_engine *e, int M, const int32_t *i, const int32_t *j, const mxg_potential *const *pots,

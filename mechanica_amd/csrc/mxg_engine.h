@@ -57,11 +57,16 @@ struct HostReplica {
   int cdim[3], span[3];
   int cell_base = 0, ncells = 0;
   int slot_base = 0, nparts = 0;
+  int gid_base = 0;  // first gid of the replica's particles
   int tile_begin = 0, tile_end = 0;
   std::vector<double> corg[3];  // cell origins per axis
 };
 
 struct DeviceState;  // device buffers (defined in mxg_engine.hip)
+
+// Default seed of replica r under the engine seed: seed itself for replica 0
+// (a single universe), then Weyl steps of the golden ratio.
+inline uint64_t replica_seed_default(uint64_t seed, int r) { return seed + (uint64_t)r * 0x9E3779B97F4A7C15ull; }
 
 class Engine {
  public:
@@ -79,10 +84,20 @@ class Engine {
   // duration, coef).
   int set_single_body(int type, int kind, const float *p);
   int set_target_energy(int type, double e);
+  // Seed of the counter-based generators.  Replica r draws from its own
+  // seed (replica_seed_default(seed, r) unless set_replica_seed overrides
+  // it); a particle's streams are keyed by its replica's seed and its rank
+  // among that replica's particles in load order, so a replica evolves as a
+  // standalone engine holding the same particles with that seed.
   int set_seed(uint64_t seed);
+  int set_replica_seed(int replica, uint64_t seed);
+  int replica_seed(int replica, uint64_t *seed) const;
   int set_dpd_counter_noise(int on);
   int set_max_dist_fraction(float f);
-  int load(int N, const double *x, const double *v, const int *type, const int *replica,
+  // engine_load: particle k has id pid[k] (k when pid is null); bonds and
+  // terms name particles by these ids, and per-particle read-backs are in
+  // load order.
+  int load(int N, const double *x, const double *v, const int *type, const int *pid, const int *replica,
            const uint16_t *flags);
   int set_bonds(int M, const int *bi, const int *bj, const Potential *const *pots,
                 const double *dissociation);
@@ -115,7 +130,7 @@ class Engine {
   int step(int nsteps);
   int sync();
 
-  // Per particle (by id) snapshots; any pointer may be null.
+  // Per particle snapshots in load order; any pointer may be null.
   int get_state(float *xlocal, int *cell, float *v, float *f, float *pf, float *nd = nullptr);
   int get_global(double *x, double *v);
   int neighbour_counts(int *counts);
@@ -225,7 +240,21 @@ class Engine {
   };
   std::vector<HostTerm> angles_, dihedrals_, exclusions_;
   std::vector<Potential> apots_, dpots_;
-  std::vector<int> prep_;  // replica of each particle id
+  // Particle ids.  Internally a particle is its gid: the replica's first
+  // gid plus its rank among the replica's particles in load order (so each
+  // replica holds a contiguous gid range, and gid - first is the key of its
+  // random streams).  The caller's ids are pids (load indices without a pid
+  // array).
+  std::vector<int> prep_;           // replica of each gid
+  std::vector<int> gid_of_load_;    // load index -> gid
+  std::vector<int> load_of_gid_;    // gid -> load index
+  std::vector<int> gid_of_pid_;     // pid -> gid, -1 for ids not loaded
+  std::vector<int> pid_of_gid_;     // gid -> the caller's id
+  int gid_of(int pid) const {
+    return (pid >= 0 && pid < (int)gid_of_pid_.size()) ? gid_of_pid_[pid] : -1;
+  }
+  std::vector<uint64_t> rep_seed_;  // per replica
+  int upload_replicas();
   struct WallInfo {
     int kind = kBoundaryPeriodic;
     float vel[3] = {0, 0, 0};

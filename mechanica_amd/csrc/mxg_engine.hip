@@ -76,6 +76,11 @@ struct DevReplica {
   // the cell's bound; drop_check is set when any offset can drop.
   float near_eps;
   int drop_check;
+  // Random streams of the replica: its seed, and the gid of its first
+  // particle (a particle's stream id is its gid minus id_base).
+  uint32_t seed_lo, seed_hi;
+  int id_base;
+  int pad;
 };
 
 enum : int { kWallPeriodicShift = 0, kWallActiveShift = 3 };
@@ -332,7 +337,8 @@ struct LoopConsts {
   int ntypes;
   float cut2_pair, cut2_self;
   float dpd_delta;
-  uint32_t seed_lo, seed_hi;
+  uint32_t seed_lo, seed_hi;  // of the replica being computed
+  int id_base;                // its first gid (stream id = gid - id_base)
   int dpd_noise;
   float cutoff, nd_norm;
 };
@@ -346,6 +352,7 @@ __device__ __forceinline__ LoopConsts loop_consts(const StepArgs &A) {
   C.dpd_delta = A.dpd_delta;
   C.seed_lo = A.seed_lo;
   C.seed_hi = A.seed_hi;
+  C.id_base = 0;
   C.dpd_noise = A.dpd_noise;
   C.cutoff = A.cutoff;
   C.nd_norm = A.nd_norm;
@@ -593,16 +600,17 @@ __device__ __forceinline__ bool pair_force(const LoopConsts &A, const PairTabs &
   if (pidx < 0) return true;
   const DevPot P = tb.pots[pidx];
   if (r2 < P.a2) {
-    int qpid = nb.pid(q);
+    // Stream ids within the replica (gid - id_base).
+    const int mine = mypid - A.id_base, other = nb.pid(q) - A.id_base;
     uint32_t ki, kj;
     float sgn = 1.0f;
     if (self_pair) {
-      ki = (uint32_t)min(mypid, qpid);
-      kj = (uint32_t)max(mypid, qpid);
-      sgn = (mypid == (int)ki) ? 1.0f : -1.0f;
+      ki = (uint32_t)min(mine, other);
+      kj = (uint32_t)max(mine, other);
+      sgn = (mine == (int)ki) ? 1.0f : -1.0f;
     } else {
-      ki = (uint32_t)(p_is_i ? mypid : qpid);
-      kj = (uint32_t)(p_is_i ? qpid : mypid);
+      ki = (uint32_t)(p_is_i ? mine : other);
+      kj = (uint32_t)(p_is_i ? other : mine);
     }
     float3 d = overlap_dir(ki, kj, t1, A.seed_lo, A.seed_hi, P.a);
     dxx = sgn * d.x;
@@ -642,7 +650,8 @@ __device__ __forceinline__ bool pair_force(const LoopConsts &A, const PairTabs &
     float fr;
     if (A.dpd_noise) {
       const int qpid = nb.pid(q);
-      fr = P.dpd_sigma * omega_c * pair_uniform((uint32_t)min(mypid, qpid), (uint32_t)max(mypid, qpid), tk) *
+      fr = P.dpd_sigma * omega_c *
+           pair_uniform((uint32_t)(min(mypid, qpid) - A.id_base), (uint32_t)(max(mypid, qpid) - A.id_base), tk) *
            A.dpd_delta;
     } else {
       fr = P.dpd_sigma * omega_c * A.dpd_delta;
@@ -807,15 +816,17 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
 // refresh: INTEGRATOR_UPDATE_PERSISTENTFORCE (set for the Euler step and the
 // first Runge-Kutta stage only).  The random vector comes from Philox keyed
 // by (id, time).
-__device__ __forceinline__ void single_body(const StepArgs &A, int s, int mypid, int ti, const DevType &T,
-                                            unsigned t1, bool refresh, const float4 &vs, float &fx, float &fy,
-                                            float &fz, float4 &pfs) {
+__device__ __forceinline__ void single_body(const StepArgs &A, const LoopConsts &C, int s, int mypid, int ti,
+                                            const DevType &T, unsigned t1, bool refresh, const float4 &vs, float &fx,
+                                            float &fy, float &fz, float4 &pfs) {
   pfs = A.with_pf ? float4(A.pf[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
   if (T.sb == kSbNone) return;
   if (T.sb == kSbGaussian || T.sb == kSbFriction) {
-    if (refresh && ((uint64_t)t1 + (uint64_t)mypid) % T.rnd_steps == 0) {
-      u32x4 ctr = {{(uint32_t)mypid, t1, kStreamSingle, 0u}};
-      u32x4 r = philox4x32_10(ctr, A.seed_lo, A.seed_hi);
+    // Stream id within the replica (MxParticle::id of a standalone engine).
+    const uint32_t sid = (uint32_t)(mypid - C.id_base);
+    if (refresh && ((uint64_t)t1 + (uint64_t)sid) % T.rnd_steps == 0) {
+      u32x4 ctr = {{sid, t1, kStreamSingle, 0u}};
+      u32x4 r = philox4x32_10(ctr, C.seed_lo, C.seed_hi);
       float g = gauss_from(r.v[0], r.v[1]);
       float mag = T.rnd_mean + T.rnd_std * g;
       float theta = 6.2831853f * u01_closed_open(r.v[2]);
@@ -1644,9 +1655,8 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
   const int lane = tid & 63, wave = tid >> 6;
   const int passthrough = A.err[2];
   const unsigned t1 = *A.time + 1u;
-  const uint32_t tk = step_key(A.seed_lo, t1);
   const PairTabs tb = stage_pair_tabs(A, reinterpret_cast<unsigned char *>(nlist + B.cap * kBT), A.npots);
-  const LoopConsts C = loop_consts(A);
+  LoopConsts C = loop_consts(A);
   (void)kOffMax;
   const DevFlux G *const fluxes = kFlux ? A.fluxes : nullptr;
   const int G *const flux_off = kFlux ? A.flux_off : nullptr;
@@ -1656,6 +1666,10 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
     const int bi = xcd_brick(g, B.nbricks);
     const int4 br = B.bricks[bi];
     const DevReplica R = A.rep[br.x];  // by value: the fields stay in registers
+    C.seed_lo = R.seed_lo;
+    C.seed_hi = R.seed_hi;
+    C.id_base = R.id_base;
+    const uint32_t tk = step_key(R.seed_lo, t1);
     const int sp0 = R.span[0], sp1 = R.span[1], sp2 = R.span[2];
     const int nb0 = min(B.bdim[0], R.cdim[0] - br.y);
     const int nb1 = min(B.bdim[1], R.cdim[1] - br.z);
@@ -1803,7 +1817,7 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
       float nd = A.nd_norm;
       float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (!passthrough) {
-        if (A.random) single_body(A, s, mypid, ti, T, t1, B.rk_stage <= 0, vs, fx, fy, fz, pfs);
+        if (A.random) single_body(A, C, s, mypid, ti, T, t1, B.rk_stage <= 0, vs, fx, fy, fz, pfs);
         if (R.wall >> kWallActiveShift) {
           const int wm = cell_walls(R, lx, ly, lz);
           if (wm) wall_forces(A, R, wm, ti, T.radius, xs, vs, C.dpd_delta, fx, fy, fz, acc[0]);
@@ -2338,9 +2352,8 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
   if ((int)blockIdx.x >= nov) return;  // usually no brick overflowed
   const int passthrough = A.err[2];
   const unsigned t1 = *A.time + 1u;
-  const uint32_t tk = step_key(A.seed_lo, t1);
   const PairTabs tb = stage_pair_tabs(A, gsmem, A.npots);
-  const LoopConsts C = loop_consts(A);
+  LoopConsts C = loop_consts(A);
   constexpr bool kFlux = (MODE & kModeFlux) != 0;
   const DevFlux G *const fluxes = kFlux ? A.fluxes : nullptr;
   const int G *const flux_off = kFlux ? A.flux_off : nullptr;
@@ -2350,6 +2363,10 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
     const int bi = B.ovf_list[i];
     const int4 br = B.bricks[bi];
     const DevReplica R = A.rep[br.x];  // by value: the fields stay in registers
+    C.seed_lo = R.seed_lo;
+    C.seed_hi = R.seed_hi;
+    C.id_base = R.id_base;
+    const uint32_t tk = step_key(R.seed_lo, t1);
     const int nb0 = min(B.bdim[0], R.cdim[0] - br.y);
     const int nb1 = min(B.bdim[1], R.cdim[1] - br.z);
     const int nb2 = min(B.bdim[2], R.cdim[2] - br.w);
@@ -2407,7 +2424,7 @@ __global__ __launch_bounds__(kBT) void k_brick_global(const StepArgs *__restrict
       float nd = A.nd_norm;
       float4 pfs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (!passthrough) {
-        if (A.random) single_body(A, s, mypid, ti, T, t1, B.rk_stage <= 0, vs, fx, fy, fz, pfs);
+        if (A.random) single_body(A, C, s, mypid, ti, T, t1, B.rk_stage <= 0, vs, fx, fy, fz, pfs);
         if (R.wall >> kWallActiveShift) {
           const int wm = cell_walls(R, lx, ly, lz);
           if (wm) wall_forces(A, R, wm, ti, T.radius, xs, vs, C.dpd_delta, fx, fy, fz, acc[0]);
@@ -2583,6 +2600,8 @@ int Engine::init(const ReplicaSpec *specs, int nrep, double cutoff, double dt) {
     base += R.ncells;
   }
   ncells_ = base;
+  rep_seed_.resize(nrep);
+  for (int r = 0; r < nrep; ++r) rep_seed_[r] = replica_seed_default(seed_, r);
   d_ = new DeviceState();
   if (int rc = check_device(hipStreamCreateWithFlags(&d_->stream, hipStreamNonBlocking), "hipStreamCreate"))
     return rc;
@@ -2707,6 +2726,9 @@ int Engine::build_geometry() {
     }
     D.cell_base = R.cell_base;
     D.wall = 7 << kWallPeriodicShift;  // periodic until set_boundaries says otherwise
+    D.seed_lo = (uint32_t)rep_seed_[r];
+    D.seed_hi = (uint32_t)(rep_seed_[r] >> 32);
+    D.id_base = R.gid_base;
     for (int c = 0; c < R.ncells; ++c) cell_rep[R.cell_base + c] = (int)r;
     // engine_advance_forward_euler constants (engine_advance.cpp:262-281).
     const float dt = (float)dt_;
@@ -2890,8 +2912,38 @@ int Engine::set_target_energy(int type, double e) {
 
 int Engine::set_seed(uint64_t seed) {
   seed_ = seed;
+  for (size_t r = 0; r < rep_seed_.size(); ++r) rep_seed_[r] = replica_seed_default(seed, (int)r);
   args_dirty_ = true;
+  return upload_replicas();
+}
+
+int Engine::set_replica_seed(int replica, uint64_t seed) {
+  if (replica < 0 || replica >= (int)reps_.size()) return reject(kErrRange, "replica index out of range");
+  rep_seed_[replica] = seed;
+  return upload_replicas();
+}
+
+int Engine::replica_seed(int replica, uint64_t *seed) const {
+  if (replica < 0 || replica >= (int)reps_.size() || !seed) return kErrRange;
+  *seed = rep_seed_[replica];
   return kErrOk;
+}
+
+// Copies the per-replica constants that change after set-up (random-stream
+// seeds and first gids) into the device table.
+int Engine::upload_replicas() {
+  if (!d_ || !d_->rep) return kErrOk;
+  DeviceState *d = d_;
+  for (size_t r = 0; r < reps_.size(); ++r) {
+    DevReplica &D = d->drep[r];
+    D.seed_lo = (uint32_t)rep_seed_[r];
+    D.seed_hi = (uint32_t)(rep_seed_[r] >> 32);
+    D.id_base = reps_[r].gid_base;
+  }
+  int rc = hipMemcpyAsync(d->rep, d->drep.data(), d->drep.size() * sizeof(DevReplica), hipMemcpyHostToDevice,
+                          d->stream);
+  rc |= hipStreamSynchronize(d->stream);
+  return check_device(rc, "replica upload");
 }
 
 int Engine::set_dpd_counter_noise(int on) {
@@ -3011,11 +3063,42 @@ int Engine::upload_potentials() {
 // engine_load / space_addpart (space.cpp:397-477): cell index
 // (x - origin) * ih in double, local coordinate x - cell origin, particles
 // kept in insertion order inside each cell.
-int Engine::load(int N, const double *x, const double *v, const int *type, const int *replica,
+int Engine::load(int N, const double *x, const double *v, const int *type, const int *pid_in, const int *replica,
                  const uint16_t *flags) {
   if (!d_) return fail(kErrNull, "engine not initialised");
   if (N <= 0 || !x || !type) return fail(kErrNull, "empty particle set");
   const int nt = (int)types_.size();
+  // Particle ids (MxParticle::id, engine_load's pid): unique, non-negative;
+  // the id -> particle table is sized by the largest id, as partlist is.
+  long long maxpid = N - 1;
+  if (pid_in) {
+    for (int i = 0; i < N; ++i) {
+      if (pid_in[i] < 0) return fail(kErrRange, "particle ids must be non-negative");
+      maxpid = std::max<long long>(maxpid, pid_in[i]);
+    }
+    if (maxpid >= 64LL * N + 65536) return fail(kErrRange, "particle ids must be below 64 N + 65536");
+  }
+  std::vector<int> rep_count(reps_.size(), 0), rank(N);
+  for (int i = 0; i < N; ++i) {
+    const int r = replica ? replica[i] : 0;
+    if (r < 0 || r >= (int)reps_.size()) return fail(kErrRange, "replica index out of range");
+    rank[i] = rep_count[r]++;
+  }
+  {
+    int base = 0;
+    for (size_t r = 0; r < reps_.size(); ++r) {
+      reps_[r].gid_base = base;
+      base += rep_count[r];
+    }
+  }
+  std::vector<int> gid(N), gid_of_pid((size_t)maxpid + 1, -1), pid_of_gid(N);
+  for (int i = 0; i < N; ++i) {
+    gid[i] = reps_[replica ? replica[i] : 0].gid_base + rank[i];
+    const int id = pid_in ? pid_in[i] : i;
+    if (gid_of_pid[id] >= 0) return fail(kErrRange, "particle id " + std::to_string(id) + " given twice");
+    gid_of_pid[id] = gid[i];
+    pid_of_gid[gid[i]] = id;
+  }
   std::vector<int> cellof(N);
   std::vector<float4> xl(N);
   for (int i = 0; i < N; ++i) {
@@ -3052,14 +3135,15 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   std::vector<int> pid(N), cell(N), slot_of(N);
   for (int i = 0; i < N; ++i) {
     const int k = fill[cellof[i]]++;
+    const int g = gid[i];
     xs[k] = xl[i];
     float idbits;
-    std::memcpy(&idbits, &i, sizeof(idbits));  // v.w carries the particle id
+    std::memcpy(&idbits, &g, sizeof(idbits));  // v.w carries the gid
     vs[k] = v ? make_float4((float)v[3 * (size_t)i], (float)v[3 * (size_t)i + 1], (float)v[3 * (size_t)i + 2], idbits)
               : make_float4(0.f, 0.f, 0.f, idbits);
-    pid[k] = i;
+    pid[k] = g;
     cell[k] = cellof[i];
-    slot_of[i] = k;
+    slot_of[g] = k;
   }
   // Tiles: per replica, slot ranges of kTile.
   std::vector<int4> tiles;
@@ -3076,8 +3160,16 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   }
   N_ = N;
   prep_.assign(N, 0);
-  if (replica)
-    for (int i = 0; i < N; ++i) prep_[i] = replica[i];
+  gid_of_load_ = gid;
+  load_of_gid_.assign(N, 0);
+  for (int i = 0; i < N; ++i) {
+    load_of_gid_[gid[i]] = i;
+    prep_[gid[i]] = replica ? replica[i] : 0;
+  }
+  gid_of_pid_.swap(gid_of_pid);
+  pid_of_gid_.swap(pid_of_gid);
+  // Bonds name particles of the previous load.
+  M_ = 0;
   // Species buffers are sized by the particle count.
   const bool had_species = species_on_;
   species_on_ = false;
@@ -3129,6 +3221,7 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
   pid_prev_valid_ = false;
   parity_ = 0;
   args_dirty_ = true;
+  if (int rc = upload_replicas()) return rc;
   if (int rc = ensure_rk_buffers()) return rc;
   if (had_species || !fluxes_.empty())
     if (int rc = ensure_species()) return rc;
@@ -3262,8 +3355,10 @@ int Engine::set_bonds(int M, const int *bi, const int *bj, const Potential *cons
   bpots_.clear();
   std::vector<const Potential *> seen;
   for (int b = 0; b < M; ++b) {
-    if (bi[b] < 0 || bi[b] >= N_ || bj[b] < 0 || bj[b] >= N_ || bi[b] == bj[b])
+    const int gi = gid_of(bi[b]), gj = gid_of(bj[b]);
+    if (gi < 0 || gj < 0 || gi == gj)
       return fail(kErrBond, "bond " + std::to_string(b) + " has an invalid particle id");
+    if (prep_[gi] != prep_[gj]) return fail(kErrBond, "bond " + std::to_string(b) + " joins two replicas");
     const Potential *p = pots[b];
     if (!p || p->kind != kPotKindTable || p->n < 1) return fail(kErrBond, "bond potential must be a table potential");
     auto it = std::find(seen.begin(), seen.end(), p);
@@ -3275,12 +3370,11 @@ int Engine::set_bonds(int M, const int *bi, const int *bj, const Potential *cons
     } else {
       idx = (int)(it - seen.begin());
     }
-    vi[b] = bi[b];
-    vj[b] = bj[b];
+    vi[b] = gi;
+    vj[b] = gj;
     vp[b] = idx;
     vd[b] = dissociation ? dissociation[b] : std::numeric_limits<double>::infinity();
   }
-  // Bond replicas must match (a bond never crosses universes).
   std::vector<int> off(N_ + 1, 0);
   for (int b = 0; b < M; ++b) {
     off[vi[b] + 1]++;
@@ -3452,7 +3546,7 @@ int Engine::set_species_values(const float *values) {
   int rc = hipMemcpy(pid.data(), d->pid[parity_], N_ * sizeof(int), hipMemcpyDeviceToHost);
   if (rc) return check_device(rc, "species upload");
   for (int s = 0; s < N_; ++s) {
-    const float *p = values + 4 * (size_t)pid[s];
+    const float *p = values + 4 * (size_t)load_of_gid_[pid[s]];
     v[s] = make_float4(p[0], p[1], p[2], p[3]);
   }
   rc |= hipMemcpy(d->spec[0], v.data(), N_ * sizeof(float4), hipMemcpyHostToDevice);
@@ -3476,18 +3570,20 @@ int Engine::get_species(float *values, float *q) {
   if (rc) return check_device(rc, "species read-back");
   for (int s = 0; s < N_; ++s) {
     const float vv[4] = {v[s].x, v[s].y, v[s].z, v[s].w}, qq[4] = {a[s].x, a[s].y, a[s].z, a[s].w};
+    const size_t i = load_of_gid_[pid[s]];
     for (int k = 0; k < 4; ++k) {
-      if (values) values[4 * (size_t)pid[s] + k] = vv[k];
-      if (q) q[4 * (size_t)pid[s] + k] = qq[k];
+      if (values) values[4 * i + k] = vv[k];
+      if (q) q[4 * i + k] = qq[k];
     }
   }
   return kErrOk;
 }
 
-// Shared checks of a term's particles: valid, distinct ids in one replica.
-static bool term_ids_ok(const int *ids, int n, int N, const std::vector<int> &prep) {
+// Shared checks of a term's particles (gids): loaded, distinct, in one
+// replica.
+static bool term_ids_ok(const int *ids, int n, const std::vector<int> &prep) {
   for (int a = 0; a < n; ++a) {
-    if (ids[a] < 0 || ids[a] >= N) return false;
+    if (ids[a] < 0) return false;
     for (int b = 0; b < a; ++b)
       if (ids[a] == ids[b]) return false;
     if (prep[ids[a]] != prep[ids[0]]) return false;
@@ -3512,8 +3608,8 @@ int Engine::set_angles(int M, const int *i, const int *j, const int *k, const Po
   std::vector<Potential> list;
   std::vector<const Potential *> seen;
   for (int a = 0; a < M; ++a) {
-    const int ids[3] = {i[a], j[a], k[a]};
-    if (!term_ids_ok(ids, 3, N_, prep_))
+    const int ids[3] = {gid_of(i[a]), gid_of(j[a]), gid_of(k[a])};
+    if (!term_ids_ok(ids, 3, prep_))
       return reject(kErrRange, "angle " + std::to_string(a) + " has invalid or repeated particle ids");
     const Potential *p = pots[a];
     if (!p || p->kind != kPotKindTable || p->n < 1)
@@ -3536,8 +3632,8 @@ int Engine::set_dihedrals(int M, const int *i, const int *j, const int *k, const
   std::vector<Potential> list;
   std::vector<const Potential *> seen;
   for (int a = 0; a < M; ++a) {
-    const int ids[4] = {i[a], j[a], k[a], l[a]};
-    if (!term_ids_ok(ids, 4, N_, prep_))
+    const int ids[4] = {gid_of(i[a]), gid_of(j[a]), gid_of(k[a]), gid_of(l[a])};
+    if (!term_ids_ok(ids, 4, prep_))
       return reject(kErrRange, "dihedral " + std::to_string(a) + " has invalid or repeated particle ids");
     const Potential *p = pots[a];
     if (!p || p->kind != kPotKindTable || p->n < 1)
@@ -3557,8 +3653,8 @@ int Engine::set_exclusions(int M, const int *i, const int *j) {
   if (M < 0 || (M > 0 && (!i || !j))) return reject(kErrNull, "exclusion arrays missing");
   std::vector<HostTerm> terms(M);
   for (int a = 0; a < M; ++a) {
-    const int ids[2] = {i[a], j[a]};
-    if (!term_ids_ok(ids, 2, N_, prep_))
+    const int ids[2] = {gid_of(i[a]), gid_of(j[a])};
+    if (!term_ids_ok(ids, 2, prep_))
       return reject(kErrRange, "exclusion " + std::to_string(a) + " has invalid or repeated particle ids");
     terms[a] = HostTerm{{ids[0], ids[1], -1, -1}, -1};
   }
@@ -4118,7 +4214,9 @@ int Engine::step(int nsteps) {
     step_ms_ = ms / nsteps;
   }
   if (herr[0] != 0) {
-    return fail(herr[0], "particle " + std::to_string(herr[1]) + " moved too fast (more than one cell in a step)");
+    const int g = herr[1];
+    const int id = (g >= 0 && g < (int)pid_of_gid_.size()) ? pid_of_gid_[g] : g;
+    return fail(herr[0], "particle " + std::to_string(id) + " moved too fast (more than one cell in a step)");
   }
   return kErrOk;
 }
@@ -4156,7 +4254,7 @@ int Engine::get_state(float *xlocal, int *cell, float *v, float *f, float *pf, f
   rc |= hipStreamSynchronize(s);
   if (rc) return check_device(rc, "state download");
   for (int k = 0; k < N_; ++k) {
-    const size_t i = hpid[k];
+    const size_t i = load_of_gid_[hpid[k]];
     if (xlocal) {
       xlocal[3 * i] = hx[k].x;
       xlocal[3 * i + 1] = hx[k].y;
@@ -4179,7 +4277,7 @@ int Engine::get_state(float *xlocal, int *cell, float *v, float *f, float *pf, f
     // the step that computed them.
     const std::vector<int> &order = pid_prev_valid_ ? hpidf : hpid;
     for (int k = 0; k < N_; ++k) {
-      const size_t i = order[k];
+      const size_t i = load_of_gid_[order[k]];
       if (f) {
         f[3 * i] = hf[k].x;
         f[3 * i + 1] = hf[k].y;
@@ -4218,9 +4316,12 @@ int Engine::neighbour_counts(int *counts) {
   StepArgs A;
   fill_step_args(&A, parity_);
   hipLaunchKernelGGL(k_ncount, dim3(ntiles_), dim3(kTile), 0, d->stream, A);
-  int rc = hipMemcpyAsync(counts, d->ncount, N_ * sizeof(int), hipMemcpyDeviceToHost, d->stream);
+  std::vector<int> by_gid(N_);
+  int rc = hipMemcpyAsync(by_gid.data(), d->ncount, N_ * sizeof(int), hipMemcpyDeviceToHost, d->stream);
   rc |= hipStreamSynchronize(d->stream);
-  return check_device(rc, "neighbour counts");
+  if (rc) return check_device(rc, "neighbour counts");
+  for (int g = 0; g < N_; ++g) counts[load_of_gid_[g]] = by_gid[g];
+  return kErrOk;
 }
 
 int Engine::observables(int replica, double *out) {

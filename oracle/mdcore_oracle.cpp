@@ -1975,6 +1975,35 @@ int oracle_load(void *h, int N, const double *x, const double *v, const int *typ
   return 0;
 }
 
+// Loads a cell-local state as the engine reports it: particle i (id i) in
+// cell[i] at local position xl[i] (no rounding through global coordinates),
+// velocity v[i] and persistent random force pf[i] (may be NULL).
+int oracle_load_local(void *h, int N, const float *xl, const int *cell, const float *v, const int *type,
+                      const uint16_t *flags, const float *pf) {
+  Oracle *o = (Oracle *)h;
+  for (auto &c : o->cells) c.parts.clear();
+  o->nparts = N;
+  for (int i = 0; i < N; ++i) {
+    if (cell[i] < 0 || cell[i] >= (int)o->cells.size()) return kErrRange;
+    Part p;
+    std::memset(&p, 0, sizeof(p));
+    const Type &t = o->types[type[i]];
+    p.radius = t.radius;
+    p.mass = t.mass;
+    p.imass = t.imass;
+    p.id = i;
+    p.type = (int16_t)type[i];
+    p.flags = flags ? flags[i] : 0;
+    for (int k = 0; k < 3; ++k) {
+      p.v[k] = v[3 * (size_t)i + k];
+      p.x[k] = xl[3 * (size_t)i + k];
+      if (pf) p.pf[k] = pf[3 * (size_t)i + k];
+    }
+    o->cells[cell[i]].parts.push_back(p);
+  }
+  return 0;
+}
+
 int oracle_set_bonds(void *h, int M, const int *bi, const int *bj, const int *pot, const double *dissoc) {
   Oracle *o = (Oracle *)h;
   o->bonds.clear();

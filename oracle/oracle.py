@@ -37,6 +37,9 @@ _SIG = {
                                       C.POINTER(C.c_float)]),
     "oracle_load": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_int), C.POINTER(C.c_uint16)]),
+    "oracle_load_local": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_uint16),
+                                    C.POINTER(C.c_float)]),
     "oracle_set_bonds": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "oracle_set_boundaries": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float),
@@ -189,6 +192,22 @@ class Oracle:
         if rc:
             raise ValueError("oracle load failed %d" % rc)
         self.N = x.shape[0]
+
+    def load_local(self, xl, cell, v, types, flags=None, pf=None):
+        """Loads a cell-local state (as the engine reports it) without going
+        through global coordinates: particle i in cell[i] at xl[i]."""
+        xl = np.ascontiguousarray(xl, np.float32).reshape(-1, 3)
+        c = np.ascontiguousarray(cell, np.int32)
+        v = np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(types, np.int32)
+        fl = None if flags is None else np.ascontiguousarray(flags, np.uint16)
+        p = None if pf is None else np.ascontiguousarray(pf, np.float32).reshape(-1, 3)
+        rc = lib().oracle_load_local(self._h, xl.shape[0], _fp(xl), _ip(c), _fp(v), _ip(t),
+                                     fl.ctypes.data_as(C.POINTER(C.c_uint16)) if fl is not None else None,
+                                     _fp(p) if p is not None else None)
+        if rc:
+            raise ValueError("oracle local load failed %d" % rc)
+        self.N = xl.shape[0]
 
     def set_bonds(self, i, j, pot, dissociation):
         inf, alpha, c = _pot_tables(pot)

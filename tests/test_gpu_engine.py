@@ -159,17 +159,19 @@ def test_brick_layout_bitwise(m, brick, name):
 
 
 def test_ensemble_matches_independent(m):
-    """Three DPD replicas of different sizes in one engine, against three
-    engines of one replica each (same brick shape)."""
+    """Three DPD replicas of different sizes in one engine, with the counter
+    noise on, against three engines of one replica each (same brick shape):
+    a replica's random streams are keyed by its own seed and its particles'
+    ranks in the replica, so replica i equals a standalone engine seeded with
+    replica i's seed."""
     sizes = [1500, 2600, 4100]
-    # DPD without the random term: no random stream is involved (those are
-    # keyed by the global particle id, which differs between the two runs).
-    ws = [W.dpd_fluid(N=n, sigma=0.0, seed=50 + i) for i, n in enumerate(sizes)]
+    ws = [W.dpd_fluid(N=n, sigma=3.0, seed=50 + i) for i, n in enumerate(sizes)]
     print("case: ensemble", sizes, flush=True)
     brick = (2, 2, 2, 0, 0)
+    rep_seeds = [8 + r * 0x9E3779B97F4A7C15 & 0xFFFFFFFFFFFFFFFF for r in range(len(ws))]
     singles = []
-    for w in ws:
-        s, o, _ = run_steps(m, w, 8, 6, brick=brick)
+    for w, sd in zip(ws, rep_seeds):
+        s, o, _ = run_steps(m, w, sd, 6, brick=brick)
         singles.append((s, o))
     w0 = ws[0]
     u = m.Universe(cutoff=w0["cutoff"], dt=w0["dt"], replicas=[(w["dim"], w["cells"]) for w in ws])
@@ -181,6 +183,8 @@ def test_ensemble_matches_independent(m):
         u.bind(p, spec[2], spec[3])
         pots.append(p)
     u.set_seed(8)
+    u.set_dpd_noise(1)
+    assert [u.replica_seed(r) for r in range(len(ws))] == rep_seeds
     x = np.concatenate([w["x"] for w in ws])
     v = np.concatenate([w["v"] for w in ws])
     t = np.concatenate([w["type"] for w in ws])

@@ -1,0 +1,116 @@
+"""The batched ensemble (many replica universes in one engine) against the
+oracle, on the cell-sorting workload of the ensemble benchmark: two types,
+Morse adhesion (AA, BB d = 3; AB d = 1; a = 6, r0 = 1, cutoff 2.5),
+overdamped, with the persistent random force as the Langevin term
+(kT = 0.1), particle counts log-uniform in [500, 20,000].
+
+Each replica draws its random force from its own seed, keyed by its
+particles' ranks in the replica, so replica r is checked against a standalone
+oracle loaded with the same particles and seeded with replica r's seed:
+  * 16 replicas: step-1 forces within 1e-5 of each replica's FP32 oracle
+    (random force included; the persistent random forces bit for bit); then
+    a relay of five steps, each replica's oracle advancing with the GPU's
+    forces, with positions and cells bit for bit;
+  * the same 16 replicas stepped from captured HIP graphs give the eager
+    states bit for bit;
+  * 256 replicas for 100 steps: no error, finite energies per replica, every
+    particle inside its replica's box.
+"""
+import numpy as np
+import pytest
+
+from mechanica_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-5
+
+
+def replica_workload(e, r):
+    rep = e["replicas"][r]
+    w = dict(rep)
+    for k in ("cutoff", "dt", "types", "pots", "random"):
+        w[k] = e[k]
+    w["name"] = "cell_sort"
+    w["v"] = np.zeros_like(rep["x"])
+    w["bonds"] = None
+    return w
+
+
+def split(e, arr):
+    out, base = [], 0
+    for rep in e["replicas"]:
+        n = len(rep["x"])
+        out.append(arr[base:base + n])
+        base += n
+    return out
+
+
+def test_ensemble_replicas_match_oracles(m, oracle_mod):
+    e = W.ensemble(R=16, seed=4)
+    sizes = [len(r["x"]) for r in e["replicas"]]
+    print("case: ensemble of 16 replicas, sizes", sizes, flush=True)
+    assert min(sizes) < 2000 and max(sizes) > 8000
+    u = W.build_ensemble(m, e, seed=12)
+    seeds = [u.replica_seed(r) for r in range(len(sizes))]
+    assert len(set(seeds)) == len(seeds)
+    cbase = np.cumsum([0] + [int(np.prod(r["cells"])) for r in e["replicas"]])
+    oracles = []
+    for r in range(len(sizes)):
+        oracles.append(oracle_mod.from_workload(replica_workload(e, r), m, seed=seeds[r]))
+    u.step(1)
+    g = u.state()
+    fs, xs, cs, vs, pfs = (split(e, g[k]) for k in ("f", "x", "cell", "v", "pf"))
+    worst = 0.0
+    for r, o in enumerate(oracles):
+        fresh = oracle_mod.from_workload(replica_workload(e, r), m, seed=seeds[r])
+        fresh.force()
+        ref = fresh.state()["f"].astype(np.float64)
+        n = np.linalg.norm(ref, axis=1)
+        floor = np.maximum(n, 0.1 * np.sqrt((n ** 2).mean()))
+        rel = (np.linalg.norm(fs[r].astype(np.float64) - ref, axis=1) / floor).max()
+        worst = max(worst, rel)
+        assert rel <= FORCE_TOL, (r, rel)
+        assert np.array_equal(pfs[r].view(np.uint32), fresh.state()["pf"].view(np.uint32)), r
+    print("step-1 worst max rel force diff over replicas %.3e" % worst)
+    for k in range(5):
+        if k > 0:
+            u.step(1)
+            g = u.state()
+            fs, xs, cs, vs, pfs = (split(e, g[q]) for q in ("f", "x", "cell", "v", "pf"))
+        for r, o in enumerate(oracles):
+            o.step_with_forces(fs[r])
+            s = o.state()
+            assert np.array_equal(cs[r] - cbase[r], s["cell"]), (k, r, "cell")
+            assert np.array_equal(xs[r].view(np.uint32), s["x"].view(np.uint32)), (k, r, "x")
+
+
+def test_ensemble_graphs_match_eager(m):
+    e = W.ensemble(R=16, seed=4)
+    print("case: ensemble graphs vs eager", flush=True)
+    out = []
+    for graphs in (False, True):
+        u = W.build_ensemble(m, e, seed=12)
+        u.set_graphs(graphs)
+        for _ in range(3):
+            u.step(4)
+            u.state()  # eager work and a synchronisation between replays
+        out.append(u.state())
+    for k in ("x", "v", "f", "cell", "pf"):
+        assert np.array_equal(out[0][k].view(np.uint32), out[1][k].view(np.uint32)), k
+
+
+def test_ensemble_256_replicas_100_steps(m):
+    e = W.ensemble(R=256, seed=4)
+    sizes = np.array([len(r["x"]) for r in e["replicas"]])
+    print("case: ensemble of 256 replicas, %d particles, 100 steps" % sizes.sum(), flush=True)
+    u = W.build_ensemble(m, e, seed=13)
+    u.step(100)
+    assert u.error == 0 and u.time == 100
+    for r in range(len(sizes)):
+        ob = u.observables(r)
+        assert np.isfinite(ob["epot_nonbond"]) and np.isfinite(ob["ekin"]), r
+    x, _ = u.unload()
+    for r, xr in enumerate(split(e, x)):
+        L = e["replicas"][r]["dim"][0]
+        assert np.isfinite(xr).all() and (xr >= 0).all() and (xr < L).all(), r
