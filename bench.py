@@ -18,10 +18,15 @@ replicas in one engine, compared with one universe of the same total size)
 and adjoint (65,536-particle Morse trajectory of 50 steps, forward and
 backward, compared with the plain step on the same system).
 
-Printed: one JSON line with the throughput, the roofline of the dominant
-kernel (timed with HIP events on the stream it runs on) and the CPU baseline
-(this repository's restatement of the reference's CPU step, multi-threaded,
-timed on a bounded sample of the same workload on the host cores).
+Printed: one JSON line with the throughput (the median of five timed runs
+of --steps steps each), the roofline of the dominant kernel (timed with HIP
+events on the stream it runs on: the HBM fraction, and beside it the FP32
+rate of the algorithmic flops and the VALU issue fraction from the measured
+instruction count) and the CPU baseline (this repository's restatement of
+the reference's CPU step, multi-threaded on every core the job may use,
+timed on a bounded sample of the same workload).  The default run also
+measures 1M-particle argon LJ, the other half of the metric, and reports it
+under "lj" with its own roofline and CPU baseline.
 """
 import argparse
 import json
@@ -33,6 +38,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+SIMDS = 1024          # 256 CUs x 4 SIMDs
+CLOCK_GHZ = 2.4       # peak engine clock
+# Algorithmic FP32 flops (add, mul, sqrt, div count 1) per unique in-cutoff
+# pair and per particle advance, from the reference's operation sequence:
+# table pair 45, DPD pair 39, Newtonian advance 21, overdamped 15.
+FLOPS_TABLE_PAIR = 45
+FLOPS_DPD_PAIR = 39
+FLOPS_ADVANCE = {"newtonian": 21, "overdamped": 15}
 # Algorithmic bytes per particle-step: read x, cell, type (20 B) and v
 # (12 B) for Newtonian or DPD particles, write and read f (24 B), write x
 # and cell (16 B) and v (12 B): 84 B; overdamped particles have no v: 60 B.
@@ -52,6 +66,10 @@ BYTES_TERM_OFFSET = 4
 # the same counters); the summary and its commands are kept in
 # profiles/k_brick_dpd1m_counters.md (bytes per launch).
 MEASURED_TRAFFIC = {("dpd", 1048576): 111.9e6}
+# Wave64 VALU instructions per force-kernel launch (rocprofv3 --pmc
+# SQ_INSTS_VALU on the same workload and build; profiles/*_valu.md).  A
+# wave64 FP32 instruction holds a 16-lane SIMD for 4 cycles.
+MEASURED_VALU = {("dpd", 1048576): 202.1e6}
 
 METRIC = "particle-steps/s (1M LJ/DPD)"
 
@@ -60,14 +78,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def timed_steps(u, steps, warmup, graphs=True):
+def timed_steps(u, steps, warmup, graphs=True, runs=5):
+    """Wall time of `steps` steps: the median of `runs` timed runs (each
+    bracketed by a synchronisation), and the list of all of them."""
     u.set_graphs(graphs)
     u.step(warmup)
     u.sync()
-    t0 = time.perf_counter()
-    u.step(steps)
-    u.sync()
-    return time.perf_counter() - t0
+    walls = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        u.step(steps)
+        u.sync()
+        walls.append(time.perf_counter() - t0)
+    return sorted(walls)[len(walls) // 2], walls
 
 
 def kernel_ms(u, steps):
@@ -80,11 +103,33 @@ def kernel_ms(u, steps):
     return f, s
 
 
-def roofline(bytes_alg, ms, kernel, traffic=None):
+def roofline(bytes_alg, ms, kernel, traffic=None, flops=None, valu=None):
+    """HBM roofline of the kernel; flops (algorithmic FP32 per launch) and
+    valu (measured VALU instructions per launch) add the FP32 rate against
+    the vector peak and the VALU issue fraction."""
     achieved = bytes_alg / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel, "kernel_ms": ms,
-            "bytes_per_launch": bytes_alg}
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel, "kernel_ms": ms,
+           "bytes_per_launch": bytes_alg}
+    if flops is not None:
+        tf = flops / (ms * 1e-3) / 1e12
+        out["fp32"] = {"flops_per_launch": flops, "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": tf / FP32_PEAK_TFLOPS}
+    if valu is not None:
+        busy = valu * 4 / (SIMDS * CLOCK_GHZ * 1e9 * ms * 1e-3)
+        out["valu"] = {"insts_per_launch": valu, "issue_frac": busy,
+                       "note": "wave64 VALU instructions x 4 cycles / (1024 SIMDs x 2.4 GHz x kernel time); "
+                               "instruction count measured with rocprofv3 on this workload"}
+    return out
+
+
+def host_threads():
+    """Threads for the CPU baseline: every core the job may use.  The GPU
+    hosts grant a job a share of the machine through OMP_NUM_THREADS; the
+    affinity mask can show more cores than that share."""
+    avail = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS")
+    return min(avail, int(share)) if share and share.isdigit() and int(share) > 0 else avail, avail
 
 
 def cpu_baseline_oracle(w, seconds, threads, dpd_noise=False, seed=1234):
@@ -106,15 +151,18 @@ def cpu_baseline_oracle(w, seconds, threads, dpd_noise=False, seed=1234):
     dt = time.perf_counter() - t0
     total_steps, total_t = steps + 1, dt + one
     return {"value": N * total_steps / total_t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": "%s N=%d, %d timed steps after 1 warm-up step, %d threads, %.1f s" % (
-                w["name"], N, total_steps, threads, total_t)}
+            "sample": "%s N=%d, %d timed steps after 1 warm-up step, %d threads (of %d cores in the affinity "
+                      "mask), %.1f s" % (w["name"], N, total_steps, threads, len(os.sched_getaffinity(0)),
+                                         total_t)}
 
 
-def line(args, value, wall, dtype, config, roof, cpu, metric=METRIC, **extra):
+def line(args, value, wall, dtype, config, roof, cpu, metric=METRIC, walls=None, **extra):
     out = {"metric": metric, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config,
            "roofline": roof, "cpu_baseline": cpu}
+    if walls:
+        out["runs_ms_per_step"] = [1e3 * t / args.steps for t in walls]
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]
     out.update(extra)
@@ -123,19 +171,20 @@ def line(args, value, wall, dtype, config, roof, cpu, metric=METRIC, **extra):
 
 # ------------------------------------------------------------------ workloads
 
-def bench_single(args, threads):
+def bench_single(args, threads, workload=None):
     import mechanica_amd as m
     from mechanica_amd import workloads as W
-    if args.workload == "dpd":
+    workload = workload or args.workload
+    if workload == "dpd":
         w = W.dpd_fluid(N=args.n, sigma=3.0, seed=2)
         w["dpd_noise"] = True
         cfg = "dpd_fluid N=%d density=3 alpha=25 gamma=4.5 sigma=3 (counter RNG) cutoff=1 dt=0.01 cells=%dx%dx%d"
         per, bonds = BYTES_NEWTONIAN, 0
-    elif args.workload == "lj":
+    elif workload == "lj":
         w = W.argon_lj(N=args.n, seed=1)
         cfg = "argon_lj N=%d density=10 LJ 12-6 on [0.275,1] tol=1e-3 cutoff=1 dt=0.01 cells=%dx%dx%d"
         per, bonds = BYTES_NEWTONIAN, 0
-    elif args.workload == "molecules":
+    elif workload == "molecules":
         side = max(2, int(round((args.n // 4) ** (1.0 / 3.0))))
         w = W.molecules(n=side, seed=8)
         cfg = ("molecules N=%d: four-bead chains on a lattice of spacing 1.6, per molecule 3 harmonic bonds "
@@ -144,13 +193,13 @@ def bench_single(args, threads):
         per, bonds = BYTES_NEWTONIAN + BYTES_TERM_OFFSET, len(w["bonds"]["i"])
         per_terms = BYTES_PER_TERM_ENTRY * (3 * len(w["angles"]["i"]) + 4 * len(w["dihedrals"]["i"])
                                             + 2 * len(w["exclusions"]["i"]))
-    elif args.workload == "channel":
+    elif workload == "channel":
         w = W.channel(N=args.n, seed=9)
         w["dpd_noise"] = True
         cfg = ("channel N=%d (examples/dpd1.py at scale): DPD alpha=1 gamma=1 sigma=1.5 (counter RNG) cutoff=1, "
                "mass 10, no-slip walls in z, constant force (0.01,0,0), dt=0.1 cells=%dx%dx%d")
         per, bonds = BYTES_NEWTONIAN, 0
-    elif args.workload == "species":
+    elif workload == "species":
         w = W.species_fluid(N=args.n, seed=10)
         w["dpd_noise"] = True
         cfg = ("species N=%d: DPD fluid at density 3 (sigma=3, counter RNG) in two types with three species, "
@@ -173,20 +222,28 @@ def bench_single(args, threads):
     elif not args.no_autotune:
         u.step(args.warmup)  # tune on a settled state, not the initial one
         tune = u.autotune(steps_per=3)
-    wall = timed_steps(u, args.steps, args.warmup)
+    wall, walls = timed_steps(u, args.steps, args.warmup)
     value = N * args.steps / wall
-    log("timed %d steps in %.3f s" % (args.steps, wall))
+    log("timed 5 x %d steps, median %.3f s" % (args.steps, wall))
     force_ms, step_ms = kernel_ms(u, args.steps)
-    bytes_alg = per * N + BYTES_PER_BOND * bonds + (per_terms if args.workload == "molecules" else 0)
+    bytes_alg = per * N + BYTES_PER_BOND * bonds + (per_terms if workload == "molecules" else 0)
+    # Algorithmic FP32 flops: unique in-cutoff pairs of the current
+    # configuration (half the neighbour-count sum) and the advance.
+    pairs = int(u.neighbour_counts().astype("int64").sum()) // 2
+    dyn = "overdamped" if w["types"][0]["dynamics"] == 1 else "newtonian"
+    pair_flops = FLOPS_DPD_PAIR if w["pots"][0][0] == "dpd" else FLOPS_TABLE_PAIR
+    flops = pairs * pair_flops + N * FLOPS_ADVANCE[dyn] + bonds * (FLOPS_TABLE_PAIR + 3)
     roof = roofline(bytes_alg, force_ms, "k_brick (LDS brick force + advance) with its global-memory fallback",
-                    MEASURED_TRAFFIC.get((args.workload, args.n)))
+                    MEASURED_TRAFFIC.get((workload, args.n)), flops=flops,
+                    valu=MEASURED_VALU.get((workload, args.n)))
     roof["step_ms_events"] = step_ms
+    roof["unique_pairs"] = pairs
     cpu = None if args.no_cpu_baseline else cpu_baseline_oracle(w, args.cpu_seconds, threads,
                                                                 dpd_noise=bool(w.get("dpd_noise")))
     config = {"workload": cfg % ((N,) + tuple(w["cells"])), "particles": N, "cells": w["cells"],
               "brick": u.brick(), "waves_per_simd": u.waves()}
     extra = {"autotune": tune} if tune else {}
-    return line(args, value, wall, "f32", config, roof, cpu, **extra)
+    return line(args, value, wall, "f32", config, roof, cpu, walls=walls, **extra)
 
 
 def bench_ensemble(args, threads):
@@ -198,7 +255,7 @@ def bench_ensemble(args, threads):
     u = W.build_ensemble(m, e, seed=1234)
     log("ensemble of %d replicas, %d particles built" % (len(sizes), N))
     tune = None if args.no_autotune else u.autotune(steps_per=2)
-    wall = timed_steps(u, args.steps, args.warmup)
+    wall, walls = timed_steps(u, args.steps, args.warmup)
     value = N * args.steps / wall
     force_ms, step_ms = kernel_ms(u, args.steps)
     ebrick = u.brick()
@@ -207,7 +264,7 @@ def bench_ensemble(args, threads):
     w = W.cell_sort(N=N, seed=5)
     p = W.build_universe(m, w, seed=1234)
     ptune = None if args.no_autotune else p.autotune(steps_per=2)
-    pwall = timed_steps(p, args.steps, args.warmup)
+    pwall, _ = timed_steps(p, args.steps, args.warmup)
     pforce, _ = kernel_ms(p, args.steps)
     plain = {"value": N * args.steps / pwall, "workload": "cell_sort N=%d cells=%dx%dx%d (one universe)" % (
         (N,) + tuple(w["cells"])), "force_kernel_ms": pforce, "brick": p.brick()}
@@ -240,7 +297,7 @@ def bench_ensemble(args, threads):
                           "dt=0.01, hipGraph-captured steps" % (len(sizes), N), "particles": N,
               "replicas": len(sizes)}
     config["brick"] = ebrick
-    return line(args, value, wall, "f32", config, roof, cpu,
+    return line(args, value, wall, "f32", config, roof, cpu, walls=walls,
                 metric="batched particle-steps/s vs plain kernel", plain_kernel=plain,
                 batched_over_plain=value / plain["value"], autotune=tune and tune["brick"])
 
@@ -295,7 +352,7 @@ def bench_adjoint(args, threads):
     u.load(w["x"], None, w["type"])
     if not args.no_autotune:
         u.autotune(steps_per=2)
-    pwall = timed_steps(u, args.steps, args.warmup)
+    pwall, _ = timed_steps(u, args.steps, args.warmup)
     plain = {"value": N * args.steps / pwall, "workload": "same system through the engine step (tabulated "
              "Morse tol 1e-3, overdamped)", "brick": u.brick()}
     step_ms = 1e3 * wall / (reps * T)
@@ -349,6 +406,7 @@ def main():
                     help="keep the default brick layout instead of timing the candidates before warm-up")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-lj", action="store_true", help="default workload only: skip the 1M LJ line")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
 
@@ -359,9 +417,15 @@ def main():
     if args.copy_calibration:
         copy_calibration(args)
         return
-    threads = min(16, os.cpu_count() or 1)
+    threads, _ = host_threads()
     if args.workload in ("dpd", "lj", "lattice", "molecules", "channel", "species"):
         out = bench_single(args, threads)
+        if args.workload == "dpd" and not args.no_lj:
+            # The metric's other half: 1M argon LJ with its own roofline and
+            # CPU baseline.
+            lj = bench_single(args, threads, workload="lj")
+            out["lj"] = {k: lj[k] for k in ("value", "unit", "ms_per_step", "runs_ms_per_step", "config",
+                                            "roofline", "cpu_baseline", "speedup_vs_cpu") if k in lj}
     elif args.workload == "ensemble":
         out = bench_ensemble(args, threads)
     else:

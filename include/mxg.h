@@ -347,12 +347,13 @@ int mxg_engine_set_timing(mxg_engine *e, int on);
 int mxg_engine_timing(mxg_engine *e, double *force_ms, double *step_ms);
 
 /* ------------------------------------------------------------------------
- * Adjoint: a differentiable overdamped Morse system and the discrete adjoint
- * of its trajectory.  No reference counterpart: the forward map restates
- * engine_step's overdamped update (engine_advance.cpp:126-239) with the
- * Morse pair term of potential_eval_ex (potential_eval.hpp:106-161, shift
- * rule of potential_create_morse, MxPotential.cpp) evaluated analytically,
- * so that its parameter derivatives are exact.  Buffers marked (device) are
+ * Adjoint: a differentiable overdamped system (Morse or LJ 12-6 pairs,
+ * harmonic bonds) and the discrete adjoint of its trajectory.  No reference
+ * counterpart: the forward map restates engine_step's overdamped update
+ * (engine_advance.cpp:126-239) with the pair term of potential_eval_ex
+ * (potential_eval.hpp:106-161) and the bond term of bond_eval
+ * (bond.cpp:99-295) evaluated analytically, so that their parameter
+ * derivatives are exact.  Buffers marked (device) are
  * device pointers; work runs on `stream` (a hipStream_t, 0 = default).
  * ------------------------------------------------------------------------ */
 typedef struct mxg_adjoint mxg_adjoint;
@@ -367,9 +368,27 @@ int mxg_adjoint_create(mxg_adjoint **out, int N, const double dim[3], const int 
 void mxg_adjoint_free(mxg_adjoint *a);
 const char *mxg_adjoint_error_message(const mxg_adjoint *a);
 
-/* theta (host): ntypes*(ntypes+1)/2 rows of (d, a, r0), one per type pair
- * ti <= tj in row-major upper-triangle order; [smin, smax] is the Morse
- * domain (potential_create_morse's min and max).
+/* Potential of a type pair (default Morse): 0 Morse, d (e^{-2a(s-1)} -
+ * 2 e^{-a(s-1)}) with s = max(r, smin) - (R_i + R_j) + r0 + 1
+ * (potential_create_morse, MxPotential.cpp:3395-3424); 1 Lennard-Jones 12-6,
+ * A s^-12 - B s^-6 with s = max(r, smin) (potential_create_LJ126,
+ * MxPotential.cpp:976-991); on its own domain [smin, smax] (the creator's
+ * min and max) when smax > smin, else on the domain the forward and
+ * backward calls pass. */
+int mxg_adjoint_set_pair_kind(mxg_adjoint *a, int ti, int tj, int kind, double smin, double smax);
+
+/* Harmonic bonds (potential_create_harmonic, MxPotential.cpp:348-366, as
+ * bond_eval evaluates it, bond.cpp:99-295: r clamped to [rmin, rmax],
+ * E = K (r - r0)^2, no dissociation) between particle ids i[b], j[b];
+ * replaces the previous set (M = 0 removes them).  With bonds, theta and
+ * dL/dtheta carry one more row (K, r0, unused) after the type pairs. */
+int mxg_adjoint_set_bonds(mxg_adjoint *a, int M, const int32_t *i, const int32_t *j, double rmin, double rmax);
+
+/* theta (host): ntypes*(ntypes+1)/2 rows, one per type pair ti <= tj in
+ * row-major upper-triangle order, of (d, a, r0) for Morse pairs or (A, B,
+ * unused) for LJ pairs, then the bond row (K, r0, unused) when bonds are
+ * set; [smin, smax] is the pair potentials' domain (the creators' min and
+ * max).
  * Forward: nsteps steps from x0 (device, N*3 FP64, rounded to FP32) to xT
  * (device, N*3 FP64); ckpt (device, ceil(nsteps/every)*N*4 FP32) receives
  * the state every `every` steps. */
@@ -378,7 +397,7 @@ int mxg_adjoint_forward(mxg_adjoint *a, const double *theta, double smin, double
 
 /* Backward: from dL/dxT (device, N*3) and the forward's checkpoints, the
  * segments are recomputed and swept in reverse; writes dL/dx0 (device,
- * N*3) and dL/dtheta (host, 3 per type pair), accumulated in FP64.
+ * N*3) and dL/dtheta (host, 3 per theta row), accumulated in FP64.
  * Synchronises the stream. */
 int mxg_adjoint_backward(mxg_adjoint *a, const double *theta, double smin, double smax, const float *ckpt, int nsteps,
                          int every, const double *gxT, double *gx0, double *gtheta, void *stream);

@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmxg.so")
+# MXG_LIBRARY names another build of the same library (for timing a
+# variant side by side); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("MXG_LIBRARY") or os.path.join(_HERE, "libmxg.so")
 
 c_double_p = C.POINTER(C.c_double)
 c_float_p = C.POINTER(C.c_float)
@@ -110,6 +112,8 @@ SIGNATURES = {
                                       C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "mxg_adjoint_backward": (C.c_int, [C.c_void_p, c_double_p, C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_int,
                                        C.c_void_p, C.c_void_p, c_double_p, C.c_void_p]),
+    "mxg_adjoint_set_pair_kind": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
+    "mxg_adjoint_set_bonds": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, C.c_double, C.c_double]),
     "mxg_adjoint_forces": (C.c_int, [C.c_void_p, c_double_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                      c_double_p, C.c_void_p]),
     "mxg_adjoint_rdf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]),

@@ -1,4 +1,4 @@
-"""Differentiable overdamped Morse trajectories on the GPU.
+"""Differentiable overdamped trajectories on the GPU.
 
 ``MorseSystem`` wraps the C ABI's adjoint object (include/mxg.h,
 ``mxg_adjoint_*``); ``trajectory`` and ``soft_rdf`` are torch.autograd
@@ -6,17 +6,21 @@ Functions whose backward passes are the hand-written HIP adjoint kernels:
 
 * ``trajectory(system, x0, theta, nsteps, every)`` runs ``nsteps``
   overdamped steps (engine_step's overdamped update,
-  src/mdcore/src/engine_advance.cpp:126-239, with the Morse pair force of
-  potential_eval_ex, src/mdcore/src/potential_eval.hpp:106-161, in closed
-  form) and keeps the state every ``every`` steps; its backward recomputes
-  each segment from its checkpoint and sweeps it in reverse, returning
-  dL/dx0 and dL/dtheta accumulated in FP64.
+  src/mdcore/src/engine_advance.cpp:126-239, with the pair force of
+  potential_eval_ex, src/mdcore/src/potential_eval.hpp:106-161, and the bond
+  force of bond_eval, src/mdcore/src/bond.cpp:99-295, in closed form) and
+  keeps the state every ``every`` steps; its backward recomputes each
+  segment from its checkpoint and sweeps it in reverse, returning dL/dx0 and
+  dL/dtheta accumulated in FP64.
 * ``soft_rdf(system, x, nbins, rmax)`` is the per-type-pair pair-distance
   histogram with biweight bins, and ``rdf_normalise`` turns it into g(r).
 
-theta holds one (d, a, r0) row per type pair (ti <= tj, row-major upper
-triangle), the arguments of potential_create_morse (MxPotential.cpp).
-torch is used for device memory and the current stream only.
+theta holds one row per type pair (ti <= tj, row-major upper triangle): the
+arguments of potential_create_morse (d, a, r0) for a Morse pair or of
+potential_create_LJ126 (A, B, unused) for a Lennard-Jones pair; with bonds,
+one more row holds the harmonic bond's (K, r0, unused)
+(potential_create_harmonic).  torch is used for device memory and the
+current stream only.
 """
 import ctypes as C
 import math
@@ -39,11 +43,17 @@ def _stream(device):
     return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+MORSE, LJ = 0, 1
+
+
 class MorseSystem:
-    """N particles of up to four types in a periodic box, overdamped Morse."""
+    """N particles of up to four types in a periodic box, overdamped, with
+    Morse or LJ 12-6 pairs (kinds: {(ti, tj): MORSE or LJ, or (kind, smin,
+    smax) with a domain of its own}; default Morse on [smin, smax]) and
+    optional harmonic bonds (bonds: (i, j, rmin, rmax))."""
 
     def __init__(self, types, dim, cutoff, dt, mass=(1.0,), radius=(1.0,), cells=None, max_dist_fraction=0.05,
-                 smin=1e-4, smax=None, device="cuda"):
+                 smin=1e-4, smax=None, device="cuda", kinds=None, bonds=None):
         L = _capi.lib()
         types = np.ascontiguousarray(types, dtype=np.int32)
         self.N = int(types.shape[0])
@@ -68,6 +78,19 @@ class MorseSystem:
                                   r.ctypes.data_as(_capi.c_double_p), types.ctypes.data_as(_capi.c_int_p))
         self._h = h
         self._check(rc)
+        self.nbonds = 0
+        for (a, b), spec in (kinds or {}).items():
+            # spec: kind, or (kind, smin, smax) for a domain of its own
+            k, lo, hi = (spec, 0.0, 0.0) if np.isscalar(spec) else spec
+            self._check(L.mxg_adjoint_set_pair_kind(self._h, int(a), int(b), int(k), float(lo), float(hi)))
+        if bonds is not None:
+            bi = np.ascontiguousarray(bonds[0], np.int32)
+            bj = np.ascontiguousarray(bonds[1], np.int32)
+            self._check(L.mxg_adjoint_set_bonds(self._h, len(bi), bi.ctypes.data_as(_capi.c_int_p),
+                                                bj.ctypes.data_as(_capi.c_int_p), float(bonds[2]),
+                                                float(bonds[3])))
+            self.nbonds = len(bi)
+        self.nrows = self.npairs + (1 if self.nbonds else 0)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -85,8 +108,8 @@ class MorseSystem:
 
     def _theta(self, theta):
         th = np.ascontiguousarray(torch.as_tensor(theta).detach().to("cpu", torch.float64).numpy()).reshape(-1)
-        if th.size != 3 * self.npairs:
-            raise ValueError("theta needs %d rows of (d, a, r0)" % self.npairs)
+        if th.size != 3 * self.nrows:
+            raise ValueError("theta needs %d rows of 3 (type pairs, then the bond row)" % self.nrows)
         return th
 
     def _x(self, x):
@@ -113,14 +136,14 @@ class MorseSystem:
         th = self._theta(theta)
         gxT = self._x(gxT)
         gx0 = torch.empty((self.N, 3), dtype=torch.float64, device=self.device)
-        gth = np.zeros(3 * self.npairs, np.float64)
+        gth = np.zeros(3 * self.nrows, np.float64)
         self._check(_capi.lib().mxg_adjoint_backward(
             self._h, th.ctypes.data_as(_capi.c_double_p), self.smin, self.smax, _ptr(ckpt), int(nsteps), int(every),
             _ptr(gxT), _ptr(gx0), gth.ctypes.data_as(_capi.c_double_p), _stream(self.device)))
-        return gx0, torch.from_numpy(gth.reshape(self.npairs, 3))
+        return gx0, torch.from_numpy(gth.reshape(self.nrows, 3))
 
     def forces(self, x, theta):
-        """Forces (N, 3) FP32 and the total Morse energy (FP64 sum)."""
+        """Forces (N, 3) FP32 and the total pair and bond energy (FP64 sum)."""
         th = self._theta(theta)
         x = self._x(x)
         f = torch.empty((self.N, 3), dtype=torch.float32, device=self.device)
@@ -191,7 +214,7 @@ class _SoftRDF(torch.autograd.Function):
 
 def trajectory(system, x0, theta, nsteps, every=10):
     """Final positions (N, 3) FP64 after nsteps steps; differentiable in x0
-    and theta (npairs, 3)."""
+    and theta (nrows, 3): the type pairs, then the bond row."""
     return _Trajectory.apply(x0, theta, system, int(nsteps), int(every))
 
 

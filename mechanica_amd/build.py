@@ -35,18 +35,20 @@ def hipcc():
     return os.path.join(rocm, "bin", "hipcc")
 
 
-def build(verbose=False):
-    """Compiles libmxg.so; raises RuntimeError with the compiler output."""
+def build(verbose=False, out=None):
+    """Compiles libmxg.so (or the library at out); raises RuntimeError with
+    the compiler output."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
+    out = out or LIB
+    tmp = out + ".tmp"
     cmd = [hipcc()] + FLAGS + ["-I", CSRC] + srcs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def build_driver(src, exe):

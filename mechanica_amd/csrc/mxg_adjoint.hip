@@ -2,21 +2,28 @@
 // checkpoints, and the discrete adjoint of the same map.
 //
 // Forward map, one step (engine_step's overdamped path,
-// src/mdcore/src/engine_advance.cpp:126-239, with the Morse pair term of
-// src/mdcore/src/potential_eval.hpp:106-161 evaluated analytically instead
-// of from its table, so that it has exact parameter derivatives):
-//   F_i   = sum_j phi(r_ij) d_ij,  d_ij = x_i - x_j (minimum image),
-//           phi = -E'(s) / max(r, FLT_EPSILON),
-//           s = max(r, smin) - (R_i + R_j) + r0 + 1,
-//           E(s) = D (u^2 - 2u),  u = exp(-A (s - 1)),
-//           pairs with r^2 <= rc^2 and s <= smax only;
+// src/mdcore/src/engine_advance.cpp:126-239, with the pair terms of
+// src/mdcore/src/potential_eval.hpp:106-161 and the bond term of bond_eval,
+// src/mdcore/src/bond.cpp:99-295, evaluated analytically instead of from
+// their tables, so that they have exact parameter derivatives):
+//   F_i   = sum_j phi(r_ij) d_ij + sum_bonds phi_b(r_ib) d_ib,
+//           d_ij = x_i - x_j (minimum image),
+//           phi = -E'(s) / max(r, FLT_EPSILON), pairs with r^2 <= rc^2 and
+//           s <= smax only, per type pair either
+//             Morse (potential_create_morse): s = max(r, smin) - (R_i + R_j)
+//             + r0 + 1, E(s) = D (u^2 - 2u), u = exp(-A (s - 1));
+//             LJ 12-6 (potential_create_LJ126): s = max(r, smin),
+//             E(s) = A s^-12 - B s^-6;
+//           bonds (potential_create_harmonic): rb = clamp(r, bmin, bmax),
+//           phi_b = -E'(rb) / rb, E(rb) = K (rb - r0)^2;
 //   dx_i  = clamp((dt F_i) / m_i, +-maxx) per component;
 //   x_i  <- x_i + dx_i, wrapped into the box.
 // Adjoint, lambda_t = dL/dx_t:
 //   mu_i      = (dt / m_i) [component not clamped] lambda_{t+1,i}
 //   lambda_t,i = lambda_{t+1,i} + sum_j H_ij (mu_i - mu_j),
 //   H_ij      = phi I + (phi'(r) / r) d d^T,
-//   dL/dtheta += sum_{pairs} (dphi/dtheta) (mu_i - mu_j) . d_ij.
+//   dL/dtheta += sum_{pairs, bonds} (dphi/dtheta) (mu_i - mu_j) . d_ij,
+// with the bonds' H and parameter terms from phi_b.
 // Every neighbour sum is a gather in a fixed order (cells in offset order,
 // particles within a cell by id), accumulated in FP64 where the adjoint
 // needs it; the forward recomputed from a checkpoint reproduces the first
@@ -38,9 +45,20 @@ constexpr int kAB = 256;  // threads per block
 static_assert(kAdjMaxTypes == 4, "k_backward accumulates four types by name");
 constexpr double kFix = 1099511627776.0;  // 2^40: fixed-point scale of the histogram
 
+enum : int { kPairMorse = 0, kPairLJ = 1 };
+
+// One type pair's potential: Morse (d, a, r0 + 1, R_i + R_j) or LJ 12-6
+// (A, B in d, a).
 struct AdjPair {
-  float d, a, r0p1, R;  // depth, width, r0 + 1, R_i + R_j
+  int kind;
+  float d, a, r0p1, R;
   float smin, smax;
+};
+
+// The harmonic bond potential shared by all bonds: K (r - r0)^2 with r
+// clamped to [bmin, bmax].
+struct AdjBond {
+  float K, r0, bmin, bmax;
 };
 
 struct AdjArgs {
@@ -53,6 +71,9 @@ struct AdjArgs {
   float imass[kAdjMaxTypes];
   float radius[kAdjMaxTypes];
   AdjPair p[kAdjMaxPairs];
+  AdjBond bond;
+  const int *boff;   // per-id bond list offsets (N + 1), null without bonds
+  const int *bent;   // entries: partner id << 1 | (this end is the bond's i)
 };
 
 __host__ __device__ inline int pair_index(int ti, int tj, int nt) {
@@ -126,24 +147,80 @@ __device__ __forceinline__ void neighbours(const AdjArgs &A, const float4 *__res
   }
 }
 
-// Morse terms of one pair: returns false when s lies beyond the domain.
-struct MorseEval {
-  float r, ro, s, u, ep, epp;  // E'(s), E''(s)
+// Terms of one pair: returns false when s lies beyond the domain.
+struct PairEval {
+  float r, ro, s, e, ep, epp;  // E(s), E'(s), E''(s)
+  float u;                     // Morse: exp(-a (s - 1))
+  float is, is6;               // LJ: 1 / s, s^-6
   bool inner;                  // r >= smin: s moves with r
 };
 
-__device__ __forceinline__ bool morse(const AdjPair &P, float r2, MorseEval &m) {
+__device__ __forceinline__ bool pair_eval(const AdjPair &P, float r2, PairEval &m) {
   m.r = sqrtf(r2);
   m.ro = fmaxf(m.r, FLT_EPSILON);
   m.inner = m.r >= P.smin;
   const float re = m.inner ? m.r : P.smin;
+  if (P.kind == kPairLJ) {
+    m.s = re;
+    if (m.s > P.smax) return false;
+    m.is = 1.0f / m.s;
+    const float is2 = m.is * m.is;
+    m.is6 = is2 * is2 * is2;
+    const float a12 = P.d * m.is6 * m.is6, b6 = P.a * m.is6;
+    m.e = a12 - b6;
+    m.ep = (-12.0f * a12 + 6.0f * b6) * m.is;
+    m.epp = (156.0f * a12 - 42.0f * b6) * is2;
+    m.u = 0.0f;
+    return true;
+  }
   m.s = re - P.R + P.r0p1;
   if (m.s > P.smax) return false;
   m.u = expf(-P.a * (m.s - 1.0f));
   const float uu = m.u * m.u;
+  m.e = P.d * (uu - 2.0f * m.u);
   m.ep = 2.0f * P.a * P.d * (m.u - uu);
   m.epp = 2.0f * P.a * P.a * P.d * (2.0f * uu - m.u);
+  m.is = m.is6 = 0.0f;
   return true;
+}
+
+// dphi/dtheta of a pair, phi = -E'(s) / ro: Morse (d, a, r0), LJ (A, B, -).
+__device__ __forceinline__ void pair_dphi(const AdjPair &P, const PairEval &m, float &g0, float &g1, float &g2) {
+  if (P.kind == kPairLJ) {
+    g0 = (12.0f * m.is6 * m.is6 * m.is) / m.ro;
+    g1 = -(6.0f * m.is6 * m.is) / m.ro;
+    g2 = 0.0f;
+    return;
+  }
+  const float w = m.u - m.u * m.u;
+  g0 = -(2.0f * P.a * w) / m.ro;
+  g1 = -(2.0f * P.d * w - 2.0f * P.a * P.d * (m.s - 1.0f) * m.u * (1.0f - 2.0f * m.u)) / m.ro;
+  g2 = -m.epp / m.ro;  // ds/dr0 = 1
+}
+
+// A bond seen from one end: d = x_self - x_other (minimum image), phi_b and
+// psi_b = (dphi_b/dr) / r (zero where r is clamped).
+struct BondEval {
+  float dx, dy, dz, rb, phi, psi;
+  bool in_range;
+};
+
+__device__ __forceinline__ float min_image(float d, float L) {
+  if (d > 0.5f * L) return d - L;
+  if (d < -0.5f * L) return d + L;
+  return d;
+}
+
+__device__ __forceinline__ void bond_eval(const AdjArgs &A, float4 xi, float4 xj, BondEval &b) {
+  b.dx = min_image(xi.x - xj.x, A.L[0]);
+  b.dy = min_image(xi.y - xj.y, A.L[1]);
+  b.dz = min_image(xi.z - xj.z, A.L[2]);
+  const float r = sqrtf(b.dx * b.dx + b.dy * b.dy + b.dz * b.dz);
+  b.in_range = r > A.bond.bmin && r < A.bond.bmax;
+  b.rb = fminf(fmaxf(r, A.bond.bmin), A.bond.bmax);
+  const float ep = 2.0f * A.bond.K * (b.rb - A.bond.r0);
+  b.phi = -ep / b.rb;
+  b.psi = b.in_range ? (-2.0f * A.bond.K + ep / b.rb) / (b.rb * b.rb) : 0.0f;
 }
 
 // ---------------------------------------------------------------- binning
@@ -234,12 +311,19 @@ __global__ __launch_bounds__(kAB) void k_cellsort(AdjArgs A, const int *__restri
   cnt[c] = 0;
 }
 
+// Sorted index of each particle id (the inverse of pid_s).
+__global__ __launch_bounds__(kAB) void k_slot_of(int N, const int *__restrict__ pid_s, int *__restrict__ slot_of) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < N) slot_of[pid_s[k]] = k;
+}
+
 // ---------------------------------------------------------------- forward
 
 // One overdamped step: forces, clamp, advance.  Optional outputs: the force
 // by id (f), the clamp bits by id (mask), and per-block FP64 energy sums.
 __global__ __launch_bounds__(kAB) void k_forward(AdjArgs A, const float4 *__restrict__ xs, const int *__restrict__ pid_s,
-                                                 const int *__restrict__ cstart, float4 *__restrict__ x_next, unsigned char *__restrict__ mask,
+                                                 const int *__restrict__ cstart, const float4 *__restrict__ xid,
+                                                 float4 *__restrict__ x_next, unsigned char *__restrict__ mask,
                                                  float *__restrict__ f, double *__restrict__ epart) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0.0;
@@ -249,15 +333,30 @@ __global__ __launch_bounds__(kAB) void k_forward(AdjArgs A, const float4 *__rest
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     neighbours(A, xs, cstart, i, xi, [&](int, float4 xj, float dx, float dy, float dz, float r2) __attribute__((always_inline)) {
       const AdjPair &P = A.p[pair_index(ti, type_of(xj), A.ntypes)];
-      MorseEval m;
-      if (!morse(P, r2, m)) return;
+      PairEval m;
+      if (!pair_eval(P, r2, m)) return;
       const float phi = -m.ep / m.ro;
       fx += phi * dx;
       fy += phi * dy;
       fz += phi * dz;
-      if (epart) e += 0.5 * (double)(P.d * (m.u * m.u - 2.0f * m.u));
+      if (epart) e += 0.5 * (double)m.e;
     });
     const int pid = pid_s[i];
+    if (A.boff) {
+      // Bonds of this particle, in list order (partners by id).
+      for (int k = A.boff[pid]; k < A.boff[pid + 1]; ++k) {
+        const int ent = A.bent[k];
+        BondEval b;
+        bond_eval(A, xi, xid[ent >> 1], b);
+        fx += b.phi * b.dx;
+        fy += b.phi * b.dy;
+        fz += b.phi * b.dz;
+        if (epart && (ent & 1)) {
+          const float q = b.rb - A.bond.r0;
+          e += (double)(A.bond.K * q * q);
+        }
+      }
+    }
     const float im = A.imass[ti];
     const float fv[3] = {fx, fy, fz};
     const float xv[3] = {xi.x, xi.y, xi.z};
@@ -349,9 +448,11 @@ __global__ __launch_bounds__(kAB) void k_zero(double *__restrict__ p, int n) {
 // lambda_t = lambda_{t+1} + sum_j H_ij (mu_i - mu_j), and per-block sums of
 // the parameter gradient (each pair once: sorted index i < j).
 __global__ __launch_bounds__(kAB) void k_backward(AdjArgs A, const float4 *__restrict__ xs, const int *__restrict__ pid_s,
-                                                  const int *__restrict__ cstart, const double4 *__restrict__ mu, double4 *__restrict__ lam,
-                                                  double *__restrict__ gpart) {
+                                                  const int *__restrict__ cstart, const int *__restrict__ slot_of,
+                                                  const double4 *__restrict__ mu,
+                                                  double4 *__restrict__ lam, double *__restrict__ gpart) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double gb0 = 0.0, gb1 = 0.0;  // bond K, r0
   double g[kAdjMaxTypes][3];
 #pragma unroll
   for (int t = 0; t < kAdjMaxTypes; ++t) g[t][0] = g[t][1] = g[t][2] = 0.0;
@@ -364,8 +465,8 @@ __global__ __launch_bounds__(kAB) void k_backward(AdjArgs A, const float4 *__res
     neighbours(A, xs, cstart, i, xi, [&](int j, float4 xj, float dx, float dy, float dz, float r2) __attribute__((always_inline)) {
       const int tj = type_of(xj);
       const AdjPair &P = A.p[pair_index(ti, tj, A.ntypes)];
-      MorseEval m;
-      if (!morse(P, r2, m)) return;
+      PairEval m;
+      if (!pair_eval(P, r2, m)) return;
       const double4 mj = mu[j];
       const double ux = mi.x - mj.x, uy = mi.y - mj.y, uz = mi.z - mj.z;
       const double dd = (double)dx * ux + (double)dy * uy + (double)dz * uz;
@@ -376,10 +477,8 @@ __global__ __launch_bounds__(kAB) void k_backward(AdjArgs A, const float4 *__res
       ay += (double)phi * uy + pp * dy;
       az += (double)phi * uz + pp * dz;
       if (i < j) {
-        const float w = m.u - m.u * m.u;
-        const float gd = -(2.0f * P.a * w) / m.ro;
-        const float ga = -(2.0f * P.d * w - 2.0f * P.a * P.d * (m.s - 1.0f) * m.u * (1.0f - 2.0f * m.u)) / m.ro;
-        const float gr = -m.epp / m.ro;  // ds/dr0 = 1
+        float gd, ga, gr;
+        pair_dphi(P, m, gd, ga, gr);
         const double c0 = dd * gd, c1 = dd * ga, c2 = dd * gr;
         // Constant indices keep g in registers.
 #define MXG_ADJ_ACC(t)  \
@@ -396,14 +495,35 @@ __global__ __launch_bounds__(kAB) void k_backward(AdjArgs A, const float4 *__res
       }
     });
     const int pid = pid_s[i];
+    if (A.boff) {
+      for (int k = A.boff[pid]; k < A.boff[pid + 1]; ++k) {
+        const int ent = A.bent[k];
+        const int sj = slot_of[ent >> 1];
+        BondEval b;
+        bond_eval(A, xi, xs[sj], b);
+        const double4 mj = mu[sj];
+        const double ux = mi.x - mj.x, uy = mi.y - mj.y, uz = mi.z - mj.z;
+        const double dd = (double)b.dx * ux + (double)b.dy * uy + (double)b.dz * uz;
+        const double pp = (double)b.psi * dd;
+        ax += (double)b.phi * ux + pp * b.dx;
+        ay += (double)b.phi * uy + pp * b.dy;
+        az += (double)b.phi * uz + pp * b.dz;
+        if (ent & 1) {
+          // dphi_b/dK = -2 (rb - r0) / rb, dphi_b/dr0 = 2 K / rb
+          gb0 += dd * (double)(-2.0f * (b.rb - A.bond.r0) / b.rb);
+          gb1 += dd * (double)(2.0f * A.bond.K / b.rb);
+        }
+      }
+    }
     double4 l = lam[pid];
     l.x += ax;
     l.y += ay;
     l.z += az;
     lam[pid] = l;
   }
-  // Block sums of the parameter gradient, slot = 3 * pair + component.
-  __shared__ double red[kAdjMaxPairs * 3][kAB / 64];
+  // Block sums of the parameter gradient, slot = 3 * pair + component, then
+  // the bond's K and r0.
+  __shared__ double red[kAdjSlots][kAB / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int np = A.ntypes * (A.ntypes + 1) / 2;
   for (int p = 0; p < np; ++p) {
@@ -418,11 +538,19 @@ __global__ __launch_bounds__(kAB) void k_backward(AdjArgs A, const float4 *__res
       if (lane == 0) red[3 * p + k][wv] = v;
     }
   }
+  {
+    const double v0 = wave_sum(gb0), v1 = wave_sum(gb1);
+    if (lane == 0) {
+      red[3 * np][wv] = v0;
+      red[3 * np + 1][wv] = v1;
+      red[3 * np + 2][wv] = 0.0;
+    }
+  }
   __syncthreads();
-  if ((int)threadIdx.x < 3 * np) {
+  if ((int)threadIdx.x < 3 * np + 3) {
     double s = 0.0;
     for (int k = 0; k < kAB / 64; ++k) s += red[threadIdx.x][k];
-    gpart[(size_t)blockIdx.x * (3 * kAdjMaxPairs) + threadIdx.x] = s;
+    gpart[(size_t)blockIdx.x * kAdjSlots + threadIdx.x] = s;
   }
 }
 
@@ -431,7 +559,7 @@ __global__ __launch_bounds__(64) void k_gsum(const double *__restrict__ gpart, i
   const int k = threadIdx.x;
   if (k >= nslots) return;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += gpart[(size_t)b * (3 * kAdjMaxPairs) + k];
+  for (int b = 0; b < nblocks; ++b) s += gpart[(size_t)b * kAdjSlots + k];
   gsum[k] += s;
 }
 
@@ -540,15 +668,17 @@ struct AdjointDev {
   int *pid_s = nullptr, *cstart = nullptr;
   unsigned char *mask = nullptr;
   double4 *lam = nullptr, *mu = nullptr;
+  int *slot_of = nullptr;  // sorted index of each id (backward sweep)
   double *part = nullptr;  // per-block partial sums
   double *gsum = nullptr;
   unsigned long long *hist = nullptr;
+  int *boff = nullptr, *bent = nullptr;  // bond lists by id
 };
 
 Adjoint::~Adjoint() {
   if (!d_) return;
   void *ptrs[] = {d_->type, d_->x[0], d_->x[1], d_->cellid, d_->cnt, d_->xs, d_->pid_s, d_->cstart,
-                  d_->mask, d_->lam, d_->mu, d_->part, d_->gsum, d_->hist};
+                  d_->mask, d_->lam, d_->mu, d_->slot_of, d_->part, d_->gsum, d_->hist, d_->boff, d_->bent};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete d_;
@@ -610,8 +740,9 @@ int Adjoint::create(int N, const double dim[3], const int cells[3], double cutof
   st |= hipMalloc(&d_->cnt, ncells_ * sizeof(int));
   st |= hipMalloc(&d_->lam, N * sizeof(double4));
   st |= hipMalloc(&d_->mu, N * sizeof(double4));
-  st |= hipMalloc(&d_->part, (size_t)nblocks_ * 3 * kAdjMaxPairs * sizeof(double));
-  st |= hipMalloc(&d_->gsum, 3 * kAdjMaxPairs * sizeof(double));
+  st |= hipMalloc(&d_->slot_of, N * sizeof(int));
+  st |= hipMalloc(&d_->part, (size_t)nblocks_ * kAdjSlots * sizeof(double));
+  st |= hipMalloc(&d_->gsum, kAdjSlots * sizeof(double));
   st |= hipMalloc(&d_->hist, kAdjMaxPairs * kAdjMaxBins * sizeof(unsigned long long));
   if (st) return device(st, "hipMalloc");
   st |= hipMemcpy(d_->type, type, N * sizeof(int), hipMemcpyHostToDevice);
@@ -649,13 +780,71 @@ int Adjoint::set_args(const double *theta, double smin, double smax) {
     for (int tj = ti; tj < ntypes_; ++tj) {
       const int p = pair_index(ti, tj, ntypes_);
       AdjPair &P = A.p[p];
+      P.kind = kind_[p];
       P.d = (float)theta[3 * p + 0];
       P.a = (float)theta[3 * p + 1];
       P.r0p1 = (float)(theta[3 * p + 2] + 1.0);
       P.R = (float)(radius_[ti] + radius_[tj]);
-      P.smin = (float)smin;
-      P.smax = (float)smax;
+      P.smin = (float)(dom_[p] ? pmin_[p] : smin);
+      P.smax = (float)(dom_[p] ? pmax_[p] : smax);
     }
+  if (M_ > 0) {
+    const int b = 3 * npairs();
+    A.bond.K = (float)theta[b];
+    A.bond.r0 = (float)theta[b + 1];
+    A.bond.bmin = (float)bmin_;
+    A.bond.bmax = (float)bmax_;
+    A.boff = d_->boff;
+    A.bent = d_->bent;
+  } else {
+    A.boff = A.bent = nullptr;
+  }
+  return 0;
+}
+
+int Adjoint::set_pair_kind(int ti, int tj, int kind, double smin, double smax) {
+  if (!d_) return fail(-1, "not created");
+  if (ti < 0 || tj < 0 || ti >= ntypes_ || tj >= ntypes_) return fail(-6, "type out of range");
+  if (kind != kPairMorse && kind != kPairLJ) return fail(-6, "pair kind is 0 (Morse) or 1 (LJ 12-6)");
+  const int p = pair_index(ti, tj, ntypes_);
+  kind_[p] = kind;
+  dom_[p] = smax > smin;
+  pmin_[p] = smin;
+  pmax_[p] = smax;
+  return 0;
+}
+
+int Adjoint::set_bonds(int M, const int *bi, const int *bj, double rmin, double rmax) {
+  if (!d_) return fail(-1, "not created");
+  if (M < 0 || (M > 0 && (!bi || !bj))) return fail(-1, "bond arrays missing");
+  if (M > 0 && !(rmax > rmin && rmin > 0)) return fail(-6, "bond domain must satisfy 0 < rmin < rmax");
+  std::vector<int> off(N_ + 1, 0);
+  for (int b = 0; b < M; ++b) {
+    if (bi[b] < 0 || bi[b] >= N_ || bj[b] < 0 || bj[b] >= N_ || bi[b] == bj[b])
+      return fail(-11, "bond " + std::to_string(b) + " has an invalid particle id");
+    off[bi[b] + 1]++;
+    off[bj[b] + 1]++;
+  }
+  for (int i = 0; i < N_; ++i) off[i + 1] += off[i];
+  std::vector<int> ent(2 * (size_t)M + 1), fill(off.begin(), off.end() - 1);
+  for (int b = 0; b < M; ++b) {
+    ent[fill[bi[b]]++] = (bj[b] << 1) | 1;
+    ent[fill[bj[b]]++] = bi[b] << 1;
+  }
+  if (d_->boff) (void)hipFree(d_->boff);
+  if (d_->bent) (void)hipFree(d_->bent);
+  d_->boff = d_->bent = nullptr;
+  M_ = 0;
+  if (M == 0) return 0;
+  int st = hipMalloc(&d_->boff, (N_ + 1) * sizeof(int));
+  st |= hipMalloc(&d_->bent, ent.size() * sizeof(int));
+  if (st) return device(st, "hipMalloc bonds");
+  st |= hipMemcpy(d_->boff, off.data(), (N_ + 1) * sizeof(int), hipMemcpyHostToDevice);
+  st |= hipMemcpy(d_->bent, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (st) return device(st, "bond upload");
+  M_ = M;
+  bmin_ = rmin;
+  bmax_ = rmax;
   return 0;
 }
 
@@ -687,7 +876,7 @@ int Adjoint::forward(const double *theta, double smin, double smax, const double
   for (int t = 0; t < nsteps; ++t) {
     if (t % every == 0) hipLaunchKernelGGL(k_copy4, dim3(nblocks_), dim3(kAB), 0, s, N_, xa, ck + (size_t)(t / every) * N_);
     bin(xa, 0, stream);
-    hipLaunchKernelGGL(k_forward, dim3(nblocks_), dim3(kAB), 0, s, A, d_->xs, d_->pid_s, d_->cstart, xb,
+    hipLaunchKernelGGL(k_forward, dim3(nblocks_), dim3(kAB), 0, s, A, d_->xs, d_->pid_s, d_->cstart, xa, xb,
                        (unsigned char *)nullptr, (float *)nullptr, (double *)nullptr);
     std::swap(xa, xb);
   }
@@ -704,9 +893,9 @@ int Adjoint::backward(const double *theta, double smin, double smax, const float
   hipStream_t s = (hipStream_t)stream;
   const AdjArgs &A = d_->args;
   const float4 *ck = (const float4 *)ckpt;
-  const int nslots = 3 * npairs();
+  const int nslots = 3 * nrows();
   hipLaunchKernelGGL(k_lam_init, dim3(nblocks_), dim3(kAB), 0, s, N_, gxT, d_->lam);
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(kAB), 0, s, d_->gsum, 3 * kAdjMaxPairs);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(kAB), 0, s, d_->gsum, kAdjSlots);
   const int nseg = (nsteps + every - 1) / every;
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int t0 = seg * every, n = std::min(every, nsteps - t0);
@@ -716,7 +905,7 @@ int Adjoint::backward(const double *theta, double smin, double smax, const float
     for (int q = 0; q < n; ++q) {
       bin(xa, q, stream);
       hipLaunchKernelGGL(k_forward, dim3(nblocks_), dim3(kAB), 0, s, A, d_->xs + (size_t)q * N_,
-                         d_->pid_s + (size_t)q * N_, d_->cstart + (size_t)q * (ncells_ + 1), xb,
+                         d_->pid_s + (size_t)q * N_, d_->cstart + (size_t)q * (ncells_ + 1), xa, xb,
                          d_->mask + (size_t)q * N_, (float *)nullptr, (double *)nullptr);
       std::swap(xa, xb);
     }
@@ -726,8 +915,9 @@ int Adjoint::backward(const double *theta, double smin, double smax, const float
       const int *cstart = d_->cstart + (size_t)q * (ncells_ + 1);
       hipLaunchKernelGGL(k_mu, dim3(nblocks_), dim3(kAB), 0, s, A, xs, pid_s, d_->mask + (size_t)q * N_, d_->lam,
                          d_->mu);
-      hipLaunchKernelGGL(k_backward, dim3(nblocks_), dim3(kAB), 0, s, A, xs, pid_s, cstart, d_->mu, d_->lam,
-                         d_->part);
+      if (M_ > 0) hipLaunchKernelGGL(k_slot_of, dim3(nblocks_), dim3(kAB), 0, s, N_, pid_s, d_->slot_of);
+      hipLaunchKernelGGL(k_backward, dim3(nblocks_), dim3(kAB), 0, s, A, xs, pid_s, cstart, d_->slot_of, d_->mu,
+                         d_->lam, d_->part);
       hipLaunchKernelGGL(k_gsum, dim3(1), dim3(64), 0, s, d_->part, nblocks_, nslots, d_->gsum);
     }
   }
@@ -747,8 +937,8 @@ int Adjoint::forces(const double *theta, double smin, double smax, const double 
   const AdjArgs &A = d_->args;
   hipLaunchKernelGGL(k_init, dim3(nblocks_), dim3(kAB), 0, s, A, x, d_->type, d_->x[0]);
   bin(d_->x[0], 0, stream);
-  hipLaunchKernelGGL(k_forward, dim3(nblocks_), dim3(kAB), 0, s, A, d_->xs, d_->pid_s, d_->cstart, d_->x[1],
-                     (unsigned char *)nullptr, f, d_->part);
+  hipLaunchKernelGGL(k_forward, dim3(nblocks_), dim3(kAB), 0, s, A, d_->xs, d_->pid_s, d_->cstart, d_->x[0],
+                     d_->x[1], (unsigned char *)nullptr, f, d_->part);
   std::vector<double> part(nblocks_);
   int st = hipMemcpyAsync(part.data(), d_->part, nblocks_ * sizeof(double), hipMemcpyDeviceToHost, s);
   st |= hipStreamSynchronize(s);

@@ -447,6 +447,16 @@ int mxg_adjoint_forward(mxg_adjoint *a, const double *theta, double smin, double
   return a->a.forward(theta, smin, smax, x0, xT, ckpt, nsteps, every, stream);
 }
 
+int mxg_adjoint_set_pair_kind(mxg_adjoint *a, int ti, int tj, int kind, double smin, double smax) {
+  if (!a) return MXG_ERR_NULL;
+  return a->a.set_pair_kind(ti, tj, kind, smin, smax);
+}
+
+int mxg_adjoint_set_bonds(mxg_adjoint *a, int M, const int32_t *i, const int32_t *j, double rmin, double rmax) {
+  if (!a) return MXG_ERR_NULL;
+  return a->a.set_bonds(M, i, j, rmin, rmax);
+}
+
 int mxg_adjoint_backward(mxg_adjoint *a, const double *theta, double smin, double smax, const float *ckpt, int nsteps,
                          int every, const double *gxT, double *gx0, double *gtheta, void *stream) {
   if (!a) return MXG_ERR_NULL;

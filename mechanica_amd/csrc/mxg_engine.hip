@@ -2709,7 +2709,19 @@ static void flag_offsets(DevReplica &D, DevOffset *off, const std::vector<float>
   D.near_eps = (float)(1.5 * std::sqrt(3.0) * (slack + 2.05 / D.dscale + 6.0 * dmax_delta));
 }
 
-// Offsets, shift tables and per-replica constants.
+// engine_advance_forward_euler's clamps (engine_advance.cpp:262-281): the
+// velocity and displacement limits of a step as a fraction of the cell.
+static void set_clamps(DevReplica &D, float dt, float frac) {
+  for (int k = 0; k < 3; ++k) {
+    D.maxv[k] = (D.h[k] * frac) / dt;
+    D.maxv2[k] = D.maxv[k] * D.maxv[k];
+    D.maxx[k] = D.h[k] * frac;
+    D.maxx2[k] = D.maxx[k] * D.maxx[k];
+  }
+}
+
+// Offsets, shift tables and per-replica constants (once, at init: it
+// allocates the geometry's device buffers).
 int Engine::build_geometry() {
   std::vector<DevReplica> drep(reps_.size());
   std::vector<DevOffset> offs;
@@ -2730,17 +2742,12 @@ int Engine::build_geometry() {
     D.seed_hi = (uint32_t)(rep_seed_[r] >> 32);
     D.id_base = R.gid_base;
     for (int c = 0; c < R.ncells; ++c) cell_rep[R.cell_base + c] = (int)r;
-    // engine_advance_forward_euler constants (engine_advance.cpp:262-281).
-    const float dt = (float)dt_;
     for (int k = 0; k < 3; ++k) {
       D.h[k] = (float)R.h[k];
       D.h2[k] = (float)(2. * R.h[k]);
-      D.maxv[k] = (D.h[k] * max_frac_) / dt;
-      D.maxv2[k] = D.maxv[k] * D.maxv[k];
-      D.maxx[k] = D.h[k] * max_frac_;
-      D.maxx2[k] = D.maxx[k] * D.maxx[k];
       D.hd[k] = R.h[k];
     }
+    set_clamps(D, (float)dt_, max_frac_);
     // Sort-key scale of runner_dosort/runner_dopair.
     D.bias = (float)std::sqrt(R.h[0] * R.h[0] + R.h[1] * R.h[1] + R.h[2] * R.h[2]);
     D.dscale = (float)SHRT_MAX / (2 * D.bias);
@@ -2956,7 +2963,12 @@ int Engine::set_max_dist_fraction(float f) {
   if (!(f > 0)) return reject(kErrRange, "max distance fraction must be positive");
   max_frac_ = f;
   args_dirty_ = true;
-  if (d_) return build_geometry();
+  // Only the advance's clamps change: the geometry and every device buffer
+  // (cell offsets, counters, time) stay as they are.
+  if (d_ && d_->rep) {
+    for (size_t r = 0; r < reps_.size(); ++r) set_clamps(d_->drep[r], (float)dt_, max_frac_);
+    return upload_replicas();
+  }
   return kErrOk;
 }
 

@@ -306,6 +306,23 @@ def adjoint_morse(N=65536, L=40.0, seed=6, min_dist=None):
             "mass": (1.0, 1.0), "radius": (1.0, 1.0), "theta": theta, "smin": 1e-4, "smax": 2.5}
 
 
+def adjoint_mixed(N=4096, L=16.0, seed=8, min_dist=None, bond_r=1.3):
+    """The adjoint's widened system: two types with Morse AA and BB (d = 3,
+    a = 6, r0 = 1 on [1e-4, 2.5]) and Lennard-Jones AB (A = 0.987,
+    B = 1.987: sigma 0.89, epsilon 1, minimum near r = 1, on [0.5, 2.5]),
+    plus harmonic bonds (K = 10, r0 = 1 on [0.2, 2.5]) between the pairs
+    closer than bond_r at the start; theta rows AA, AB, BB, bond."""
+    from scipy.spatial import cKDTree
+    w = adjoint_morse(N=N, L=L, seed=seed, min_dist=min_dist)
+    tree = cKDTree(np.mod(w["x"], L), boxsize=L)
+    pairs = tree.query_pairs(bond_r, output_type="ndarray")
+    w["name"] = "adjoint_mixed"
+    w["theta"] = np.array([[3.0, 6.0, 1.0], [0.987, 1.987, 0.0], [3.0, 6.0, 1.0], [10.0, 1.0, 0.0]])
+    w["kinds"] = {(0, 1): (1, 0.5, 2.5)}
+    w["bonds"] = (pairs[:, 0].astype(np.int32), pairs[:, 1].astype(np.int32), 0.2, 2.5)
+    return w
+
+
 def rdf_target(w, nbins=64, rmax=2.5, seed=7):
     """A seeded target g(r): the soft g(r) of an independent uniform
     configuration of the same size and composition, on the host."""

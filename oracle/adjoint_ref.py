@@ -1,4 +1,4 @@
-"""FP64 reference of the differentiable overdamped Morse trajectory.
+"""FP64 reference of the differentiable overdamped trajectory.
 
 Test infrastructure only: the product never imports this module.  It
 restates, in torch float64 on the CPU, the map the GPU adjoint
@@ -8,10 +8,15 @@ differentiates, and takes exact gradients of it with torch autograd:
   loops of runner_dopair/runner_doself, src/mdcore/src/runner_dopair.cpp:
   91-340, 446-682), and s <= smax (potential_eval_ex's range test,
   src/mdcore/src/potential_eval.hpp:106-161);
-* pair force: phi(r) d with phi = -E'(s) / max(r, FLT_EPSILON),
-  s = max(r, smin) - (R_i + R_j) + r0 + 1 (the Morse shift rule of
-  potential_create_morse, MxPotential.cpp), E(s) = D (u^2 - 2u),
-  u = exp(-A (s - 1)) -- the closed form of the reference's Morse table;
+* pair force: phi(r) d with phi = -E'(s) / max(r, FLT_EPSILON), per type
+  pair either Morse, s = max(r, smin) - (R_i + R_j) + r0 + 1 (the shift rule
+  of potential_create_morse, MxPotential.cpp:3395-3424), E(s) = D (u^2 - 2u),
+  u = exp(-A (s - 1)), or Lennard-Jones 12-6, s = max(r, smin),
+  E(s) = A s^-12 - B s^-6 (potential_create_LJ126, MxPotential.cpp:976-991)
+  -- the closed forms of the reference's tables;
+* bond force (bond_eval, src/mdcore/src/bond.cpp:99-295, on a
+  potential_create_harmonic table): rb = clamp(r, bmin, bmax),
+  phi = -E'(rb) / rb, E = K (rb - r0)^2;
 * overdamped update (src/mdcore/src/engine_advance.cpp:126-239): per
   component dx = (dt F) / m clamped to +-max_dist_fraction * h, x += dx.
 
@@ -28,7 +33,7 @@ FLT_EPSILON = float(np.finfo(np.float32).eps)
 
 class RefSystem:
     def __init__(self, types, dim, cutoff, dt, mass=(1.0,), radius=(1.0,), cells=None, max_dist_fraction=0.05,
-                 smin=1e-4, smax=None):
+                 smin=1e-4, smax=None, kinds=None, bonds=None):
         self.types = torch.as_tensor(np.asarray(types), dtype=torch.long)
         self.N = len(self.types)
         self.ntypes = len(mass)
@@ -49,6 +54,21 @@ class RefSystem:
             for b in range(a, self.ntypes):
                 self.pair_of[a, b] = self.pair_of[b, a] = k
                 k += 1
+        self.npairs = k
+        # kind of each type pair: 0 Morse, 1 LJ 12-6
+        self.kind = torch.zeros(k, dtype=torch.long)
+        self.pmin = torch.full((k,), self.smin, dtype=torch.float64)
+        self.pmax = torch.full((k,), self.smax, dtype=torch.float64)
+        for (a, b), spec in (kinds or {}).items():
+            # spec: kind, or (kind, smin, smax) for a domain of its own
+            kd, lo, hi = (spec, self.smin, self.smax) if np.isscalar(spec) else spec
+            self.kind[self.pair_of[a, b]] = int(kd)
+            self.pmin[self.pair_of[a, b]] = float(lo)
+            self.pmax[self.pair_of[a, b]] = float(hi)
+        self.bonds = None
+        if bonds is not None:
+            self.bonds = (torch.as_tensor(np.asarray(bonds[0]), dtype=torch.long),
+                          torch.as_tensor(np.asarray(bonds[1]), dtype=torch.long), float(bonds[2]), float(bonds[3]))
 
     def min_image(self, d):
         dim = self.dim.to(d.dtype)
@@ -72,30 +92,57 @@ class RefSystem:
         return torch.cat(I), torch.cat(J)
 
     def pair_terms(self, x, theta):
+        """Pairs in range with E(s) and E'(s) of their potentials."""
         i, j = self.pairs(x)
         d = self.min_image(x[i] - x[j])
         r = torch.sqrt((d * d).sum(-1))
         p = self.pair_of[self.types[i], self.types[j]]
-        D, A, r0 = theta[p, 0], theta[p, 1], theta[p, 2]
+        P0, P1, r0 = theta[p, 0], theta[p, 1], theta[p, 2]
+        lj = self.kind[p] == 1
         R = (self.radius[self.types[i]] + self.radius[self.types[j]]).to(x.dtype)
-        s = torch.clamp(r, min=self.smin) - R + r0 + 1.0
+        pmin = self.pmin[p].to(x.dtype)
+        re = torch.maximum(r, pmin)
+        s = torch.where(lj, re, re - R + r0 + 1.0)
         with torch.no_grad():
-            ok = s <= self.smax
-        i, j, d, r, s, D, A = i[ok], j[ok], d[ok], r[ok], s[ok], D[ok], A[ok]
-        u = torch.exp(-A * (s - 1.0))
-        return i, j, d, r, u, D, A
+            ok = s <= self.pmax[p].to(x.dtype)
+        i, j, d, r, s, P0, P1, lj = i[ok], j[ok], d[ok], r[ok], s[ok], P0[ok], P1[ok], lj[ok]
+        # Morse: D = P0, A = P1; LJ: A = P0, B = P1.  Each form is evaluated
+        # on a harmless argument where the other applies, so that neither
+        # puts a non-finite value into the gradient.
+        sm = torch.where(lj, torch.ones_like(s), s)
+        sl = torch.where(lj, s, torch.ones_like(s))
+        u = torch.exp(-P1 * (sm - 1.0))
+        is6 = sl ** -6
+        e = torch.where(lj, P0 * is6 * is6 - P1 * is6, P0 * (u * u - 2.0 * u))
+        ep = torch.where(lj, (-12.0 * P0 * is6 * is6 + 6.0 * P1 * is6) / sl, 2.0 * P1 * P0 * (u - u * u))
+        return i, j, d, r, e, ep
+
+    def bond_terms(self, x, theta):
+        """Bonds with E(rb), E'(rb) and rb = clamp(r, bmin, bmax)."""
+        i, j, bmin, bmax = self.bonds
+        d = self.min_image(x[i] - x[j])
+        r = torch.sqrt((d * d).sum(-1))
+        rb = torch.clamp(r, min=bmin, max=bmax)
+        K, r0 = theta[self.npairs, 0], theta[self.npairs, 1]
+        return i, j, d, rb, K * (rb - r0) ** 2, 2.0 * K * (rb - r0)
 
     def energy(self, x, theta):
-        _, _, _, _, u, D, _ = self.pair_terms(x, theta)
-        return (D * (u * u - 2.0 * u)).sum()
+        e = self.pair_terms(x, theta)[4].sum()
+        if self.bonds is not None:
+            e = e + self.bond_terms(x, theta)[4].sum()
+        return e
 
     def forces(self, x, theta):
-        i, j, d, r, u, D, A = self.pair_terms(x, theta)
-        ep = 2.0 * A * D * (u - u * u)
+        i, j, d, r, _, ep = self.pair_terms(x, theta)
         phi = -ep / torch.clamp(r, min=FLT_EPSILON)  # also runs in float32 to size FP32 rounding
         fp = phi[:, None] * d
         F = torch.zeros_like(x)
-        return F.index_add(0, i, fp).index_add(0, j, -fp)
+        F = F.index_add(0, i, fp).index_add(0, j, -fp)
+        if self.bonds is not None:
+            bi, bj, bd, rb, _, bep = self.bond_terms(x, theta)
+            fb = (-bep / rb)[:, None] * bd
+            F = F.index_add(0, bi, fb).index_add(0, bj, -fb)
+        return F
 
     def step(self, x, theta):
         F = self.forces(x, theta)

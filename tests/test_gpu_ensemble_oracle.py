@@ -8,7 +8,8 @@ Each replica draws its random force from its own seed, keyed by its
 particles' ranks in the replica, so replica r is checked against a standalone
 oracle loaded with the same particles and seeded with replica r's seed:
   * 16 replicas: step-1 forces within 1e-5 of each replica's FP32 oracle
-    (random force included; the persistent random forces bit for bit); then
+    (random force included; the persistent random forces within 1e-6, the
+    device's and the host's sinf/cosf/logf differing by an ulp or two); then
     a relay of five steps, each replica's oracle advancing with the GPU's
     forces, with positions and cells bit for bit;
   * the same 16 replicas stepped from captured HIP graphs give the eager
@@ -71,7 +72,10 @@ def test_ensemble_replicas_match_oracles(m, oracle_mod):
         rel = (np.linalg.norm(fs[r].astype(np.float64) - ref, axis=1) / floor).max()
         worst = max(worst, rel)
         assert rel <= FORCE_TOL, (r, rel)
-        assert np.array_equal(pfs[r].view(np.uint32), fresh.state()["pf"].view(np.uint32)), r
+        # The persistent random vector uses sinf/cosf/logf: the device's and
+        # the host's libm differ by an ulp or two.
+        pf_ref = fresh.state()["pf"]
+        assert np.abs(pfs[r] - pf_ref).max() <= 1e-6 * np.abs(pf_ref).max(), r
     print("step-1 worst max rel force diff over replicas %.3e" % worst)
     for k in range(5):
         if k > 0:
