@@ -415,6 +415,24 @@ int mxg_adjoint_rdf(mxg_adjoint *a, const double *x, int nbins, double rmax, dou
 int mxg_adjoint_rdf_backward(mxg_adjoint *a, const double *x, int nbins, double rmax, const double *ghist,
                              double *gx, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  mxg_adjoint_forces without an mxg_adjoint and without
+ * allocating: the cell binning, sorted positions and energy partials live in
+ * a file-scope device array of about 7.5 MB per GPU, held until the process
+ * ends.  N <= 131072 particles and cells[0]*cells[1]*cells[2] <= 131072
+ * (larger sizes return MXG_ERR_RANGE); cells of at least 3 per axis, no
+ * narrower than the cutoff (else MXG_ERR_CUTOFF).  type (N int32, by id), x
+ * (N*3 FP64), f (N*3 FP32) and epot (one FP64, the total Morse energy) are
+ * device pointers; radius (ntypes) and theta (3 per type pair, as
+ * mxg_adjoint_forward) are host arrays.  The work is queued on `stream`
+ * (a hipStream_t, 0 = default) and nothing synchronises.  Calls on one GPU
+ * share the scratch and leave the last call's contents in it: calls that
+ * overlap on one GPU (on other streams or threads) race. */
+int mxg_morse_forces_noalloc(int N, const double dim[3], const int cells[3], double cutoff, int ntypes,
+                             const double *radius, const double *theta, double smin, double smax, const int *type,
+                             const double *x, float *f, double *epot, void *stream);
+
 /* Number of visible HIP devices (0 when none). */
 int mxg_device_count(void);
 

@@ -116,6 +116,9 @@ SIGNATURES = {
     "mxg_adjoint_set_bonds": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, C.c_double, C.c_double]),
     "mxg_adjoint_forces": (C.c_int, [C.c_void_p, c_double_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                      c_double_p, C.c_void_p]),
+    "mxg_morse_forces_noalloc": (C.c_int, [C.c_int, c_double_p, c_int_p, C.c_double, C.c_int, c_double_p,
+                                           c_double_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]),
     "mxg_adjoint_rdf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]),
     "mxg_adjoint_rdf_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                            C.c_void_p]),

@@ -14,6 +14,19 @@ constexpr int kAdjMaxBins = 256;
 // Gradient slots: (d, a, r0) or (A, B, -) per type pair, then the bond
 // potential's (K, r0, -).
 constexpr int kAdjSlots = 3 * kAdjMaxPairs + 3;
+// Limits of morse_forces_static's file-scope scratch (about 7.5 MB per GPU).
+constexpr int kStaticMaxParticles = 131072;
+constexpr int kStaticMaxCells = 131072;
+
+// Adjoint::forces without an Adjoint and without allocating: the binning,
+// sorted positions and energy partials live in a file-scope device array;
+// x, type (by id), f and epot (one double) are device pointers.  Returns 0,
+// -6 beyond the limits above, -27 for cells narrower than the cutoff, -21
+// on a device error.  Calls on one GPU share the scratch: calls that overlap
+// on one GPU race.
+int morse_forces_static(int N, const double dim[3], const int cells[3], double cutoff, int ntypes,
+                        const double *radius, const double *theta, double smin, double smax, const int *type,
+                        const double *x, float *f, double *epot, void *stream);
 
 struct AdjointDev;
 

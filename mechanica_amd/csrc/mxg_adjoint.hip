@@ -34,6 +34,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "mxg_adjoint.h"
 
@@ -978,6 +979,132 @@ int Adjoint::rdf_backward(const double *x, int nbins, double rmax, const double 
   hipLaunchKernelGGL(k_rdf_backward, dim3(nblocks_), dim3(kAB), 0, s, A, d_->xs, d_->pid_s, d_->cstart, nbins,
                      (float)(rmax * rmax), nbins / rmax, ghist, gx);
   return device(hipGetLastError(), "rdf backward launch");
+}
+
+// ---------------------------------------------------------------- no-alloc
+
+namespace {
+
+// Scratch of morse_forces_static: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code never passes the array's name
+// to a kernel (on the host it is a stand-in, not a device address): the
+// address comes from hipGetSymbolAddress, once per device.
+constexpr size_t kStaticPartBytes = (size_t)kStaticMaxParticles * (3 * sizeof(float4) + 2 * sizeof(int));
+constexpr size_t kStaticCellBytes = (size_t)(2 * kStaticMaxCells + 1) * sizeof(int);
+constexpr size_t kStaticPartials = (kStaticMaxParticles + kAB - 1) / kAB;
+constexpr size_t kStaticBytes = kStaticPartBytes + (kStaticCellBytes + 255) / 256 * 256 + kStaticPartials * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch (looked up on the first call on the
+// device, with the device current).
+int scratch_of_current_device(unsigned char **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -21;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -21;
+    g_scratch_ptr[dev] = (unsigned char *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+__global__ __launch_bounds__(kAB) void k_fill0(int *__restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Fixed-order sum of the per-block energies into one double.
+__global__ __launch_bounds__(kAB) void k_esum(const double *__restrict__ part, int n, double *__restrict__ out) {
+  __shared__ double red[kAB / 64];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < n; k += kAB) s += part[k];
+  const double w = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kAB / 64; ++k) t += red[k];
+    *out = t;
+  }
+}
+
+}  // namespace
+
+int morse_forces_static(int N, const double dim[3], const int cells[3], double cutoff, int ntypes,
+                        const double *radius, const double *theta, double smin, double smax, const int *type,
+                        const double *x, float *f, double *epot, void *stream) {
+  if (N <= 0 || !dim || !cells || !radius || !theta || !type || !x || !f || !epot) return -1;
+  if (N > kStaticMaxParticles) return -6;
+  if (ntypes < 1 || ntypes > kAdjMaxTypes) return -6;
+  if (!(smax > smin)) return -6;
+  long long nc = 1;
+  for (int k = 0; k < 3; ++k) {
+    if (cells[k] < 3 || !(dim[k] > 0)) return -6;
+    if (dim[k] / cells[k] < cutoff) return -27;
+    nc *= cells[k];
+  }
+  if (nc > kStaticMaxCells) return -6;
+  unsigned char *base = nullptr;
+  if (int rc = scratch_of_current_device(&base)) return rc;
+  AdjArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.N = N;
+  A.ntypes = ntypes;
+  A.ncells = (int)nc;
+  for (int k = 0; k < 3; ++k) {
+    A.cdim[k] = cells[k];
+    A.L[k] = (float)dim[k];
+    A.ih[k] = (float)(cells[k] / dim[k]);
+    A.maxx[k] = 0.0f;
+    A.maxx2[k] = 0.0f;
+  }
+  A.rc2 = (float)(cutoff * cutoff);
+  for (int t = 0; t < ntypes; ++t) {
+    A.imass[t] = 1.0f;
+    A.radius[t] = (float)radius[t];
+  }
+  for (int ti = 0; ti < ntypes; ++ti)
+    for (int tj = ti; tj < ntypes; ++tj) {
+      const int p = pair_index(ti, tj, ntypes);
+      AdjPair &P = A.p[p];
+      P.kind = kPairMorse;
+      P.d = (float)theta[3 * p + 0];
+      P.a = (float)theta[3 * p + 1];
+      P.r0p1 = (float)(theta[3 * p + 2] + 1.0);
+      P.R = (float)(radius[ti] + radius[tj]);
+      P.smin = (float)smin;
+      P.smax = (float)smax;
+    }
+  // Scratch layout: positions in, sorted, next; cell ids, sorted ids; cell
+  // counts and offsets; per-block energies.
+  float4 *xin = (float4 *)base;
+  float4 *xs = xin + kStaticMaxParticles;
+  float4 *xnext = xs + kStaticMaxParticles;
+  int *cellid = (int *)(xnext + kStaticMaxParticles);
+  int *pid_s = cellid + kStaticMaxParticles;
+  int *cnt = pid_s + kStaticMaxParticles;
+  int *cstart = cnt + kStaticMaxCells;
+  double *epart = (double *)(base + kStaticPartBytes + (kStaticCellBytes + 255) / 256 * 256);
+  const int nb = (N + kAB - 1) / kAB;
+  hipStream_t s = (hipStream_t)stream;
+  // The scratch keeps the previous call's contents: the counters start at
+  // zero by a fill kernel (no memset node under graph capture).
+  hipLaunchKernelGGL(k_fill0, dim3((A.ncells + kAB - 1) / kAB), dim3(kAB), 0, s, cnt, A.ncells);
+  hipLaunchKernelGGL(k_init, dim3(nb), dim3(kAB), 0, s, A, x, type, xin);
+  hipLaunchKernelGGL(k_count, dim3(nb), dim3(kAB), 0, s, A, xin, cellid, cnt);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, cnt, cstart, A.ncells);
+  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kAB), 0, s, A, cellid, cnt, pid_s);
+  hipLaunchKernelGGL(k_cellsort, dim3((A.ncells + kAB - 1) / kAB), dim3(kAB), 0, s, A, cstart, pid_s, xin, xs, cnt);
+  hipLaunchKernelGGL(k_forward, dim3(nb), dim3(kAB), 0, s, A, xs, pid_s, cstart, xin, xnext, (unsigned char *)nullptr,
+                     f, epart);
+  hipLaunchKernelGGL(k_esum, dim3(1), dim3(kAB), 0, s, epart, nb, epot);
+  return hipGetLastError() == hipSuccess ? 0 : -21;
 }
 
 }  // namespace mxg

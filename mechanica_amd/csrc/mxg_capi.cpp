@@ -469,6 +469,12 @@ int mxg_adjoint_forces(mxg_adjoint *a, const double *theta, double smin, double 
   return a->a.forces(theta, smin, smax, x, f, epot, stream);
 }
 
+int mxg_morse_forces_noalloc(int N, const double dim[3], const int cells[3], double cutoff, int ntypes,
+                             const double *radius, const double *theta, double smin, double smax, const int *type,
+                             const double *x, float *f, double *epot, void *stream) {
+  return mxg::morse_forces_static(N, dim, cells, cutoff, ntypes, radius, theta, smin, smax, type, x, f, epot, stream);
+}
+
 int mxg_adjoint_rdf(mxg_adjoint *a, const double *x, int nbins, double rmax, double *hist, void *stream) {
   if (!a) return MXG_ERR_NULL;
   return a->a.rdf(x, nbins, rmax, hist, stream);
