@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -293,23 +294,39 @@ __global__ __launch_bounds__(kAB) void k_scatter(AdjArgs A, const int *__restric
 }
 
 // Per cell: order its slots by particle id, gather the positions, and clear
-// the cell's counter for the next binning.
+// the cell's counter for the next binning.  One wave per cell: each lane
+// holds one slot's id and finds its rank among the cell's ids with wave
+// shuffles (ids are distinct), so the order is the same as a sort's; a cell
+// of more than 64 particles is sorted by its first lane.
+constexpr int kCellsPerBlock = kAB / 64;
+
 __global__ __launch_bounds__(kAB) void k_cellsort(AdjArgs A, const int *__restrict__ cstart, int *__restrict__ pid_s, const float4 *__restrict__ x,
                                                    float4 *__restrict__ xs, int *__restrict__ cnt) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.x * kCellsPerBlock + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (c >= A.ncells) return;
-  const int a = cstart[c], b = cstart[c + 1];
-  for (int k = a + 1; k < b; ++k) {
-    const int v = pid_s[k];
-    int m = k - 1;
-    while (m >= a && pid_s[m] > v) {
-      pid_s[m + 1] = pid_s[m];
-      --m;
+  const int a = cstart[c], b = cstart[c + 1], n = b - a;
+  if (n <= 64) {
+    const int mine = lane < n ? pid_s[a + lane] : INT_MAX;
+    int rank = 0;
+    for (int m = 0; m < n; ++m) rank += __shfl(mine, m) < mine ? 1 : 0;
+    if (lane < n) {
+      pid_s[a + rank] = mine;
+      xs[a + rank] = x[mine];
     }
-    pid_s[m + 1] = v;
+  } else if (lane == 0) {
+    for (int k = a + 1; k < b; ++k) {
+      const int v = pid_s[k];
+      int m = k - 1;
+      while (m >= a && pid_s[m] > v) {
+        pid_s[m + 1] = pid_s[m];
+        --m;
+      }
+      pid_s[m + 1] = v;
+    }
+    for (int k = a; k < b; ++k) xs[k] = x[pid_s[k]];
   }
-  for (int k = a; k < b; ++k) xs[k] = x[pid_s[k]];
-  cnt[c] = 0;
+  if (lane == 0) cnt[c] = 0;
 }
 
 // Sorted index of each particle id (the inverse of pid_s).
@@ -555,13 +572,22 @@ __global__ __launch_bounds__(kAB) void k_backward(AdjArgs A, const float4 *__res
   }
 }
 
-// gsum[k] += sum over blocks of gpart[b][k], in block order.
-__global__ __launch_bounds__(64) void k_gsum(const double *__restrict__ gpart, int nblocks, int nslots, double *__restrict__ gsum) {
-  const int k = threadIdx.x;
+// gsum[k] += sum over blocks of gpart[b][k]: one block per slot, each
+// thread summing a fixed stride of blocks, then a fixed-order block sum.
+__global__ __launch_bounds__(kAB) void k_gsum(const double *__restrict__ gpart, int nblocks, int nslots, double *__restrict__ gsum) {
+  __shared__ double red[kAB / 64];
+  const int k = blockIdx.x;
   if (k >= nslots) return;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += gpart[(size_t)b * kAdjSlots + k];
-  gsum[k] += s;
+  for (int b = threadIdx.x; b < nblocks; b += kAB) s += gpart[(size_t)b * kAdjSlots + k];
+  const double w = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kAB / 64; ++q) t += red[q];
+    gsum[k] += t;
+  }
 }
 
 // ---------------------------------------------------------------- histogram
@@ -861,7 +887,8 @@ void Adjoint::bin(const void *xv, int slot, void *stream) {
   hipLaunchKernelGGL(k_count, dim3(nblocks_), dim3(kAB), 0, s, A, x, d_->cellid, d_->cnt);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, d_->cnt, cstart, ncells_);
   hipLaunchKernelGGL(k_scatter, dim3(nblocks_), dim3(kAB), 0, s, A, d_->cellid, d_->cnt, pid_s);
-  hipLaunchKernelGGL(k_cellsort, dim3(blocks(ncells_)), dim3(kAB), 0, s, A, cstart, pid_s, x, xs, d_->cnt);
+  hipLaunchKernelGGL(k_cellsort, dim3((ncells_ + kCellsPerBlock - 1) / kCellsPerBlock), dim3(kAB), 0, s, A, cstart,
+                     pid_s, x, xs, d_->cnt);
 }
 
 int Adjoint::forward(const double *theta, double smin, double smax, const double *x0, double *xT, float *ckpt,
@@ -919,7 +946,7 @@ int Adjoint::backward(const double *theta, double smin, double smax, const float
       if (M_ > 0) hipLaunchKernelGGL(k_slot_of, dim3(nblocks_), dim3(kAB), 0, s, N_, pid_s, d_->slot_of);
       hipLaunchKernelGGL(k_backward, dim3(nblocks_), dim3(kAB), 0, s, A, xs, pid_s, cstart, d_->slot_of, d_->mu,
                          d_->lam, d_->part);
-      hipLaunchKernelGGL(k_gsum, dim3(1), dim3(64), 0, s, d_->part, nblocks_, nslots, d_->gsum);
+      hipLaunchKernelGGL(k_gsum, dim3(nslots), dim3(kAB), 0, s, d_->part, nblocks_, nslots, d_->gsum);
     }
   }
   hipLaunchKernelGGL(k_lam_out, dim3(nblocks_), dim3(kAB), 0, s, N_, d_->lam, gx0);
@@ -1100,7 +1127,8 @@ int morse_forces_static(int N, const double dim[3], const int cells[3], double c
   hipLaunchKernelGGL(k_count, dim3(nb), dim3(kAB), 0, s, A, xin, cellid, cnt);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, cnt, cstart, A.ncells);
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kAB), 0, s, A, cellid, cnt, pid_s);
-  hipLaunchKernelGGL(k_cellsort, dim3((A.ncells + kAB - 1) / kAB), dim3(kAB), 0, s, A, cstart, pid_s, xin, xs, cnt);
+  hipLaunchKernelGGL(k_cellsort, dim3((A.ncells + kCellsPerBlock - 1) / kCellsPerBlock), dim3(kAB), 0, s, A, cstart,
+                     pid_s, xin, xs, cnt);
   hipLaunchKernelGGL(k_forward, dim3(nb), dim3(kAB), 0, s, A, xs, pid_s, cstart, xin, xnext, (unsigned char *)nullptr,
                      f, epart);
   hipLaunchKernelGGL(k_esum, dim3(1), dim3(kAB), 0, s, epart, nb, epot);
