@@ -814,6 +814,11 @@ void single_body(Oracle &o, Part *p) {
   }
 }
 
+// DPD's 1/sqrt(dt): a function-static in the reference, fixed by the dt of
+// its first evaluation (dpd_eval.hpp:24, 74).  Before the first force phase
+// it is the value that phase will latch.
+float dpd_delta_of(const Oracle &o) { return o.dpd_delta_set ? o.dpd_delta : (float)(1.f / std::sqrt(o.dt_d)); }
+
 // potential_eval_ex in FP64 (the FP64 statement of the table potentials).
 bool table_ex64(const Pot &p, double ri, double rj, double r2, double *e, double *f) {
   double r = std::sqrt(r2);
@@ -874,7 +879,7 @@ void boundary_eval(const Oracle &o, const Cell &c, const Part &p, T *epot, Add a
       const T fc = (T)pot.dpd_alpha * omega_c;
       const T omega_d = omega_c * omega_c;
       const T fd = -(T)pot.dpd_gamma * omega_d * (e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
-      const T fr = (T)pot.dpd_sigma * omega_c * (T)o.dpd_delta;
+      const T fr = (T)pot.dpd_sigma * omega_c * (T)dpd_delta_of(o);
       const T f = fc + fd + fr;
       for (int k = 0; k < 3; ++k) add(k, f * e[k]);
     } else {
@@ -1383,7 +1388,7 @@ std::vector<Loc> locate(const Oracle &o) {
 // non-bonded tasks, bonds.
 void force(Oracle &o) {
   if (!o.dpd_delta_set) {
-    o.dpd_delta = 1.f / std::sqrt(o.dt_d);
+    o.dpd_delta = dpd_delta_of(o);
     o.dpd_delta_set = true;
   }
   double total = 0;
@@ -1535,7 +1540,7 @@ void forces64(const Oracle &o, double *F /* 3 * nparts, by id */, double *epot_o
       double sr = shifted ? r - ri - rj : r;
       double wc = sr < 0 ? 1.0 : 1 - sr / cut;
       double f = p.dpd_alpha * wc - p.dpd_gamma * wc * wc * (e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
-      if (!o.dpd_counter_noise) f += p.dpd_sigma * wc * (double)o.dpd_delta;
+      if (!o.dpd_counter_noise) f += p.dpd_sigma * wc * (double)dpd_delta_of(o);
       const double fi[3] = {f * e[0], f * e[1], f * e[2]};
       for (int k = 0; k < 3; ++k) {
         F[3 * (size_t)pi.id + k] += fi[k];

@@ -219,3 +219,28 @@ def test_philox_known_answers(oracle_mod):
     assert oracle_mod.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
     assert oracle_mod.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+@pytest.mark.parametrize("r", [0.35, 0.7, 0.95, 1.05])
+def test_two_body_dpd_shifted(m, oracle_mod, r):
+    """Shifted DPD (dpd_eval.hpp:21-69): cutoff b + R_i + R_j, omega_c from
+    r - R_i - R_j (1 inside the radii); with b = 0.6 and radii 0.2 the
+    cutoff is 1.0; a pair beyond it gets no force."""
+    alpha, gamma, sigma, dt, b, R = 25.0, 4.5, 3.0, 0.01, 0.6, 0.2
+    pot = m.Potential.dpd(alpha=alpha, gamma=gamma, sigma=sigma, cutoff=b, shifted=True)
+    d = np.array([0.6, -0.48, 0.64])
+    x1 = np.array([2.2, 2.5, 2.5])
+    x2 = x1 + r * d
+    v1 = np.array([0.3, -0.2, 0.1])
+    v2 = np.array([-0.1, 0.4, 0.0])
+    f, _ = two_body(m, oracle_mod, pot, x1, x2, v1, v2, L=5.0, cells=5, rc=1.2, radius=R, dt=dt)
+    cut = b + 2 * R
+    if r > cut:
+        assert np.abs(f).max() == 0.0
+        return
+    e = (x1 - x2) / r
+    sr = r - 2 * R
+    w = 1.0 if sr < 0 else 1 - sr / cut
+    fs = alpha * w - gamma * w * w * (e @ (v1 - v2)) + sigma * w / math.sqrt(dt)
+    assert np.allclose(f[0], fs * e, rtol=1e-5, atol=1e-6)
+    assert np.allclose(f[1], -fs * e, rtol=1e-5, atol=1e-6)

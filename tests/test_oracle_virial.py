@@ -68,3 +68,24 @@ def test_periodic_translation_invariance(m, oracle_mod):
     W0 = oracle_mod.from_workload(system(x), m).virial64()
     W1 = oracle_mod.from_workload(system((x + [2.3, 5.1, 7.7]) % 8.0), m).virial64()
     assert np.allclose(W0, W1, rtol=0, atol=1e-5 * np.abs(W0).max())
+
+
+def test_dpd_sigma_term_before_first_force(m, oracle_mod):
+    """DPD with the reference's deterministic sigma term (sigma * w / sqrt(dt),
+    dpd_eval.hpp:24): in a walled box the FP64 virial taken before any force
+    phase equals sum_i x_i (x) f_i of the FP32 force phase that follows
+    (itself pinned pair by pair in test_oracle.py), and is unchanged by it."""
+    rng = np.random.default_rng(9)
+    N = 900
+    x = W._min_dist_fill(rng, N, 8.0, 0.3)
+    w = system(x, kinds=[4] * 6, pots=(("dpd", (25.0, 4.5, 3.0, 1.0, 0), 0, 0),))
+    w["v"] = rng.normal(size=(N, 3))
+    o = oracle_mod.from_workload(w, m)
+    W0 = o.virial64()
+    o.force()
+    W1 = o.virial64()
+    f = o.state()["f"].astype(np.float64)
+    ref = o.global_positions().T @ f
+    assert np.abs(W0).max() > 1.0
+    assert np.array_equal(W0, W1)
+    assert np.allclose(W0, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
