@@ -419,7 +419,7 @@ int mxg_adjoint_rdf_backward(mxg_adjoint *a, const double *x, int nbins, double 
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  mxg_adjoint_forces without an mxg_adjoint and without
  * allocating: the cell binning, sorted positions and energy partials live in
- * a file-scope device array of about 7.5 MB per GPU, held until the process
+ * a file-scope device array of about 8.4 MB per GPU, held until the process
  * ends.  N <= 131072 particles and cells[0]*cells[1]*cells[2] <= 131072
  * (larger sizes return MXG_ERR_RANGE); cells of at least 3 per axis, no
  * narrower than the cutoff (else MXG_ERR_CUTOFF).  type (N int32, by id), x

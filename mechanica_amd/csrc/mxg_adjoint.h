@@ -14,7 +14,7 @@ constexpr int kAdjMaxBins = 256;
 // Gradient slots: (d, a, r0) or (A, B, -) per type pair, then the bond
 // potential's (K, r0, -).
 constexpr int kAdjSlots = 3 * kAdjMaxPairs + 3;
-// Limits of morse_forces_static's file-scope scratch (about 7.5 MB per GPU).
+// Limits of morse_forces_static's file-scope scratch (about 8.4 MB per GPU).
 constexpr int kStaticMaxParticles = 131072;
 constexpr int kStaticMaxCells = 131072;
 
@@ -67,7 +67,7 @@ class Adjoint {
   void bin(const void *x, int slot, void *stream);
 
   AdjointDev *d_ = nullptr;
-  int N_ = 0, ntypes_ = 0, ncells_ = 0, ws_cap_ = 0, nblocks_ = 0, M_ = 0;
+  int N_ = 0, ntypes_ = 0, ncells_ = 0, ws_cap_ = 0, nblocks_ = 0, nlanes_ = 0, M_ = 0;
   int kind_[kAdjMaxPairs] = {};
   bool dom_[kAdjMaxPairs] = {};
   double pmin_[kAdjMaxPairs] = {}, pmax_[kAdjMaxPairs] = {};

@@ -97,11 +97,26 @@ __device__ __forceinline__ int cell_of(const AdjArgs &A, float4 x, int &cx, int 
 
 __device__ __forceinline__ int type_of(float4 x) { return __float_as_int(x.w); }
 
+// Lanes per particle in k_forward and k_backward: lane `sub` of a particle's
+// group walks candidates sub, sub + kLPP, ... of every neighbour cell, and the
+// group's sums are combined by an xor butterfly (every lane ends with the
+// same bits), so the result depends on kLPP but not on scheduling.
+constexpr int kLPP = 4;
+
+// Sum over a particle's kLPP lanes (aligned groups within a wave).
+template <typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = 1; o < kLPP; o <<= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 // Calls fn(j, dx, dy, dz, r2) for every particle j != i (sorted index) of
-// the 27 cells around x_i with r^2 <= rc^2, cells in offset order.
+// the 27 cells around x_i with r^2 <= rc^2, cells in offset order, taking
+// the candidates sub, sub + stride, ... of each cell.
 template <typename Fn>
 __device__ __forceinline__ void neighbours(const AdjArgs &A, const float4 *__restrict__ xs, const int *__restrict__ cstart, int i,
-                                           float4 xi, Fn &&fn) {
+                                           float4 xi, int sub, int stride, Fn &&fn) {
   int cx, cy, cz;
   cell_of(A, xi, cx, cy, cz);
   for (int ox = -1; ox <= 1; ++ox) {
@@ -136,7 +151,7 @@ __device__ __forceinline__ void neighbours(const AdjArgs &A, const float4 *__res
         }
         const int c = (nx * A.cdim[1] + ny) * A.cdim[2] + nz;
         const int j1 = cstart[c + 1];
-        for (int j = cstart[c]; j < j1; ++j) {
+        for (int j = cstart[c] + sub; j < j1; j += stride) {
           if (j == i) continue;
           const float4 xj = xs[j];
           const float dx = xi.x - (xj.x + sx), dy = xi.y - (xj.y + sy), dz = xi.z - (xj.z + sz);
@@ -337,19 +352,23 @@ __global__ __launch_bounds__(kAB) void k_slot_of(int N, const int *__restrict__ 
 
 // ---------------------------------------------------------------- forward
 
-// One overdamped step: forces, clamp, advance.  Optional outputs: the force
-// by id (f), the clamp bits by id (mask), and per-block FP64 energy sums.
+// One overdamped step: forces, clamp, advance; kLPP lanes per particle (the
+// grid covers N * kLPP threads).  Optional outputs: the force by id (f), the
+// clamp bits by id (mask), and per-block FP64 energy sums.
 __global__ __launch_bounds__(kAB) void k_forward(AdjArgs A, const float4 *__restrict__ xs, const int *__restrict__ pid_s,
                                                  const int *__restrict__ cstart, const float4 *__restrict__ xid,
                                                  float4 *__restrict__ x_next, unsigned char *__restrict__ mask,
                                                  float *__restrict__ f, double *__restrict__ epart) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / kLPP, sub = t % kLPP;
   double e = 0.0;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  float4 xi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  int pid = 0;
   if (i < A.N) {
-    const float4 xi = xs[i];
+    xi = xs[i];
     const int ti = type_of(xi);
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    neighbours(A, xs, cstart, i, xi, [&](int, float4 xj, float dx, float dy, float dz, float r2) __attribute__((always_inline)) {
+    neighbours(A, xs, cstart, i, xi, sub, kLPP, [&](int, float4 xj, float dx, float dy, float dz, float r2) __attribute__((always_inline)) {
       const AdjPair &P = A.p[pair_index(ti, type_of(xj), A.ntypes)];
       PairEval m;
       if (!pair_eval(P, r2, m)) return;
@@ -359,10 +378,11 @@ __global__ __launch_bounds__(kAB) void k_forward(AdjArgs A, const float4 *__rest
       fz += phi * dz;
       if (epart) e += 0.5 * (double)m.e;
     });
-    const int pid = pid_s[i];
+    pid = pid_s[i];
     if (A.boff) {
-      // Bonds of this particle, in list order (partners by id).
-      for (int k = A.boff[pid]; k < A.boff[pid + 1]; ++k) {
+      // Bonds of this particle, in list order (partners by id), split over
+      // the group's lanes like the candidates.
+      for (int k = A.boff[pid] + sub; k < A.boff[pid + 1]; k += kLPP) {
         const int ent = A.bent[k];
         BondEval b;
         bond_eval(A, xi, xid[ent >> 1], b);
@@ -375,6 +395,13 @@ __global__ __launch_bounds__(kAB) void k_forward(AdjArgs A, const float4 *__rest
         }
       }
     }
+  }
+  // Groups are aligned and share i, so whole groups are in or out of range.
+  fx = group_sum(fx);
+  fy = group_sum(fy);
+  fz = group_sum(fz);
+  if (i < A.N && sub == 0) {
+    const int ti = type_of(xi);
     const float im = A.imass[ti];
     const float fv[3] = {fx, fy, fz};
     const float xv[3] = {xi.x, xi.y, xi.z};
@@ -464,23 +491,25 @@ __global__ __launch_bounds__(kAB) void k_zero(double *__restrict__ p, int n) {
 }
 
 // lambda_t = lambda_{t+1} + sum_j H_ij (mu_i - mu_j), and per-block sums of
-// the parameter gradient (each pair once: sorted index i < j).
+// the parameter gradient (each pair once: sorted index i < j); kLPP lanes
+// per particle, as in k_forward.
 __global__ __launch_bounds__(kAB) void k_backward(AdjArgs A, const float4 *__restrict__ xs, const int *__restrict__ pid_s,
                                                   const int *__restrict__ cstart, const int *__restrict__ slot_of,
                                                   const double4 *__restrict__ mu,
                                                   double4 *__restrict__ lam, double *__restrict__ gpart) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int th = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = th / kLPP, sub = th % kLPP;
   double gb0 = 0.0, gb1 = 0.0;  // bond K, r0
   double g[kAdjMaxTypes][3];
 #pragma unroll
   for (int t = 0; t < kAdjMaxTypes; ++t) g[t][0] = g[t][1] = g[t][2] = 0.0;
-  int ti = 0;
+  int ti = 0, pid = 0;
+  double ax = 0.0, ay = 0.0, az = 0.0;
   if (i < A.N) {
     const float4 xi = xs[i];
     ti = type_of(xi);
     const double4 mi = mu[i];
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    neighbours(A, xs, cstart, i, xi, [&](int j, float4 xj, float dx, float dy, float dz, float r2) __attribute__((always_inline)) {
+    neighbours(A, xs, cstart, i, xi, sub, kLPP, [&](int j, float4 xj, float dx, float dy, float dz, float r2) __attribute__((always_inline)) {
       const int tj = type_of(xj);
       const AdjPair &P = A.p[pair_index(ti, tj, A.ntypes)];
       PairEval m;
@@ -512,9 +541,9 @@ __global__ __launch_bounds__(kAB) void k_backward(AdjArgs A, const float4 *__res
 #undef MXG_ADJ_ACC
       }
     });
-    const int pid = pid_s[i];
+    pid = pid_s[i];
     if (A.boff) {
-      for (int k = A.boff[pid]; k < A.boff[pid + 1]; ++k) {
+      for (int k = A.boff[pid] + sub; k < A.boff[pid + 1]; k += kLPP) {
         const int ent = A.bent[k];
         const int sj = slot_of[ent >> 1];
         BondEval b;
@@ -533,6 +562,11 @@ __global__ __launch_bounds__(kAB) void k_backward(AdjArgs A, const float4 *__res
         }
       }
     }
+  }
+  ax = group_sum(ax);
+  ay = group_sum(ay);
+  az = group_sum(az);
+  if (i < A.N && sub == 0) {
     double4 l = lam[pid];
     l.x += ax;
     l.y += ay;
@@ -616,7 +650,7 @@ __global__ __launch_bounds__(kAB) void k_rdf(AdjArgs A, const float4 *__restrict
   if (i < A.N) {
     const float4 xi = xs[i];
     const int ti = type_of(xi);
-    neighbours(A, xs, cstart, i, xi, [&](int j, float4 xj, float, float, float, float r2) __attribute__((always_inline)) {
+    neighbours(A, xs, cstart, i, xi, 0, 1, [&](int j, float4 xj, float, float, float, float r2) __attribute__((always_inline)) {
       if (j < i || r2 >= rmax2) return;
       const int p = pair_index(ti, type_of(xj), A.ntypes);
       int k0;
@@ -654,7 +688,7 @@ __global__ __launch_bounds__(kAB) void k_rdf_backward(AdjArgs A, const float4 *_
   const float4 xi = xs[i];
   const int ti = type_of(xi);
   double ax = 0.0, ay = 0.0, az = 0.0;
-  neighbours(A, xs, cstart, i, xi, [&](int, float4 xj, float dx, float dy, float dz, float r2) __attribute__((always_inline)) {
+  neighbours(A, xs, cstart, i, xi, 0, 1, [&](int, float4 xj, float dx, float dy, float dz, float r2) __attribute__((always_inline)) {
     if (r2 >= rmax2 || r2 <= 0.0f) return;
     const int p = pair_index(ti, type_of(xj), A.ntypes);
     const float r = sqrtf(r2);
@@ -721,6 +755,7 @@ int Adjoint::create(int N, const double dim[3], const int cells[3], double cutof
                     int ntypes, const double *mass, const double *radius, const int *type) {
   if (d_) return fail(-1, "already created");
   if (N <= 0 || !dim || !cells || !mass || !radius || !type) return fail(-1, "missing argument");
+  if (N > INT_MAX / kLPP) return fail(-6, "too many particles for one thread group each");
   if (ntypes < 1 || ntypes > kAdjMaxTypes) return fail(-6, "between 1 and 4 particle types");
   for (int k = 0; k < 3; ++k) {
     if (cells[k] < 3) return fail(-6, "a periodic grid needs at least 3 cells per axis");
@@ -737,6 +772,7 @@ int Adjoint::create(int N, const double dim[3], const int cells[3], double cutof
   radius_.assign(radius, radius + ntypes);
   ncells_ = cells[0] * cells[1] * cells[2];
   nblocks_ = blocks(N);
+  nlanes_ = blocks(N * kLPP);
   d_ = new AdjointDev();
   AdjArgs &A = d_->args;
   std::memset(&A, 0, sizeof(A));
@@ -768,7 +804,7 @@ int Adjoint::create(int N, const double dim[3], const int cells[3], double cutof
   st |= hipMalloc(&d_->lam, N * sizeof(double4));
   st |= hipMalloc(&d_->mu, N * sizeof(double4));
   st |= hipMalloc(&d_->slot_of, N * sizeof(int));
-  st |= hipMalloc(&d_->part, (size_t)nblocks_ * kAdjSlots * sizeof(double));
+  st |= hipMalloc(&d_->part, (size_t)nlanes_ * kAdjSlots * sizeof(double));
   st |= hipMalloc(&d_->gsum, kAdjSlots * sizeof(double));
   st |= hipMalloc(&d_->hist, kAdjMaxPairs * kAdjMaxBins * sizeof(unsigned long long));
   if (st) return device(st, "hipMalloc");
@@ -904,7 +940,7 @@ int Adjoint::forward(const double *theta, double smin, double smax, const double
   for (int t = 0; t < nsteps; ++t) {
     if (t % every == 0) hipLaunchKernelGGL(k_copy4, dim3(nblocks_), dim3(kAB), 0, s, N_, xa, ck + (size_t)(t / every) * N_);
     bin(xa, 0, stream);
-    hipLaunchKernelGGL(k_forward, dim3(nblocks_), dim3(kAB), 0, s, A, d_->xs, d_->pid_s, d_->cstart, xa, xb,
+    hipLaunchKernelGGL(k_forward, dim3(nlanes_), dim3(kAB), 0, s, A, d_->xs, d_->pid_s, d_->cstart, xa, xb,
                        (unsigned char *)nullptr, (float *)nullptr, (double *)nullptr);
     std::swap(xa, xb);
   }
@@ -932,7 +968,7 @@ int Adjoint::backward(const double *theta, double smin, double smax, const float
     // Recompute the segment, keeping each step's binning and clamp bits.
     for (int q = 0; q < n; ++q) {
       bin(xa, q, stream);
-      hipLaunchKernelGGL(k_forward, dim3(nblocks_), dim3(kAB), 0, s, A, d_->xs + (size_t)q * N_,
+      hipLaunchKernelGGL(k_forward, dim3(nlanes_), dim3(kAB), 0, s, A, d_->xs + (size_t)q * N_,
                          d_->pid_s + (size_t)q * N_, d_->cstart + (size_t)q * (ncells_ + 1), xa, xb,
                          d_->mask + (size_t)q * N_, (float *)nullptr, (double *)nullptr);
       std::swap(xa, xb);
@@ -944,9 +980,9 @@ int Adjoint::backward(const double *theta, double smin, double smax, const float
       hipLaunchKernelGGL(k_mu, dim3(nblocks_), dim3(kAB), 0, s, A, xs, pid_s, d_->mask + (size_t)q * N_, d_->lam,
                          d_->mu);
       if (M_ > 0) hipLaunchKernelGGL(k_slot_of, dim3(nblocks_), dim3(kAB), 0, s, N_, pid_s, d_->slot_of);
-      hipLaunchKernelGGL(k_backward, dim3(nblocks_), dim3(kAB), 0, s, A, xs, pid_s, cstart, d_->slot_of, d_->mu,
+      hipLaunchKernelGGL(k_backward, dim3(nlanes_), dim3(kAB), 0, s, A, xs, pid_s, cstart, d_->slot_of, d_->mu,
                          d_->lam, d_->part);
-      hipLaunchKernelGGL(k_gsum, dim3(nslots), dim3(kAB), 0, s, d_->part, nblocks_, nslots, d_->gsum);
+      hipLaunchKernelGGL(k_gsum, dim3(nslots), dim3(kAB), 0, s, d_->part, nlanes_, nslots, d_->gsum);
     }
   }
   hipLaunchKernelGGL(k_lam_out, dim3(nblocks_), dim3(kAB), 0, s, N_, d_->lam, gx0);
@@ -965,10 +1001,10 @@ int Adjoint::forces(const double *theta, double smin, double smax, const double 
   const AdjArgs &A = d_->args;
   hipLaunchKernelGGL(k_init, dim3(nblocks_), dim3(kAB), 0, s, A, x, d_->type, d_->x[0]);
   bin(d_->x[0], 0, stream);
-  hipLaunchKernelGGL(k_forward, dim3(nblocks_), dim3(kAB), 0, s, A, d_->xs, d_->pid_s, d_->cstart, d_->x[0],
+  hipLaunchKernelGGL(k_forward, dim3(nlanes_), dim3(kAB), 0, s, A, d_->xs, d_->pid_s, d_->cstart, d_->x[0],
                      d_->x[1], (unsigned char *)nullptr, f, d_->part);
-  std::vector<double> part(nblocks_);
-  int st = hipMemcpyAsync(part.data(), d_->part, nblocks_ * sizeof(double), hipMemcpyDeviceToHost, s);
+  std::vector<double> part(nlanes_);
+  int st = hipMemcpyAsync(part.data(), d_->part, nlanes_ * sizeof(double), hipMemcpyDeviceToHost, s);
   st |= hipStreamSynchronize(s);
   if (st) return device(st, "forces");
   double e = 0.0;
@@ -1018,7 +1054,7 @@ namespace {
 // address comes from hipGetSymbolAddress, once per device.
 constexpr size_t kStaticPartBytes = (size_t)kStaticMaxParticles * (3 * sizeof(float4) + 2 * sizeof(int));
 constexpr size_t kStaticCellBytes = (size_t)(2 * kStaticMaxCells + 1) * sizeof(int);
-constexpr size_t kStaticPartials = (kStaticMaxParticles + kAB - 1) / kAB;
+constexpr size_t kStaticPartials = ((size_t)kStaticMaxParticles * kLPP + kAB - 1) / kAB;  // k_forward's blocks
 constexpr size_t kStaticBytes = kStaticPartBytes + (kStaticCellBytes + 255) / 256 * 256 + kStaticPartials * sizeof(double);
 static __device__ __align__(256) unsigned char scratch[kStaticBytes];
 
@@ -1129,9 +1165,10 @@ int morse_forces_static(int N, const double dim[3], const int cells[3], double c
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kAB), 0, s, A, cellid, cnt, pid_s);
   hipLaunchKernelGGL(k_cellsort, dim3((A.ncells + kCellsPerBlock - 1) / kCellsPerBlock), dim3(kAB), 0, s, A, cstart,
                      pid_s, xin, xs, cnt);
-  hipLaunchKernelGGL(k_forward, dim3(nb), dim3(kAB), 0, s, A, xs, pid_s, cstart, xin, xnext, (unsigned char *)nullptr,
+  const int nlb = (N * kLPP + kAB - 1) / kAB;
+  hipLaunchKernelGGL(k_forward, dim3(nlb), dim3(kAB), 0, s, A, xs, pid_s, cstart, xin, xnext, (unsigned char *)nullptr,
                      f, epart);
-  hipLaunchKernelGGL(k_esum, dim3(1), dim3(kAB), 0, s, epart, nb, epot);
+  hipLaunchKernelGGL(k_esum, dim3(1), dim3(kAB), 0, s, epart, nlb, epot);
   return hipGetLastError() == hipSuccess ? 0 : -21;
 }
 
