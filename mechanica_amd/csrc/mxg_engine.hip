@@ -1929,7 +1929,11 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
               bool full = false;
               // Candidates go in chunks of kChunk: their LDS loads are issued
               // together, and the list is flushed before a chunk that might
-              // not fit, so the visiting order is unchanged.
+              // not fit, so the visiting order is unchanged.  A chunk's tail
+              // past the cell reads the next staged particles (or, past the
+              // last, the next LDS arrays of this block: at most kChunk - 1
+              // entries beyond hx); those entries are never kept (q + u < qe),
+              // and unclamped indices let the chunk share one address.
               if (oside == 0) {
                 for (; q < qe; q += kChunk) {
                   if (nn > B.cap - kChunk) {
@@ -1938,7 +1942,7 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                   }
                   float4 xq[kChunk];
 #pragma unroll
-                  for (int u = 0; u < kChunk; ++u) xq[u] = hx[min(q + u, qe - 1)];
+                  for (int u = 0; u < kChunk; ++u) xq[u] = hx[q + u];
 #pragma unroll
                   for (int u = 0; u < kChunk; ++u) {
                     const float dx = xs.x - xq[u].x, dy = xs.y - xq[u].y, dz = xs.z - xq[u].z;
@@ -1969,7 +1973,7 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                     }
                     float4 xq[kChunk];
 #pragma unroll
-                    for (int u = 0; u < kChunk; ++u) xq[u] = hx[min(q + u, qe - 1)];
+                    for (int u = 0; u < kChunk; ++u) xq[u] = hx[q + u];
 #pragma unroll
                     for (int u = 0; u < kChunk; ++u) {
                       const float dx = pix - xq[u].x, dy = piy - xq[u].y, dz = piz - xq[u].z;
@@ -1986,7 +1990,7 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                     }
                     float4 xq[kChunk];
 #pragma unroll
-                    for (int u = 0; u < kChunk; ++u) xq[u] = hx[min(q + u, qe - 1)];
+                    for (int u = 0; u < kChunk; ++u) xq[u] = hx[q + u];
 #pragma unroll
                     for (int u = 0; u < kChunk; ++u) {
                       const float dx = (xq[u].x - sx) - xs.x, dy = (xq[u].y - sy) - xs.y, dz = (xq[u].z - sz) - xs.z;
