@@ -1892,22 +1892,35 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
           };
           // Gather in-cutoff neighbours into the per-thread list, process
           // the list when it is full or the scan is done (one call site, so
-          // the pair code is instantiated once).
+          // the pair code is instantiated once).  The control flow is the
+          // wave's: every lane visits the offsets in the same order, a chunk
+          // loop runs while any lane has candidates left in its cell (a lane
+          // that is done keeps its position and appends nothing), and the
+          // whole wave flushes when any lane's list might not hold the next
+          // chunk.  Flushing early does not change a lane's list order, so
+          // each particle's pair order is the same as a per-lane loop's.
           int o = 0, q = -1;
           const int hm = R.drop_check ? hmask[h] : 0;  // this cell's near-bound bits
           for (;;) {
             int nn = 0;
+            bool full = false;
             while (o < R.n_off) {
-              const int4 off = hoff[o];
+              // The offset is the same for the whole wave: its descriptor
+              // goes to scalar registers.
+              const int4 offv = hoff[o];
+              const int4 off = make_int4(__builtin_amdgcn_readfirstlane(offv.x), __builtin_amdgcn_readfirstlane(offv.y),
+                                         __builtin_amdgcn_readfirstlane(offv.z), 0);
               const int h2 = h + off.x;
               const unsigned geo = (unsigned)off.y;
               const int oside = (int)((geo >> 9) & 3u) - 1;
-              const int qb = hstart[h2], qe = hstart[h2 + 1];
+              const int qe = hstart[h2 + 1];
               if (q < 0) {
+                const int qb = hstart[h2];
                 q = qb;
                 // A cell pair that runner_dopair drops whole: only when both
                 // cells hold particles near the bounds the offset's keys
-                // favour (rare), the exact key test.
+                // favour (rare), the exact key test; a dropped pair leaves
+                // this lane nothing to scan in the cell.
                 if ((off.z & kOffDrop) && (hm & off.z & 0x3f) == (off.z & 0x3f)) {
                   const int need = (off.z >> 8) & 0x3f;
                   if ((hmask[h2] & need) == need) {
@@ -1918,25 +1931,22 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                     offset_dir(ox, oy, oz, oside, nx, ny, nz);
                     const int hb = hstart[h], he = hstart[h + 1];
                     if (oside > 0 ? pair_dropped(R, nx, ny, nz, sx, sy, sz, hx, hb, he, qb, qe)
-                                  : pair_dropped(R, nx, ny, nz, sx, sy, sz, hx, qb, qe, hb, he)) {
-                      ++o;
-                      q = -1;
-                      continue;
-                    }
+                                  : pair_dropped(R, nx, ny, nz, sx, sy, sz, hx, qb, qe, hb, he))
+                      q = qe;
                   }
                 }
               }
-              bool full = false;
               // Candidates go in chunks of kChunk: their LDS loads are issued
               // together, and the list is flushed before a chunk that might
-              // not fit, so the visiting order is unchanged.  A chunk's tail
-              // past the cell reads the next staged particles (or, past the
-              // last, the next LDS arrays of this block: at most kChunk - 1
-              // entries beyond hx); those entries are never kept (q + u < qe),
-              // and unclamped indices let the chunk share one address.
+              // not fit.  A chunk's tail past the cell reads the next staged
+              // particles (or, past the last, the next LDS arrays of this
+              // block: a lane that is done stays at most kChunk - 1 entries
+              // past its cell, so reads end 2 kChunk - 2 entries beyond hx);
+              // those entries are never kept (q + u < qe), and unclamped
+              // indices let the chunk share one address.
               if (oside == 0) {
-                for (; q < qe; q += kChunk) {
-                  if (nn > B.cap - kChunk) {
+                while (__any(q < qe)) {
+                  if (__any(nn > B.cap - kChunk)) {
                     full = true;
                     break;
                   }
@@ -1953,6 +1963,7 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                     nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
                     nn += (q + u < qe && q + u != p && r2 <= C.cut2_self) ? 1 : 0;
                   }
+                  q += q < qe ? kChunk : 0;
                 }
               } else {
                 float sx, sy, sz;
@@ -1966,8 +1977,8 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                 // of runner_dopair is applied when the list is processed.
                 if (oside > 0) {
                   const float pix = xs.x - sx, piy = xs.y - sy, piz = xs.z - sz;
-                  for (; q < qe; q += kChunk) {
-                    if (nn > B.cap - kChunk) {
+                  while (__any(q < qe)) {
+                    if (__any(nn > B.cap - kChunk)) {
                       full = true;
                       break;
                     }
@@ -1981,10 +1992,11 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                       nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
                       nn += (q + u < qe && r2 <= C.cut2_pair) ? 1 : 0;
                     }
+                    q += q < qe ? kChunk : 0;
                   }
                 } else {
-                  for (; q < qe; q += kChunk) {
-                    if (nn > B.cap - kChunk) {
+                  while (__any(q < qe)) {
+                    if (__any(nn > B.cap - kChunk)) {
                       full = true;
                       break;
                     }
@@ -1998,6 +2010,7 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                       nlist[nn * kBT + tid] = ((unsigned)(q + u) << 12) | geo;
                       nn += (q + u < qe && r2 <= C.cut2_pair) ? 1 : 0;
                     }
+                    q += q < qe ? kChunk : 0;
                   }
                 }
               }
@@ -2006,7 +2019,7 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
               q = -1;
             }
             process(nn);
-            if (o >= R.n_off) break;
+            if (!full) break;
           }
         }
         if (MODE & kModeBonds) {
