@@ -61,15 +61,19 @@ BYTES_PER_TERM_ENTRY = 16
 BYTES_TERM_OFFSET = 4
 
 # HBM traffic per force-kernel launch measured with rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE on the default workload (DPD, 1M particles),
-# scaled by the copy-kernel calibration (bench.py --copy-calibration under
-# the same counters); the summary and its commands are kept in
-# profiles/k_brick_dpd1m_counters.md (bytes per launch).
-MEASURED_TRAFFIC = {("dpd", 1048576): 111.9e6}
+# FETCH_SIZE and WRITE_SIZE on the 1M DPD and LJ workloads at the brick the
+# autotuner picks for them ([5,5,3] list 12 and [6,4,1] list 16), scaled by
+# the copy-kernel calibration (bench.py --copy-calibration under the same
+# counters); the summaries and their commands are kept in
+# profiles/k_brick_dpd1m_counters.md and profiles/k_brick_lj1m_counters.md
+# (bytes per launch).
+MEASURED_TRAFFIC = {("dpd", 1048576): 112.6e6, ("lj", 1048576): 128.7e6}
 # Wave64 VALU instructions per force-kernel launch (rocprofv3 --pmc
-# SQ_INSTS_VALU on the same workload and build; profiles/*_valu.md).  A
+# SQ_INSTS_VALU on the same workloads, bricks and build;
+# profiles/k_brick_dpd1m_valu.md, profiles/k_brick_lj1m_counters.md).  A
 # wave64 FP32 instruction holds a 16-lane SIMD for 4 cycles.
-MEASURED_VALU = {("dpd", 1048576): 202.1e6}
+MEASURED_VALU = {("dpd", 1048576): 175.7e6, ("lj", 1048576): 587.1e6}
+MEASURED_BRICK = {("dpd", 1048576): [[5, 5, 3], 12], ("lj", 1048576): [[6, 4, 1], 16]}
 
 METRIC = "particle-steps/s (1M LJ/DPD)"
 
@@ -238,6 +242,9 @@ def bench_single(args, threads, workload=None):
                     valu=MEASURED_VALU.get((workload, args.n)))
     roof["step_ms_events"] = step_ms
     roof["unique_pairs"] = pairs
+    if (workload, args.n) in MEASURED_BRICK:
+        # The counters were read at one layout; the autotuner may pick another.
+        roof["counters_brick"] = MEASURED_BRICK[(workload, args.n)]
     cpu = None if args.no_cpu_baseline else cpu_baseline_oracle(w, args.cpu_seconds, threads,
                                                                 dpd_noise=bool(w.get("dpd_noise")))
     config = {"workload": cfg % ((N,) + tuple(w["cells"])), "particles": N, "cells": w["cells"],

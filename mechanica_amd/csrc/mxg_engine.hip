@@ -4061,9 +4061,12 @@ int Engine::autotune(int steps_per, int *taken) {
   if (!d_ || N_ == 0) return fail(kErrNull, "no particles loaded");
   if (steps_per < 1) return reject(kErrRange, "autotune needs at least one step per candidate");
   // Shapes whose home-particle count fills the 256-thread block at common
-  // densities, and the power-of-two series for dense or small systems.
-  static const int shapes[][3] = {{6, 4, 4}, {5, 5, 4}, {5, 4, 4}, {4, 4, 4}, {4, 4, 3}, {4, 3, 3}, {3, 3, 3},
-                                  {4, 4, 2}, {4, 2, 2}, {3, 3, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
+  // densities (the measured best for DPD at density 3, [5,5,3], and for LJ
+  // at density 10, [6,4,1], among them), and the power-of-two series for
+  // dense or small systems.
+  static const int shapes[][3] = {{6, 4, 4}, {5, 5, 4}, {5, 4, 4}, {5, 5, 3}, {6, 4, 3}, {7, 5, 2}, {6, 6, 2},
+                                  {4, 4, 4}, {4, 4, 3}, {4, 3, 3}, {3, 3, 3}, {4, 4, 2}, {6, 4, 1}, {6, 2, 2},
+                                  {5, 2, 2}, {4, 2, 2}, {3, 3, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
   static const int caps[] = {8, 12, 16, 24};
   const bool was_timing = timing_;
   int best[4] = {bdim_[0], bdim_[1], bdim_[2], cap_};
