@@ -1901,15 +1901,17 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
           // each particle's pair order is the same as a per-lane loop's.
           int o = 0, q = -1;
           const int hm = R.drop_check ? hmask[h] : 0;  // this cell's near-bound bits
+          int4 offn = hoff[0];  // descriptor of offset o, read one offset ahead
           for (;;) {
             int nn = 0;
             bool full = false;
             while (o < R.n_off) {
               // The offset is the same for the whole wave: its descriptor
-              // goes to scalar registers.
-              const int4 offv = hoff[o];
-              const int4 off = make_int4(__builtin_amdgcn_readfirstlane(offv.x), __builtin_amdgcn_readfirstlane(offv.y),
-                                         __builtin_amdgcn_readfirstlane(offv.z), 0);
+              // goes to scalar registers.  The next one is read now, off
+              // the path of this offset's cell bounds.
+              const int4 off = make_int4(__builtin_amdgcn_readfirstlane(offn.x), __builtin_amdgcn_readfirstlane(offn.y),
+                                         __builtin_amdgcn_readfirstlane(offn.z), 0);
+              offn = hoff[min(o + 1, R.n_off - 1)];
               const int h2 = h + off.x;
               const unsigned geo = (unsigned)off.y;
               const int oside = (int)((geo >> 9) & 3u) - 1;
@@ -2020,6 +2022,7 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
             }
             process(nn);
             if (!full) break;
+            offn = hoff[o];  // resume at the same offset
           }
         }
         if (MODE & kModeBonds) {

@@ -143,10 +143,13 @@ def layout_workload(name, seed):
 
 
 @pytest.mark.parametrize("name", ["argon", "dpd"])
-@pytest.mark.parametrize("brick", [(1, 1, 1, 0, 0), (2, 2, 2, 0, 0), (4, 2, 1, 8, 0), (2, 2, 2, 0, 64)])
+@pytest.mark.parametrize("brick", [(1, 1, 1, 0, 0), (2, 2, 2, 0, 0), (4, 2, 1, 8, 0), (2, 2, 2, 0, 64),
+                                   (3, 3, 2, 3, 0), (5, 5, 3, 12, 0)])
 def test_brick_layout_bitwise(m, brick, name):
     """Brick shapes, list capacities and the global fallback (pmax = 64
-    sends nearly every brick there) give the same per-particle results."""
+    sends nearly every brick there) give the same per-particle results.
+    Capacity 3 makes the wave flush its lists before almost every chunk of
+    the scan."""
     w = layout_workload(name, 45)
     print("case: brick", name, brick, flush=True)
     a, oa, _ = run_steps(m, w, 6, 5)
