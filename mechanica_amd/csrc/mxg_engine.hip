@@ -393,6 +393,13 @@ __device__ __forceinline__ float3 overlap_dir(uint32_t pi, uint32_t pj, uint32_t
   return make_float3(g0 * len / l, g1 * len / l, g2 * len / l);
 }
 
+// x / y to about one ulp from the hardware reciprocal (v_rcp_f32).  This
+// build passes -fhip-fp32-correctly-rounded-divide-sqrt, which makes both `/`
+// and __fdividef the IEEE division sequence (about ten instructions); the
+// pair forces and the number density only have to agree to 1e-5 and 1e-3,
+// while r^2, the cutoff tests and the advance stay exact.
+__device__ __forceinline__ float div_1ulp(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+
 // potential_eval_ex (potential_eval.hpp:106-161).
 __device__ __forceinline__ bool table_eval_ex(const DevPot &P, const float *__restrict__ coef, float ri,
                                               float rj, float r_in, float &e, float &f) {
@@ -421,9 +428,7 @@ __device__ __forceinline__ bool table_eval_ex(const DevPot &P, const float *__re
   eff = eff * x + ee;
   ee = ee * x + c1.w;
   e = ee;
-  // One-ulp division: the force only has to agree to 1e-5; r^2, the cutoff
-  // test and the advance stay exact.
-  f = __fdividef(eff * c0.y, ro);
+  f = div_1ulp(eff * c0.y, ro);
   return true;
 }
 
@@ -489,7 +494,7 @@ __device__ __forceinline__ int near_bits(const DevReplica &R, const float4 &p) {
 // Cubic-spline number-density kernel W(r^2, h) (smoothing_kernel.hpp:28-50)
 // with the exact square root (the reference uses an approximate x86 rsqrt).
 __device__ __forceinline__ float w_cubic(float r, float h, float inv_norm) {
-  const float x = __fdividef(r, h);
+  const float x = div_1ulp(r, h);
   float y;
   if (x < 1.f) {
     const float x2 = x * x;
@@ -628,7 +633,7 @@ __device__ __forceinline__ bool pair_force(const LoopConsts &A, const PairTabs &
     // branch; the flux mode keeps the branch, where it measures faster).
     const bool in_range = r2 <= cutoff * cutoff;
     if ((MODE & kModeFlux) && !in_range) return false;
-    const float ir = __fdividef(1.0f, r);
+    const float ir = __builtin_amdgcn_rcpf(r);
     float ex = dxx * ir, ey = dxy * ir, ez = dxz * ir;
     float4 vq = nb.v(q);
     float vx, vy, vz;
@@ -642,7 +647,7 @@ __device__ __forceinline__ bool pair_force(const LoopConsts &A, const PairTabs &
       vz = vq.z - vs.z;
     }
     float shifted_r = shifted ? r - ri - rj : r;
-    float omega_c = shifted_r < 0.f ? 1.f : (1 - __fdividef(shifted_r, cutoff));
+    float omega_c = shifted_r < 0.f ? 1.f : (1 - div_1ulp(shifted_r, cutoff));
     float fc = P.dpd_alpha * omega_c;
     float omega_d = omega_c * omega_c;
     float ev = ex * vx + ey * vy + ez * vz;
