@@ -416,17 +416,22 @@ __device__ __forceinline__ bool table_eval_ex(const DevPot &P, const float *__re
   if (r > P.b || ind > (unsigned)P.n) return false;
   const float4 *c4 = reinterpret_cast<const float4 *>(coef + P.coff + (size_t)ind * kPotChunk);
   float4 c0 = c4[0], c1 = c4[1];
-  float x = (r - c0.x) * c0.y;
-  float ee = c0.z * x + c0.w;
+  // The interval's quintic and its derivative by Horner's rule in fused
+  // multiply-adds: half the instructions and fewer roundings than the
+  // reference's separate products and sums (the build keeps contraction off
+  // for the pair-set test and the advance, which must round as the
+  // reference's do).
+  const float x = (r - c0.x) * c0.y;
+  float ee = __builtin_fmaf(c0.z, x, c0.w);
   float eff = c0.z;
-  eff = eff * x + ee;
-  ee = ee * x + c1.x;
-  eff = eff * x + ee;
-  ee = ee * x + c1.y;
-  eff = eff * x + ee;
-  ee = ee * x + c1.z;
-  eff = eff * x + ee;
-  ee = ee * x + c1.w;
+  eff = __builtin_fmaf(eff, x, ee);
+  ee = __builtin_fmaf(ee, x, c1.x);
+  eff = __builtin_fmaf(eff, x, ee);
+  ee = __builtin_fmaf(ee, x, c1.y);
+  eff = __builtin_fmaf(eff, x, ee);
+  ee = __builtin_fmaf(ee, x, c1.z);
+  eff = __builtin_fmaf(eff, x, ee);
+  ee = __builtin_fmaf(ee, x, c1.w);
   e = ee;
   f = div_1ulp(eff * c0.y, ro);
   return true;
