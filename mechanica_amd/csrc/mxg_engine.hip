@@ -641,16 +641,10 @@ __device__ __forceinline__ bool pair_force(const LoopConsts &A, const PairTabs &
     const float ir = __builtin_amdgcn_rcpf(r);
     float ex = dxx * ir, ey = dxy * ir, ez = dxz * ir;
     float4 vq = nb.v(q);
-    float vx, vy, vz;
-    if (p_is_i) {
-      vx = vs.x - vq.x;
-      vy = vs.y - vq.y;
-      vz = vs.z - vq.z;
-    } else {
-      vx = vq.x - vs.x;
-      vy = vq.y - vs.y;
-      vz = vq.z - vs.z;
-    }
+    // v_i - v_j: the owner's difference, negated when the owner is j (the
+    // negation is exact, b - a = -(a - b)), so no branch on the end.
+    const float sg = p_is_i ? 1.0f : -1.0f;
+    const float vx = sg * (vs.x - vq.x), vy = sg * (vs.y - vq.y), vz = sg * (vs.z - vq.z);
     float shifted_r = shifted ? r - ri - rj : r;
     float omega_c = shifted_r < 0.f ? 1.f : (1 - div_1ulp(shifted_r, cutoff));
     float fc = P.dpd_alpha * omega_c;
@@ -668,15 +662,9 @@ __device__ __forceinline__ bool pair_force(const LoopConsts &A, const PairTabs &
     }
     float f = in_range ? fc + fd + fr : 0.0f;
     float wx = f * ex, wy = f * ey, wz = f * ez;  // force on i
-    if (p_is_i) {
-      fx += wx;
-      fy += wy;
-      fz += wz;
-    } else {
-      fx -= wx;
-      fy -= wy;
-      fz -= wz;
-    }
+    fx += sg * wx;
+    fy += sg * wy;
+    fz += sg * wz;
     if (MODE & kModeVirial) {
       acc[3] += 0.5 * dxx * wx;
       acc[4] += 0.5 * dxx * wy;
