@@ -225,7 +225,7 @@ def bench_single(args, threads, workload=None):
         u.set_brick(*[int(v) for v in args.brick.split(",")])
     elif not args.no_autotune:
         u.step(args.warmup)  # tune on a settled state, not the initial one
-        tune = u.autotune(steps_per=3)
+        tune = u.autotune(steps_per=5)
     wall, walls = timed_steps(u, args.steps, args.warmup)
     value = N * args.steps / wall
     log("timed 5 x %d steps, median %.3f s" % (args.steps, wall))
