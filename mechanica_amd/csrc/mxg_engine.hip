@@ -4068,7 +4068,7 @@ int Engine::autotune(int steps_per, int *taken) {
   static const int shapes[][3] = {{6, 4, 4}, {5, 5, 4}, {5, 4, 4}, {5, 5, 3}, {6, 4, 3}, {7, 5, 2}, {6, 6, 2},
                                   {4, 4, 4}, {4, 4, 3}, {4, 3, 3}, {3, 3, 3}, {4, 4, 2}, {6, 4, 1}, {6, 2, 2},
                                   {5, 2, 2}, {4, 2, 2}, {3, 3, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
-  static const int caps[] = {8, 12, 16, 24};
+  static const int caps[] = {8, 10, 12, 14, 16, 20, 24};
   const bool was_timing = timing_;
   int best[4] = {bdim_[0], bdim_[1], bdim_[2], cap_};
   int best_waves = waves_;
