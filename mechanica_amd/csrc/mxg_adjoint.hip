@@ -101,7 +101,7 @@ __device__ __forceinline__ int type_of(float4 x) { return __float_as_int(x.w); }
 // group walks candidates sub, sub + kLPP, ... of every neighbour cell, and the
 // group's sums are combined by an xor butterfly (every lane ends with the
 // same bits), so the result depends on kLPP but not on scheduling.
-constexpr int kLPP = 4;
+constexpr int kLPP = 8;
 
 // Sum over a particle's kLPP lanes (aligned groups within a wave).
 template <typename T>
