@@ -216,7 +216,11 @@ int mxg_engine_set_graphs(mxg_engine *e, int on);
  * angles, dihedrals and exclusions name particles by these ids.  replica
  * and flags may be NULL.  Per-particle read-backs (state, unload, number
  * density, neighbour counts, species) are in argument order.  A load
- * replaces the particles and drops the previous bonds and terms. */
+ * replaces the particles and drops the previous bonds and terms.  The local
+ * shuffle that begins the reference's next step (space_shuffle_local,
+ * space.cpp:333-393) is applied at the load: a particle whose float local
+ * coordinate rounds to the cell width (just below a face) is reported in
+ * the next cell at 0. */
 int mxg_engine_load(mxg_engine *e, int N, const double *x, const double *v, const int *type, const int *pid,
                     const int *replica, const uint16_t *flags);
 

@@ -1351,6 +1351,30 @@ __device__ __forceinline__ bool reflect(const StepArgs &A, const DevReplica &R, 
   return enforced;
 }
 
+// space_shuffle_local (space.cpp:333-393), with which the reference's step
+// begins: a particle whose cell-local coordinate lies outside [0, h) moves
+// to the neighbouring cell (x -= delta h in float, cell index modulo the
+// grid).  The load already places particles so, and a step's own move
+// leaves a coordinate outside [0, h) only one way: x + h rounding to h (x a
+// tiny negative; x - h of an x in [h, 2h) is exact and non-negative).  Such
+// a particle starts the next advance from the upper neighbour at 0, as the
+// reference's does.  Returns the cell the particle starts in.
+__device__ __forceinline__ int shuffle_local(const DevReplica &R, int c, float X[3], float P[3], int &lx, int &ly,
+                                             int &lz) {
+  const bool ux = X[0] >= R.h[0], uy = X[1] >= R.h[1], uz = X[2] >= R.h[2];
+  if (__builtin_expect(!(ux | uy | uz), 1)) return c;
+  const int d[3] = {ux ? 1 : 0, uy ? 1 : 0, uz ? 1 : 0};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    X[k] -= d[k] * R.h[k];
+    if (P) P[k] -= d[k] * R.h[k];
+  }
+  lx = wrap(lx + d[0], R.cdim[0]);
+  ly = wrap(ly + d[1], R.cdim[1]);
+  lz = wrap(lz + d[2], R.cdim[2]);
+  return R.cell_base + (lx * R.cdim[1] + ly) * R.cdim[2] + lz;
+}
+
 // Advance of one particle (cell_advance_forward_euler,
 // engine_advance.cpp:126-239) and the store of its next-step state, the
 // migration flag and the per-cell counts.
@@ -1362,7 +1386,9 @@ __device__ __forceinline__ bool advance_store(const StepArgs &A, const DevReplic
   float X[3] = {xs.x, xs.y, xs.z};
   float V[3] = {vs.x, vs.y, vs.z};
   const float F[3] = {fx, fy, fz};
-  int newc = c;
+  // c: the slot's cell; the step moves the particle from cb.
+  const int cb = passthrough ? c : shuffle_local(R, c, X, nullptr, lx, ly, lz);
+  int newc = cb;
   const bool all_frozen = (pflags & kFrozenX) && (pflags & kFrozenY) && (pflags & kFrozenZ);
   if (!passthrough && !all_frozen) {
     const float mask[3] = {(pflags & kFrozenX) ? 0.0f : 1.0f, (pflags & kFrozenY) ? 0.0f : 1.0f,
@@ -1412,9 +1438,10 @@ __device__ __forceinline__ bool advance_store(const StepArgs &A, const DevReplic
     if (delta[0] | delta[1] | delta[2]) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) X[k] -= delta[k] * R.h[k];
-      const int nx = (lx + delta[0] + R.cdim[0]) % R.cdim[0];
-      const int ny = (ly + delta[1] + R.cdim[1]) % R.cdim[1];
-      const int nz = (lz + delta[2] + R.cdim[2]) % R.cdim[2];
+      // delta is -1, 0 or 1: a compare wraps the index (no division).
+      const int nx = wrap(lx + delta[0], R.cdim[0]);
+      const int ny = wrap(ly + delta[1], R.cdim[1]);
+      const int nz = wrap(lz + delta[2], R.cdim[2]);
       newc = R.cell_base + (nx * R.cdim[1] + ny) * R.cdim[2] + nz;
     }
   }
@@ -1449,7 +1476,7 @@ __device__ __forceinline__ bool rk_store(const StepArgs &A, const DevReplica &R,
   const float imf = (float)T.imass;
   const bool newton = T.dynamics == kDynNewtonian;
   const float F[3] = {fx, fy, fz};
-  const float X[3] = {xs.x, xs.y, xs.z};
+  float X[3] = {xs.x, xs.y, xs.z};
   const float V[3] = {vs.x, vs.y, vs.z};  // v0: velocities change only at the last stage
   float P0[3], AX[3], AV[3], VK[3], Xn[3], Vn[3] = {V[0], V[1], V[2]};
   if (stage == 0) {
@@ -1462,11 +1489,13 @@ __device__ __forceinline__ bool rk_store(const StepArgs &A, const DevReplica &R,
     P0[1] = p0.y;
     P0[2] = p0.z;
   }
+  // c: the slot's cell; the stage moves the particle (and p0) from cb.
+  const int cb = passthrough ? c : shuffle_local(R, c, X, P0, lx, ly, lz);
   const float4 axv = stage ? float4(A.rk_ax[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 avv = stage ? float4(A.rk_av[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 vkv = stage ? float4(A.rk_vk[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
   const float ax0[3] = {axv.x, axv.y, axv.z}, av0[3] = {avv.x, avv.y, avv.z}, vkp[3] = {vkv.x, vkv.y, vkv.z};
-  int newc = c;
+  int newc = cb;
   if (!passthrough) {
     int delta[3];
     bool toofast = false;
@@ -1500,9 +1529,10 @@ __device__ __forceinline__ bool rk_store(const StepArgs &A, const DevReplica &R,
         Xn[k] -= delta[k] * R.h[k];
         P0[k] -= delta[k] * R.h[k];
       }
-      const int nx = (lx + delta[0] + R.cdim[0]) % R.cdim[0];
-      const int ny = (ly + delta[1] + R.cdim[1]) % R.cdim[1];
-      const int nz = (lz + delta[2] + R.cdim[2]) % R.cdim[2];
+      // delta is -1, 0 or 1: a compare wraps the index (no division).
+      const int nx = wrap(lx + delta[0], R.cdim[0]);
+      const int ny = wrap(ly + delta[1], R.cdim[1]);
+      const int nz = wrap(lz + delta[2], R.cdim[2]);
       newc = R.cell_base + (nx * R.cdim[1] + ny) * R.cdim[2] + nz;
     }
   } else {
@@ -3155,6 +3185,18 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
         return fail(kErrSpace, "particle " + std::to_string(i) + " lies outside the space");
     float loc[3];
     for (int k = 0; k < 3; ++k) loc[k] = x[3 * (size_t)i + k] - R.corg[k][ind[k]];
+    // The first step's space_shuffle_local (space.cpp:333-393), applied
+    // here: a local coordinate that rounded to the cell width (a particle
+    // just below a face) or fell below zero moves the particle to the
+    // neighbouring cell, x -= delta h in float, the index modulo the grid.
+    for (int k = 0; k < 3; ++k) {
+      const float hf = (float)R.h[k];
+      const int d = (loc[k] >= hf ? 1 : 0) - (loc[k] < 0.0f ? 1 : 0);
+      if (d) {
+        loc[k] -= d * hf;
+        ind[k] = (ind[k] + d + R.cdim[k]) % R.cdim[k];
+      }
+    }
     cellof[i] = R.cell_base + (ind[0] * R.cdim[1] + ind[1]) * R.cdim[2] + ind[2];
     const uint16_t fl = flags ? flags[i] : 0;
     const unsigned bits = (unsigned)t | ((unsigned)fl << 16);

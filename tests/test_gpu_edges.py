@@ -1,0 +1,164 @@
+"""Edge cases of the step against the oracle: a lone particle, a single pair
+(inside one box and across the periodic seam), a sparse box where nearly
+every cell is empty, a clustered fluid whose cells hold hundreds of
+particles (bricks overflow LDS and run the global-memory kernel), particles
+lying exactly on cell faces or loaded just below one (the first step's
+local shuffle moves them on), a particle whose advance ends on a face (x + h
+rounds to h, and the next step starts from the neighbouring cell), and an
+empty load.
+
+Bars as in test_gpu_parity: neighbour counts bit for bit; forces within 1e-5
+of the FP32 oracle per particle (floor 0.1 rms |F|; all zero where the
+oracle's are); the number density within 1e-3 of the largest (the oracle
+keeps the reference's approximate rsqrt); a five-step relay (the GPU's
+forces fed to the oracle's advance) bit for bit in positions, velocities
+and cells."""
+import math
+
+import numpy as np
+import pytest
+
+from mechanica_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _dpd(x, L, cells, sigma=0.0, seed=1):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = len(x)
+    return {
+        "name": "dpd_edge",
+        "dim": [L, L, L], "cells": [cells] * 3, "cutoff": 1.0, "dt": 0.01,
+        "types": [{"mass": 1.0, "radius": 1.0, "dynamics": 0}],
+        "x": np.asarray(x, np.float64), "v": rng.normal(0.0, 1.0, size=(n, 3)),
+        "type": np.zeros(n, np.int32),
+        "pots": [("dpd", (25.0, 4.5, sigma, 1.0, 0), 0, 0)],
+        "bonds": None,
+    }
+
+
+def _lj(x, L, cells, seed=1):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = len(x)
+    return {
+        "name": "lj_edge",
+        "dim": [L, L, L], "cells": [cells] * 3, "cutoff": 1.0, "dt": 0.01,
+        "types": [{"mass": 39.4, "radius": 0.1, "dynamics": 0}],
+        "x": np.asarray(x, np.float64), "v": rng.normal(0.0, math.sqrt(1.0 / 39.4), size=(n, 3)),
+        "type": np.zeros(n, np.int32),
+        "pots": [("lj126", (0.275, 1.0, W.LJ_A, W.LJ_B, 1e-3), 0, 0)],
+        "bonds": None,
+    }
+
+
+def check(m, oracle_mod, w, relay_steps=5):
+    print("case:", w["name"], "N=%d" % len(w["x"]), flush=True)
+    u = W.build_universe(m, w, seed=5)
+    o = oracle_mod.from_workload(w, m, seed=5)
+    counts = u.neighbour_counts()
+    u.step(1)
+    g = u.state()
+    o.force()
+    r = o.state()
+    assert np.array_equal(counts, r["ncount"])
+    fr = r["f"].astype(np.float64)
+    fg = g["f"].astype(np.float64)
+    n = np.linalg.norm(fr, axis=1)
+    rms = math.sqrt((n ** 2).mean())
+    if rms == 0.0:
+        assert not fg.any()
+    else:
+        rel = (np.linalg.norm(fg - fr, axis=1) / np.maximum(n, 0.1 * rms)).max()
+        print("max rel force diff %.3e" % rel)
+        assert rel <= 1e-5
+    nd_g, nd_r = u.number_density(), o.number_density()
+    assert np.abs(nd_g - nd_r).max() <= 1e-3 * np.abs(nd_r).max()
+    u2 = W.build_universe(m, w, seed=5)
+    o2 = oracle_mod.from_workload(w, m, seed=5)
+    for k in range(relay_steps):
+        u2.step(1)
+        s = u2.state()
+        o2.step_with_forces(s["f"])
+        t = o2.state()
+        assert np.array_equal(s["cell"], t["cell"]), k
+        assert np.array_equal(s["x"].view(np.uint32), t["x"].view(np.uint32)), k
+        assert np.array_equal(s["v"].view(np.uint32), t["v"].view(np.uint32)), k
+    return counts, g
+
+
+def test_lone_particle(m, oracle_mod):
+    counts, g = check(m, oracle_mod, _dpd([[2.5, 1.25, 4.75]], 5.0, 5))
+    assert counts.tolist() == [0]
+    assert not g["f"].any()
+
+
+@pytest.mark.parametrize("where", ["inside", "across_seam"])
+def test_single_pair(m, oracle_mod, where):
+    """One LJ pair 0.6 apart: in neighbouring cells, or across the periodic
+    boundary (x = 0.2 and L - 0.4).  Each end computes the pair in the
+    reference orientation, so the forces are exact negatives."""
+    x = [[1.7, 2.5, 2.5], [2.3, 2.5, 2.5]] if where == "inside" else [[0.2, 2.5, 2.5], [4.6, 2.5, 2.5]]
+    counts, g = check(m, oracle_mod, _lj(x, 5.0, 5))
+    assert counts.tolist() == [1, 1]
+    f = g["f"]
+    assert np.array_equal(f[0], -f[1])
+    assert np.abs(f[0]).max() > 0
+
+
+def test_sparse_box(m, oracle_mod):
+    """60 particles in 12^3 cells: nearly every cell and most bricks are
+    empty, and most particles have no neighbour."""
+    rng = np.random.Generator(np.random.PCG64(31))
+    check(m, oracle_mod, _dpd(rng.random((60, 3)) * 12.0, 12.0, 12))
+
+
+def test_clustered_cells(m, oracle_mod):
+    """1000 DPD particles in a 2 x 2 x 2-cell block of a 10^3-cell box:
+    about 125 particles per occupied cell and several hundred neighbours
+    per particle, so the bricks around the block overflow LDS (the
+    global-memory kernel computes them) while the rest of the box is
+    empty."""
+    rng = np.random.Generator(np.random.PCG64(32))
+    x = 4.0 + 2.0 * rng.random((1000, 3))
+    counts, _ = check(m, oracle_mod, _dpd(x, 10.0, 10))
+    assert counts.max() > 300
+
+
+def test_particles_on_cell_faces(m, oracle_mod):
+    """LJ on a grid of spacing 0.5 in unit cells: every particle lies on a
+    cell face, edge or corner (cell-local coordinate 0), with half of them
+    jittered off the grid by up to 0.03, and four 1e-9 below the box's upper
+    faces, at least 0.3 from every other particle (no pair comes closer than
+    the table's 0.275)."""
+    rng = np.random.Generator(np.random.PCG64(33))
+    idx = np.arange(12) * 0.5
+    x = np.stack(np.meshgrid(idx, idx, idx, indexing="ij"), -1).reshape(-1, 3)
+    move = rng.random(len(x)) < 0.5
+    x[move] += rng.uniform(-0.03, 0.03, size=(move.sum(), 3))
+    x %= 6.0
+    x[:4] = [[6.0 - 1e-9, 0.25, 0.25], [0.75, 6.0 - 1e-9, 0.75], [1.75, 1.75, 6.0 - 1e-9],
+             [6.0 - 1e-9, 6.0 - 1e-9, 6.0 - 1e-9]]
+    check(m, oracle_mod, _lj(x, 6.0, 6))
+
+
+def test_advance_lands_on_face(m, oracle_mod):
+    """A particle 1e-10 above a cell's lower face moving down by 2e-10 a
+    step: the advance puts it at -1e-10, moves it to the lower cell at
+    -1e-10 + h, which rounds to h, and the reference's next step begins by
+    moving it back (space_shuffle_local) before it advances again.  A second
+    particle does the same across the periodic seam, a third on two axes."""
+    x = [[2.0 + 1e-10, 2.5, 2.5], [1e-10, 0.5, 4.5], [4.5, 3.0 + 1e-10, 1.0 + 1e-10]]
+    w = _dpd(x, 5.0, 5)
+    w["v"] = np.array([[-2e-8, 0.0, 0.0], [-2e-8, 0.0, 0.0], [0.0, -2e-8, -2e-8]])
+    counts, g = check(m, oracle_mod, w)
+    assert counts.tolist() == [0, 0, 0]
+
+
+def test_empty_load_rejected(m):
+    u = m.Universe(dim=[5.0, 5.0, 5.0], cells=[5, 5, 5], cutoff=1.0, dt=0.01)
+    u.add_type(1.0, 1.0, 0)
+    with pytest.raises(m.EngineError) as e:
+        u.load(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.int32))
+    assert e.value.code == -1  # MXG_ERR_NULL
+    with pytest.raises(m.EngineError):
+        u.step(1)
