@@ -1489,8 +1489,11 @@ __device__ __forceinline__ bool rk_store(const StepArgs &A, const DevReplica &R,
     P0[1] = p0.y;
     P0[2] = p0.z;
   }
-  // c: the slot's cell; the stage moves the particle (and p0) from cb.
-  const int cb = passthrough ? c : shuffle_local(R, c, X, P0, lx, ly, lz);
+  // c: the slot's cell; the stage moves the particle (and p0) from cb.  The
+  // first stage follows the previous step's update, which the reference's
+  // next engine_force shuffles; each later stage follows this kernel's own
+  // move at the end of the stage before, which is that force's shuffle.
+  const int cb = (passthrough || stage != 0) ? c : shuffle_local(R, c, X, P0, lx, ly, lz);
   const float4 axv = stage ? float4(A.rk_ax[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 avv = stage ? float4(A.rk_av[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 vkv = stage ? float4(A.rk_vk[s]) : make_float4(0.f, 0.f, 0.f, 0.f);

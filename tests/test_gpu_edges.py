@@ -154,6 +154,27 @@ def test_advance_lands_on_face(m, oracle_mod):
     assert counts.tolist() == [0, 0, 0]
 
 
+def test_advance_lands_on_face_rk4(m, oracle_mod):
+    """The same particles under Runge-Kutta 4 (each stage's move, and the
+    local shuffle that begins each force stage, carry p0 along).  With no
+    pair forces the stages are exact, so the states match the oracle's RK4
+    bit for bit, step after step."""
+    x = [[2.0 + 1e-10, 2.5, 2.5], [1e-10, 0.5, 4.5], [4.5, 3.0 + 1e-10, 1.0 + 1e-10]]
+    w = _dpd(x, 5.0, 5)
+    w["v"] = np.array([[-2e-8, 0.0, 0.0], [-2e-8, 0.0, 0.0], [0.0, -2e-8, -2e-8]])
+    w["integrator"] = 1
+    print("case: rk4 face landing", flush=True)
+    u = W.build_universe(m, w, seed=5)
+    o = oracle_mod.from_workload(w, m, seed=5)
+    for k in range(4):
+        u.step(1)
+        o.step(1)
+        s, t = u.state(), o.state()
+        assert np.array_equal(s["cell"], t["cell"]), k
+        assert np.array_equal(s["x"].view(np.uint32), t["x"].view(np.uint32)), k
+        assert np.array_equal(s["v"].view(np.uint32), t["v"].view(np.uint32)), k
+
+
 def test_empty_load_rejected(m):
     u = m.Universe(dim=[5.0, 5.0, 5.0], cells=[5, 5, 5], cutoff=1.0, dt=0.01)
     u.add_type(1.0, 1.0, 0)
