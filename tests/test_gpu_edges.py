@@ -4,7 +4,8 @@ every cell is empty, a clustered fluid whose cells hold hundreds of
 particles (bricks overflow LDS and run the global-memory kernel), particles
 lying exactly on cell faces or loaded just below one (the first step's
 local shuffle moves them on), a particle whose advance ends on a face (x + h
-rounds to h, and the next step starts from the neighbouring cell), and an
+rounds to h, and the next step starts from the neighbouring cell, under
+Euler and Runge-Kutta 4), bonded chains across the periodic seam, and an
 empty load.
 
 Bars as in test_gpu_parity: neighbour counts bit for bit; forces within 1e-5
@@ -173,6 +174,37 @@ def test_advance_lands_on_face_rk4(m, oracle_mod):
         assert np.array_equal(s["cell"], t["cell"]), k
         assert np.array_equal(s["x"].view(np.uint32), t["x"].view(np.uint32)), k
         assert np.array_equal(s["v"].view(np.uint32), t["v"].view(np.uint32)), k
+
+
+def test_bonded_terms_across_the_seam(m, oracle_mod):
+    """Two four-bead chains (the molecules' bonds, angles, dihedral,
+    exclusions and LJ) that cross the periodic boundary: one across the x
+    face, one through the corner where all three axes wrap, so bonds, angles
+    and dihedrals take the cell-offset folding across the seam on every
+    axis."""
+    th0 = math.radians(110.0)
+    sig, eps = 0.25, 0.5
+    A, B = 4 * eps * sig ** 12, 4 * eps * sig ** 6
+    x = np.array([[5.70, 2.50, 2.50], [5.95, 2.60, 2.50], [0.20, 2.50, 2.60], [0.45, 2.60, 2.75],
+                  [5.80, 5.85, 5.90], [0.05, 5.95, 0.10], [0.15, 0.20, 0.05], [0.40, 0.30, 0.25]])
+    rng = np.random.Generator(np.random.PCG64(34))
+    base = np.array([0, 4], np.int32)
+    w = {
+        "name": "seam_chains", "dim": [6.0, 6.0, 6.0], "cells": [6, 6, 6], "cutoff": 1.0, "dt": 0.002,
+        "types": [{"mass": 1.0, "radius": 0.1, "dynamics": 0}],
+        "x": x, "v": rng.normal(0.0, 0.5, size=(8, 3)), "type": np.zeros(8, np.int32),
+        "pots": [("lj126", (0.1, 1.0, A, B, 1e-4), 0, 0)],
+        "bonds": {"i": np.concatenate([base, base + 1, base + 2]), "j": np.concatenate([base + 1, base + 2, base + 3]),
+                  "pot": ("harmonic", (0.0, 0.6, 100.0, 0.3, 1e-4)), "dissociation": 1e30},
+        "angles": {"i": np.concatenate([base, base + 1]), "j": np.concatenate([base + 1, base + 2]),
+                   "k": np.concatenate([base + 2, base + 3]),
+                   "pot": ("harmonic_angle", (th0 - math.radians(60), th0 + math.radians(60), 20.0, th0, 1e-4))},
+        "dihedrals": {"i": base, "j": base + 1, "k": base + 2, "l": base + 3,
+                      "pot": ("harmonic_dihedral", (1.0, 3, 0.0, 1e-4))},
+        "exclusions": {"i": np.concatenate([base, base + 1, base + 2, base, base + 1]),
+                       "j": np.concatenate([base + 1, base + 2, base + 3, base + 2, base + 3])},
+    }
+    check(m, oracle_mod, w)
 
 
 def test_empty_load_rejected(m):
