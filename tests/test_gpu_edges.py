@@ -5,8 +5,9 @@ particles (bricks overflow LDS and run the global-memory kernel), particles
 lying exactly on cell faces or loaded just below one (the first step's
 local shuffle moves them on), a particle whose advance ends on a face (x + h
 rounds to h, and the next step starts from the neighbouring cell, under
-Euler and Runge-Kutta 4), bonded chains across the periodic seam, and an
-empty load.
+Euler and Runge-Kutta 4), bonded chains across the periodic seam, an
+ensemble with one-particle, empty and two-particle replicas, and an empty
+load.
 
 Bars as in test_gpu_parity: neighbour counts bit for bit; forces within 1e-5
 of the FP32 oracle per particle (floor 0.1 rms |F|; all zero where the
@@ -205,6 +206,69 @@ def test_bonded_terms_across_the_seam(m, oracle_mod):
                        "j": np.concatenate([base + 1, base + 2, base + 3, base + 2, base + 3])},
     }
     check(m, oracle_mod, w)
+
+
+def test_ensemble_tiny_and_empty_replicas(m, oracle_mod):
+    """One engine with four cell-sorting replicas (Morse, overdamped, boxes
+    of 3^3 cells of width 2.5): one particle, none, a pair 1.2 apart, and
+    300 particles.  Each non-empty replica against its own oracle: neighbour
+    counts, forces, then a five-step relay bit for bit; the empty replica's
+    observables stay zero."""
+    params = W.cell_sort_params()
+    params.pop("random")
+    rng = np.random.Generator(np.random.PCG64(35))
+    pts = [np.array([[3.0, 4.0, 5.0]]), np.zeros((0, 3)), np.array([[1.0, 3.7, 3.7], [2.2, 3.7, 3.7]]),
+           rng.random((300, 3)) * 7.5]
+    reps = [{"dim": [7.5] * 3, "cells": [3] * 3, "x": x, "type": (np.arange(len(x)) % 2).astype(np.int32)}
+            for x in pts]
+    e = {"name": "ensemble_edges", "replicas": reps}
+    e.update(params)
+    print("case: ensemble with replicas of", [len(x) for x in pts], "particles", flush=True)
+
+    def replica(r):
+        w = dict(reps[r])
+        for k in ("cutoff", "dt", "types", "pots"):
+            w[k] = e[k]
+        w["name"] = "cell_sort"
+        w["v"] = np.zeros_like(reps[r]["x"])
+        w["bonds"] = None
+        return w
+
+    bounds = np.cumsum([0] + [len(x) for x in pts])
+    cbase = np.cumsum([0] + [27] * len(pts))
+    u = W.build_ensemble(m, e, seed=12)
+    counts = u.neighbour_counts()
+    u.step(1)
+    g = u.state()
+    for r in range(len(pts)):
+        if len(pts[r]) == 0:
+            ob = u.observables(r)
+            assert ob["epot_nonbond"] == 0.0 and ob["ekin"] == 0.0
+            continue
+        sl = slice(bounds[r], bounds[r + 1])
+        o = oracle_mod.from_workload(replica(r), m)
+        o.force()
+        t = o.state()
+        assert np.array_equal(counts[sl], t["ncount"]), r
+        fr = t["f"].astype(np.float64)
+        n = np.linalg.norm(fr, axis=1)
+        rms = math.sqrt((n ** 2).mean())
+        fg = g["f"][sl].astype(np.float64)
+        if rms == 0.0:
+            assert not fg.any(), r
+        else:
+            assert (np.linalg.norm(fg - fr, axis=1) / np.maximum(n, 0.1 * rms)).max() <= 1e-5, r
+    u2 = W.build_ensemble(m, e, seed=12)
+    oracles = {r: oracle_mod.from_workload(replica(r), m) for r in range(len(pts)) if len(pts[r])}
+    for k in range(5):
+        u2.step(1)
+        s = u2.state()
+        for r, o in oracles.items():
+            sl = slice(bounds[r], bounds[r + 1])
+            o.step_with_forces(s["f"][sl])
+            t = o.state()
+            assert np.array_equal(s["cell"][sl] - cbase[r], t["cell"]), (k, r)
+            assert np.array_equal(s["x"][sl].view(np.uint32), t["x"].view(np.uint32)), (k, r)
 
 
 def test_empty_load_rejected(m):
