@@ -34,10 +34,12 @@ extern "C" {
 #define MXG_NEWTONIAN 0
 #define MXG_OVERDAMPED 1
 
-/* Particle flags honoured by the advance (MxParticle.h PARTICLE_FROZEN_*). */
-#define MXG_FROZEN_X (1 << 2)
-#define MXG_FROZEN_Y (1 << 3)
-#define MXG_FROZEN_Z (1 << 4)
+/* Particle flags honoured by the advance (MxParticle.h:58-60,
+ * PARTICLE_FROZEN_X/Y/Z); the other MxParticleFlags bits are carried and
+ * ignored. */
+#define MXG_FROZEN_X (1 << 3)
+#define MXG_FROZEN_Y (1 << 4)
+#define MXG_FROZEN_Z (1 << 5)
 
 /* Potential kinds and flags (MxPotential.h:55-110). */
 #define MXG_POTENTIAL_KIND_TABLE 0

@@ -23,9 +23,10 @@ from . import _capi
 
 NEWTONIAN = 0
 OVERDAMPED = 1
-FROZEN_X = 1 << 2
-FROZEN_Y = 1 << 3
-FROZEN_Z = 1 << 4
+# MxParticleFlags PARTICLE_FROZEN_X/Y/Z (MxParticle.h:58-60)
+FROZEN_X = 1 << 3
+FROZEN_Y = 1 << 4
+FROZEN_Z = 1 << 5
 
 ERR_TOOFAST = -29
 

@@ -43,8 +43,14 @@ enum : int {
   kBoundaryActive = kBoundaryFreeslip | kBoundaryVelocity | kBoundaryPotential,
 };
 
-// Particle flags understood by the advance (MxParticle.h PARTICLE_FROZEN_*).
-enum : uint16_t { kFrozenX = 1 << 2, kFrozenY = 1 << 3, kFrozenZ = 1 << 4 };
+// Particle flags understood by the advance (MxParticle.h:58-60,
+// PARTICLE_FROZEN_X/Y/Z; bit 2 is PARTICLE_BOUND, which the advance ignores).
+enum : uint16_t { kFrozenX = 1 << 3, kFrozenY = 1 << 4, kFrozenZ = 1 << 5 };
+// Internal flag (the reference's flags use bits 0-6): the load's local
+// shuffle, which stands for the first step's, left a coordinate at the cell
+// width (a coordinate just below the cell's lower face, moved down a cell),
+// so the first advance starts from there instead of shuffling again.
+enum : uint16_t { kShuffledToFace = 1 << 15 };
 
 struct ReplicaSpec {
   double origin[3];

@@ -1387,7 +1387,7 @@ __device__ __forceinline__ bool advance_store(const StepArgs &A, const DevReplic
   float V[3] = {vs.x, vs.y, vs.z};
   const float F[3] = {fx, fy, fz};
   // c: the slot's cell; the step moves the particle from cb.
-  const int cb = passthrough ? c : shuffle_local(R, c, X, nullptr, lx, ly, lz);
+  const int cb = (passthrough || (pflags & kShuffledToFace)) ? c : shuffle_local(R, c, X, nullptr, lx, ly, lz);
   int newc = cb;
   const bool all_frozen = (pflags & kFrozenX) && (pflags & kFrozenY) && (pflags & kFrozenZ);
   if (!passthrough && !all_frozen) {
@@ -1445,7 +1445,8 @@ __device__ __forceinline__ bool advance_store(const StepArgs &A, const DevReplic
       newc = R.cell_base + (nx * R.cdim[1] + ny) * R.cdim[2] + nz;
     }
   }
-  A.x_out[s] = make_float4(X[0], X[1], X[2], xs.w);
+  const float xw = passthrough ? xs.w : __uint_as_float(__float_as_uint(xs.w) & ~((unsigned)kShuffledToFace << 16));
+  A.x_out[s] = make_float4(X[0], X[1], X[2], xw);
   A.v_out[s] = make_float4(V[0], V[1], V[2], vs.w);  // w carries the particle id
   A.cell_out[s] = newc != c ? (int)((unsigned)newc | 0x80000000u) : newc;
   // Movers are counted into their new cell here; the caller counts stayers
@@ -1493,7 +1494,8 @@ __device__ __forceinline__ bool rk_store(const StepArgs &A, const DevReplica &R,
   // first stage follows the previous step's update, which the reference's
   // next engine_force shuffles; each later stage follows this kernel's own
   // move at the end of the stage before, which is that force's shuffle.
-  const int cb = (passthrough || stage != 0) ? c : shuffle_local(R, c, X, P0, lx, ly, lz);
+  const bool at_face = (((unsigned)__float_as_int(xs.w) >> 16) & kShuffledToFace) != 0;
+  const int cb = (passthrough || stage != 0 || at_face) ? c : shuffle_local(R, c, X, P0, lx, ly, lz);
   const float4 axv = stage ? float4(A.rk_ax[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 avv = stage ? float4(A.rk_av[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 vkv = stage ? float4(A.rk_vk[s]) : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1547,7 +1549,8 @@ __device__ __forceinline__ bool rk_store(const StepArgs &A, const DevReplica &R,
       VK[k] = vkp[k];
     }
   }
-  A.x_out[s] = make_float4(Xn[0], Xn[1], Xn[2], xs.w);
+  const float xw = passthrough ? xs.w : __uint_as_float(__float_as_uint(xs.w) & ~((unsigned)kShuffledToFace << 16));
+  A.x_out[s] = make_float4(Xn[0], Xn[1], Xn[2], xw);
   A.v_out[s] = make_float4(Vn[0], Vn[1], Vn[2], vs.w);
   A.cell_out[s] = newc != c ? (int)((unsigned)newc | 0x80000000u) : newc;
   if (newc != c) atomicAdd(&A.inc[newc], 1);
@@ -3192,16 +3195,18 @@ int Engine::load(int N, const double *x, const double *v, const int *type, const
     // here: a local coordinate that rounded to the cell width (a particle
     // just below a face) or fell below zero moves the particle to the
     // neighbouring cell, x -= delta h in float, the index modulo the grid.
+    bool at_face = false;
     for (int k = 0; k < 3; ++k) {
       const float hf = (float)R.h[k];
       const int d = (loc[k] >= hf ? 1 : 0) - (loc[k] < 0.0f ? 1 : 0);
       if (d) {
         loc[k] -= d * hf;
         ind[k] = (ind[k] + d + R.cdim[k]) % R.cdim[k];
+        at_face = at_face || loc[k] >= hf;
       }
     }
     cellof[i] = R.cell_base + (ind[0] * R.cdim[1] + ind[1]) * R.cdim[2] + ind[2];
-    const uint16_t fl = flags ? flags[i] : 0;
+    const uint16_t fl = (uint16_t)(((flags ? flags[i] : 0) & ~kShuffledToFace) | (at_face ? kShuffledToFace : 0));
     const unsigned bits = (unsigned)t | ((unsigned)fl << 16);
     float wbits;
     std::memcpy(&wbits, &bits, sizeof(wbits));

@@ -492,7 +492,7 @@ def build_universe(m, w, seed=None):
             u.add_friction(t, params[3], params[0], params[1], params[2])
     for t, e in (w.get("target_energy") or {}).items():
         u.set_target_energy(t, e)
-    u.load(w["x"], w["v"], w["type"])
+    u.load(w["x"], w["v"], w["type"], flags=w.get("flags"))
     if w.get("bonds"):
         b = w["bonds"]
         bp = make_potential(m, b["pot"])

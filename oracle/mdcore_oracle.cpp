@@ -87,7 +87,8 @@ constexpr int kChunk = 8;
 
 enum : uint32_t { kFlagScaled = 1u << 11, kFlagShifted = 1u << 12 };
 enum : uint32_t { kKindTable = 0, kKindDPD = 1 };
-enum : uint16_t { kFrozenX = 1 << 2, kFrozenY = 1 << 3, kFrozenZ = 1 << 4 };
+// PARTICLE_FROZEN_X/Y/Z (MxParticle.h:58-60).
+enum : uint16_t { kFrozenX = 1 << 3, kFrozenY = 1 << 4, kFrozenZ = 1 << 5 };
 enum { kNewtonian = 0, kOverdamped = 1 };
 enum { kTaskSort = 0, kTaskSelf = 1, kTaskPair = 2 };
 

@@ -366,7 +366,7 @@ def from_workload(w, m, threads=1, seed=None):
         o.set_single_body(t, kind, params)
     for t, e in (w.get("target_energy") or {}).items():
         o.set_target_energy(t, e)
-    o.load(w["x"], w["v"], w["type"])
+    o.load(w["x"], w["v"], w["type"], w.get("flags"))
     if w.get("bonds"):
         b = w["bonds"]
         bp = W.make_potential(m, b["pot"])

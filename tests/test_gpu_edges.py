@@ -6,8 +6,9 @@ lying exactly on cell faces or loaded just below one (the first step's
 local shuffle moves them on), a particle whose advance ends on a face (x + h
 rounds to h, and the next step starts from the neighbouring cell, under
 Euler and Runge-Kutta 4), bonded chains across the periodic seam, an
-ensemble with one-particle, empty and two-particle replicas, and an empty
-load.
+ensemble with one-particle, empty and two-particle replicas, particles
+loaded just below a wall or just below zero, frozen-axis flags, and loads
+the engine refuses.
 
 Bars as in test_gpu_parity: neighbour counts bit for bit; forces within 1e-5
 of the FP32 oracle per particle (floor 0.1 rms |F|; all zero where the
@@ -269,6 +270,85 @@ def test_ensemble_tiny_and_empty_replicas(m, oracle_mod):
             t = o.state()
             assert np.array_equal(s["cell"][sl] - cbase[r], t["cell"]), (k, r)
             assert np.array_equal(s["x"][sl].view(np.uint32), t["x"].view(np.uint32)), (k, r)
+
+
+def test_wall_face_load(m, oracle_mod):
+    """DPD between free-slip walls in z (periodic in x and y) with particles
+    loaded 1e-9 below the top wall: their local z rounds to the cell width,
+    and the reference's local shuffle moves them on modulo the grid, as on a
+    periodic axis, into the bottom cell at z = 0 (space.cpp:333-393 wraps
+    every axis)."""
+    rng = np.random.Generator(np.random.PCG64(36))
+    x = rng.random((400, 3)) * 8.0
+    x[:3, 2] = 8.0 - 1e-9
+    w = _dpd(x, 8.0, 8)
+    w["boundaries"] = {"kinds": [2, 2, 2, 2, 4, 4]}
+    check(m, oracle_mod, w)
+
+
+def test_load_just_below_zero(m, oracle_mod):
+    """space_addpart truncates (x - origin) / h toward zero, so a coordinate
+    in (-h, 0) is accepted into cell 0 with a negative local coordinate, and
+    the local shuffle then moves it to the last cell (x + h)."""
+    rng = np.random.Generator(np.random.PCG64(38))
+    x = rng.random((200, 3)) * 5.0
+    x[:3] = [[1.0, -0.1, 1.0], [-0.5, 2.0, 3.0], [-1e-9, -1e-9, 4.5]]
+    counts, _ = check(m, oracle_mod, _dpd(x, 5.0, 5))
+
+
+def test_frozen_flags(m, oracle_mod):
+    """MxParticleFlags on a DPD fluid: PARTICLE_FROZEN_X, FROZEN_Y | FROZEN_Z
+    and FROZEN (1 << 3 ... 1 << 5, MxParticle.h:58-60) hold those
+    coordinates, PARTICLE_BOUND (1 << 2) holds nothing; against the oracle,
+    and the frozen coordinates unchanged over five steps."""
+    rng = np.random.Generator(np.random.PCG64(39))
+    x = rng.random((600, 3)) * 6.0
+    w = _dpd(x, 6.0, 6)
+    fl = np.zeros(600, np.uint16)
+    fl[0:40] = m.FROZEN_X
+    fl[40:80] = m.FROZEN_Y | m.FROZEN_Z
+    fl[80:120] = m.FROZEN_X | m.FROZEN_Y | m.FROZEN_Z
+    fl[120:160] = 1 << 2
+    w["flags"] = fl
+    u = W.build_universe(m, w, seed=5)
+    x0, _ = u.unload()
+    check(m, oracle_mod, w)
+    u.step(5)
+    x1, _ = u.unload()
+    for lo, axes in ((0, [0]), (40, [1, 2]), (80, [0, 1, 2])):
+        assert np.array_equal(x1[lo:lo + 40, axes], x0[lo:lo + 40, axes]), lo
+        moving = [k for k in range(3) if k not in axes]
+        if moving:
+            assert (x1[lo:lo + 40, moving] != x0[lo:lo + 40, moving]).all(), lo
+    assert (x1[120:160] != x0[120:160]).all()
+
+
+def test_load_errors(m):
+    """Loads the engine refuses, leaving it unloaded: a particle outside the
+    box, beyond it or a cell width or more below it (MXG_ERR_SPACE), an id
+    given twice and a type out of range
+    (MXG_ERR_RANGE), and 65,536 particles in one cell (MXG_ERR_MAXPARTS,
+    mdcore's per-cell limit)."""
+    def universe():
+        u = m.Universe(dim=[5.0, 5.0, 5.0], cells=[5, 5, 5], cutoff=1.0, dt=0.01)
+        u.add_type(1.0, 1.0, 0)
+        return u
+
+    cases = [
+        (-3, dict(x=[[5.5, 1.0, 1.0]])),
+        (-3, dict(x=[[1.0, -1.5, 1.0]])),
+        (-6, dict(x=[[1.0, 1.0, 1.0], [2.0, 2.0, 2.0]], pid=[3, 3])),
+        (-6, dict(x=[[1.0, 1.0, 1.0]], types=[1])),
+        (-24, dict(x=np.random.default_rng(37).random((65536, 3)) * 0.9)),
+    ]
+    for code, kw in cases:
+        u = universe()
+        x = np.asarray(kw.pop("x"), np.float64)
+        with pytest.raises(m.EngineError) as e:
+            u.load(x, None, kw.pop("types", np.zeros(len(x), np.int32)), **kw)
+        assert e.value.code == code, (code, e.value)
+        with pytest.raises(m.EngineError):
+            u.step(1)
 
 
 def test_empty_load_rejected(m):

@@ -163,7 +163,7 @@ def test_advance_overdamped_clamp_and_frozen(m, oracle_mod):
     o.add_type(1.0, 0.1, 1)
     o.bind(pot, 0, 0)
     x = np.array([[2.5, 2.5, 2.5], [2.8, 2.5, 2.5]])
-    o.load(x, np.zeros_like(x), np.zeros(2, np.int32), np.array([0, 1 << 2], np.uint16))
+    o.load(x, np.zeros_like(x), np.zeros(2, np.int32), np.array([0, m.FROZEN_X], np.uint16))
     before = o.state()["x"].copy()
     o.step(1)
     s = o.state()
@@ -172,6 +172,29 @@ def test_advance_overdamped_clamp_and_frozen(m, oracle_mod):
     assert s["x"][0, 0] == np.float32(before[0, 0] - np.float32(0.05))
     # particle 1 is frozen in x
     assert s["x"][1, 0] == before[1, 0]
+
+
+def test_particle_flag_values(m):
+    """PARTICLE_FROZEN_X/Y/Z are 1 << 3, 1 << 4, 1 << 5 (MxParticle.h:58-60;
+    1 << 2 is PARTICLE_BOUND), in the Python module and the C header."""
+    assert (m.FROZEN_X, m.FROZEN_Y, m.FROZEN_Z) == (1 << 3, 1 << 4, 1 << 5)
+    import os
+    import re
+    text = open(os.path.join(os.path.dirname(__file__), "..", "include", "mxg.h")).read()
+    vals = {k: int(v) for k, v in re.findall(r"#define MXG_FROZEN_([XYZ]) \(1 << (\d+)\)", text)}
+    assert vals == {"X": 3, "Y": 4, "Z": 5}
+
+
+def test_advance_frozen_axes_and_bound_flag(oracle_mod):
+    """A particle frozen in y keeps its y and moves in x and z; PARTICLE_BOUND
+    (1 << 2) does not freeze anything."""
+    for flags, still in ((1 << 4, [1]), (1 << 2, []), ((1 << 3) | (1 << 5), [0, 2])):
+        o = one_particle(oracle_mod, [2.5, 2.5, 2.5], [1.0, -2.0, 3.0], flags=flags)
+        before = o.state()["x"][0].copy()
+        o.step(1)
+        after = o.state()["x"][0]
+        for k in range(3):
+            assert (after[k] == before[k]) == (k in still), (flags, k)
 
 
 def test_toofast_error(oracle_mod):
