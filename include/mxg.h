@@ -256,6 +256,9 @@ int mxg_engine_set_exclusions(mxg_engine *e, int M, const int32_t *i, const int3
 #define MXG_BOUNDARY_PERIODIC (1 << 1)
 #define MXG_BOUNDARY_FREESLIP (1 << 2)
 #define MXG_BOUNDARY_POTENTIAL (1 << 3)
+/* BOUNDARY_NO_SLIP: a velocity wall at rest (MxBoundaryConditions.cpp:415-418);
+ * BOUNDARY_RESETTING (1 << 5) is not supported and refused. */
+#define MXG_BOUNDARY_NO_SLIP (1 << 4)
 #define MXG_SIDE_LEFT 0
 #define MXG_SIDE_RIGHT 1
 #define MXG_SIDE_FRONT 2

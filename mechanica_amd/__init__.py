@@ -184,6 +184,7 @@ BOUNDARY_VELOCITY = 1 << 0
 BOUNDARY_PERIODIC = 1 << 1
 BOUNDARY_FREESLIP = 1 << 2
 BOUNDARY_POTENTIAL = 1 << 3
+BOUNDARY_NO_SLIP = 1 << 4  # a velocity wall at rest
 SIDES = ("left", "right", "front", "back", "bottom", "top")
 
 

@@ -41,6 +41,9 @@ enum : int {
   kBoundaryFreeslip = 1 << 2,
   kBoundaryPotential = 1 << 3,
   kBoundaryActive = kBoundaryFreeslip | kBoundaryVelocity | kBoundaryPotential,
+  // BOUNDARY_NO_SLIP: accepted by set_boundaries as a velocity wall at rest
+  // (MxBoundaryConditions.cpp:415-418), never stored.
+  kBoundaryNoSlip = 1 << 4,
 };
 
 // Particle flags understood by the advance (MxParticle.h:58-60,

@@ -3509,9 +3509,21 @@ int Engine::set_bonds(int M, const int *bi, const int *bj, const Potential *cons
 // sides together.  An axis is periodic when both its sides are; a lone
 // periodic side is refused (the reference warns and makes both periodic,
 // check_periodicy).
-int Engine::set_boundaries(const int kind[6], const float *vel, const float *restore, const float *radius) {
+int Engine::set_boundaries(const int kind_in[6], const float *vel_in, const float *restore, const float *radius) {
   if (!d_) return fail(kErrNull, "engine not initialised");
-  if (!kind) return reject(kErrNull, "boundary kinds missing");
+  if (!kind_in) return reject(kErrNull, "boundary kinds missing");
+  // A no-slip side is a velocity wall at rest, as the reference's
+  // configuration makes it (MxBoundaryConditions.cpp:415-418, 438-441).
+  int kind[6];
+  float vel[18];
+  for (int side = 0; side < 6; ++side) {
+    kind[side] = kind_in[side];
+    for (int k = 0; k < 3; ++k) vel[3 * side + k] = vel_in ? vel_in[3 * side + k] : 0.0f;
+    if (kind[side] & kBoundaryNoSlip) {
+      kind[side] = (kind[side] & ~kBoundaryNoSlip) | kBoundaryVelocity;
+      for (int k = 0; k < 3; ++k) vel[3 * side + k] = 0.0f;
+    }
+  }
   for (int side = 0; side < 6; ++side) {
     const int k = kind[side];
     if (k & ~(kBoundaryVelocity | kBoundaryPeriodic | kBoundaryFreeslip | kBoundaryPotential))
@@ -3525,7 +3537,7 @@ int Engine::set_boundaries(const int kind[6], const float *vel, const float *res
   for (int side = 0; side < 6; ++side) {
     WallInfo &W = walls_[side];
     W.kind = kind[side];
-    for (int k = 0; k < 3; ++k) W.vel[k] = vel ? vel[3 * side + k] : 0.0f;
+    for (int k = 0; k < 3; ++k) W.vel[k] = vel[3 * side + k];
     W.restore = restore ? restore[side] : 1.0f;
     W.radius = radius ? radius[side] : 0.0f;
   }

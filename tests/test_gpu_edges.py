@@ -323,6 +323,22 @@ def test_frozen_flags(m, oracle_mod):
     assert (x1[120:160] != x0[120:160]).all()
 
 
+def test_no_slip_kind(m):
+    """BOUNDARY_NO_SLIP sides (1 << 4) behave as velocity walls at rest, as
+    the reference's configuration makes them: five steps bit for bit."""
+    rng = np.random.Generator(np.random.PCG64(40))
+    x = rng.random((500, 3)) * 8.0
+    states = []
+    for kinds in ([2, 2, 2, 2, m.BOUNDARY_NO_SLIP, m.BOUNDARY_NO_SLIP], [2, 2, 2, 2, 1, 1]):
+        w = _dpd(x, 8.0, 8)
+        w["boundaries"] = {"kinds": kinds}
+        u = W.build_universe(m, w, seed=5)
+        u.step(5)
+        states.append(u.state())
+    for k in ("x", "v", "f", "cell"):
+        assert np.array_equal(states[0][k].view(np.uint32), states[1][k].view(np.uint32)), k
+
+
 def test_load_errors(m):
     """Loads the engine refuses, leaving it unloaded: a particle outside the
     box, beyond it or a cell width or more below it (MXG_ERR_SPACE), an id
