@@ -4200,8 +4200,10 @@ int Engine::autotune(int steps_per, int *taken) {
     waves_ = saved;
   }
   // A fresh load relaxes over the sweep's first steps, which favours the
-  // layouts timed late: the four fastest are timed again, one after
-  // another, and the best of that round is kept.
+  // layouts timed late, and five steps are short against the clock's
+  // wanderings: the four fastest are timed again in two passes of four
+  // times as many steps, and the layout with the best time in either pass
+  // is kept.
   {
     std::vector<TuneEntry> top = tune_log_;
     std::sort(top.begin(), top.end(), [](const TuneEntry &a, const TuneEntry &b) {
@@ -4210,35 +4212,37 @@ int Engine::autotune(int steps_per, int *taken) {
     top.resize(std::min<size_t>(top.size(), 4));
     const int saved = waves_;
     double round_ms = 1e30;
-    for (const TuneEntry &t : top) {
-      if (!(t.force_ms > 0)) continue;
-      if (t.waves != waves_) {
-        waves_ = t.waves;
-        ++config_gen_;
-      }
-      if (set_brick(t.bx, t.by, t.bz, t.cap, pmax_req_) != kErrOk) continue;
-      timing_ = false;
-      int rc = step(1);
-      timing_ = true;
-      if (rc == kErrOk) rc = step(steps_per);
-      steps += steps_per + 1;
-      if (rc != kErrOk) {
-        if (waves_ != saved) {
-          waves_ = saved;
+    for (int pass = 0; pass < 2; ++pass) {
+      for (const TuneEntry &t : top) {
+        if (!(t.force_ms > 0)) continue;
+        if (t.waves != waves_) {
+          waves_ = t.waves;
           ++config_gen_;
         }
-        timing_ = was_timing;
-        if (taken) *taken = steps;
-        return rc;
-      }
-      tune_log_.push_back({t.bx, t.by, t.bz, t.cap, t.waves, force_ms_});
-      if (force_ms_ > 0 && force_ms_ < round_ms) {
-        round_ms = force_ms_;
-        best[0] = t.bx;
-        best[1] = t.by;
-        best[2] = t.bz;
-        best[3] = t.cap;
-        best_waves = t.waves;
+        if (set_brick(t.bx, t.by, t.bz, t.cap, pmax_req_) != kErrOk) continue;
+        timing_ = false;
+        int rc = step(1);
+        timing_ = true;
+        if (rc == kErrOk) rc = step(4 * steps_per);
+        steps += 4 * steps_per + 1;
+        if (rc != kErrOk) {
+          if (waves_ != saved) {
+            waves_ = saved;
+            ++config_gen_;
+          }
+          timing_ = was_timing;
+          if (taken) *taken = steps;
+          return rc;
+        }
+        tune_log_.push_back({t.bx, t.by, t.bz, t.cap, t.waves, force_ms_});
+        if (force_ms_ > 0 && force_ms_ < round_ms) {
+          round_ms = force_ms_;
+          best[0] = t.bx;
+          best[1] = t.by;
+          best[2] = t.bz;
+          best[3] = t.cap;
+          best_waves = t.waves;
+        }
       }
     }
     if (waves_ != saved) {

@@ -223,7 +223,12 @@ def test_autotune_keeps_trajectory(m, name):
         u.set_dpd_noise(1)
     res = u.autotune(steps_per=1)
     print("autotune", res["brick"], res["steps"], res["log"][:3])
-    assert len(res["log"]) > 0 and res["steps"] == 2 * len(res["log"])  # one untimed step each
+    # The sweep takes one untimed and steps_per timed steps per layout; the
+    # final round times the (up to) four fastest again, twice, with one
+    # untimed and 4 steps_per timed steps each.
+    n = len(res["log"])
+    sweep = next(k for k in range(n + 1) if k + 2 * min(4, k) == n)
+    assert sweep > 0 and res["steps"] == 2 * sweep + 5 * (n - sweep)
     total = res["steps"] + 3
     u.step(3)
     a = u.state()
