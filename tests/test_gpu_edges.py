@@ -159,12 +159,15 @@ def test_advance_lands_on_face(m, oracle_mod):
 
 def test_advance_lands_on_face_rk4(m, oracle_mod):
     """The same particles under Runge-Kutta 4 (each stage's move, and the
-    local shuffle that begins each force stage, carry p0 along).  With no
-    pair forces the stages are exact, so the states match the oracle's RK4
-    bit for bit, step after step."""
-    x = [[2.0 + 1e-10, 2.5, 2.5], [1e-10, 0.5, 4.5], [4.5, 3.0 + 1e-10, 1.0 + 1e-10]]
+    local shuffle that begins each force stage, carry p0 along), and two
+    loaded 1e-9 below zero, whose load shuffle leaves them on the face of
+    the last cell.  With no pair forces the stages are exact, so the states
+    match the oracle's RK4 bit for bit, step after step."""
+    x = [[2.0 + 1e-10, 2.5, 2.5], [1e-10, 0.5, 4.5], [4.5, 3.0 + 1e-10, 1.0 + 1e-10], [-1e-9, 2.5, 2.5],
+         [3.5, -1e-9, 0.5]]
     w = _dpd(x, 5.0, 5)
-    w["v"] = np.array([[-2e-8, 0.0, 0.0], [-2e-8, 0.0, 0.0], [0.0, -2e-8, -2e-8]])
+    w["v"] = np.array([[-2e-8, 0.0, 0.0], [-2e-8, 0.0, 0.0], [0.0, -2e-8, -2e-8], [1e-8, 0.0, 0.0],
+                       [0.0, -1e-8, 0.0]])
     w["integrator"] = 1
     print("case: rk4 face landing", flush=True)
     u = W.build_universe(m, w, seed=5)
