@@ -197,6 +197,25 @@ def test_advance_frozen_axes_and_bound_flag(oracle_mod):
             assert (after[k] == before[k]) == (k in still), (flags, k)
 
 
+def test_local_shuffle_at_step_start(oracle_mod):
+    """space_shuffle_local (space.cpp:333-393) at the start of the step's
+    force phase, by hand: x - origin truncates toward zero, so x = -1e-9 loads
+    into cell 0 at -1e-9 and moves to the last cell at -1e-9 + h, which rounds
+    to h; x = L - 1e-9 loads into the last cell at 1 - 1e-9, which rounds to
+    h, and moves to cell 0 at h - h = 0."""
+    o = oracle_mod.Oracle([5.0] * 3, [5] * 3, 1.0, 0.01)
+    o.add_type(1.0, 0.1, 0)
+    x = np.array([[-1e-9, 2.5, 2.5], [5.0 - 1e-9, 2.5, 2.5]])
+    o.load(x, np.zeros_like(x), np.zeros(2, np.int32))
+    s = o.state()
+    assert s["cell"].tolist() == [(0 * 5 + 2) * 5 + 2, (4 * 5 + 2) * 5 + 2]
+    assert s["x"][0, 0] == np.float32(-1e-9) and s["x"][1, 0] == np.float32(1.0)
+    o.force()
+    s = o.state()
+    assert s["cell"].tolist() == [(4 * 5 + 2) * 5 + 2, (0 * 5 + 2) * 5 + 2]
+    assert s["x"][0, 0] == np.float32(1.0) and s["x"][1, 0] == np.float32(0.0)
+
+
 def test_toofast_error(oracle_mod):
     o = one_particle(oracle_mod, [2.5, 2.5, 2.5], [250.0, 0.0, 0.0], frac=3.0)
     rc = o.step(1)
