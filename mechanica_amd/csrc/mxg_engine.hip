@@ -3403,6 +3403,18 @@ int Engine::set_brick(int bx, int by, int bz, int cap, int pmax) {
         for (int z = 0; z < R.cdim[2]; z += bd[2]) bricks.push_back(make_int4((int)r, x, y, z));
     rep_bricks[r].y = (int)bricks.size();
   }
+  // One universe: each XCD walks an eighth of the list in order (xcd_brick),
+  // so the bricks cut short by the grid's upper edges go to the end of each
+  // eighth, and the step's last workgroups are the cheap ones.
+  if (reps_.size() == 1) {
+    const HostReplica &R = reps_[0];
+    const size_t part = (bricks.size() - bricks.size() % 8) / 8;
+    auto whole = [&](const int4 &b) {
+      return b.y + bd[0] <= R.cdim[0] && b.z + bd[1] <= R.cdim[1] && b.w + bd[2] <= R.cdim[2];
+    };
+    for (int k = 0; k < 8 && part > 0; ++k)
+      std::stable_partition(bricks.begin() + k * part, bricks.begin() + (k + 1) * part, whole);
+  }
   DeviceState *d = d_;
   if (int st = dalloc(d, &d->bricks, bricks.size())) return check_device(st, "hipMalloc");
   if (int st = dalloc(d, &d->rep_bricks, rep_bricks.size())) return check_device(st, "hipMalloc");
