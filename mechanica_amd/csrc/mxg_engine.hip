@@ -1576,6 +1576,7 @@ struct BrickArgs {
   int bdim[3];  // brick size in cells
   int pmax;     // LDS particle capacity
   int cap;      // neighbour-list entries per thread
+  int drain;    // entries a full list keeps when the wave drains (clamped to [0, cap - chunk])
   int hmax;     // halo-cell capacity
   int hdmax;    // halo cells per axis (capacity)
   int noff;     // neighbour offsets (capacity)
@@ -1882,7 +1883,23 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
             sy = side >= 0 ? y : -y;
             sz = side >= 0 ? z : -z;
           };
-          auto process = [&](int n) {
+          // Each thread's neighbour list: entry k of thread t at
+          // nlist[k * kBT + t], nn entries held.
+          int nn = 0;
+          // Evaluates the first k entries of every list, k the wave's
+          // longest list less `target`, so that no list keeps more than
+          // `target`, then moves what is left to the lists' fronts.  A wave
+          // whose fullest list triggers the drain stops while most lists
+          // still have entries, so most lanes work in each of the k passes;
+          // each list is evaluated in the order it was appended, so each
+          // particle's pair order, and its force bits, do not depend on when
+          // the wave drains.
+          auto process = [&](int target) {
+            int longest = 0;  // the wave's longest list (nn <= cap <= 64), bit by bit
+            for (int b = 64; b >= 1; b >>= 1)
+              if (__any(nn >= longest + b)) longest += b;
+            const int k = max(longest - target, 0);
+            const int n = min(nn, k);
             for (int e = 0; e < n; ++e) {
               // Entry: LDS index << 12 | window << 11 | (side + 1) << 9 |
               // (ox + 2) << 6 | (oy + 2) << 3 | (oz + 2).
@@ -1923,21 +1940,22 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
                 flux_pair(fluxes, flux_off, C.ntypes, flux_cut, ti, tj, side == 0 ? p < q : side > 0, r, sme,
                           hsp[q], sq);
             }
+            for (int j = k; j < nn; ++j) nlist[(j - k) * kBT + tid] = nlist[j * kBT + tid];
+            nn -= n;
           };
           // Gather in-cutoff neighbours into the per-thread list, process
-          // the list when it is full or the scan is done (one call site, so
-          // the pair code is instantiated once).  The control flow is the
+          // it when it is full or the scan is done (one call site, so the
+          // pair code is instantiated once).  The control flow is the
           // wave's: every lane visits the offsets in the same order, a chunk
           // loop runs while any lane has candidates left in its cell (a lane
           // that is done keeps its position and appends nothing), and the
-          // whole wave flushes when any lane's list might not hold the next
-          // chunk.  Flushing early does not change a lane's list order, so
-          // each particle's pair order is the same as a per-lane loop's.
+          // wave drains its lists when any lane's list might not hold the
+          // next chunk.  Draining early does not change a lane's list order,
+          // so each particle's pair order is the same as a per-lane loop's.
           int o = 0, q = -1;
           const int hm = R.drop_check ? hmask[h] : 0;  // this cell's near-bound bits
           int4 offn = hoff[0];  // descriptor of offset o, read one offset ahead
           for (;;) {
-            int nn = 0;
             bool full = false;
             while (o < R.n_off) {
               // The offset is the same for the whole wave: its descriptor
@@ -2054,7 +2072,10 @@ __global__ __launch_bounds__(kBT, WAVES) void k_brick(const StepArgs *__restrict
               ++o;
               q = -1;
             }
-            process(nn);
+            // A full list drains to B.drain entries (at most cap - kChunk, so
+            // that the next chunk fits); at the end of the scan every list
+            // drains.
+            process(full ? max(0, min(B.drain, B.cap - kChunk)) : 0);
             if (!full) break;
             offn = hoff[o];  // resume at the same offset
           }
@@ -4028,6 +4049,7 @@ int Engine::launch_substep(int rk_stage, int tick) {
   for (int k = 0; k < 3; ++k) Bk.bdim[k] = bdim_[k];
   Bk.pmax = pmax_;
   Bk.cap = cap_;
+  Bk.drain = cap_ * 2 / 3;
   Bk.hmax = hmax_;
   Bk.hdmax = hdmax_;
   Bk.noff = noff_;
