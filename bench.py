@@ -62,17 +62,17 @@ BYTES_TERM_OFFSET = 4
 
 # HBM traffic per force-kernel launch measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE on the 1M DPD and LJ workloads at the brick the
-# autotuner picks for them ([5,5,3] list 12 and [6,4,1] list 16), scaled by
+# autotuner has picked for them ([5,5,3] list 12 and [6,4,1] list 16), scaled by
 # the copy-kernel calibration (bench.py --copy-calibration under the same
 # counters); the summaries and their commands are kept in
 # profiles/k_brick_dpd1m_counters.md and profiles/k_brick_lj1m_counters.md
 # (bytes per launch).
-MEASURED_TRAFFIC = {("dpd", 1048576): 108.5e6, ("lj", 1048576): 118.5e6}
+MEASURED_TRAFFIC = {("dpd", 1048576): 130.4e6, ("lj", 1048576): 139.1e6}
 # Wave64 VALU instructions per force-kernel launch (rocprofv3 --pmc
 # SQ_INSTS_VALU on the same workloads, bricks and build;
-# profiles/k_brick_dpd1m_valu.md, profiles/k_brick_lj1m_counters.md).  A
+# profiles/k_brick_lanes_dpd1m_lj1m.md).  A
 # wave64 FP32 instruction holds a 16-lane SIMD for 4 cycles.
-MEASURED_VALU = {("dpd", 1048576): 163.8e6, ("lj", 1048576): 532.1e6}
+MEASURED_VALU = {("dpd", 1048576): 141.5e6, ("lj", 1048576): 455.1e6}
 MEASURED_BRICK = {("dpd", 1048576): [[5, 5, 3], 12], ("lj", 1048576): [[6, 4, 1], 16]}
 
 METRIC = "particle-steps/s (1M LJ/DPD)"
