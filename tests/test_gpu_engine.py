@@ -148,8 +148,9 @@ def layout_workload(name, seed):
 def test_brick_layout_bitwise(m, brick, name):
     """Brick shapes, list capacities and the global fallback (pmax = 64
     sends nearly every brick there) give the same per-particle results.
-    Capacity 3 makes the wave flush its lists before almost every chunk of
-    the scan."""
+    Capacity 3 makes the wave drain its lists before almost every chunk of
+    the scan; capacities 8 and 12 drain them only down to two thirds in the
+    middle of the scan, so the entries left must keep their list order."""
     w = layout_workload(name, 45)
     print("case: brick", name, brick, flush=True)
     a, oa, _ = run_steps(m, w, 6, 5)
